@@ -1,0 +1,205 @@
+/*
+ * mettleq_sv.h -- C ABI of the MI355X (gfx950) statevector engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * The Python host binds it with ctypes (mettleq_amd/_native.py); any other
+ * FFI (cffi, a C++ runtime, a serving process) can bind the same symbols.
+ *
+ * Conventions (identical to the reference engine):
+ *   - qubit q of an n-qubit register is bit (n-1-q) of the basis index,
+ *     so qubit 0 is the most significant bit (reference sim.py:10-14,
+ *     shaders/xor_affine.py:12);
+ *   - a 2x2 matrix is passed as 8 doubles, row-major, (re, im) pairs:
+ *     u00.re u00.im u01.re u01.im u10.re u10.im u11.re u11.im;
+ *   - amplitudes are interleaved (re, im) complex64 (MSV_C64) or complex128
+ *     (MSV_C128); indices are 64-bit (the reference stops at 31 qubits
+ *     through uint32 indices, execution.py:21-29).
+ *
+ * Errors: every call returns MSV_OK (0) or a negative-free positive code;
+ * msv_last_error() returns the calling thread's last message.
+ * Ownership: the library owns device buffers, callers own host buffers.
+ * Threading: calls on distinct states may run concurrently from different
+ * host threads; each state owns one HIP stream and all its work is ordered
+ * on it. Calls on ONE state must not overlap.
+ */
+#ifndef METTLEQ_SV_H
+#define METTLEQ_SV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MSV_OK 0
+#define MSV_E_INVALID 1     /* bad argument (ValueError on the Python side)  */
+#define MSV_E_OOM 2         /* device allocation failed                       */
+#define MSV_E_HIP 3         /* HIP runtime error                              */
+#define MSV_E_REFUSED 4     /* refused by a size/limit check                  */
+#define MSV_E_UNSUPPORTED 5 /* valid request this build does not implement    */
+
+/* ---- amplitude types --------------------------------------------------- */
+#define MSV_C64 0
+#define MSV_C128 1
+
+/* ---- phase-term kinds for msv_apply_phase_layer ------------------------ */
+#define MSV_PHASE_ZZ 0     /* exp(-i*theta*Z_a Z_b)   (ZZPHASE, zz.py:78-84)  */
+#define MSV_PHASE_CPHASE 1 /* diag(1,1,1,e^{i theta}) (CPHASE/CZ, diag.py:89) */
+
+/* ---- gate kinds for msv_apply_gates ------------------------------------ */
+#define MSV_G_U 0        /* 2x2 on q0 with ncontrols controls in q[1..]      */
+#define MSV_G_DIAG1 1    /* diag(d0, d1) on q0: m[0..3]                      */
+#define MSV_G_DIAG2 2    /* diag(d00,d01,d10,d11) on (q0 = MSB, q1): m[0..7] */
+#define MSV_G_SWAP 3     /* SWAP(q0, q1), optionally controlled by q[2..]    */
+#define MSV_G_QFT_STAGE 4 /* one QFT stage, see msv_apply_qft_stage          */
+
+typedef struct msv_state msv_state;
+
+/* One gate of a generic gate list (msv_apply_gates). */
+typedef struct msv_gate {
+  int32_t kind;       /* MSV_G_*                                           */
+  int32_t ncontrols;  /* MSV_G_U / MSV_G_SWAP: number of control qubits    */
+  int32_t q[6];       /* target(s) first, then controls                    */
+  double m[8];        /* matrix / diagonal entries (see MSV_G_*)           */
+} msv_gate;
+
+/* Device memory and identity, for the statevector preflight. */
+typedef struct msv_device_info {
+  uint64_t total_bytes;      /* HBM capacity                              */
+  uint64_t free_bytes;       /* free HBM reported by the runtime now      */
+  uint64_t max_alloc_bytes;  /* largest single allocation the runtime allows */
+  int32_t compute_units;
+  int32_t lds_bytes_per_block;
+  char name[96];
+  char arch[32];
+} msv_device_info;
+
+/* ---- library ----------------------------------------------------------- */
+const char *msv_last_error(void);
+int msv_version(void);
+int msv_device_count(int *out);
+int msv_get_device_info(int device, msv_device_info *out);
+
+/* ---- state lifecycle (reference sim.py:17-45) -------------------------- */
+/* Allocates 2^n amplitudes on `device` and sets |0...0>. */
+int msv_state_create(int n_qubits, int dtype, int device, msv_state **out);
+int msv_state_destroy(msv_state *s);
+int msv_state_reset(msv_state *s);
+int msv_state_set_basis(msv_state *s, uint64_t index);
+int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *bytes);
+/* Copies `count` amplitudes at amplitude `offset` between host and device
+ * (host layout = the state's dtype). Synchronous. */
+int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t count);
+int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count);
+/* dst <- src (same qubit count; converts between c64 and c128). */
+int msv_state_copy(msv_state *dst, msv_state *src);
+/* Multiplies every amplitude by (re + i im): global phase (_common.py:231-238). */
+int msv_state_scale(msv_state *s, double re, double im);
+int msv_state_sync(msv_state *s);
+/* Releases the out-of-place scratch buffer kept after an XOR permutation. */
+int msv_state_release_scratch(msv_state *s);
+/* Raw device pointer of the current amplitude buffer (for zero-copy
+ * interop, e.g. DLPack); valid until the next call that permutes out of
+ * place on this state. */
+void *msv_state_device_ptr(msv_state *s);
+
+/* ---- structured layers -------------------------------------------------- */
+/* Per-qubit 2x2 layer: mats[q*8 .. q*8+7] acts on qubit q; active[q] == 0
+ * skips qubit q (NULL = all active). Replaces u2_list_layer_all /
+ * u2_layer_all / rx_layer_all (shaders/single_qubit.py:420-673). */
+int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active);
+
+/* Commuting two-qubit phase terms in one pass: term b acts on qubits
+ * (bonds[2b], bonds[2b+1]) with angle thetas[b]; kinds[b] is MSV_PHASE_*.
+ * Replaces diag_pair_layer / diag_weighted_layer (shaders/diag.py:123-217)
+ * and zz_chain_layer / zz_weighted_layer (shaders/zz.py:99-185). */
+int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bonds,
+                          const double *thetas, int n_terms);
+
+/* Generic diagonal: n1 one-qubit diagonals (qubit q1[k], entries d1[4k..4k+3]
+ * = d0.re d0.im d1.re d1.im) and n2 two-qubit diagonals (qubits q2[2k] (MSB),
+ * q2[2k+1], entries d2[8k..8k+7]) applied as one pass. */
+int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
+                       const int32_t *q2, const double *d2, int n2);
+
+/* GF(2) affine permutation out[y] = state[M'y ^ c]: qubit k of the source
+ * index is parity(rows[k] & y) ^ bit(n-1-k) of c. Replaces
+ * xor_affine_gather (shaders/xor_affine.py:79-116). */
+int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c);
+
+/* QFT without terminal swaps on the contiguous qubits first..m-1 (first =
+ * MSB of the subregister); inverse != 0 applies its exact inverse.
+ * Replaces qft_stage_all (shaders/qft.py:207-236) and sim.qft/iqft
+ * (sim.py:522-563) on contiguous ascending wires. */
+int msv_apply_qft(msv_state *s, int first, int m, int inverse);
+
+/* One QFT stage on qubit j of the subregister [j..m-1]: Hadamard butterfly
+ * plus the controlled-phase ladder toward j+1..m-1 (inverse: adjoint).
+ * Replaces qft_stage_sub (shaders/qft.py:144-181). */
+int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse);
+
+/* Ordered list of generic gates, fused into tile sweeps by the library.
+ * Replaces the per-gate StateVectorSimulator.apply_* calls dispatched by
+ * Device._apply (device.py:1138-1290) and controlled_u_groups
+ * (shaders/controlled_single.py:89-134). */
+int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates);
+
+/* ---- reductions ----------------------------------------------------------- */
+/* FP64 probabilities of the requested wires, first wire = MSB of the output
+ * index (sim.py:322-343); wires == NULL means all qubits in order. `out`
+ * holds 2^nw doubles. Not renormalized. */
+int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out);
+
+/* Draws `shots` basis indices from |a|^2 with a counter-based Philox4x32-10
+ * stream keyed by (seed, stream_id) and writes, per shot, the requested
+ * wires' bits packed MSB-first (wire 0 of the list = highest bit) into
+ * out[shot]. Deterministic for a given state, seed and stream_id. */
+int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots,
+               uint64_t seed, uint64_t stream_id, uint64_t *out);
+
+/* <psi|P_w|psi> for Pauli words P_w = i^{ny_w} X^{xmask_w} Z^{zmask_w}
+ * (masks over qubits: bit q = qubit q; ny_w = number of Y factors). Returns
+ * real and imaginary parts. Replaces pauli_product_expectations
+ * (integrations/_common.py:384-427). */
+int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask,
+                     int n_words, double *out_re, double *out_im);
+
+/* FP64 <a|b> over two states of equal qubit count (any dtypes). */
+int msv_inner(msv_state *a, msv_state *b, double *re, double *im);
+/* FP64 sum |a_i|^2. */
+int msv_norm2(msv_state *s, double *out);
+
+/* ---- tuning knobs ------------------------------------------------------- */
+/* Sets an integer tuning parameter (MSV_TUNE_*); returns MSV_E_INVALID for
+ * unknown keys. Values apply to subsequent launches from any state. */
+#define MSV_TUNE_TILE_LOCAL_BITS 1   /* local bits per tile sweep (c64)        */
+#define MSV_TUNE_TILE_GATE_BITS 2    /* max non-column bits per high sweep     */
+#define MSV_TUNE_PHASE_BLOCKS 3      /* persistent grid size of phase kernels   */
+#define MSV_TUNE_SWEEP_SPLIT 4       /* 1q-layer split policy (0 = auto)        */
+int msv_set_tuning(int key, int64_t value);
+int msv_get_tuning(int key, int64_t *value);
+
+/* ---- launch accounting (execution report) ------------------------------- */
+/* Number of full-state sweep kernels launched on this state since creation
+ * or the last reset of the counter. */
+int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset);
+
+/* ---- timing ------------------------------------------------------------- */
+/* HIP events on the state's own stream: start records the first event,
+ * stop records the second, waits for it and returns the elapsed ms. */
+int msv_state_timer_start(msv_state *s);
+int msv_state_timer_stop(msv_state *s, double *ms);
+
+/* ---- diagnostics -------------------------------------------------------- */
+/* Times a plain device-to-device copy kernel of `bytes` (read + write) and
+ * returns the mean milliseconds per copy; used to calibrate the achievable
+ * HBM bandwidth and hardware byte counters. No reference counterpart. */
+int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* METTLEQ_SV_H */

@@ -1,0 +1,878 @@
+// gfx950 kernels of the statevector engine.
+//
+// Every structured layer is an in-place sweep over the amplitude buffer
+// except the XOR permutation, which gathers into a second buffer. The
+// reference engine launches one out-of-place Metal kernel per 2-4 qubits
+// (shaders/single_qubit.py:165-210, shaders/qft.py:68-109); here one "tile
+// sweep" applies up to a dozen qubits' worth of gates per HBM round trip:
+//
+//   tile      = 2^nlocal amplitudes whose index differs only in the tile's
+//               local bits (the lowest "column" bits for coalescing plus the
+//               gate bits); the remaining index bits number the tiles.
+//   segment   = one assignment of RB local bits to registers: each thread
+//               holds the 2^RB amplitudes that differ in those bits, so any
+//               gate whose target is a register bit is a register-only
+//               butterfly. Segments exchange through LDS.
+//   program   = ordered ops (2x2 with optional controls, QFT stages,
+//               one/two-bit diagonals) executed segment by segment.
+#include <hip/hip_runtime.h>
+
+#include "msv_internal.h"
+
+namespace msv {
+
+namespace {
+
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  V r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) {
+  V r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) {
+  V r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t pdep32(uint32_t v, uint32_t mask) {
+  uint32_t r = 0;
+  for (uint32_t m = mask; m; m &= m - 1u) {
+    if (v & 1u) r |= m & (0u - m);
+    v >>= 1;
+  }
+  return r;
+}
+
+template <typename T> struct CisPi;
+template <> struct CisPi<float> {
+  // exp(i*pi*x/2^w) for an integer x < 2^w; x/2^w is formed exactly up to the
+  // 24-bit rounding of x, and sincospif reduces the argument in units of pi.
+  __device__ static float2 eval(uint64_t x, uint32_t w) {
+    float f = ldexpf(static_cast<float>(x), -static_cast<int>(w));
+    float s, c;
+    sincospif(f, &s, &c);
+    return make_float2(c, s);
+  }
+};
+template <> struct CisPi<double> {
+  __device__ static double2 eval(uint64_t x, uint32_t w) {
+    double f = ldexp(static_cast<double>(x), -static_cast<int>(w));
+    double s, c;
+    sincospi(f, &s, &c);
+    return make_double2(c, s);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// tile-program building blocks (RB register bits, J = target register bit)
+// ---------------------------------------------------------------------------
+
+template <typename T, int RB, int J, bool CTRL>
+__device__ __forceinline__ void u2_pairs(typename Vec2<T>::type *a, const T *m, bool tok,
+                                         uint32_t creg) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    const V v0 = a[r], v1 = a[r1];
+    V n0, n1;
+    n0.x = m[0] * v0.x - m[1] * v0.y + m[2] * v1.x - m[3] * v1.y;
+    n0.y = m[0] * v0.y + m[1] * v0.x + m[2] * v1.y + m[3] * v1.x;
+    n1.x = m[4] * v0.x - m[5] * v0.y + m[6] * v1.x - m[7] * v1.y;
+    n1.y = m[4] * v0.y + m[5] * v0.x + m[6] * v1.y + m[7] * v1.x;
+    if (CTRL) {
+      const bool ok = tok && ((static_cast<uint32_t>(r) & creg) == creg);
+      a[r].x = ok ? n0.x : v0.x;
+      a[r].y = ok ? n0.y : v0.y;
+      a[r1].x = ok ? n1.x : v1.x;
+      a[r1].y = ok ? n1.y : v1.y;
+    } else {
+      a[r] = n0;
+      a[r1] = n1;
+    }
+  }
+}
+
+template <typename T, int RB, bool CTRL>
+__device__ __forceinline__ void u2_dispatch(typename Vec2<T>::type *a, const T *m, int j, bool tok,
+                                            uint32_t creg) {
+  switch (j) {
+    case 0: u2_pairs<T, RB, 0, CTRL>(a, m, tok, creg); break;
+    case 1: if constexpr (RB > 1) u2_pairs<T, RB, 1, CTRL>(a, m, tok, creg); break;
+    case 2: if constexpr (RB > 2) u2_pairs<T, RB, 2, CTRL>(a, m, tok, creg); break;
+    case 3: if constexpr (RB > 3) u2_pairs<T, RB, 3, CTRL>(a, m, tok, creg); break;
+    case 4: if constexpr (RB > 4) u2_pairs<T, RB, 4, CTRL>(a, m, tok, creg); break;
+    default: break;
+  }
+}
+
+// QFT stage on register bit J. The ladder phase of the amplitude with the
+// stage bit set is exp(i*pi*L/2^(p-lb)), L = index bits [lb, p). It factors
+// into the thread part wt (bits outside the registers) and the register
+// part from the JL register bits directly below J, which are index bits
+// p-1 .. p-JL: exp(i*pi*v/2^JL) = w32[v << (4-JL)].
+template <typename T, int RB, int J, int JL, bool INV>
+__device__ __forceinline__ void qft_pairs(typename Vec2<T>::type *a, typename Vec2<T>::type wt,
+                                          const T *w32) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    V w = wt;
+    if constexpr (JL > 0) {
+      const int v = (r >> (J - JL)) & ((1 << JL) - 1);
+      const int mi = v << (4 - JL);
+      if (mi != 0) {
+        V e;
+        e.x = w32[2 * mi];
+        e.y = w32[2 * mi + 1];
+        w = cmul(wt, e);
+      }
+    }
+    if (INV) w.y = -w.y;
+    const V v0 = a[r], v1 = a[r1];
+    if (!INV) {
+      a[r] = cadd(v0, v1);
+      a[r1] = cmul(csub(v0, v1), w);
+    } else {
+      const V t = cmul(v1, w);
+      a[r] = cadd(v0, t);
+      a[r1] = csub(v0, t);
+    }
+  }
+}
+
+#define MSV_QFT_CASE(JJ, LL)                                                   \
+  case (JJ) * 8 + (LL):                                                        \
+    if constexpr (RB > (JJ)) {                                                 \
+      if (inv) qft_pairs<T, RB, JJ, LL, true>(a, wt, w32);                     \
+      else qft_pairs<T, RB, JJ, LL, false>(a, wt, w32);                        \
+    }                                                                          \
+    break;
+
+template <typename T, int RB>
+__device__ __forceinline__ void qft_dispatch(typename Vec2<T>::type *a, typename Vec2<T>::type wt,
+                                             const T *w32, int j, int jl, bool inv) {
+  switch (j * 8 + jl) {
+    MSV_QFT_CASE(0, 0)
+    MSV_QFT_CASE(1, 0) MSV_QFT_CASE(1, 1)
+    MSV_QFT_CASE(2, 0) MSV_QFT_CASE(2, 1) MSV_QFT_CASE(2, 2)
+    MSV_QFT_CASE(3, 0) MSV_QFT_CASE(3, 1) MSV_QFT_CASE(3, 2) MSV_QFT_CASE(3, 3)
+    MSV_QFT_CASE(4, 0) MSV_QFT_CASE(4, 1) MSV_QFT_CASE(4, 2) MSV_QFT_CASE(4, 3)
+    MSV_QFT_CASE(4, 4)
+    default: break;
+  }
+}
+#undef MSV_QFT_CASE
+
+template <typename T, int RB>
+__device__ __forceinline__ void diag_apply(typename Vec2<T>::type *a, const TileOp<T> &op,
+                                           uint64_t gb) {
+  using V = typename Vec2<T>::type;
+  const int ra = op.ra, rb = op.rb;
+  const bool two = op.nbits == 2;
+  const uint32_t ta = (gb & op.amask) ? 1u : 0u;
+  const uint32_t tb = (gb & op.bmask) ? 1u : 0u;
+  V d0, d1, d2, d3;
+  d0.x = op.m[0]; d0.y = op.m[1];
+  d1.x = op.m[2]; d1.y = op.m[3];
+  d2.x = op.m[4]; d2.y = op.m[5];
+  d3.x = op.m[6]; d3.y = op.m[7];
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    const uint32_t ba = (ra >= 0) ? ((static_cast<uint32_t>(r) >> ra) & 1u) : ta;
+    V dv;
+    if (two) {
+      const uint32_t bb = (rb >= 0) ? ((static_cast<uint32_t>(r) >> rb) & 1u) : tb;
+      const T lox = bb ? d1.x : d0.x, loy = bb ? d1.y : d0.y;
+      const T hix = bb ? d3.x : d2.x, hiy = bb ? d3.y : d2.y;
+      dv.x = ba ? hix : lox;
+      dv.y = ba ? hiy : loy;
+    } else {
+      dv.x = ba ? d1.x : d0.x;
+      dv.y = ba ? d1.y : d0.y;
+    }
+    a[r] = cmul(a[r], dv);
+  }
+}
+
+template <typename T, int RB>
+__device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArgs<T> &A,
+                                        uint32_t ob, uint32_t oe, uint64_t gb) {
+  using V = typename Vec2<T>::type;
+  for (uint32_t o = ob; o < oe; ++o) {
+    const TileOp<T> &op = A.ops[o];
+    const uint8_t kind = op.kind;
+    if (kind == OP_U2) {
+      T m[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) m[k] = op.m[k];
+      if (!op.has_ctrl) {
+        u2_dispatch<T, RB, false>(a, m, op.rpos, true, 0u);
+      } else {
+        const bool tok = (gb & op.cmask_t) == op.cmask_t;
+        u2_dispatch<T, RB, true>(a, m, op.rpos, tok, op.cmask_r);
+      }
+    } else if (kind == OP_QFT_F || kind == OP_QFT_I) {
+      const uint32_t w = static_cast<uint32_t>(op.pbit) - op.lb;
+      const uint64_t x = (gb >> op.lb) & ((w >= 64) ? ~0ull : ((1ull << w) - 1ull));
+      const V wt = CisPi<T>::eval(x, w);
+      qft_dispatch<T, RB>(a, wt, A.w32, op.rpos, op.jl, kind == OP_QFT_I);
+    } else if (kind == OP_DIAG) {
+      diag_apply<T, RB>(a, op, gb);
+    }
+  }
+}
+
+template <int RB>
+__device__ __forceinline__ uint64_t reg_goff(int r, const uint64_t *gstr) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+    if ((r >> j) & 1) o += gstr[j];
+  return o;
+}
+template <int RB>
+__device__ __forceinline__ uint32_t reg_loff(int r, const uint32_t *lstr) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+    if ((r >> j) & 1) o |= lstr[j];
+  return o;
+}
+
+// XOR swizzle of LDS slots: when lanes step by 2^RB slots (registers on the
+// lowest local bits) they would all hit one bank pair; folding bits above
+// RB into the low bits spreads them while staying a bijection.
+template <int RB>
+__device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
+  constexpr uint32_t kMask = (RB >= 5) ? 15u : ((RB == 4) ? 7u : 0u);
+  return l ^ ((l >> RB) & kMask);
+}
+
+template <typename T, int RB>
+__global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict__ st,
+                                              const TileArgs<T> A) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nlocal = A.nlocal;
+  const uint32_t lmask = (nlocal >= 32) ? 0xffffffffu : ((1u << nlocal) - 1u);
+  for (uint64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+    uint64_t base = 0;
+    {
+      uint64_t m = A.nonlocal_mask, t = tile;
+      while (m) {
+        if (t & 1ull) base |= m & (0ull - m);
+        t >>= 1;
+        m &= m - 1ull;
+      }
+    }
+    V a[R];
+    for (uint32_t s = 0; s < A.nseg; ++s) {
+      const TileSeg sg = A.seg[s];
+      uint32_t regmask = 0;
+      uint64_t gstr[RB];
+      uint32_t lstr[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        regmask |= 1u << sg.reg[j];
+        gstr[j] = 1ull << A.lpos[sg.reg[j]];
+        lstr[j] = 1u << sg.reg[j];
+      }
+      const uint32_t tl = pdep32(tid, lmask & ~regmask);
+      uint64_t gb = base;
+      for (uint32_t k = 0; k < nlocal; ++k)
+        if ((tl >> k) & 1u) gb |= 1ull << A.lpos[k];
+      if (s == 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = st[gb + reg_goff<RB>(r, gstr)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = lds[lds_swz<RB>(tl | reg_loff<RB>(r, lstr))];
+      }
+      run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, gb);
+      if (s + 1 == A.nseg) {
+        if (A.scale != T(1)) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            a[r].x *= A.scale;
+            a[r].y *= A.scale;
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) st[gb + reg_goff<RB>(r, gstr)] = a[r];
+      } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_swz<RB>(tl | reg_loff<RB>(r, lstr))] = a[r];
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// diagonal phase layer: product of per-window lookup tables
+// ---------------------------------------------------------------------------
+
+template <typename T> struct Pack;
+template <> struct Pack<float> { using P = float4; static constexpr int kAmps = 2; };
+template <> struct Pack<double> { using P = double2; static constexpr int kAmps = 1; };
+
+__device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) {
+  const uint32_t a = static_cast<uint32_t>(i >> w.s1) & ((1u << w.w1) - 1u);
+  const uint32_t b = static_cast<uint32_t>(i >> w.s2) & ((1u << w.w2) - 1u);
+  return w.off + (a | (b << w.w1));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restrict__ st,
+                                               const typename Vec2<T>::type *__restrict__ lut_g,
+                                               const PhaseArgs A) {
+  using V = typename Vec2<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  V *lut = reinterpret_cast<V *>(smem_raw);
+  for (uint32_t i = threadIdx.x; i < A.lut_len; i += blockDim.x) lut[i] = lut_g[i];
+  __syncthreads();
+  constexpr int KA = Pack<T>::kAmps;
+  const uint64_t npack = A.n_amps / KA;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t pk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; pk < npack;
+       pk += stride) {
+    const uint64_t i0 = pk * KA;
+    V ph;
+    ph.x = T(1);
+    ph.y = T(0);
+    for (uint32_t w = A.nlow; w < A.nwin; ++w) ph = cmul(ph, lut[win_index(A.win[w], i0)]);
+    V amp[KA];
+    if constexpr (KA == 2) {
+      const float4 p = reinterpret_cast<const float4 *>(st)[pk];
+      amp[0] = make_float2(p.x, p.y);
+      amp[1] = make_float2(p.z, p.w);
+    } else {
+      amp[0] = st[i0];
+    }
+#pragma unroll
+    for (int d = 0; d < KA; ++d) {
+      V p = ph;
+      for (uint32_t w = 0; w < A.nlow; ++w) p = cmul(p, lut[win_index(A.win[w], i0 + d)]);
+      amp[d] = cmul(amp[d], p);
+    }
+    if constexpr (KA == 2) {
+      reinterpret_cast<float4 *>(st)[pk] = make_float4(amp[0].x, amp[0].y, amp[1].x, amp[1].y);
+    } else {
+      st[i0] = amp[0];
+    }
+  }
+  // a 1-amplitude register (n = 0 never occurs; n = 1 with KA = 2 is even)
+}
+
+// ---------------------------------------------------------------------------
+// GF(2) affine gather: out[y] = in[c ^ T0[y&255] ^ T1[(y>>8)&255] ^ ...]
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type *__restrict__ src,
+                                                    typename Vec2<T>::type *__restrict__ dst,
+                                                    uint64_t n_amps,
+                                                    const uint64_t *__restrict__ tables, int ntab,
+                                                    uint64_t c) {
+  using V = typename Vec2<T>::type;
+  __shared__ uint64_t tab[8 * 256];
+  for (int i = threadIdx.x; i < ntab * 256; i += blockDim.x) tab[i] = tables[i];
+  __syncthreads();
+  constexpr int KA = Pack<T>::kAmps;
+  const uint64_t npack = n_amps / KA;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t pk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; pk < npack;
+       pk += stride) {
+    const uint64_t y0 = pk * KA;
+    uint64_t s0 = c;
+    for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
+    if constexpr (KA == 2) {
+      // y0 is even: src(y0 + 1) = src(y0) ^ column of index bit 0
+      const uint64_t s1 = s0 ^ tab[1];
+      const V a0 = src[s0];
+      const V a1 = src[s1];
+      reinterpret_cast<float4 *>(dst)[pk] = make_float4(a0.x, a0.y, a1.x, a1.y);
+    } else {
+      dst[y0] = src[s0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise helpers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_basis(typename Vec2<T>::type *__restrict__ st,
+                                                    uint64_t n_amps, uint64_t index) {
+  using V = typename Vec2<T>::type;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride) {
+    V v;
+    v.x = (i == index) ? T(1) : T(0);
+    v.y = T(0);
+    st[i] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scale(typename Vec2<T>::type *__restrict__ st,
+                                               uint64_t n_amps, T re, T im) {
+  using V = typename Vec2<T>::type;
+  V f;
+  f.x = re;
+  f.y = im;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride)
+    st[i] = cmul(st[i], f);
+}
+
+template <typename Ts, typename Td>
+__global__ void __launch_bounds__(256) k_convert(const typename Vec2<Ts>::type *__restrict__ src,
+                                                 typename Vec2<Td>::type *__restrict__ dst,
+                                                 uint64_t n_amps) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride) {
+    typename Vec2<Td>::type v;
+    v.x = static_cast<Td>(src[i].x);
+    v.y = static_cast<Td>(src[i].y);
+    dst[i] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ src,
+                                                    float4 *__restrict__ dst, uint64_t n_vec) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_vec;
+       i += stride)
+    dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions. Summation orders are fixed so that a CPU restatement of
+// the same order reproduces the sums bit for bit (sampling depends on it).
+// ---------------------------------------------------------------------------
+
+template <typename V>
+__device__ __forceinline__ double amp_prob(V v) {
+#pragma clang fp contract(off)
+  const double re = static_cast<double>(v.x);
+  const double im = static_cast<double>(v.y);
+  const double a = re * re;
+  const double b = im * im;
+  return a + b;
+}
+
+// Lane l of the wave owning block b sums elements b*4096 + j*64 + l for
+// j = 0..63 in order; the block sum adds the 64 lane sums in lane order.
+template <typename T>
+__device__ __forceinline__ double lane_sum(const typename Vec2<T>::type *st, uint64_t n_amps,
+                                           uint64_t b, uint32_t lane) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  const uint64_t base = b * kSampleBlock + lane;
+  for (int j = 0; j < kSampleBlock / kSampleLanes; ++j) {
+    const uint64_t idx = base + static_cast<uint64_t>(j) * kSampleLanes;
+    const double p = (idx < n_amps) ? amp_prob(st[idx]) : 0.0;
+    s = s + p;
+  }
+  return s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_block_probsum(const typename Vec2<T>::type *__restrict__ st,
+                                                       uint64_t n_amps, double *__restrict__ out,
+                                                       uint64_t n_blocks) {
+#pragma clang fp contract(off)
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= n_blocks) return;  // uniform per wave
+  const double s = lane_sum<T>(st, n_amps, b, lane);
+  double tot = 0.0;
+  for (int l = 0; l < 64; ++l) tot = tot + __shfl(s, l, 64);
+  if (lane == 0) out[b] = tot;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sample_resolve(const typename Vec2<T>::type *__restrict__ st,
+                                                        uint64_t n_amps,
+                                                        const uint64_t *__restrict__ shot_block,
+                                                        const double *__restrict__ shot_resid,
+                                                        uint64_t *__restrict__ shot_index,
+                                                        uint64_t shots) {
+#pragma clang fp contract(off)
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t shot = static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (shot >= shots) return;  // uniform per wave
+  const uint64_t b = shot_block[shot];
+  const double r = shot_resid[shot];
+  const double s = lane_sum<T>(st, n_amps, b, lane);
+  double c = 0.0, r2 = 0.0;
+  int L = -1, last_pos = -1;
+  for (int l = 0; l < 64; ++l) {
+    const double v = __shfl(s, l, 64);
+    if (v > 0.0) last_pos = l;
+    const double cn = c + v;
+    if (L < 0 && cn > r) {
+      L = l;
+      r2 = r - c;
+    }
+    c = cn;
+  }
+  if (L < 0) {
+    L = (last_pos < 0) ? 0 : last_pos;
+    r2 = __builtin_inf();
+  }
+  if (static_cast<int>(lane) == L) {
+    double c2 = 0.0;
+    int J = -1, last_j = -1;
+    const uint64_t base = b * kSampleBlock + lane;
+    for (int j = 0; j < kSampleBlock / kSampleLanes; ++j) {
+      const uint64_t idx = base + static_cast<uint64_t>(j) * kSampleLanes;
+      const double p = (idx < n_amps) ? amp_prob(st[idx]) : 0.0;
+      if (p > 0.0) last_j = j;
+      const double cn = c2 + p;
+      if (J < 0 && cn > r2) J = j;
+      c2 = cn;
+    }
+    if (J < 0) J = (last_j < 0) ? 0 : last_j;
+    shot_index[shot] = base + static_cast<uint64_t>(J) * kSampleLanes;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_probs_full(const typename Vec2<T>::type *__restrict__ st,
+                                                    uint64_t n_amps, double *__restrict__ out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride)
+    out[i] = amp_prob(st[i]);
+}
+
+// partials[chunk * 2^nw + k] = sum over the chunk's "other" indices o of
+// p[scatter(k) | pdep(o)]; thread = (k, chunk), k fastest.
+template <typename T>
+__global__ void __launch_bounds__(256) k_marginal(const typename Vec2<T>::type *__restrict__ st,
+                                                  int n, const uint64_t *__restrict__ out_bits,
+                                                  int nw, double *__restrict__ partials,
+                                                  uint64_t chunks, uint64_t chunk_len) {
+#pragma clang fp contract(off)
+  const uint64_t K = 1ull << nw;
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= K * chunks) return;
+  const uint64_t k = t % K, chunk = t / K;
+  uint64_t kb = 0, wmask = 0;
+  for (int w = 0; w < nw; ++w) {
+    const uint64_t bit = 1ull << out_bits[w];
+    wmask |= bit;
+    if ((k >> (nw - 1 - w)) & 1ull) kb |= bit;
+  }
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  const uint64_t omask = all & ~wmask;
+  // o0 = pdep(chunk * chunk_len, omask)
+  uint64_t o = 0;
+  {
+    uint64_t v = chunk * chunk_len, m = omask;
+    while (m) {
+      if (v & 1ull) o |= m & (0ull - m);
+      v >>= 1;
+      m &= m - 1ull;
+    }
+  }
+  double s = 0.0;
+  for (uint64_t q = 0; q < chunk_len; ++q) {
+    s = s + amp_prob(st[kb | o]);
+    o = (o - omask) & omask;  // next subset of omask in increasing order
+  }
+  partials[chunk * K + k] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pauli(const typename Vec2<T>::type *__restrict__ st,
+                                               uint64_t n_amps, const uint64_t *__restrict__ xm,
+                                               const uint64_t *__restrict__ zm, int nwords,
+                                               double *__restrict__ partials) {
+#pragma clang fp contract(off)
+  using V = typename Vec2<T>::type;
+  __shared__ double red[2 * 256];
+  double acc_re[kMaxWords], acc_im[kMaxWords];
+  uint64_t xs[kMaxWords], zs[kMaxWords];
+#pragma unroll
+  for (int w = 0; w < kMaxWords; ++w) {
+    acc_re[w] = 0.0;
+    acc_im[w] = 0.0;
+    xs[w] = (w < nwords) ? xm[w] : 0ull;
+    zs[w] = (w < nwords) ? zm[w] : 0ull;
+  }
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride) {
+    const V ai = st[i];
+    const double ar = ai.x, aim = ai.y;
+#pragma unroll
+    for (int w = 0; w < kMaxWords; ++w) {
+      if (w < nwords) {
+        double br = ar, bi = aim;
+        if (xs[w]) {
+          const V aj = st[i ^ xs[w]];
+          br = aj.x;
+          bi = aj.y;
+        }
+        // conj(b) * a
+        double tr = br * ar + bi * aim;
+        double ti = br * aim - bi * ar;
+        if (__popcll(i & zs[w]) & 1) {
+          tr = -tr;
+          ti = -ti;
+        }
+        acc_re[w] = acc_re[w] + tr;
+        acc_im[w] = acc_im[w] + ti;
+      }
+    }
+  }
+  for (int w = 0; w < nwords; ++w) {
+    red[threadIdx.x] = acc_re[w];
+    red[256 + threadIdx.x] = acc_im[w];
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+        red[256 + threadIdx.x] = red[256 + threadIdx.x] + red[256 + threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partials[(static_cast<uint64_t>(blockIdx.x) * nwords + w) * 2] = red[0];
+      partials[(static_cast<uint64_t>(blockIdx.x) * nwords + w) * 2 + 1] = red[256];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename Ta, typename Tb>
+__global__ void __launch_bounds__(256) k_inner(const typename Vec2<Ta>::type *__restrict__ a,
+                                               const typename Vec2<Tb>::type *__restrict__ b,
+                                               uint64_t n_amps, double *__restrict__ partials) {
+#pragma clang fp contract(off)
+  __shared__ double red[2 * 256];
+  double sr = 0.0, si = 0.0;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
+       i += stride) {
+    const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+    const double tr = ar * br + ai * bi;
+    const double ti = ar * bi - ai * br;
+    sr = sr + tr;
+    si = si + ti;
+  }
+  red[threadIdx.x] = sr;
+  red[256 + threadIdx.x] = si;
+  __syncthreads();
+  for (uint32_t s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+      red[256 + threadIdx.x] = red[256 + threadIdx.x] + red[256 + threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = red[0];
+    partials[2 * blockIdx.x + 1] = red[256];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, int rb,
+                       uint32_t grid, hipStream_t stream) {
+  using V = typename Vec2<T>::type;
+  const uint32_t threads = 1u << (args.nlocal - rb);
+  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  switch (rb) {
+    case 1: hipLaunchKernelGGL((k_tile<T, 1>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 2: hipLaunchKernelGGL((k_tile<T, 2>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 3: hipLaunchKernelGGL((k_tile<T, 3>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 4: hipLaunchKernelGGL((k_tile<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 5:
+      if constexpr (TileCfg<T>::kRB >= 5) {
+        hipLaunchKernelGGL((k_tile<T, 5>), dim3(grid), dim3(threads), lds, stream, state, args);
+        break;
+      }
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_phase(typename Vec2<T>::type *state, const typename Vec2<T>::type *lut,
+                        const PhaseArgs &args, uint32_t grid, hipStream_t stream) {
+  using V = typename Vec2<T>::type;
+  hipLaunchKernelGGL((k_phase<T>), dim3(grid), dim3(256), sizeof(V) * args.lut_len, stream, state,
+                     lut, args);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xor_gather(const typename Vec2<T>::type *src, typename Vec2<T>::type *dst,
+                             uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
+                             uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_xor_gather<T>), dim3(grid), dim3(256), 0, stream, src, dst, n_amps, tables,
+                     ntab, c);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_init_basis(typename Vec2<T>::type *state, uint64_t n_amps, uint64_t index,
+                             uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_init_basis<T>), dim3(grid), dim3(256), 0, stream, state, n_amps, index);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale(typename Vec2<T>::type *state, uint64_t n_amps, double re, double im,
+                        uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_scale<T>), dim3(grid), dim3(256), 0, stream, state, n_amps,
+                     static_cast<T>(re), static_cast<T>(im));
+  return hipGetLastError();
+}
+
+template <typename Ts, typename Td>
+hipError_t launch_convert(const typename Vec2<Ts>::type *src, typename Vec2<Td>::type *dst,
+                          uint64_t n_amps, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_convert<Ts, Td>), dim3(grid), dim3(256), 0, stream, src, dst, n_amps);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_block_probsum(const typename Vec2<T>::type *state, uint64_t n_amps,
+                                double *block_sums, uint64_t n_blocks, hipStream_t stream) {
+  const uint64_t grid = (n_blocks + 3) / 4;
+  hipLaunchKernelGGL((k_block_probsum<T>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, stream,
+                     state, n_amps, block_sums, n_blocks);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sample_resolve(const typename Vec2<T>::type *state, uint64_t n_amps,
+                                 const uint64_t *shot_block, const double *shot_resid,
+                                 uint64_t *shot_index, uint64_t shots, hipStream_t stream) {
+  const uint64_t grid = (shots + 3) / 4;
+  hipLaunchKernelGGL((k_sample_resolve<T>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0,
+                     stream, state, n_amps, shot_block, shot_resid, shot_index, shots);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_probs_full(const typename Vec2<T>::type *state, uint64_t n_amps, double *out,
+                             uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_probs_full<T>), dim3(grid), dim3(256), 0, stream, state, n_amps, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_marginal(const typename Vec2<T>::type *state, int n, const uint64_t *out_bits,
+                           int nw, double *partials, uint64_t chunks, uint64_t chunk_len,
+                           hipStream_t stream) {
+  const uint64_t threads = (1ull << nw) * chunks;
+  const uint64_t grid = (threads + 255) / 256;
+  hipLaunchKernelGGL((k_marginal<T>), dim3(static_cast<uint32_t>(grid)), dim3(256), 0, stream,
+                     state, n, out_bits, nw, partials, chunks, chunk_len);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pauli(const typename Vec2<T>::type *state, uint64_t n_amps,
+                        const uint64_t *xmask, const uint64_t *zmask, int nwords,
+                        double *partials, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_pauli<T>), dim3(grid), dim3(256), 0, stream, state, n_amps, xmask, zmask,
+                     nwords, partials);
+  return hipGetLastError();
+}
+
+template <typename Ta, typename Tb>
+hipError_t launch_inner(const typename Vec2<Ta>::type *a, const typename Vec2<Tb>::type *b,
+                        uint64_t n_amps, double *partials, uint32_t grid, hipStream_t stream) {
+  hipLaunchKernelGGL((k_inner<Ta, Tb>), dim3(grid), dim3(256), 0, stream, a, b, n_amps, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uint32_t grid,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, stream, src, dst, n_vec);
+  return hipGetLastError();
+}
+
+#define MSV_INST_T(T)                                                                            \
+  template hipError_t launch_tile<T>(Vec2<T>::type *, const TileArgs<T> &, int, uint32_t,        \
+                                     hipStream_t);                                               \
+  template hipError_t launch_phase<T>(Vec2<T>::type *, const Vec2<T>::type *, const PhaseArgs &, \
+                                      uint32_t, hipStream_t);                                    \
+  template hipError_t launch_xor_gather<T>(const Vec2<T>::type *, Vec2<T>::type *, uint64_t,     \
+                                           const uint64_t *, int, uint64_t, uint32_t,            \
+                                           hipStream_t);                                         \
+  template hipError_t launch_init_basis<T>(Vec2<T>::type *, uint64_t, uint64_t, uint32_t,        \
+                                           hipStream_t);                                         \
+  template hipError_t launch_scale<T>(Vec2<T>::type *, uint64_t, double, double, uint32_t,       \
+                                      hipStream_t);                                              \
+  template hipError_t launch_block_probsum<T>(const Vec2<T>::type *, uint64_t, double *,         \
+                                              uint64_t, hipStream_t);                            \
+  template hipError_t launch_sample_resolve<T>(const Vec2<T>::type *, uint64_t,                  \
+                                               const uint64_t *, const double *, uint64_t *,     \
+                                               uint64_t, hipStream_t);                           \
+  template hipError_t launch_probs_full<T>(const Vec2<T>::type *, uint64_t, double *, uint32_t,  \
+                                           hipStream_t);                                         \
+  template hipError_t launch_marginal<T>(const Vec2<T>::type *, int, const uint64_t *, int,      \
+                                         double *, uint64_t, uint64_t, hipStream_t);             \
+  template hipError_t launch_pauli<T>(const Vec2<T>::type *, uint64_t, const uint64_t *,         \
+                                      const uint64_t *, int, double *, uint32_t, hipStream_t);
+
+MSV_INST_T(float)
+MSV_INST_T(double)
+#undef MSV_INST_T
+
+template hipError_t launch_convert<float, double>(const float2 *, double2 *, uint64_t, uint32_t,
+                                                  hipStream_t);
+template hipError_t launch_convert<double, float>(const double2 *, float2 *, uint64_t, uint32_t,
+                                                  hipStream_t);
+template hipError_t launch_convert<float, float>(const float2 *, float2 *, uint64_t, uint32_t,
+                                                 hipStream_t);
+template hipError_t launch_convert<double, double>(const double2 *, double2 *, uint64_t, uint32_t,
+                                                   hipStream_t);
+template hipError_t launch_inner<float, float>(const float2 *, const float2 *, uint64_t, double *,
+                                               uint32_t, hipStream_t);
+template hipError_t launch_inner<float, double>(const float2 *, const double2 *, uint64_t,
+                                                double *, uint32_t, hipStream_t);
+template hipError_t launch_inner<double, float>(const double2 *, const float2 *, uint64_t,
+                                                double *, uint32_t, hipStream_t);
+template hipError_t launch_inner<double, double>(const double2 *, const double2 *, uint64_t,
+                                                 double *, uint32_t, hipStream_t);
+
+}  // namespace msv

@@ -1,0 +1,1670 @@
+// Host runtime of the statevector engine: state lifecycle, sweep planning,
+// parameter staging and the C ABI declared in include/mettleq_sv.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mettleq_sv.h"
+#include "msv_internal.h"
+#include "msv_philox.h"
+
+using namespace msv;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define MSV_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(MSV_E_HIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define MSV_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != MSV_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// tuning
+// ---------------------------------------------------------------------------
+
+struct Tuning {
+  int local_bits_c64 = 13;  // 2^13 complex64 = 64 KiB LDS per tile
+  int local_bits_c128 = 12; // 2^12 complex128 = 64 KiB
+  int gate_bits = 9;        // index bits above the columns per high sweep
+  int phase_blocks = 0;     // 0 = CUs * 8
+  int sweep_split = 0;
+  int tile_blocks_per_cu = 8;
+};
+Tuning g_tune;
+std::mutex g_tune_mu;
+
+struct DevProps {
+  int cu = 0;
+  bool ok = false;
+};
+DevProps g_props[64];
+std::mutex g_props_mu;
+
+int dev_cu(int device) {
+  std::lock_guard<std::mutex> lk(g_props_mu);
+  if (device < 0 || device >= 64) return 256;
+  if (!g_props[device].ok) {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cu <= 0)
+      cu = 256;
+    g_props[device].cu = cu;
+    g_props[device].ok = true;
+  }
+  return g_props[device].cu;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// state
+// ---------------------------------------------------------------------------
+
+struct msv_state {
+  int n = 0;
+  int dtype = MSV_C64;
+  int device = 0;
+  int cu = 256;
+  uint64_t amps = 0;
+  void *buf = nullptr;  // current amplitudes
+  void *alt = nullptr;  // scratch for out-of-place permutations
+  hipStream_t stream = nullptr;
+  char *h_stage = nullptr;  // pinned host staging ring for kernel parameters
+  char *d_stage = nullptr;  // device side of the ring
+  size_t stage_cap = 0;
+  size_t stage_head = 0;
+  void *d_red = nullptr;  // reduction scratch
+  size_t red_cap = 0;
+  uint64_t sweeps = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timer on this state's stream
+};
+
+namespace {
+
+size_t esize(int dtype) { return dtype == MSV_C128 ? 16 : 8; }
+
+int check_state(const msv_state *s) {
+  if (!s) return fail(MSV_E_INVALID, "null state handle");
+  if (!s->buf) return fail(MSV_E_INVALID, "state has no amplitude buffer");
+  return MSV_OK;
+}
+
+int set_dev(const msv_state *s) {
+  MSV_HIP(hipSetDevice(s->device));
+  return MSV_OK;
+}
+
+// Copies `bytes` of host data into the staging ring and enqueues the upload
+// on the state's stream; returns the device address. Wrapping the ring first
+// drains the stream, so no in-flight copy or kernel still reads a slot.
+int stage(msv_state *s, const void *src, size_t bytes, void **dev) {
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (need > s->stage_cap) {
+    MSV_HIP(hipStreamSynchronize(s->stream));
+    if (s->h_stage) MSV_HIP(hipHostFree(s->h_stage));
+    if (s->d_stage) MSV_HIP(hipFree(s->d_stage));
+    s->h_stage = nullptr;
+    s->d_stage = nullptr;
+    size_t cap = std::max<size_t>(need, size_t(1) << 20);
+    if (hipHostMalloc(reinterpret_cast<void **>(&s->h_stage), cap, hipHostMallocDefault) !=
+        hipSuccess)
+      return fail(MSV_E_OOM, "pinned staging allocation failed");
+    if (hipMalloc(reinterpret_cast<void **>(&s->d_stage), cap) != hipSuccess)
+      return fail(MSV_E_OOM, "device staging allocation failed");
+    s->stage_cap = cap;
+    s->stage_head = 0;
+  }
+  if (s->stage_head + need > s->stage_cap) {
+    MSV_HIP(hipStreamSynchronize(s->stream));
+    s->stage_head = 0;
+  }
+  char *h = s->h_stage + s->stage_head;
+  char *d = s->d_stage + s->stage_head;
+  std::memcpy(h, src, bytes);
+  MSV_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s->stream));
+  s->stage_head += need;
+  *dev = d;
+  return MSV_OK;
+}
+
+int ensure_red(msv_state *s, size_t bytes) {
+  if (bytes <= s->red_cap) return MSV_OK;
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  if (s->d_red) MSV_HIP(hipFree(s->d_red));
+  s->d_red = nullptr;
+  s->red_cap = 0;
+  const size_t cap = std::max<size_t>(bytes, size_t(1) << 20);
+  if (hipMalloc(&s->d_red, cap) != hipSuccess)
+    return fail(MSV_E_OOM, "reduction scratch allocation failed (" + std::to_string(cap) + " B)");
+  s->red_cap = cap;
+  return MSV_OK;
+}
+
+int ensure_alt(msv_state *s) {
+  if (s->alt) return MSV_OK;
+  const size_t bytes = s->amps * esize(s->dtype);
+  if (hipMalloc(&s->alt, bytes) != hipSuccess) {
+    s->alt = nullptr;
+    (void)hipGetLastError();
+    return fail(MSV_E_OOM, "out-of-place permutation needs a second " + std::to_string(bytes) +
+                               "-byte state buffer and the allocation failed");
+  }
+  return MSV_OK;
+}
+
+uint32_t elem_grid(const msv_state *s, uint64_t work) {
+  const uint64_t want = (work + 255) / 256;
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * 16;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+}
+
+int bit_of(const msv_state *s, int q) { return s->n - 1 - q; }
+
+int check_qubit(const msv_state *s, int q, const char *what) {
+  if (q < 0 || q >= s->n)
+    return fail(MSV_E_INVALID, std::string(what) + ": qubit " + std::to_string(q) +
+                                   " out of range for " + std::to_string(s->n) + " qubits");
+  return MSV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tile-sweep planning
+// ---------------------------------------------------------------------------
+
+struct PlanOp {
+  uint8_t kind = 0;   // OP_U2 / OP_QFT_F / OP_QFT_I / OP_DIAG
+  int tbit = -1;      // target index bit (U2, QFT)
+  uint64_t cmask = 0; // controls (U2), index bits
+  int abit = -1, bbit = -1;  // DIAG bits (bbit < 0: one-bit diagonal)
+  int lb = 0;         // QFT ladder low bit
+  double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct SegPlan {
+  int reg0 = 0;  // registers hold local bits [reg0, reg0 + rb)
+  std::vector<PlanOp> ops;
+};
+
+struct SweepPlan {
+  std::vector<int> local;  // index bits of the tile, ascending
+  std::vector<SegPlan> segs;
+  double scale = 1.0;
+};
+
+template <typename T> int rb_of() { return TileCfg<T>::kRB; }
+template <typename T> int local_max() {
+  return std::is_same<T, float>::value ? g_tune.local_bits_c64 : g_tune.local_bits_c128;
+}
+template <typename T> int col_bits() { return std::is_same<T, float>::value ? 4 : 3; }
+
+// Partition of index bits [0, n) into a low sweep [0, L) and high chunks of
+// between rb and gate_bits bits each (ascending).
+template <typename T>
+void partition(int n, int *low_len, std::vector<std::pair<int, int>> *chunks) {
+  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int RB = rb_of<T>();
+  const int G = std::max(RB, std::min(g_tune.gate_bits, NL - col_bits<T>()));
+  chunks->clear();
+  if (n <= NL) {
+    *low_len = n;
+    return;
+  }
+  int H = n - NL;
+  int L = NL;
+  if (H < RB) {
+    L = n - RB;
+    H = RB;
+  }
+  const int k = (H + G - 1) / G;
+  int start = L;
+  for (int c = 0; c < k; ++c) {
+    const int sz = H / k + (c < H % k ? 1 : 0);
+    chunks->push_back({start, sz});
+    start += sz;
+  }
+  *low_len = L;
+}
+
+template <typename T>
+std::vector<int> sweep_local_bits(int n, int low_len, bool is_low, std::pair<int, int> chunk) {
+  std::vector<int> local;
+  if (is_low) {
+    for (int b = 0; b < low_len; ++b) local.push_back(b);
+  } else {
+    const int C = std::min(col_bits<T>(), chunk.first);
+    for (int b = 0; b < C; ++b) local.push_back(b);
+    for (int b = chunk.first; b < chunk.first + chunk.second; ++b) local.push_back(b);
+  }
+  (void)n;
+  return local;
+}
+
+// Windows of rb consecutive local bits covering the given local bits,
+// taken from the top down; windows never start below `floor`.
+std::vector<int> cover_top_down(const std::vector<int> &gate_locals, int nlocal, int rb, int floor) {
+  std::vector<int> sorted = gate_locals;
+  std::sort(sorted.begin(), sorted.end());
+  std::vector<int> wins;
+  std::vector<bool> covered(nlocal, false);
+  for (int i = static_cast<int>(sorted.size()) - 1; i >= 0; --i) {
+    const int b = sorted[i];
+    if (covered[b]) continue;
+    int lo = std::max(floor, b - rb + 1);
+    lo = std::min(lo, nlocal - rb);
+    lo = std::max(lo, 0);
+    wins.push_back(lo);
+    for (int k = lo; k < lo + rb && k < nlocal; ++k) covered[k] = true;
+  }
+  return wins;
+}
+
+// Global loads/stores of a segment are coalesced when the lanes run over
+// the lowest local bits; that holds when the register window starts at or
+// above local bit `good_from`.
+int good_from(int nlocal, int rb) { return std::max(0, std::min(4, nlocal - rb)); }
+
+template <typename T>
+int launch_sweep(msv_state *s, const SweepPlan &plan) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
+  const int nlocal = static_cast<int>(plan.local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  if (plan.segs.empty()) return MSV_OK;
+  if (static_cast<int>(plan.segs.size()) > kMaxSegs)
+    return fail(MSV_E_INVALID, "internal: too many tile segments");
+  TileArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  A.nlocal = static_cast<uint32_t>(nlocal);
+  A.nseg = static_cast<uint32_t>(plan.segs.size());
+  int loc_of[64];
+  for (int i = 0; i < 64; ++i) loc_of[i] = -1;
+  uint64_t lmask = 0;
+  for (int k = 0; k < nlocal; ++k) {
+    A.lpos[k] = static_cast<uint8_t>(plan.local[k]);
+    loc_of[plan.local[k]] = k;
+    lmask |= 1ull << plan.local[k];
+  }
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  A.nonlocal_mask = all & ~lmask;
+  A.n_tiles = 1ull << (n - nlocal);
+  A.scale = static_cast<T>(plan.scale);
+  for (int m = 0; m < 32; ++m) {
+    A.w32[2 * m] = static_cast<T>(std::cos(2.0 * M_PI * m / 32.0));
+    A.w32[2 * m + 1] = static_cast<T>(std::sin(2.0 * M_PI * m / 32.0));
+  }
+  int nops = 0;
+  for (size_t si = 0; si < plan.segs.size(); ++si) {
+    const SegPlan &sp = plan.segs[si];
+    TileSeg &sg = A.seg[si];
+    if (sp.reg0 < 0 || sp.reg0 + rb > nlocal)
+      return fail(MSV_E_INVALID, "internal: register window outside the tile");
+    for (int j = 0; j < 5; ++j) sg.reg[j] = static_cast<uint8_t>(j < rb ? sp.reg0 + j : 0);
+    sg.op_begin = static_cast<uint8_t>(nops);
+    for (const PlanOp &po : sp.ops) {
+      if (nops >= kMaxTileOps) return fail(MSV_E_INVALID, "internal: too many tile ops");
+      TileOp<T> &op = A.ops[nops++];
+      op.kind = po.kind;
+      op.rpos = -1;
+      op.ra = -1;
+      op.rb = -1;
+      for (int k = 0; k < 8; ++k) op.m[k] = static_cast<T>(po.m[k]);
+      auto reg_pos = [&](int bit) -> int {
+        const int k = loc_of[bit];
+        if (k < 0) return -1;
+        const int j = k - sp.reg0;
+        return (j >= 0 && j < rb) ? j : -1;
+      };
+      if (po.kind == OP_U2 || po.kind == OP_QFT_F || po.kind == OP_QFT_I) {
+        const int j = reg_pos(po.tbit);
+        if (j < 0) return fail(MSV_E_INVALID, "internal: op target is not a register bit");
+        op.rpos = static_cast<int8_t>(j);
+      }
+      if (po.kind == OP_U2) {
+        uint64_t cm = po.cmask;
+        while (cm) {
+          const int bit = __builtin_ctzll(cm);
+          cm &= cm - 1;
+          const int j = reg_pos(bit);
+          if (j >= 0) op.cmask_r |= 1u << j;
+          else op.cmask_t |= 1ull << bit;
+        }
+        op.has_ctrl = po.cmask != 0;
+      } else if (po.kind == OP_QFT_F || po.kind == OP_QFT_I) {
+        const int j = op.rpos;
+        const int p = po.tbit;
+        int jl = 0;
+        for (int d = 1; d <= j; ++d) {
+          const int idx = plan.local[sp.reg0 + j - d];
+          if (idx == p - d && idx >= po.lb) ++jl;
+          else break;
+        }
+        for (int d = jl + 1; d <= j; ++d) {
+          if (plan.local[sp.reg0 + j - d] >= po.lb)
+            return fail(MSV_E_INVALID, "internal: QFT register window not contiguous");
+        }
+        op.jl = static_cast<uint8_t>(jl);
+        op.pbit = static_cast<uint8_t>(p);
+        op.lb = static_cast<uint8_t>(po.lb);
+      } else if (po.kind == OP_DIAG) {
+        op.nbits = po.bbit >= 0 ? 2 : 1;
+        const int ja = reg_pos(po.abit);
+        op.ra = static_cast<int8_t>(ja);
+        op.amask = ja >= 0 ? 0ull : (1ull << po.abit);
+        if (po.bbit >= 0) {
+          const int jb = reg_pos(po.bbit);
+          op.rb = static_cast<int8_t>(jb);
+          op.bmask = jb >= 0 ? 0ull : (1ull << po.bbit);
+        }
+      }
+    }
+    sg.op_end = static_cast<uint8_t>(nops);
+  }
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+  MSV_HIP(launch_tile<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+// Commuting per-bit ops (one op per gate local bit, any order): windows from
+// the top, then the bottom window moved inside so that the first and last
+// segments keep coalesced lanes.
+template <typename T>
+int run_commuting_sweep(msv_state *s, const std::vector<int> &local,
+                        const std::vector<std::pair<int, PlanOp>> &ops_by_local) {
+  const int nlocal = static_cast<int>(local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  std::vector<int> gl;
+  for (auto &p : ops_by_local) gl.push_back(p.first);
+  std::vector<int> wins = cover_top_down(gl, nlocal, rb, 0);
+  const int gf = good_from(nlocal, rb);
+  std::vector<int> good, bad;
+  for (int w : wins) (w >= gf ? good : bad).push_back(w);
+  std::vector<int> order;
+  std::vector<bool> has_ops;
+  if (good.empty()) {
+    const int top = nlocal - rb;
+    order.push_back(top);
+    for (int w : bad) order.push_back(w);
+    order.push_back(top);
+  } else {
+    order.push_back(good[0]);
+    for (int w : bad) order.push_back(w);
+    for (size_t i = 1; i < good.size(); ++i) order.push_back(good[i]);
+    if (!bad.empty() && good.size() == 1) order.push_back(good[0]);
+  }
+  SweepPlan plan;
+  plan.local = local;
+  std::vector<bool> used(ops_by_local.size(), false);
+  for (int w : order) {
+    SegPlan sp;
+    sp.reg0 = w;
+    for (size_t i = 0; i < ops_by_local.size(); ++i) {
+      const int l = ops_by_local[i].first;
+      if (!used[i] && l >= w && l < w + rb) {
+        sp.ops.push_back(ops_by_local[i].second);
+        used[i] = true;
+      }
+    }
+    plan.segs.push_back(sp);
+  }
+  return launch_sweep<T>(s, plan);
+}
+
+template <typename T>
+int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
+  const int n = s->n;
+  int L = 0;
+  std::vector<std::pair<int, int>> chunks;
+  partition<T>(n, &L, &chunks);
+  std::vector<bool> act_bit(n, false);
+  for (int q = 0; q < n; ++q)
+    if (!active || active[q]) act_bit[bit_of(s, q)] = true;
+  auto make_op = [&](int bit) {
+    PlanOp po;
+    po.kind = OP_U2;
+    po.tbit = bit;
+    const int q = n - 1 - bit;
+    for (int k = 0; k < 8; ++k) po.m[k] = mats[8 * q + k];
+    return po;
+  };
+  // low sweep
+  {
+    std::vector<int> local = sweep_local_bits<T>(n, L, true, {0, 0});
+    std::vector<std::pair<int, PlanOp>> ops;
+    for (int k = 0; k < static_cast<int>(local.size()); ++k)
+      if (act_bit[local[k]]) ops.push_back({k, make_op(local[k])});
+    if (!ops.empty()) MSV_TRY(run_commuting_sweep<T>(s, local, ops));
+  }
+  for (auto &ch : chunks) {
+    std::vector<int> local = sweep_local_bits<T>(n, L, false, ch);
+    std::vector<std::pair<int, PlanOp>> ops;
+    for (int k = 0; k < static_cast<int>(local.size()); ++k)
+      if (local[k] >= ch.first && act_bit[local[k]]) ops.push_back({k, make_op(local[k])});
+    if (!ops.empty()) MSV_TRY(run_commuting_sweep<T>(s, local, ops));
+  }
+  return MSV_OK;
+}
+
+// QFT (no terminal swaps) on index bits [lb, hb]: forward stages run from
+// hb down to lb; the inverse runs the adjoint stages from lb up to hb.
+template <typename T>
+int apply_qft_T(msv_state *s, int hb, int lb, bool inverse, int only_stage_bit) {
+  const int n = s->n;
+  int L = 0;
+  std::vector<std::pair<int, int>> chunks;
+  partition<T>(n, &L, &chunks);
+  struct Sw {
+    std::vector<int> local;
+    int gate_lo, gate_hi;  // index range of gate bits in this sweep
+  };
+  std::vector<Sw> sweeps;
+  sweeps.push_back({sweep_local_bits<T>(n, L, true, {0, 0}), 0, L - 1});
+  for (auto &ch : chunks)
+    sweeps.push_back({sweep_local_bits<T>(n, L, false, ch), ch.first, ch.first + ch.second - 1});
+  // forward: high sweeps first
+  std::vector<int> order;
+  for (int i = static_cast<int>(sweeps.size()) - 1; i >= 0; --i) order.push_back(i);
+  if (inverse) std::reverse(order.begin(), order.end());
+  for (int si : order) {
+    const Sw &sw = sweeps[si];
+    const int nlocal = static_cast<int>(sw.local.size());
+    const int rb = std::min(rb_of<T>(), nlocal);
+    std::vector<int> stage_bits;  // index bits, in execution order
+    for (int p = std::min(hb, sw.gate_hi); p >= std::max(lb, sw.gate_lo); --p)
+      if (only_stage_bit < 0 || p == only_stage_bit) stage_bits.push_back(p);
+    if (stage_bits.empty()) continue;
+    if (inverse) std::reverse(stage_bits.begin(), stage_bits.end());
+    int loc_of[64];
+    for (int i = 0; i < 64; ++i) loc_of[i] = -1;
+    for (int k = 0; k < nlocal; ++k) loc_of[sw.local[k]] = k;
+    std::vector<int> gl;
+    for (int p : stage_bits) gl.push_back(loc_of[p]);
+    // windows stay inside the gate bits so that register bits are
+    // contiguous index bits
+    const int floor = (si == 0) ? 0 : std::min(col_bits<T>(), sw.gate_lo);
+    std::vector<int> wins = cover_top_down(gl, nlocal, rb, floor);
+    if (inverse) std::reverse(wins.begin(), wins.end());
+    SweepPlan plan;
+    plan.local = sw.local;
+    std::vector<bool> used(stage_bits.size(), false);
+    for (int w : wins) {
+      SegPlan sp;
+      sp.reg0 = w;
+      for (size_t i = 0; i < stage_bits.size(); ++i) {
+        const int l = loc_of[stage_bits[i]];
+        if (!used[i] && l >= w && l < w + rb) {
+          PlanOp po;
+          po.kind = inverse ? OP_QFT_I : OP_QFT_F;
+          po.tbit = stage_bits[i];
+          po.lb = lb;
+          sp.ops.push_back(po);
+          used[i] = true;
+        } else if (!used[i]) {
+          break;  // keep stage order: later stages wait for a later window
+        }
+      }
+      plan.segs.push_back(sp);
+    }
+    for (size_t i = 0; i < used.size(); ++i)
+      if (!used[i]) return fail(MSV_E_INVALID, "internal: QFT stage left unscheduled");
+    const int gf = good_from(nlocal, rb);
+    if (plan.segs.front().reg0 < gf) {
+      SegPlan ld;
+      ld.reg0 = nlocal - rb;
+      plan.segs.insert(plan.segs.begin(), ld);
+    }
+    if (plan.segs.back().reg0 < gf) {
+      SegPlan st;
+      st.reg0 = nlocal - rb;
+      plan.segs.push_back(st);
+    }
+    plan.scale = std::pow(2.0, -0.5 * static_cast<double>(stage_bits.size()));
+    MSV_TRY(launch_sweep<T>(s, plan));
+  }
+  return MSV_OK;
+}
+
+// Generic ordered gate list -> tile programs. A program holds at most
+// `max_targets` distinct non-diagonal target bits and kMaxTileOps ops.
+template <typename T>
+int flush_program(msv_state *s, const std::vector<PlanOp> &ops) {
+  if (ops.empty()) return MSV_OK;
+  const int n = s->n;
+  const int NL = std::min(std::min(local_max<T>(), TileCfg<T>::kMaxLocal), n);
+  std::vector<int> targets;
+  for (const PlanOp &po : ops)
+    if (po.tbit >= 0 && std::find(targets.begin(), targets.end(), po.tbit) == targets.end())
+      targets.push_back(po.tbit);
+  std::vector<int> local = targets;
+  for (int b = 0; b < n && static_cast<int>(local.size()) < NL; ++b)
+    if (std::find(local.begin(), local.end(), b) == local.end()) local.push_back(b);
+  std::sort(local.begin(), local.end());
+  const int nlocal = static_cast<int>(local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  int loc_of[64];
+  for (int i = 0; i < 64; ++i) loc_of[i] = -1;
+  for (int k = 0; k < nlocal; ++k) loc_of[local[k]] = k;
+  SweepPlan plan;
+  plan.local = local;
+  std::vector<PlanOp> pending_diag;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const PlanOp &po = ops[i];
+    if (po.tbit < 0) {
+      if (plan.segs.empty()) pending_diag.push_back(po);
+      else plan.segs.back().ops.push_back(po);
+      continue;
+    }
+    const int kt = loc_of[po.tbit];
+    if (!plan.segs.empty()) {
+      const int w = plan.segs.back().reg0;
+      if (kt >= w && kt < w + rb) {
+        plan.segs.back().ops.push_back(po);
+        continue;
+      }
+    }
+    int best = -1, best_score = -1;
+    const int lo_min = std::max(0, kt - rb + 1), lo_max = std::min(kt, nlocal - rb);
+    for (int lo = lo_max; lo >= lo_min; --lo) {
+      int score = 0;
+      for (size_t k = i; k < ops.size(); ++k) {
+        if (ops[k].tbit < 0) continue;
+        const int l = loc_of[ops[k].tbit];
+        if (l >= lo && l < lo + rb) ++score;
+        else break;
+      }
+      if (score > best_score) {
+        best_score = score;
+        best = lo;
+      }
+    }
+    SegPlan sp;
+    sp.reg0 = best;
+    if (plan.segs.empty()) {
+      sp.ops = pending_diag;
+      pending_diag.clear();
+    }
+    sp.ops.push_back(po);
+    plan.segs.push_back(sp);
+  }
+  if (plan.segs.empty()) {
+    SegPlan sp;
+    sp.reg0 = nlocal - rb;
+    sp.ops = pending_diag;
+    plan.segs.push_back(sp);
+  }
+  const int gf = good_from(nlocal, rb);
+  if (plan.segs.front().reg0 < gf) {
+    SegPlan ld;
+    ld.reg0 = nlocal - rb;
+    plan.segs.insert(plan.segs.begin(), ld);
+  }
+  if (plan.segs.back().reg0 < gf) {
+    SegPlan st;
+    st.reg0 = nlocal - rb;
+    plan.segs.push_back(st);
+  }
+  return launch_sweep<T>(s, plan);
+}
+
+// number of segments flush_program would produce (without launching)
+template <typename T>
+int count_segments(const msv_state *s, const std::vector<PlanOp> &ops) {
+  const int n = s->n;
+  const int NL = std::min(std::min(local_max<T>(), TileCfg<T>::kMaxLocal), n);
+  std::vector<int> targets;
+  for (const PlanOp &po : ops)
+    if (po.tbit >= 0 && std::find(targets.begin(), targets.end(), po.tbit) == targets.end())
+      targets.push_back(po.tbit);
+  std::vector<int> local = targets;
+  for (int b = 0; b < n && static_cast<int>(local.size()) < NL; ++b)
+    if (std::find(local.begin(), local.end(), b) == local.end()) local.push_back(b);
+  std::sort(local.begin(), local.end());
+  const int nlocal = static_cast<int>(local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  int loc_of[64];
+  for (int i = 0; i < 64; ++i) loc_of[i] = -1;
+  for (int k = 0; k < nlocal; ++k) loc_of[local[k]] = k;
+  int segs = 0, cur = -1, first = -1, last = -1;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    if (ops[i].tbit < 0) continue;
+    const int kt = loc_of[ops[i].tbit];
+    if (cur >= 0 && kt >= cur && kt < cur + rb) continue;
+    int best = -1, best_score = -1;
+    const int lo_min = std::max(0, kt - rb + 1), lo_max = std::min(kt, nlocal - rb);
+    for (int lo = lo_max; lo >= lo_min; --lo) {
+      int score = 0;
+      for (size_t k = i; k < ops.size(); ++k) {
+        if (ops[k].tbit < 0) continue;
+        const int l = loc_of[ops[k].tbit];
+        if (l >= lo && l < lo + rb) ++score;
+        else break;
+      }
+      if (score > best_score) {
+        best_score = score;
+        best = lo;
+      }
+    }
+    cur = best;
+    if (first < 0) first = best;
+    last = best;
+    ++segs;
+  }
+  if (segs == 0) return 1;
+  const int gf = good_from(nlocal, rb);
+  if (first < gf) ++segs;
+  if (last < gf) ++segs;
+  return segs;
+}
+
+int max_program_targets() { return 10; }
+
+template <typename T>
+int apply_plan_ops(msv_state *s, const std::vector<PlanOp> &all) {
+  std::vector<PlanOp> cur;
+  std::vector<int> targets;
+  for (const PlanOp &po : all) {
+    std::vector<PlanOp> trial = cur;
+    trial.push_back(po);
+    std::vector<int> t2 = targets;
+    if (po.tbit >= 0 && std::find(t2.begin(), t2.end(), po.tbit) == t2.end()) t2.push_back(po.tbit);
+    const bool fits = static_cast<int>(trial.size()) <= kMaxTileOps &&
+                      static_cast<int>(t2.size()) <= max_program_targets() &&
+                      count_segments<T>(s, trial) <= kMaxSegs;
+    if (!fits) {
+      MSV_TRY(flush_program<T>(s, cur));
+      cur.clear();
+      targets.clear();
+      cur.push_back(po);
+      if (po.tbit >= 0) targets.push_back(po.tbit);
+    } else {
+      cur = std::move(trial);
+      targets = std::move(t2);
+    }
+  }
+  return flush_program<T>(s, cur);
+}
+
+// ---------------------------------------------------------------------------
+// diagonal phase layers
+// ---------------------------------------------------------------------------
+
+struct DiagTerm {
+  int bits[2];  // index bits; bits[1] < 0 for one-bit terms; bits[0] is the MSB of the table
+  double re[4], im[4];
+};
+
+struct Win {
+  std::vector<int> bits;  // distinct index bits, ascending
+  int s1 = 0, w1 = 0, s2 = 0, w2 = 0;
+  std::vector<int> terms;
+};
+
+// Cover a sorted bit set with at most two runs of total width <= wmax.
+bool fit_runs(const std::vector<int> &bits, int wmax, int *s1, int *w1, int *s2, int *w2) {
+  if (bits.empty()) return false;
+  const int lo = bits.front(), hi = bits.back();
+  if (hi - lo + 1 <= wmax) {
+    *s1 = lo;
+    *w1 = hi - lo + 1;
+    *s2 = 0;
+    *w2 = 0;
+    return true;
+  }
+  int gap_at = -1, gap = -1;
+  for (size_t i = 0; i + 1 < bits.size(); ++i) {
+    const int g = bits[i + 1] - bits[i];
+    if (g > gap) {
+      gap = g;
+      gap_at = static_cast<int>(i);
+    }
+  }
+  const int a_lo = lo, a_hi = bits[gap_at], b_lo = bits[gap_at + 1], b_hi = hi;
+  const int wa = a_hi - a_lo + 1, wb = b_hi - b_lo + 1;
+  if (wa + wb > wmax) return false;
+  *s1 = a_lo;
+  *w1 = wa;
+  *s2 = b_lo;
+  *w2 = wb;
+  return true;
+}
+
+template <typename T>
+int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
+  using V = typename Vec2<T>::type;
+  if (terms.empty()) return MSV_OK;
+  const int wmax = 8;
+  std::vector<Win> wins;
+  for (size_t ti = 0; ti < terms.size(); ++ti) {
+    const DiagTerm &t = terms[ti];
+    std::vector<int> tb = {t.bits[0]};
+    if (t.bits[1] >= 0) tb.push_back(t.bits[1]);
+    bool placed = false;
+    for (Win &w : wins) {
+      std::vector<int> u = w.bits;
+      for (int b : tb)
+        if (std::find(u.begin(), u.end(), b) == u.end()) u.push_back(b);
+      std::sort(u.begin(), u.end());
+      int s1, w1, s2, w2;
+      if (fit_runs(u, wmax, &s1, &w1, &s2, &w2)) {
+        w.bits = u;
+        w.s1 = s1; w.w1 = w1; w.s2 = s2; w.w2 = w2;
+        w.terms.push_back(static_cast<int>(ti));
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) {
+      Win w;
+      w.bits = tb;
+      std::sort(w.bits.begin(), w.bits.end());
+      fit_runs(w.bits, wmax, &w.s1, &w.w1, &w.s2, &w.w2);
+      w.terms.push_back(static_cast<int>(ti));
+      wins.push_back(w);
+    }
+  }
+  // order: windows reading index bit 0 first (evaluated per amplitude)
+  std::stable_sort(wins.begin(), wins.end(), [](const Win &a, const Win &b) {
+    const bool la = a.bits.front() == 0, lb2 = b.bits.front() == 0;
+    return la && !lb2;
+  });
+  size_t wi = 0;
+  while (wi < wins.size()) {
+    PhaseArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.n_amps = s->amps;
+    std::vector<V> lut;
+    uint32_t nw = 0, nlow = 0;
+    while (wi < wins.size() && nw < static_cast<uint32_t>(kMaxPhaseWins)) {
+      const Win &w = wins[wi];
+      const uint32_t len = 1u << (w.w1 + w.w2);
+      if (lut.size() + len > static_cast<size_t>(kPhaseLutMax) && nw > 0) break;
+      PhaseWin pw;
+      pw.s1 = static_cast<uint8_t>(w.s1);
+      pw.w1 = static_cast<uint8_t>(w.w1);
+      pw.s2 = static_cast<uint8_t>(w.s2);
+      pw.w2 = static_cast<uint8_t>(w.w2);
+      pw.off = static_cast<uint32_t>(lut.size());
+      for (uint32_t v = 0; v < len; ++v) {
+        auto bit_val = [&](int bit) -> int {
+          if (bit >= w.s1 && bit < w.s1 + w.w1) return (v >> (bit - w.s1)) & 1;
+          return (v >> (w.w1 + bit - w.s2)) & 1;
+        };
+        double pr = 1.0, pi = 0.0;
+        for (int ti : w.terms) {
+          const DiagTerm &t = terms[ti];
+          int sel = bit_val(t.bits[0]);
+          if (t.bits[1] >= 0) sel = sel * 2 + bit_val(t.bits[1]);
+          const double nr = pr * t.re[sel] - pi * t.im[sel];
+          const double ni = pr * t.im[sel] + pi * t.re[sel];
+          pr = nr;
+          pi = ni;
+        }
+        V e;
+        e.x = static_cast<T>(pr);
+        e.y = static_cast<T>(pi);
+        lut.push_back(e);
+      }
+      A.win[nw++] = pw;
+      if (w.bits.front() == 0) nlow = nw;
+      ++wi;
+    }
+    A.nwin = nw;
+    A.nlow = nlow;
+    A.lut_len = static_cast<uint32_t>(lut.size());
+    void *d_lut = nullptr;
+    MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &d_lut));
+    const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
+                                                  : static_cast<uint32_t>(s->cu * 8);
+    MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A,
+                            grid, s->stream));
+    s->sweeps += 1;
+  }
+  return MSV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// XOR permutation
+// ---------------------------------------------------------------------------
+
+template <typename T>
+int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
+  bool ident = (c == 0);
+  for (int k = 0; k < n && ident; ++k) ident = rows[k] == (1ull << (n - 1 - k));
+  if (ident) return MSV_OK;
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  for (int k = 0; k < n; ++k)
+    if (rows[k] & ~all) return fail(MSV_E_INVALID, "xor_affine: row mask exceeds the register");
+  if (c & ~all) return fail(MSV_E_INVALID, "xor_affine: flip vector exceeds the register");
+  // column j of the inverse map: source bits set by index bit j of y
+  std::vector<uint64_t> col(n, 0);
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < n; ++j)
+      if ((rows[k] >> j) & 1ull) col[j] |= 1ull << (n - 1 - k);
+  // invertibility check (a non-invertible M' would not be a permutation)
+  {
+    std::vector<uint64_t> m(col);
+    int rank = 0;
+    for (int bit = 0; bit < n && rank < n; ++bit) {
+      int piv = -1;
+      for (int r = rank; r < n; ++r)
+        if ((m[r] >> bit) & 1ull) {
+          piv = r;
+          break;
+        }
+      if (piv < 0) continue;
+      std::swap(m[rank], m[piv]);
+      for (int r = 0; r < n; ++r)
+        if (r != rank && ((m[r] >> bit) & 1ull)) m[r] ^= m[rank];
+      ++rank;
+    }
+    if (rank != n) return fail(MSV_E_INVALID, "xor_affine: matrix is singular over GF(2)");
+  }
+  const int ntab = (n + 7) / 8;
+  std::vector<uint64_t> tab(static_cast<size_t>(ntab) * 256, 0);
+  for (int t = 0; t < ntab; ++t)
+    for (int v = 0; v < 256; ++v) {
+      uint64_t x = 0;
+      for (int b = 0; b < 8; ++b) {
+        const int j = 8 * t + b;
+        if (j < n && ((v >> b) & 1)) x ^= col[j];
+      }
+      tab[static_cast<size_t>(t) * 256 + v] = x;
+    }
+  MSV_TRY(ensure_alt(s));
+  void *d_tab = nullptr;
+  MSV_TRY(stage(s, tab.data(), tab.size() * sizeof(uint64_t), &d_tab));
+  MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
+                               s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
+                               elem_grid(s, s->amps / 2), s->stream));
+  std::swap(s->buf, s->alt);
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+
+template <typename T>
+int block_sums_T(msv_state *s, std::vector<double> *out) {
+  using V = typename Vec2<T>::type;
+  const uint64_t nb = (s->amps + kSampleBlock - 1) / kSampleBlock;
+  MSV_TRY(ensure_red(s, nb * sizeof(double)));
+  MSV_HIP(launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps,
+                                  reinterpret_cast<double *>(s->d_red), nb, s->stream));
+  out->resize(nb);
+  MSV_HIP(hipMemcpyAsync(out->data(), s->d_red, nb * sizeof(double), hipMemcpyDeviceToHost,
+                         s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  return MSV_OK;
+}
+
+template <typename T>
+int sample_T(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_t seed,
+             uint64_t stream_id, uint64_t *out) {
+  using V = typename Vec2<T>::type;
+  if (shots == 0) return MSV_OK;
+  std::vector<double> bs;
+  MSV_TRY(block_sums_T<T>(s, &bs));
+  std::vector<double> prefix(bs.size() + 1, 0.0);
+  for (size_t b = 0; b < bs.size(); ++b) prefix[b + 1] = prefix[b] + bs[b];
+  const double total = prefix.back();
+  if (!(total > 0.0)) return fail(MSV_E_INVALID, "sample: state has zero norm");
+  std::vector<uint64_t> shot_block(shots);
+  std::vector<double> shot_resid(shots);
+  for (uint64_t k = 0; k < shots; ++k) {
+    double t = philox_u01(seed, stream_id, k) * total;
+    if (t >= total) t = std::nextafter(total, 0.0);
+    // first block b with prefix[b+1] > t
+    const size_t b = static_cast<size_t>(
+        std::upper_bound(prefix.begin() + 1, prefix.end(), t) - (prefix.begin() + 1));
+    const size_t bb = std::min(b, bs.size() - 1);
+    shot_block[k] = bb;
+    shot_resid[k] = t - prefix[bb];
+  }
+  const size_t need = shots * (sizeof(uint64_t) * 2 + sizeof(double));
+  MSV_TRY(ensure_red(s, need));
+  char *base = reinterpret_cast<char *>(s->d_red);
+  uint64_t *d_block = reinterpret_cast<uint64_t *>(base);
+  double *d_resid = reinterpret_cast<double *>(base + shots * sizeof(uint64_t));
+  uint64_t *d_index = reinterpret_cast<uint64_t *>(base + shots * (sizeof(uint64_t) + sizeof(double)));
+  MSV_HIP(hipMemcpyAsync(d_block, shot_block.data(), shots * sizeof(uint64_t),
+                         hipMemcpyHostToDevice, s->stream));
+  MSV_HIP(hipMemcpyAsync(d_resid, shot_resid.data(), shots * sizeof(double),
+                         hipMemcpyHostToDevice, s->stream));
+  MSV_HIP(launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
+                                   d_index, shots, s->stream));
+  std::vector<uint64_t> idx(shots);
+  MSV_HIP(hipMemcpyAsync(idx.data(), d_index, shots * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  const int n = s->n;
+  for (uint64_t k = 0; k < shots; ++k) {
+    uint64_t v = 0;
+    for (int w = 0; w < nw; ++w) {
+      const int bit = n - 1 - wires[w];
+      v = (v << 1) | ((idx[k] >> bit) & 1ull);
+    }
+    out[k] = v;
+  }
+  return MSV_OK;
+}
+
+template <typename T>
+int probabilities_T(msv_state *s, const int32_t *wires, int nw, double *out) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
+  bool identity = (wires == nullptr);
+  if (!identity && nw == n) {
+    identity = true;
+    for (int w = 0; w < n; ++w) identity = identity && wires[w] == w;
+  }
+  if (identity) {
+    MSV_TRY(ensure_red(s, s->amps * sizeof(double)));
+    MSV_HIP(launch_probs_full<T>(reinterpret_cast<const V *>(s->buf), s->amps,
+                                 reinterpret_cast<double *>(s->d_red), elem_grid(s, s->amps),
+                                 s->stream));
+    MSV_HIP(hipMemcpyAsync(out, s->d_red, s->amps * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+    MSV_HIP(hipStreamSynchronize(s->stream));
+    return MSV_OK;
+  }
+  if (nw == 0) {
+    double tot = 0.0;
+    std::vector<double> bs;
+    MSV_TRY(block_sums_T<T>(s, &bs));
+    for (double v : bs) tot += v;
+    out[0] = tot;
+    return MSV_OK;
+  }
+  std::vector<uint64_t> obits(nw);
+  for (int w = 0; w < nw; ++w) obits[w] = static_cast<uint64_t>(n - 1 - wires[w]);
+  const uint64_t K = 1ull << nw;
+  const uint64_t O = 1ull << (n - nw);
+  uint64_t chunks = std::max<uint64_t>(1, (1ull << 18) / K);
+  chunks = std::min(chunks, O);
+  const uint64_t chunk_len = O / chunks;
+  const size_t part_bytes = K * chunks * sizeof(double);
+  MSV_TRY(ensure_red(s, part_bytes + 64 * sizeof(uint64_t)));
+  double *d_part = reinterpret_cast<double *>(s->d_red);
+  uint64_t *d_bits = reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(s->d_red) + part_bytes);
+  MSV_HIP(hipMemcpyAsync(d_bits, obits.data(), nw * sizeof(uint64_t), hipMemcpyHostToDevice,
+                         s->stream));
+  MSV_HIP(launch_marginal<T>(reinterpret_cast<const V *>(s->buf), n, d_bits, nw, d_part, chunks,
+                             chunk_len, s->stream));
+  std::vector<double> part(K * chunks);
+  MSV_HIP(hipMemcpyAsync(part.data(), d_part, part_bytes, hipMemcpyDeviceToHost, s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  for (uint64_t k = 0; k < K; ++k) {
+    double acc = 0.0;
+    for (uint64_t c = 0; c < chunks; ++c) acc += part[c * K + k];
+    out[k] = acc;
+  }
+  return MSV_OK;
+}
+
+template <typename T>
+int pauli_T(msv_state *s, const uint64_t *xq, const uint64_t *zq, int nwords, double *ore,
+            double *oim) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
+  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  for (int w0 = 0; w0 < nwords; w0 += kMaxWords) {
+    const int cnt = std::min(kMaxWords, nwords - w0);
+    std::vector<uint64_t> xm(cnt), zm(cnt);
+    for (int w = 0; w < cnt; ++w) {
+      if ((xq[w0 + w] & ~all) || (zq[w0 + w] & ~all))
+        return fail(MSV_E_INVALID, "expval_pauli: mask names a qubit outside the register");
+      uint64_t xi = 0, zi = 0;
+      for (int q = 0; q < n; ++q) {
+        if ((xq[w0 + w] >> q) & 1ull) xi |= 1ull << (n - 1 - q);
+        if ((zq[w0 + w] >> q) & 1ull) zi |= 1ull << (n - 1 - q);
+      }
+      xm[w] = xi;
+      zm[w] = zi;
+    }
+    const uint32_t grid = std::min<uint32_t>(elem_grid(s, s->amps), static_cast<uint32_t>(s->cu * 4));
+    const size_t part_bytes = static_cast<size_t>(grid) * cnt * 2 * sizeof(double);
+    MSV_TRY(ensure_red(s, part_bytes + 2 * kMaxWords * sizeof(uint64_t)));
+    double *d_part = reinterpret_cast<double *>(s->d_red);
+    uint64_t *d_x = reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(s->d_red) + part_bytes);
+    uint64_t *d_z = d_x + kMaxWords;
+    MSV_HIP(hipMemcpyAsync(d_x, xm.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    MSV_HIP(hipMemcpyAsync(d_z, zm.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    MSV_HIP(launch_pauli<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_x, d_z, cnt, d_part,
+                            grid, s->stream));
+    std::vector<double> part(static_cast<size_t>(grid) * cnt * 2);
+    MSV_HIP(hipMemcpyAsync(part.data(), d_part, part_bytes, hipMemcpyDeviceToHost, s->stream));
+    MSV_HIP(hipStreamSynchronize(s->stream));
+    for (int w = 0; w < cnt; ++w) {
+      double sr = 0.0, si = 0.0;
+      for (uint32_t b = 0; b < grid; ++b) {
+        sr += part[(static_cast<size_t>(b) * cnt + w) * 2];
+        si += part[(static_cast<size_t>(b) * cnt + w) * 2 + 1];
+      }
+      // multiply by i^{ny}
+      const int ny = __builtin_popcountll(xq[w0 + w] & zq[w0 + w]) & 3;
+      double rr = sr, ri = si;
+      for (int k = 0; k < ny; ++k) {
+        const double t = rr;
+        rr = -ri;
+        ri = t;
+      }
+      ore[w0 + w] = rr;
+      oim[w0 + w] = ri;
+    }
+  }
+  return MSV_OK;
+}
+
+template <typename Ta, typename Tb>
+int inner_T(msv_state *a, msv_state *b, double *re, double *im) {
+  using Va = typename Vec2<Ta>::type;
+  using Vb = typename Vec2<Tb>::type;
+  MSV_HIP(hipStreamSynchronize(b->stream));
+  const uint32_t grid = std::min<uint32_t>(elem_grid(a, a->amps), static_cast<uint32_t>(a->cu * 4));
+  MSV_TRY(ensure_red(a, static_cast<size_t>(grid) * 2 * sizeof(double)));
+  MSV_HIP((launch_inner<Ta, Tb>(reinterpret_cast<const Va *>(a->buf),
+                                reinterpret_cast<const Vb *>(b->buf), a->amps,
+                                reinterpret_cast<double *>(a->d_red), grid, a->stream)));
+  std::vector<double> part(static_cast<size_t>(grid) * 2);
+  MSV_HIP(hipMemcpyAsync(part.data(), a->d_red, part.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, a->stream));
+  MSV_HIP(hipStreamSynchronize(a->stream));
+  double sr = 0.0, si = 0.0;
+  for (uint32_t k = 0; k < grid; ++k) {
+    sr += part[2 * k];
+    si += part[2 * k + 1];
+  }
+  *re = sr;
+  *im = si;
+  return MSV_OK;
+}
+
+template <typename T>
+int init_basis_T(msv_state *s, uint64_t index) {
+  using V = typename Vec2<T>::type;
+  MSV_HIP(launch_init_basis<T>(reinterpret_cast<V *>(s->buf), s->amps, index,
+                               elem_grid(s, s->amps), s->stream));
+  return MSV_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+#define MSV_DISPATCH(s, fn, ...) \
+  ((s)->dtype == MSV_C128 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" {
+
+const char *msv_last_error(void) { return g_err.c_str(); }
+
+int msv_version(void) { return 10000; }
+
+int msv_device_count(int *out) {
+  if (!out) return fail(MSV_E_INVALID, "null output");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *out = 0;
+    (void)hipGetLastError();
+    return fail(MSV_E_HIP, std::string("hipGetDeviceCount failed: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return MSV_OK;
+}
+
+int msv_get_device_info(int device, msv_device_info *out) {
+  if (!out) return fail(MSV_E_INVALID, "null output");
+  std::memset(out, 0, sizeof(*out));
+  MSV_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  MSV_HIP(hipGetDeviceProperties(&prop, device));
+  size_t fr = 0, tot = 0;
+  MSV_HIP(hipMemGetInfo(&fr, &tot));
+  out->total_bytes = tot;
+  out->free_bytes = fr;
+  out->max_alloc_bytes = tot;
+  out->compute_units = prop.multiProcessorCount;
+  out->lds_bytes_per_block = static_cast<int32_t>(prop.sharedMemPerBlock);
+  std::snprintf(out->name, sizeof(out->name), "%s", prop.name);
+  std::snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+  return MSV_OK;
+}
+
+int msv_state_create(int n_qubits, int dtype, int device, msv_state **out) {
+  if (!out) return fail(MSV_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (n_qubits < 1 || n_qubits > 40)
+    return fail(MSV_E_INVALID, "n_qubits must be in [1, 40], got " + std::to_string(n_qubits));
+  if (dtype != MSV_C64 && dtype != MSV_C128) return fail(MSV_E_INVALID, "unknown dtype");
+  MSV_HIP(hipSetDevice(device));
+  msv_state *s = new msv_state();
+  s->n = n_qubits;
+  s->dtype = dtype;
+  s->device = device;
+  s->cu = dev_cu(device);
+  s->amps = 1ull << n_qubits;
+  const size_t bytes = s->amps * esize(dtype);
+  if (hipMalloc(&s->buf, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return fail(MSV_E_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes for a " +
+                               std::to_string(n_qubits) + "-qubit state failed");
+  }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipFree(s->buf);
+    delete s;
+    return fail(MSV_E_HIP, "hipStreamCreate failed");
+  }
+  int rc = MSV_DISPATCH(s, init_basis_T, s, 0ull);
+  if (rc == MSV_OK && hipStreamSynchronize(s->stream) != hipSuccess)
+    rc = fail(MSV_E_HIP, "state initialization failed");
+  if (rc != MSV_OK) {
+    msv_state_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return MSV_OK;
+}
+
+int msv_state_destroy(msv_state *s) {
+  if (!s) return MSV_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->buf) (void)hipFree(s->buf);
+  if (s->alt) (void)hipFree(s->alt);
+  if (s->d_stage) (void)hipFree(s->d_stage);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->d_red) (void)hipFree(s->d_red);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return MSV_OK;
+}
+
+int msv_state_reset(msv_state *s) { return msv_state_set_basis(s, 0); }
+
+int msv_state_set_basis(msv_state *s, uint64_t index) {
+  MSV_TRY(check_state(s));
+  if (index >= s->amps) return fail(MSV_E_INVALID, "basis index out of range");
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, init_basis_T, s, index);
+}
+
+int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *bytes) {
+  MSV_TRY(check_state(s));
+  if (n_qubits) *n_qubits = s->n;
+  if (dtype) *dtype = s->dtype;
+  if (bytes) *bytes = s->amps * esize(s->dtype);
+  return MSV_OK;
+}
+
+int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t count) {
+  MSV_TRY(check_state(s));
+  if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
+  if (offset > s->amps || count > s->amps - offset)
+    return fail(MSV_E_INVALID, "upload range exceeds the state");
+  MSV_TRY(set_dev(s));
+  const size_t es = esize(s->dtype);
+  MSV_HIP(hipMemcpyAsync(static_cast<char *>(s->buf) + offset * es, host, count * es,
+                         hipMemcpyHostToDevice, s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  return MSV_OK;
+}
+
+int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count) {
+  MSV_TRY(check_state(s));
+  if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
+  if (offset > s->amps || count > s->amps - offset)
+    return fail(MSV_E_INVALID, "download range exceeds the state");
+  MSV_TRY(set_dev(s));
+  const size_t es = esize(s->dtype);
+  MSV_HIP(hipMemcpyAsync(host, static_cast<const char *>(s->buf) + offset * es, count * es,
+                         hipMemcpyDeviceToHost, s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  return MSV_OK;
+}
+
+int msv_state_copy(msv_state *dst, msv_state *src) {
+  MSV_TRY(check_state(dst));
+  MSV_TRY(check_state(src));
+  if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
+  MSV_TRY(set_dev(dst));
+  MSV_HIP(hipStreamSynchronize(src->stream));
+  const uint32_t grid = elem_grid(dst, dst->amps);
+  if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
+    MSV_HIP((launch_convert<float, float>(static_cast<const float2 *>(src->buf),
+                                          static_cast<float2 *>(dst->buf), dst->amps, grid,
+                                          dst->stream)));
+  else if (src->dtype == MSV_C64)
+    MSV_HIP((launch_convert<float, double>(static_cast<const float2 *>(src->buf),
+                                           static_cast<double2 *>(dst->buf), dst->amps, grid,
+                                           dst->stream)));
+  else if (dst->dtype == MSV_C64)
+    MSV_HIP((launch_convert<double, float>(static_cast<const double2 *>(src->buf),
+                                           static_cast<float2 *>(dst->buf), dst->amps, grid,
+                                           dst->stream)));
+  else
+    MSV_HIP((launch_convert<double, double>(static_cast<const double2 *>(src->buf),
+                                            static_cast<double2 *>(dst->buf), dst->amps, grid,
+                                            dst->stream)));
+  MSV_HIP(hipStreamSynchronize(dst->stream));
+  return MSV_OK;
+}
+
+int msv_state_scale(msv_state *s, double re, double im) {
+  MSV_TRY(check_state(s));
+  MSV_TRY(set_dev(s));
+  if (s->dtype == MSV_C128)
+    MSV_HIP(launch_scale<double>(static_cast<double2 *>(s->buf), s->amps, re, im,
+                                 elem_grid(s, s->amps), s->stream));
+  else
+    MSV_HIP(launch_scale<float>(static_cast<float2 *>(s->buf), s->amps, re, im,
+                                elem_grid(s, s->amps), s->stream));
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+int msv_state_sync(msv_state *s) {
+  MSV_TRY(check_state(s));
+  MSV_TRY(set_dev(s));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  return MSV_OK;
+}
+
+int msv_state_release_scratch(msv_state *s) {
+  MSV_TRY(check_state(s));
+  MSV_TRY(set_dev(s));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  if (s->alt) MSV_HIP(hipFree(s->alt));
+  s->alt = nullptr;
+  return MSV_OK;
+}
+
+void *msv_state_device_ptr(msv_state *s) { return s ? s->buf : nullptr; }
+
+int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) {
+  MSV_TRY(check_state(s));
+  if (!mats) return fail(MSV_E_INVALID, "null matrix array");
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, apply_1q_layer_T, s, mats, active);
+}
+
+int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bonds,
+                          const double *thetas, int n_terms) {
+  MSV_TRY(check_state(s));
+  if (n_terms < 0 || (n_terms > 0 && (!kinds || !bonds || !thetas)))
+    return fail(MSV_E_INVALID, "phase_layer: null term arrays");
+  std::vector<DiagTerm> terms;
+  for (int t = 0; t < n_terms; ++t) {
+    const int a = bonds[2 * t], b = bonds[2 * t + 1];
+    if (a == b || a < 0 || b < 0 || a >= s->n || b >= s->n)
+      return fail(MSV_E_INVALID, "phase_layer: invalid bond (" + std::to_string(a) + ", " +
+                                     std::to_string(b) + ")");
+    DiagTerm d;
+    d.bits[0] = bit_of(s, a);
+    d.bits[1] = bit_of(s, b);
+    const double th = thetas[t];
+    if (kinds[t] == MSV_PHASE_ZZ) {
+      const double c = std::cos(th), sn = std::sin(th);
+      // exp(-i th Z_a Z_b): e^{-i th} when the bits agree, e^{+i th} otherwise
+      d.re[0] = c; d.im[0] = -sn;
+      d.re[1] = c; d.im[1] = sn;
+      d.re[2] = c; d.im[2] = sn;
+      d.re[3] = c; d.im[3] = -sn;
+    } else if (kinds[t] == MSV_PHASE_CPHASE) {
+      for (int k = 0; k < 3; ++k) {
+        d.re[k] = 1.0;
+        d.im[k] = 0.0;
+      }
+      if (std::fabs(th - M_PI) < 1e-12) {
+        d.re[3] = -1.0;
+        d.im[3] = 0.0;
+      } else {
+        d.re[3] = std::cos(th);
+        d.im[3] = std::sin(th);
+      }
+    } else {
+      return fail(MSV_E_INVALID, "phase_layer: unknown term kind");
+    }
+    terms.push_back(d);
+  }
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms);
+}
+
+int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
+                       const int32_t *q2, const double *d2, int n2) {
+  MSV_TRY(check_state(s));
+  std::vector<DiagTerm> terms;
+  for (int t = 0; t < n1; ++t) {
+    MSV_TRY(check_qubit(s, q1[t], "diagonal"));
+    DiagTerm d;
+    d.bits[0] = bit_of(s, q1[t]);
+    d.bits[1] = -1;
+    for (int k = 0; k < 2; ++k) {
+      d.re[k] = d1[4 * t + 2 * k];
+      d.im[k] = d1[4 * t + 2 * k + 1];
+    }
+    terms.push_back(d);
+  }
+  for (int t = 0; t < n2; ++t) {
+    MSV_TRY(check_qubit(s, q2[2 * t], "diagonal"));
+    MSV_TRY(check_qubit(s, q2[2 * t + 1], "diagonal"));
+    if (q2[2 * t] == q2[2 * t + 1]) return fail(MSV_E_INVALID, "diagonal: duplicate qubits");
+    DiagTerm d;
+    d.bits[0] = bit_of(s, q2[2 * t]);
+    d.bits[1] = bit_of(s, q2[2 * t + 1]);
+    for (int k = 0; k < 4; ++k) {
+      d.re[k] = d2[8 * t + 2 * k];
+      d.im[k] = d2[8 * t + 2 * k + 1];
+    }
+    terms.push_back(d);
+  }
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms);
+}
+
+int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c) {
+  MSV_TRY(check_state(s));
+  if (!rows) return fail(MSV_E_INVALID, "null rows");
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, apply_xor_T, s, rows, c);
+}
+
+int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
+  MSV_TRY(check_state(s));
+  if (!(0 <= first && first < m && m <= s->n))
+    return fail(MSV_E_INVALID, "qft: invalid qubit range [" + std::to_string(first) + ", " +
+                                   std::to_string(m) + ") for " + std::to_string(s->n) +
+                                   " qubits");
+  MSV_TRY(set_dev(s));
+  const int hb = s->n - 1 - first, lb = s->n - m;
+  return MSV_DISPATCH(s, apply_qft_T, s, hb, lb, inverse != 0, -1);
+}
+
+int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse) {
+  MSV_TRY(check_state(s));
+  if (!(0 <= j && j < m && m <= s->n))
+    return fail(MSV_E_INVALID, "qft_stage: invalid j=" + std::to_string(j) + ", m=" +
+                                   std::to_string(m) + ", n=" + std::to_string(s->n));
+  MSV_TRY(set_dev(s));
+  const int p = s->n - 1 - j, lb = s->n - m;
+  return MSV_DISPATCH(s, apply_qft_T, s, p, lb, inverse != 0, p);
+}
+
+int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
+  MSV_TRY(check_state(s));
+  if (n_gates < 0 || (n_gates > 0 && !gates)) return fail(MSV_E_INVALID, "null gate list");
+  std::vector<PlanOp> ops;
+  ops.reserve(n_gates);
+  for (int g = 0; g < n_gates; ++g) {
+    const msv_gate &G = gates[g];
+    if (G.kind == MSV_G_U) {
+      if (G.ncontrols < 0 || G.ncontrols > 5) return fail(MSV_E_INVALID, "gate: bad control count");
+      MSV_TRY(check_qubit(s, G.q[0], "gate"));
+      PlanOp po;
+      po.kind = OP_U2;
+      po.tbit = bit_of(s, G.q[0]);
+      for (int c = 0; c < G.ncontrols; ++c) {
+        MSV_TRY(check_qubit(s, G.q[1 + c], "gate control"));
+        const int cb = bit_of(s, G.q[1 + c]);
+        if (cb == po.tbit || ((po.cmask >> cb) & 1ull))
+          return fail(MSV_E_INVALID, "gate: control and target wires must be distinct");
+        po.cmask |= 1ull << cb;
+      }
+      for (int k = 0; k < 8; ++k) po.m[k] = G.m[k];
+      ops.push_back(po);
+    } else if (G.kind == MSV_G_DIAG1) {
+      MSV_TRY(check_qubit(s, G.q[0], "diagonal gate"));
+      PlanOp po;
+      po.kind = OP_DIAG;
+      po.abit = bit_of(s, G.q[0]);
+      for (int k = 0; k < 4; ++k) po.m[k] = G.m[k];
+      ops.push_back(po);
+    } else if (G.kind == MSV_G_DIAG2) {
+      MSV_TRY(check_qubit(s, G.q[0], "diagonal gate"));
+      MSV_TRY(check_qubit(s, G.q[1], "diagonal gate"));
+      if (G.q[0] == G.q[1]) return fail(MSV_E_INVALID, "diagonal gate: duplicate qubits");
+      PlanOp po;
+      po.kind = OP_DIAG;
+      po.abit = bit_of(s, G.q[0]);
+      po.bbit = bit_of(s, G.q[1]);
+      for (int k = 0; k < 8; ++k) po.m[k] = G.m[k];
+      ops.push_back(po);
+    } else if (G.kind == MSV_G_SWAP) {
+      if (G.ncontrols < 0 || G.ncontrols > 4) return fail(MSV_E_INVALID, "swap: bad control count");
+      MSV_TRY(check_qubit(s, G.q[0], "swap"));
+      MSV_TRY(check_qubit(s, G.q[1], "swap"));
+      if (G.q[0] == G.q[1]) continue;
+      const int a = bit_of(s, G.q[0]), b = bit_of(s, G.q[1]);
+      uint64_t cm = 0;
+      for (int c = 0; c < G.ncontrols; ++c) {
+        MSV_TRY(check_qubit(s, G.q[2 + c], "swap control"));
+        const int cb = bit_of(s, G.q[2 + c]);
+        if (cb == a || cb == b) return fail(MSV_E_INVALID, "swap: control overlaps a target");
+        cm |= 1ull << cb;
+      }
+      // SWAP = CX(a->b) CX(b->a) CX(a->b), each carrying the extra controls
+      const int seq[3][2] = {{a, b}, {b, a}, {a, b}};
+      for (auto &cx : seq) {
+        PlanOp po;
+        po.kind = OP_U2;
+        po.tbit = cx[1];
+        po.cmask = cm | (1ull << cx[0]);
+        po.m[2] = 1.0;
+        po.m[4] = 1.0;
+        ops.push_back(po);
+      }
+    } else {
+      return fail(MSV_E_INVALID, "gate: unknown kind " + std::to_string(G.kind));
+    }
+  }
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, apply_plan_ops, s, ops);
+}
+
+int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out) {
+  MSV_TRY(check_state(s));
+  if (!out) return fail(MSV_E_INVALID, "null output");
+  if (wires) {
+    if (nw < 0 || nw > s->n) return fail(MSV_E_INVALID, "probabilities: bad wire count");
+    uint64_t seen = 0;
+    for (int w = 0; w < nw; ++w) {
+      MSV_TRY(check_qubit(s, wires[w], "probabilities"));
+      if ((seen >> wires[w]) & 1ull) return fail(MSV_E_INVALID, "Duplicate qubit indices");
+      seen |= 1ull << wires[w];
+    }
+  }
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, probabilities_T, s, wires, nw, out);
+}
+
+int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_t seed,
+               uint64_t stream_id, uint64_t *out) {
+  MSV_TRY(check_state(s));
+  if (shots && !out) return fail(MSV_E_INVALID, "null output");
+  std::vector<int32_t> all;
+  if (!wires) {
+    for (int q = 0; q < s->n; ++q) all.push_back(q);
+    wires = all.data();
+    nw = s->n;
+  }
+  if (nw < 0 || nw > s->n || nw > 64) return fail(MSV_E_INVALID, "sample: bad wire count");
+  uint64_t seen = 0;
+  for (int w = 0; w < nw; ++w) {
+    MSV_TRY(check_qubit(s, wires[w], "sample"));
+    if ((seen >> wires[w]) & 1ull) return fail(MSV_E_INVALID, "Duplicate qubit indices");
+    seen |= 1ull << wires[w];
+  }
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, sample_T, s, wires, nw, shots, seed, stream_id, out);
+}
+
+int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask, int n_words,
+                     double *out_re, double *out_im) {
+  MSV_TRY(check_state(s));
+  if (n_words < 0 || (n_words > 0 && (!xmask || !zmask || !out_re || !out_im)))
+    return fail(MSV_E_INVALID, "expval_pauli: null arrays");
+  if (n_words == 0) return MSV_OK;
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, pauli_T, s, xmask, zmask, n_words, out_re, out_im);
+}
+
+int msv_inner(msv_state *a, msv_state *b, double *re, double *im) {
+  MSV_TRY(check_state(a));
+  MSV_TRY(check_state(b));
+  if (!re || !im) return fail(MSV_E_INVALID, "null output");
+  if (a->n != b->n) return fail(MSV_E_INVALID, "inner: qubit counts differ");
+  if (a->device != b->device) return fail(MSV_E_INVALID, "inner: states on different devices");
+  MSV_TRY(set_dev(a));
+  if (a->dtype == MSV_C64 && b->dtype == MSV_C64) return inner_T<float, float>(a, b, re, im);
+  if (a->dtype == MSV_C64) return inner_T<float, double>(a, b, re, im);
+  if (b->dtype == MSV_C64) return inner_T<double, float>(a, b, re, im);
+  return inner_T<double, double>(a, b, re, im);
+}
+
+int msv_norm2(msv_state *s, double *out) {
+  MSV_TRY(check_state(s));
+  if (!out) return fail(MSV_E_INVALID, "null output");
+  MSV_TRY(set_dev(s));
+  std::vector<double> bs;
+  int rc = s->dtype == MSV_C128 ? block_sums_T<double>(s, &bs) : block_sums_T<float>(s, &bs);
+  if (rc != MSV_OK) return rc;
+  double t = 0.0;
+  for (double v : bs) t += v;
+  *out = t;
+  return MSV_OK;
+}
+
+int msv_set_tuning(int key, int64_t value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  switch (key) {
+    case MSV_TUNE_TILE_LOCAL_BITS:
+      if (value < 6 || value > TileCfg<float>::kMaxLocal)
+        return fail(MSV_E_INVALID, "tile local bits out of range");
+      g_tune.local_bits_c64 = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_TILE_GATE_BITS:
+      if (value < 5 || value > 10) return fail(MSV_E_INVALID, "gate bits out of range");
+      g_tune.gate_bits = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_PHASE_BLOCKS:
+      if (value < 0 || value > (1 << 20)) return fail(MSV_E_INVALID, "phase blocks out of range");
+      g_tune.phase_blocks = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_SWEEP_SPLIT:
+      g_tune.sweep_split = static_cast<int>(value);
+      return MSV_OK;
+    default: return fail(MSV_E_INVALID, "unknown tuning key");
+  }
+}
+
+int msv_get_tuning(int key, int64_t *value) {
+  if (!value) return fail(MSV_E_INVALID, "null output");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  switch (key) {
+    case MSV_TUNE_TILE_LOCAL_BITS: *value = g_tune.local_bits_c64; return MSV_OK;
+    case MSV_TUNE_TILE_GATE_BITS: *value = g_tune.gate_bits; return MSV_OK;
+    case MSV_TUNE_PHASE_BLOCKS: *value = g_tune.phase_blocks; return MSV_OK;
+    case MSV_TUNE_SWEEP_SPLIT: *value = g_tune.sweep_split; return MSV_OK;
+    default: return fail(MSV_E_INVALID, "unknown tuning key");
+  }
+}
+
+int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset) {
+  MSV_TRY(check_state(s));
+  if (out) *out = s->sweeps;
+  if (reset) s->sweeps = 0;
+  return MSV_OK;
+}
+
+int msv_state_timer_start(msv_state *s) {
+  MSV_TRY(check_state(s));
+  MSV_TRY(set_dev(s));
+  if (!s->ev0) MSV_HIP(hipEventCreate(&s->ev0));
+  if (!s->ev1) MSV_HIP(hipEventCreate(&s->ev1));
+  MSV_HIP(hipEventRecord(s->ev0, s->stream));
+  return MSV_OK;
+}
+
+int msv_state_timer_stop(msv_state *s, double *ms) {
+  MSV_TRY(check_state(s));
+  if (!ms) return fail(MSV_E_INVALID, "null output");
+  if (!s->ev0 || !s->ev1) return fail(MSV_E_INVALID, "timer was not started");
+  MSV_TRY(set_dev(s));
+  MSV_HIP(hipEventRecord(s->ev1, s->stream));
+  MSV_HIP(hipEventSynchronize(s->ev1));
+  float f = 0.f;
+  MSV_HIP(hipEventElapsedTime(&f, s->ev0, s->ev1));
+  *ms = static_cast<double>(f);
+  return MSV_OK;
+}
+
+int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_per_copy) {
+  if (!ms_per_copy || iters < 1 || bytes < 16) return fail(MSV_E_INVALID, "probe: bad arguments");
+  MSV_HIP(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    (void)hipGetLastError();
+    return fail(MSV_E_OOM, "probe: allocation failed");
+  }
+  hipStream_t st;
+  MSV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  MSV_HIP(hipMemsetAsync(a, 0, bytes, st));
+  const uint64_t nvec = bytes / 16;
+  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((nvec + 255) / 256, dev_cu(device) * 16ull));
+  MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, grid, st));
+  hipEvent_t e0, e1;
+  MSV_HIP(hipEventCreate(&e0));
+  MSV_HIP(hipEventCreate(&e1));
+  MSV_HIP(hipEventRecord(e0, st));
+  for (int i = 0; i < iters; ++i)
+    MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, grid, st));
+  MSV_HIP(hipEventRecord(e1, st));
+  MSV_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  MSV_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_copy = static_cast<double>(ms) / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return MSV_OK;
+}
+
+}  // extern "C"

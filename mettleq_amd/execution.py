@@ -1,0 +1,384 @@
+"""Statevector memory preflight, runtime capability report and execution plan.
+
+Restates the reference's preflight contract (execution.py:186-407) for one
+MI355X: the same decision strings, report keys, override policy and
+``StatevectorMemoryError``, re-sized for 288 GB HBM and for this engine's
+kernels, which are in place except the XOR permutation (a gather into a
+second buffer). Two lower bounds therefore exist:
+
+  in_place      one state + the library's fixed scratch (staging ring and
+                reduction buffers) -- circuits without XOR layers
+  out_of_place  two states (the reference's model, execution.py:195-223)
+
+Device limits come from an injectable provider (``_device_info``) so tests
+can fake a constrained GPU exactly like the reference's tests do
+(tests/mlxQExecutionPlanTest.py:69-81).
+"""
+from __future__ import annotations
+
+import math
+import os
+from collections import Counter
+from typing import Any, Dict, Iterable, List, Optional
+
+STATE_DTYPE = "complex64"
+BYTES_PER_AMPLITUDE = {"complex64": 8, "complex128": 16}
+INDEX_BITS = 64
+QUBIT_LIMIT = 40
+# staging ring (1 MiB) + reduction scratch (up to 1 MiB for the usual
+# reductions) + LUTs, rounded up; full probability vectors are separate
+FIXED_SCRATCH_BYTES = 8 << 20
+STATEVECTOR_UNSAFE_OVERRIDE_ENV = "METTLEQ_ALLOW_UNSAFE_STATEVECTOR"
+HIP_KERNELS_ENV = "METTLEQ_HIP_KERNELS"
+
+_TRUE_VALUES = {"1", "true", "on", "yes", "enabled"}
+_FALSE_VALUES = {"0", "false", "off", "no", "disabled", ""}
+
+
+class StatevectorMemoryError(MemoryError):
+    """Raised before allocation when reported device limits are insufficient."""
+
+    def __init__(self, message: str, report: Dict[str, Any]):
+        super().__init__(message)
+        self.report = report
+
+
+def _device_info() -> Dict[str, Any]:
+    """Device memory limits of GPU 0 from the HIP runtime (empty if none)."""
+    try:
+        from . import _native
+        if _native.device_count() < 1:
+            return {}
+        info = _native.device_info(0)
+    except Exception:
+        return {}
+    return {
+        "device_name": info["name"],
+        "architecture": info["arch"],
+        "max_buffer_length": info["max_alloc_bytes"],
+        "max_recommended_working_set_size": info["free_bytes"],
+        "total_bytes": info["total_bytes"],
+        "compute_units": info["compute_units"],
+    }
+
+
+def _limit(value) -> Optional[int]:
+    if value is None:
+        return None
+    value = int(value)
+    return value if value > 0 else None
+
+
+def state_memory_estimate(n_qubits: int, dtype: str = STATE_DTYPE) -> Dict[str, Any]:
+    """State bytes and the two lower bounds (execution.py:195-223)."""
+    n = int(n_qubits)
+    if n < 0:
+        raise ValueError("n_qubits must be non-negative")
+    if dtype not in BYTES_PER_AMPLITUDE:
+        raise ValueError(f"dtype must be one of {sorted(BYTES_PER_AMPLITUDE)}")
+    bpa = BYTES_PER_AMPLITUDE[dtype]
+    state_bytes = (1 << n) * bpa
+    return {
+        "dtype": dtype,
+        "amplitudes": 1 << n,
+        "bytes_per_amplitude": bpa,
+        "state_bytes": state_bytes,
+        "state_mib": state_bytes / (1024.0 * 1024.0),
+        "temporary_memory_expectation_bytes": state_bytes,
+        "minimum_input_plus_output_bytes": 2 * state_bytes,
+        "minimum_in_place_bytes": state_bytes + FIXED_SCRATCH_BYTES,
+        "expectation_basis": (
+            "in-place layer kernels need one state plus fixed scratch; an XOR "
+            "permutation gathers into a second state buffer"
+        ),
+    }
+
+
+def statevector_unsafe_override_policy(allow_unsafe: Optional[bool] = None) -> Dict[str, Any]:
+    """Constructor argument first, then the environment; invalid values fail
+    closed (execution.py:226-267)."""
+    raw = os.environ.get(STATEVECTOR_UNSAFE_OVERRIDE_ENV)
+    if allow_unsafe is not None:
+        if not isinstance(allow_unsafe, bool):
+            raise TypeError("allow_unsafe_statevector must be a boolean or None")
+        return {"enabled": allow_unsafe, "source": "constructor_argument",
+                "environment_variable": STATEVECTOR_UNSAFE_OVERRIDE_ENV, "environment_value": raw}
+    if raw is None:
+        return {"enabled": False, "source": "default_disabled",
+                "environment_variable": STATEVECTOR_UNSAFE_OVERRIDE_ENV, "environment_value": None}
+    normalized = raw.strip().lower()
+    if normalized in _TRUE_VALUES:
+        enabled = True
+    elif normalized in _FALSE_VALUES:
+        enabled = False
+    else:
+        raise ValueError(f"{STATEVECTOR_UNSAFE_OVERRIDE_ENV} must be a boolean value")
+    return {"enabled": enabled, "source": "environment",
+            "environment_variable": STATEVECTOR_UNSAFE_OVERRIDE_ENV, "environment_value": raw}
+
+
+def statevector_preflight(
+    n_qubits: int,
+    *,
+    dtype: str = STATE_DTYPE,
+    allow_unsafe: Optional[bool] = None,
+    mode: str = "out_of_place",
+    reserved_bytes: int = 0,
+) -> Dict[str, Any]:
+    """One-sided check before allocating a statevector (execution.py:270-374).
+
+    ``mode`` selects the lower bound: "out_of_place" (two states, the
+    reference model) or "in_place" (one state plus fixed scratch).
+    ``reserved_bytes`` is memory already promised to other resident states
+    (the serving loop's admission control).
+    """
+    n = int(n_qubits)
+    if n < 0:
+        raise ValueError("n_qubits must be non-negative")
+    if mode not in ("out_of_place", "in_place"):
+        raise ValueError("mode must be 'out_of_place' or 'in_place'")
+    memory = state_memory_estimate(n, dtype=dtype)
+    override = statevector_unsafe_override_policy(allow_unsafe)
+    info = _device_info()
+    max_buffer = _limit(info.get("max_buffer_length"))
+    working_set = _limit(info.get("max_recommended_working_set_size"))
+    if working_set is not None:
+        working_set = max(0, working_set - int(reserved_bytes))
+    state_bytes = memory["state_bytes"]
+    lower = (memory["minimum_input_plus_output_bytes"] if mode == "out_of_place"
+             else memory["minimum_in_place_bytes"])
+    buffer_check = state_bytes <= max_buffer if max_buffer is not None else None
+    working_check = lower <= working_set if working_set is not None else None
+    index_check = n <= QUBIT_LIMIT
+    failures: List[str] = []
+    if buffer_check is False:
+        failures.append("single_state_exceeds_max_buffer_length")
+    if working_check is False:
+        failures.append("two_state_lower_bound_exceeds_recommended_working_set"
+                        if mode == "out_of_place"
+                        else "in_place_lower_bound_exceeds_recommended_working_set")
+    if not index_check:
+        failures.append("qubit_count_exceeds_engine_limit")
+    refused = bool(failures)
+    overridden = refused and bool(override["enabled"]) and index_check
+    allowed = (not refused) or overridden
+    available = buffer_check is not None and working_check is not None
+    if overridden:
+        decision = "allowed_with_unsafe_override"
+        reason = ("reported device limits reject the allocation, but the explicit "
+                  "unsafe override permits it")
+    elif refused:
+        decision = "refused"
+        reason = "reported device limits are below the statevector lower bound"
+    elif available:
+        decision = "allowed_within_reported_lower_bounds"
+        reason = "statevector lower bounds fit within the reported device limits"
+    else:
+        decision = "allowed_with_unverified_device_limits"
+        reason = "one or more device memory limits are unavailable"
+    second_key = ("two_state_lower_bound_within_recommended_working_set" if mode == "out_of_place"
+                  else "in_place_lower_bound_within_recommended_working_set")
+    return {
+        "schema_version": 1,
+        "allowed": allowed,
+        "decision": decision,
+        "reason": reason,
+        "overridden": overridden,
+        "override": override,
+        "failure_reasons": failures,
+        "qubits": n,
+        "dtype": dtype,
+        "mode": mode,
+        "device": {
+            "name": info.get("device_name"),
+            "architecture": info.get("architecture"),
+            "max_buffer_length": max_buffer,
+            "max_recommended_working_set_size": working_set,
+            "reserved_bytes": int(reserved_bytes),
+        },
+        "checks": {
+            "single_state_within_max_buffer_length": buffer_check,
+            second_key: working_check,
+        },
+        "cost_model": {
+            **memory,
+            "lower_bound_bytes": lower,
+            "recommended_working_set_headroom_bytes": (working_set - lower
+                                                       if working_set is not None else None),
+            "scope": "allocation preflight lower bound",
+            "not_included": ["full probability vectors requested by the caller",
+                             "memory used by other processes"],
+        },
+        "passing_guarantee": (
+            "Passing does not guarantee that an arbitrary circuit fits; it only "
+            "establishes that the reported lower bounds do not already rule it out."
+        ),
+    }
+
+
+def require_statevector_preflight(n_qubits: int, *, dtype: str = STATE_DTYPE,
+                                  allow_unsafe: Optional[bool] = None,
+                                  mode: str = "out_of_place",
+                                  reserved_bytes: int = 0) -> Dict[str, Any]:
+    """Return the report or raise StatevectorMemoryError (execution.py:377-407)."""
+    report = statevector_preflight(n_qubits, dtype=dtype, allow_unsafe=allow_unsafe, mode=mode,
+                                   reserved_bytes=reserved_bytes)
+    if report["allowed"]:
+        return report
+    cost = report["cost_model"]
+    device = report["device"]
+    gib = 1024.0 ** 3
+
+    def _lim(v):
+        return f"{float(v) / gib:.2f} GiB" if v is not None else "unknown"
+
+    bound = "out-of-place" if mode == "out_of_place" else "in-place"
+    message = (
+        f"Refusing {n_qubits}-qubit {dtype} statevector before allocation: one state is "
+        f"{cost['state_bytes'] / gib:.2f} GiB and the {bound} lower bound is "
+        f"{cost['lower_bound_bytes'] / gib:.2f} GiB; device max buffer is "
+        f"{_lim(device['max_buffer_length'])} and free working set is "
+        f"{_lim(device['max_recommended_working_set_size'])}. Reduce qubits, use complex64, "
+        f"or set {STATEVECTOR_UNSAFE_OVERRIDE_ENV}=1 to deliberately bypass this preflight."
+    )
+    raise StatevectorMemoryError(message, report)
+
+
+# ---------------------------------------------------------------------------
+# runtime status and execution plan
+# ---------------------------------------------------------------------------
+
+
+def hip_runtime_status(n_qubits: Optional[int] = None, *, dtype: str = STATE_DTYPE) -> Dict[str, Any]:
+    """Whether the HIP engine can run here (the counterpart of
+    metal_runtime_status, execution.py:476-591)."""
+    raw = os.environ.get(HIP_KERNELS_ENV)
+    policy = "auto_default" if raw is None else raw.strip().lower()
+    checks = {"library_loaded": False, "device_present": False, "architecture_gfx950": False,
+              "index_width": n_qubits is None or int(n_qubits) <= QUBIT_LIMIT,
+              "dtype_supported": dtype in BYTES_PER_AMPLITUDE}
+    arch = None
+    try:
+        from . import _native
+        _native.lib()
+        checks["library_loaded"] = True
+        if _native.device_count() > 0:
+            checks["device_present"] = True
+            arch = _native.device_info(0)["arch"]
+            checks["architecture_gfx950"] = arch.split(":")[0] == "gfx950"
+    except Exception:
+        pass
+    if policy not in ("auto_default",) and policy not in _TRUE_VALUES and policy not in _FALSE_VALUES:
+        return {"policy": "invalid", "requested": False, "enabled": False, "checks": checks,
+                "reason": f"invalid {HIP_KERNELS_ENV} value", "architecture": arch}
+    requested = policy == "auto_default" or policy in _TRUE_VALUES
+    enabled = requested and all(checks.values())
+    reason = ("HIP kernels selected" if enabled else
+              ("disabled by policy" if not requested else
+               "missing: " + ", ".join(k for k, v in checks.items() if not v)))
+    return {"policy": policy, "requested": requested, "enabled": enabled, "checks": checks,
+            "reason": reason, "architecture": arch,
+            "kernel_constraints": {"index_bits": INDEX_BITS, "index_qubit_limit": QUBIT_LIMIT}}
+
+
+def layer_sweeps(op: Dict[str, Any], n: int, dtype: str = STATE_DTYPE) -> int:
+    """Full-state sweeps an optimized op costs on this engine (planning model
+    of the native tile planner: low sweep of 13 (c64) / 12 (c128) local bits,
+    high chunks of at most 9 bits)."""
+    name = str(op.get("name", "")).upper()
+    nl = 13 if dtype == "complex64" else 12
+    if name in ("_U2LISTLAYER", "_U2LAYER", "_RXLAYER"):
+        active = op.get("active")
+        bits = sorted(n - 1 - q for q in (active if active is not None else range(n)))
+        return _sweeps_for_bits(bits, n, nl)
+    if name in ("_QFTLAYER",):
+        return _sweeps_for_bits(list(range(n)), n, nl)
+    if name in ("_QFTSTAGE",):
+        return 1
+    if name in ("_DIAGLAYER", "_DIAGWEIGHTED", "_ZZLAYER", "_ZZWEIGHTED", "_DIAGONAL"):
+        return 1
+    if name in ("_XORLAYER",):
+        return 1
+    if name in ("_XXLAYER", "_YYLAYER", "_CHAINPHASE_RXLAYER"):
+        return 2 * _sweeps_for_bits(list(range(n)), n, nl) + 1
+    if name == "_GATEBLOCK":
+        return int(op.get("sweeps", 1))
+    return 1
+
+
+def _sweeps_for_bits(bits: List[int], n: int, nl: int) -> int:
+    if not bits:
+        return 0
+    if n <= nl:
+        return 1
+    rb = 5 if nl == 13 else 4
+    h = n - nl
+    low = nl
+    if h < rb:
+        low = n - rb
+        h = rb
+    k = (h + 8) // 9
+    chunks = []
+    start = low
+    for c in range(k):
+        sz = h // k + (1 if c < h % k else 0)
+        chunks.append((start, start + sz))
+        start += sz
+    count = 1 if any(b < low for b in bits) else 0
+    for lo, hi in chunks:
+        if any(lo <= b < hi for b in bits):
+            count += 1
+    return count
+
+
+def build_execution_plan(*, n_qubits: int, operations: List[Dict[str, Any]],
+                         optimized_operations: List[Dict[str, Any]],
+                         statevector_preflight_data: Optional[Dict[str, Any]] = None,
+                         dtype: str = STATE_DTYPE) -> Dict[str, Any]:
+    """Launch-accounting report (execution.py:783-1029): matched structured
+    patterns, predicted sweeps and their byte traffic."""
+    n = int(n_qubits)
+    status = hip_runtime_status(n, dtype=dtype)
+    memory = state_memory_estimate(n, dtype)
+    matched = []
+    for idx, op in enumerate(optimized_operations):
+        name = str(op.get("name", ""))
+        if name.startswith("_"):
+            s = layer_sweeps(op, n, dtype)
+            matched.append({"optimized_operation_index": idx, "kernel_family": name.lower().lstrip("_"),
+                            "expected_sweeps": s})
+    sweep_bytes = 2 * memory["state_bytes"]
+    predicted = sum(m["expected_sweeps"] for m in matched)
+    return {
+        "schema_version": 1,
+        "engine": "mettleq_amd HIP statevector (gfx950)",
+        "qubits": n,
+        "dtype": dtype,
+        "input_operation_count": len(operations),
+        "optimized_operation_count": len(optimized_operations),
+        "matched_structured_patterns": matched,
+        "pattern_counts": dict(Counter(m["kernel_family"] for m in matched)),
+        "expected_custom_kernel_launches": predicted,
+        "capabilities": status,
+        "memory": memory,
+        "statevector_preflight": statevector_preflight_data,
+        "execution_cost_model": {
+            "scope": "full-state sweeps of structured layers",
+            "state_bytes": memory["state_bytes"],
+            "bytes_per_sweep": sweep_bytes,
+            "predicted_sweeps": predicted,
+            "predicted_input_output_bytes": predicted * sweep_bytes,
+        },
+        "observed_custom_dispatches": [],
+        "execution_status": "planned",
+        "synchronized": False,
+    }
+
+
+def mark_synchronized(plan: Optional[Dict[str, Any]], observed_sweeps: Optional[int] = None) -> None:
+    if plan is None:
+        return
+    plan["execution_status"] = "synchronized"
+    plan["synchronized"] = True
+    if observed_sweeps is not None:
+        plan["observed_sweeps"] = int(observed_sweeps)

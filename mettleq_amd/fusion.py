@@ -1,0 +1,424 @@
+"""Runtime gate fusion: the reference's pattern matcher over canonical ops.
+
+Restates device.py:556-1087 of the reference (paths relative to its
+src/mettleq) so the same gate streams become the same structured layers:
+
+  ASAP reschedule of ready one-qubit ops when U2/U3 are present  (556-606, 628-634)
+  one-qubit windows -> _RXLAYER / _U2LAYER / _U2LISTLAYER / X-flip _XORLAYER,
+      per-wire complex128 products with exact-identity collapse   (643-755)
+  CH/CRX/CRY/CRZ runs -> _CULISTLAYER                             (759-806)
+  X/CNOT/SWAP runs -> _XORLAYER via the inverse GF(2) affine map  (810-834)
+  QFT stages -> _QFTSTAGE; complete forward QFT -> _QFTLAYER      (842-894, 992-1028)
+  CZ/CPHASE runs -> _DIAGLAYER / _DIAGWEIGHTED                    (898-938)
+  ZZPHASE runs -> _ZZLAYER / _ZZWEIGHTED                          (939-965)
+  uniform XX/YY runs -> _XXLAYER / _YYLAYER                       (970-989)
+
+Ops left unmatched become "_GATE" entries holding device gates
+(``gates_for_op``, the routing of device.py:1138-1290); the dispatcher
+hands consecutive ones to the native planner in one call.
+"""
+from __future__ import annotations
+
+import math
+import os
+from typing import Any, Dict, List
+
+import numpy as np
+
+from . import gates as G
+from . import layers as L
+from .native_state import gate_diag1, gate_diag2, gate_swap, gate_u
+
+_FALSE = {"0", "false", "off", "no"}
+
+
+def _env_on(name: str, default: str = "1") -> bool:
+    return os.environ.get(name, default).strip().lower() not in _FALSE
+
+
+def _pauli_pair_gates(name, theta, a, b):
+    from .sim import _pauli_pair_gates as f
+    return f(name, theta, a, b)
+
+
+def dependency_schedule(operations: List[Dict[str, Any]]) -> List[Dict[str, Any]]:
+    """Per-wire DAG, ready one-qubit ops first (device.py:556-606)."""
+    count = len(operations)
+    if count < 2:
+        return list(operations)
+    indegree = [0] * count
+    successors = [set() for _ in range(count)]
+    last_on_wire: Dict[int, int] = {}
+    for index, operation in enumerate(operations):
+        preds = {last_on_wire[w] for w in operation.get("wires", []) if w in last_on_wire}
+        indegree[index] = len(preds)
+        for p in preds:
+            successors[p].add(index)
+        for w in operation.get("wires", []):
+            last_on_wire[w] = index
+    ready = {i for i, d in enumerate(indegree) if d == 0}
+    scheduled: List[Dict[str, Any]] = []
+    while ready:
+        single = [i for i in ready if len(operations[i].get("wires", [])) == 1]
+        batch = sorted(single if single else ready)
+        for i in batch:
+            ready.remove(i)
+            scheduled.append(operations[i])
+        for i in batch:
+            for s in successors[i]:
+                indegree[s] -= 1
+                if indegree[s] == 0:
+                    ready.add(s)
+    if len(scheduled) != count:
+        raise RuntimeError("operation dependency graph contains a cycle")
+    return scheduled
+
+def predicted_sweeps(fused: List[Dict[str, Any]], n: int) -> float:
+    """Cost model used to choose between fusion candidates: full-state sweeps
+    of the structured layers plus about one sweep per eight queued gates."""
+    from .execution import layer_sweeps
+    cost = 0.0
+    queued = 0
+    for op in fused:
+        if op["name"] == "_GATE":
+            queued += max(1, len(op["gates"]))
+            continue
+        cost += layer_sweeps(op, n) + (queued + 7) // 8
+        queued = 0
+    return cost + (queued + 7) // 8
+
+
+def fuse_operations(operations: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:
+    """Reference pattern matching (device.py:608-1087) over canonical ops.
+
+    When U2/U3 are present the reference always reschedules ready one-qubit
+    ops first (device.py:628-634); that exposes SDK-serialised one-qubit
+    layers but also pulls the Hadamards out of QFT ladders. Both orders are
+    matched here and the one with fewer predicted sweeps is kept."""
+    if os.environ.get("METTLEQ_DENSE_ONLY", "0") == "1":
+        return [{"name": "_GATE", "gates": gates_for_op(op, n)} for op in operations]
+    direct = _fuse(list(operations), n)
+    if any(str(op.get("name", "")).upper() in {"U2", "U3"} for op in operations):
+        rescheduled = _fuse(dependency_schedule(operations), n)
+        if predicted_sweeps(rescheduled, n) < predicted_sweeps(direct, n):
+            return rescheduled
+    return direct
+
+
+def _fuse(operations: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:
+    fused: List[Dict[str, Any]] = []
+    i = 0
+    n_ops = len(operations)
+    while i < n_ops:
+        op = operations[i]
+        name = str(op.get("name", "")).upper()
+        wires = list(op.get("wires", []))
+        # the closing Hadamard of a forward QFT stays a single gate so the
+        # complete-QFT collapse below can see it
+        if (name == "H" and wires == [n - 1] and fused and fused[-1].get("name") == "_QFTSTAGE"
+                and fused[-1].get("j") == n - 2 and fused[-1].get("m") == n
+                and not fused[-1].get("inverse", False)):
+            fused.append({"name": "_GATE", "gates": gates_for_op(op, n), "source": name,
+                          "wires": wires})
+            i += 1
+            continue
+        # one-qubit windows (device.py:643-755)
+        if len(wires) == 1:
+            j = i
+            per_wire: Dict[int, list] = {}
+            while j < n_ops:
+                w = operations[j].get("wires", [])
+                if len(w) != 1 or not (0 <= w[0] < n):
+                    break
+                per_wire.setdefault(w[0], []).append(
+                    (str(operations[j].get("name", "")).upper(),
+                     tuple(operations[j].get("parameters", []) or [])))
+                j += 1
+            k_ops = j - i
+            seqs = list(per_wire.values())
+            uniform = sorted(per_wire) == list(range(n)) and all(s == seqs[0] for s in seqs)
+            if uniform and len(seqs[0]) == 1:
+                gname, gparams = seqs[0][0]
+                if gname == "RX":
+                    fused.append({"name": "_RXLAYER", "theta": gparams[0]})
+                elif gname == "X":
+                    fused.append({"name": "_XORLAYER", "rows": [1 << (n - 1 - k) for k in range(n)],
+                                  "c": (1 << n) - 1})
+                elif gname == "I":
+                    pass
+                else:
+                    fused.append({"name": "_U2LAYER", "gate": gname, "params": list(gparams)})
+                i = j
+                continue
+            if k_ops >= 2:
+                prods = {}
+                ok = True
+                for wq, seq in per_wire.items():
+                    m = np.eye(2, dtype=np.complex128)
+                    for gname, gparams in seq:
+                        try:
+                            m = G.single_qubit_matrix(gname, list(gparams)) @ m
+                        except (ValueError, IndexError):
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                    prods[wq] = m
+                if ok:
+                    eye2 = np.eye(2, dtype=np.complex128)
+                    xmat = G.CONST_1Q["X"]
+
+                    def _is(m, ref):
+                        return np.allclose(m, ref, rtol=0.0, atol=G.IDENTITY_ATOL)
+
+                    active = sorted(wq for wq, m in prods.items() if not _is(m, eye2))
+                    if not active:
+                        i = j
+                        continue
+                    if active == list(range(n)) and all(_is(m, xmat) for m in prods.values()):
+                        fused.append({"name": "_XORLAYER", "rows": [1 << (n - 1 - k) for k in range(n)],
+                                      "c": (1 << n) - 1})
+                        i = j
+                        continue
+                    mats = np.tile(np.eye(2, dtype=np.complex128).reshape(1, 4), (n, 1))
+                    for wq, m in prods.items():
+                        mats[wq] = m.reshape(4)
+                    fused.append({"name": "_U2LISTLAYER", "mats": mats, "active": active})
+                    i = j
+                    continue
+        # controlled one-qubit runs (device.py:759-806)
+        cu = _controlled_entry(n, op)
+        if cu is not None:
+            entries = []
+            j = i
+            while j < n_ops:
+                e = _controlled_entry(n, operations[j])
+                if e is None:
+                    break
+                entries.append(e)
+                j += 1
+            groups = []
+            k = 0
+            while k < len(entries):
+                if (k + 1 < len(entries)
+                        and {entries[k]["control"], entries[k]["target"]}.isdisjoint(
+                            {entries[k + 1]["control"], entries[k + 1]["target"]})):
+                    groups.append(entries[k:k + 2])
+                    k += 2
+                else:
+                    groups.append(entries[k:k + 1])
+                    k += 1
+            fused.append({"name": "_CULISTLAYER", "groups": groups})
+            i = j
+            continue
+        # X/CNOT/SWAP runs (device.py:810-834)
+        if name in ("X", "CNOT", "CX", "SWAP"):
+            j = i
+            while j < n_ops and _affine_ok(n, operations[j]):
+                j += 1
+            if j - i >= 2:
+                rows, c = L.compose_inverse_affine(operations[i:j], n)
+                fused.append({"name": "_XORLAYER", "rows": rows, "c": c})
+                i = j
+                continue
+        # QFT stages (device.py:842-894)
+        if name == "H" and len(wires) == 1:
+            jq = wires[0]
+            j2, kk = i + 1, jq + 1
+            while j2 < n_ops:
+                cp = _cphase_of(operations[j2])
+                if cp is not None and cp[0] == [kk, jq] and cp[1] == math.pi / (1 << (kk - jq)):
+                    kk += 1
+                    j2 += 1
+                else:
+                    break
+            if kk > jq + 1:
+                fused.append({"name": "_QFTSTAGE", "j": jq, "m": kk, "inverse": False})
+                i = j2
+                continue
+        cp0 = _cphase_of(op)
+        if cp0 is not None and cp0[1] < 0:
+            (k0, jq), _t = cp0
+            kk, j2, okrun = k0, i, True
+            while kk > jq:
+                cp = _cphase_of(operations[j2]) if j2 < n_ops else None
+                if cp is not None and cp[0] == [kk, jq] and cp[1] == -math.pi / (1 << (kk - jq)):
+                    kk -= 1
+                    j2 += 1
+                else:
+                    okrun = False
+                    break
+            if (okrun and j2 < n_ops and str(operations[j2].get("name", "")).upper() == "H"
+                    and list(operations[j2].get("wires", [])) == [jq]):
+                fused.append({"name": "_QFTSTAGE", "j": jq, "m": k0 + 1, "inverse": True})
+                i = j2 + 1
+                continue
+        # CZ / CPHASE runs (device.py:898-938)
+        dname = "CPHASE" if name == "CP" else name
+        if dname in ("CZ", "CPHASE") and _pair_ok(n, op):
+            bonds, thetas = [], []
+            j = i
+            while j < n_ops:
+                nxt = operations[j]
+                nn = str(nxt.get("name", "")).upper()
+                nn = "CPHASE" if nn == "CP" else nn
+                if nn == "CZ" and _pair_ok(n, nxt):
+                    thetas.append(math.pi)
+                elif nn == "CPHASE" and _pair_ok(n, nxt):
+                    thetas.append(list(nxt.get("parameters", [None]))[0])
+                else:
+                    break
+                bonds.append(tuple(nxt.get("wires", [])))
+                j += 1
+            if len(bonds) >= 2:
+                if all(t == thetas[0] for t in thetas):
+                    fused.append({"name": "_DIAGLAYER", "theta": thetas[0], "bonds": bonds})
+                else:
+                    fused.append({"name": "_DIAGWEIGHTED", "bonds": bonds, "thetas": thetas})
+                i = j
+                continue
+        # ZZPHASE runs (device.py:939-965)
+        if name == "ZZPHASE":
+            bonds, thetas = [], []
+            j = i
+            while j < n_ops and str(operations[j].get("name", "")).upper() == "ZZPHASE":
+                bonds.append(tuple(operations[j].get("wires", [])))
+                thetas.append(list(operations[j].get("parameters", [None]))[0])
+                j += 1
+            if len(bonds) >= 2:
+                if all(t == thetas[0] for t in thetas):
+                    fused.append({"name": "_ZZLAYER", "theta": thetas[0], "bonds": bonds})
+                else:
+                    fused.append({"name": "_ZZWEIGHTED", "bonds": bonds, "thetas": thetas})
+                i = j
+                continue
+        # uniform XX / YY runs (device.py:970-989)
+        if name in ("XXPHASE", "YYPHASE"):
+            theta = list(op.get("parameters", [None]))[0]
+            bonds = [tuple(wires)]
+            j = i + 1
+            while j < n_ops:
+                nxt = operations[j]
+                if (str(nxt.get("name", "")).upper() == name
+                        and list(nxt.get("parameters", [None]))[0] == theta):
+                    bonds.append(tuple(nxt.get("wires", [])))
+                    j += 1
+                else:
+                    break
+            if len(bonds) >= 2:
+                fused.append({"name": "_XXLAYER" if name == "XXPHASE" else "_YYLAYER",
+                              "theta": theta, "bonds": bonds})
+                i = j
+                continue
+        fused.append({"name": "_GATE", "gates": gates_for_op(op, n), "source": name,
+                      "wires": wires})
+        i += 1
+    # complete forward QFT (device.py:992-1028)
+    if n >= 2 and _env_on("METTLEQ_FULL_QFT_RADIX4"):
+        collapsed: List[Dict[str, Any]] = []
+        index = 0
+        while index < len(fused):
+            cand = fused[index:index + n]
+            stages, final = cand[:-1], (cand[-1] if cand else None)
+            if (len(cand) == n
+                    and all(s.get("name") == "_QFTSTAGE" and s.get("j") == off and s.get("m") == n
+                            and not s.get("inverse", False) for off, s in enumerate(stages))
+                    and final is not None and final.get("name") == "_GATE"
+                    and final.get("source") == "H" and final.get("wires") == [n - 1]):
+                collapsed.append({"name": "_QFTLAYER"})
+                index += n
+                continue
+            collapsed.append(fused[index])
+            index += 1
+        fused = collapsed
+    return fused
+
+def _pair_ok(n: int, o) -> bool:
+    w = o.get("wires", [])
+    return len(w) == 2 and w[0] != w[1] and all(0 <= q < n for q in w)
+
+def _affine_ok(n: int, o) -> bool:
+    nm = str(o.get("name", "")).upper()
+    w = o.get("wires", [])
+    if nm == "X":
+        return len(w) == 1 and 0 <= w[0] < n
+    if nm in ("CNOT", "CX", "SWAP"):
+        return _pair_ok(n, o)
+    return False
+
+def _cphase_of(o):
+    nm = str(o.get("name", "")).upper()
+    if nm not in ("CP", "CPHASE"):
+        return None
+    w = list(o.get("wires", []))
+    pr = list(o.get("parameters", []) or [])
+    if len(w) != 2 or len(pr) != 1:
+        return None
+    return w, pr[0]
+
+def _controlled_entry(n: int, o):
+    nm = str(o.get("name", "")).upper()
+    w = list(o.get("wires", []))
+    pr = list(o.get("parameters", []) or [])
+    if nm not in ("CH", "CRX", "CRY", "CRZ") or not _pair_ok(n, o):
+        return None
+    if nm == "CH" and not pr:
+        m = G.CONST_1Q["H"]
+    elif nm in ("CRX", "CRY", "CRZ") and len(pr) == 1:
+        m = G.controlled_target_matrix(nm, pr)
+    else:
+        return None
+    return {"control": w[0], "target": w[1], "matrix": m.reshape(4)}
+
+def gates_for_op(op: Dict[str, Any], n: int) -> List:
+    """Device gates of one unfused op, following the reference routing
+    (device.py:1138-1290)."""
+    name = str(op.get("name", "")).upper()
+    wires = [int(w) for w in op.get("wires", [])]
+    params = [float(p) for p in (op.get("parameters", []) or [])]
+    for w in wires:
+        if not (0 <= w < n):
+            raise ValueError("Qubit index out of range")
+    if name in ("SDAG", "S†"):
+        name = "SDG"
+    elif name in ("TDAG", "T†"):
+        name = "TDG"
+    elif name == "CX":
+        name = "CNOT"
+    elif name == "CP":
+        name = "CPHASE"
+    if name == "I":
+        return []
+    if len(wires) == 1:
+        q = wires[0]
+        m = G.single_qubit_matrix(name, params)
+        if name in G.DIAGONAL_1Q:
+            return [gate_diag1(q, m[0, 0], m[1, 1])]
+        return [gate_u(q, m)]
+    if len(wires) == 2:
+        a, b = wires
+        if a == b:
+            raise ValueError("Control and target must differ")
+        if name in ("CPHASE",) and len(params) != 1:
+            raise ValueError("CPHASE requires one parameter")
+        d = G.two_qubit_diagonal(name, params)
+        if d is not None:
+            return [gate_diag2(a, b, d)]
+        t = G.controlled_target_matrix(name, params)
+        if t is not None:
+            return [gate_u(b, t, [a])]
+        if name == "SWAP":
+            return [gate_swap(a, b)]
+        if name == "ISWAP":
+            return [gate_diag2(a, b, (1, 1j, 1j, 1)), gate_swap(a, b)]
+        if name in ("XXPHASE", "YYPHASE"):
+            return _pauli_pair_gates(name, params[0], a, b)
+        raise ValueError(f"Unsupported two-qubit op: {name}")
+    if len(wires) == 3:
+        if name in ("CCX", "TOFFOLI"):
+            return [gate_u(wires[2], G.CONST_1Q["X"], wires[:2])]
+        if name in ("CSWAP", "FREDKIN"):
+            return [gate_swap(wires[1], wires[2], [wires[0]])]
+        raise ValueError(f"Unsupported three-qubit op: {name}")
+    raise ValueError(f"Unsupported operation arity for: {name}")
+

@@ -1,0 +1,391 @@
+"""HIP kernels vs the CPU oracle (oracle/sv_oracle.py) through the C ABI.
+
+Tolerances: complex64 amplitudes within 5e-6 max-abs of the complex128
+oracle (the reference's kernel A/B budget, tests/mlxQMetalKernelsTest.py);
+complex128 within 1e-12; XOR permutations and sampled indices bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import sv_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 5e-6
+TOL128 = 1e-12
+
+
+def _state(n, dtype="complex64"):
+    from mettleq_amd.native_state import NativeState
+    return NativeState(n, dtype)
+
+
+def _rand_state(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+def _rand_u(rng):
+    t, p, l = rng.uniform(0, 2 * math.pi, 3)
+    return O.U3(t, p, l)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 5, 6, 9, 13, 14, 16, 17, 20, 22])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_1q_layer_matches_oracle(gpu, n, dtype):
+    rng = np.random.default_rng(100 + n)
+    psi = _rand_state(n, n)
+    mats = [_rand_u(rng) for _ in range(n)]
+    want = O.apply_1q_layer(psi, n, mats)
+    with _state(n, dtype) as s:
+        s.from_numpy(psi)
+        s.apply_1q_layer(mats)
+        got = s.to_numpy()
+    tol = TOL64 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
+@pytest.mark.parametrize("n", [6, 13, 17, 21])
+def test_1q_layer_active_subset(gpu, n):
+    rng = np.random.default_rng(7)
+    psi = _rand_state(n, 3)
+    active = sorted(rng.choice(n, size=max(1, n // 3), replace=False).tolist())
+    mats = [O.I() for _ in range(n)]
+    for q in active:
+        mats[q] = _rand_u(rng)
+    want = O.apply_1q_layer(psi, n, mats)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        # garbage matrices on inactive qubits must be ignored
+        sent = [m if q in active else np.full((2, 2), 7.0) for q, m in enumerate(mats)]
+        s.apply_1q_layer(sent, active=active)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 13, 14, 16, 18, 20])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_full_qft_matches_closed_form(gpu, n, dtype):
+    psi = _rand_state(n, 11 + n)
+    want = O.qft_matrix(n) @ psi if n <= 10 else O.qft_fft(psi, n)
+    with _state(n, dtype) as s:
+        s.from_numpy(psi)
+        s.apply_qft(0, n)
+        got = s.to_numpy()
+        s.apply_qft(0, n, inverse=True)
+        back = s.to_numpy()
+    tol = 2e-5 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+    assert np.max(np.abs(back - psi)) < tol
+
+
+def test_qft_uniform_to_zero(gpu):
+    """QFT(uniform) = |0> (tests/mlxQCoreTest.py:469-477, convention: no swaps
+    maps the uniform state to |0>)."""
+    n = 15
+    psi = np.full(1 << n, 1 / math.sqrt(1 << n), dtype=np.complex128)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_qft(0, n)
+        got = s.to_numpy()
+    want = np.zeros(1 << n)
+    want[0] = 1
+    assert np.max(np.abs(got - want)) < 1e-5
+
+
+@pytest.mark.parametrize("n,first,m", [(6, 1, 5), (10, 2, 9), (16, 0, 7), (16, 5, 16), (18, 3, 17)])
+def test_subregister_qft_matches_gate_decomposition(gpu, n, first, m):
+    psi = _rand_state(n, n + first)
+    wires = list(range(first, m))
+    want = O.qft(psi, n, wires)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_qft(first, m)
+        got = s.to_numpy()
+        s.apply_qft(first, m, inverse=True)
+        back = s.to_numpy()
+    assert np.max(np.abs(got - want)) < 2e-5
+    assert np.max(np.abs(back - psi)) < 2e-5
+
+
+@pytest.mark.parametrize("n,j,m,inv", [(5, 0, 5, False), (9, 2, 7, False), (9, 2, 7, True),
+                                       (15, 4, 15, True), (15, 14, 15, False)])
+def test_single_qft_stage(gpu, n, j, m, inv):
+    """qft_stage_sub semantics (shaders/qft.py:144-181): H(j) then the ladder
+    CPHASE(k -> j, pi/2^(k-j)); inverse = adjoint."""
+    psi = _rand_state(n, 5)
+    want = psi
+    if not inv:
+        want = O.apply_dense(want, n, O.H(), [j])
+        for k in range(j + 1, m):
+            want = O.apply_dense(want, n, O.CPHASE(math.pi / 2 ** (k - j)), [k, j])
+    else:
+        for k in reversed(range(j + 1, m)):
+            want = O.apply_dense(want, n, O.CPHASE(-math.pi / 2 ** (k - j)), [k, j])
+        want = O.apply_dense(want, n, O.H(), [j])
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_qft_stage(j, m, inverse=inv)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+@pytest.mark.parametrize("n", [2, 5, 12, 17, 20])
+def test_zz_ring_distinct_angles(gpu, n):
+    rng = np.random.default_rng(n)
+    bonds = [(i, i + 1) for i in range(n - 1)] + ([(n - 1, 0)] if n > 2 else [])
+    thetas = rng.uniform(0, math.pi, len(bonds)).tolist()
+    psi = _rand_state(n, 9)
+    want = psi * O.zz_layer_diag(n, bonds, thetas)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_phase_layer([0] * len(bonds), bonds, thetas)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+@pytest.mark.parametrize("n", [3, 8, 14, 19])
+def test_cphase_random_bonds_and_cz(gpu, n):
+    rng = np.random.default_rng(n + 1)
+    bonds = []
+    while len(bonds) < 3 * n:
+        a, b = rng.choice(n, 2, replace=False)
+        bonds.append((int(a), int(b)))
+    thetas = [math.pi if k % 3 == 0 else float(rng.uniform(-3, 3)) for k in range(len(bonds))]
+    psi = _rand_state(n, 2)
+    want = psi * O.cphase_layer_diag(n, bonds, thetas)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_phase_layer([1] * len(bonds), bonds, thetas)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+def test_cz_brick_exact_signs(gpu):
+    """CZ layers use exact +-1 entries (diag.py:89-98): on a real state the
+    result equals the oracle's sign flips exactly."""
+    n = 12
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=1 << n).astype(np.complex64)
+    bonds = [(q, q + 1) for q in range(0, n - 1, 2)]
+    want = (psi.astype(np.complex128) * O.cphase_layer_diag(n, bonds, [math.pi] * len(bonds))).astype(np.complex64)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_phase_layer([1] * len(bonds), bonds, [math.pi] * len(bonds))
+        got = s.to_numpy()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [2, 6, 11, 16, 21])
+def test_xor_cnot_chain_bit_exact(gpu, n):
+    ops = [{"name": "CNOT", "wires": [q, q + 1]} for q in range(n - 1)]
+    rows, c = O.compose_inverse_affine(ops, n)
+    rng = np.random.default_rng(n)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    want = psi[O.affine_source_indices(n, rows, c).astype(np.int64)]
+    # Gray-code closed form of the chain's inverse: src = y ^ (y >> 1)
+    y = np.arange(1 << n)
+    assert np.array_equal(want, psi[y ^ (y >> 1)])
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_xor_affine(rows, c)
+        got = s.to_numpy()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [3, 9, 17])
+def test_xor_random_affine_bit_exact(gpu, n):
+    rng = np.random.default_rng(42 + n)
+    ops = []
+    for _ in range(4 * n):
+        kind = rng.choice(["X", "CNOT", "SWAP"])
+        if kind == "X":
+            ops.append({"name": "X", "wires": [int(rng.integers(n))]})
+        else:
+            a, b = rng.choice(n, 2, replace=False)
+            ops.append({"name": str(kind), "wires": [int(a), int(b)]})
+    rows, c = O.compose_inverse_affine(ops, n)
+    psi = (rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)).astype(np.complex64)
+    want_dense = O.run_ops(n, ops, psi.astype(np.complex128))
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_xor_affine(rows, c)
+        got = s.to_numpy()
+    assert np.array_equal(got, psi[O.affine_source_indices(n, rows, c).astype(np.int64)])
+    assert np.max(np.abs(got - want_dense)) < 1e-6
+
+
+def _random_gate_list(n, count, rng):
+    from mettleq_amd.native_state import gate_u, gate_diag1, gate_diag2, gate_swap
+    gates, ops = [], []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.4:
+            q = int(rng.integers(n))
+            t, p, l = rng.uniform(0, 2 * math.pi, 3)
+            gates.append(gate_u(q, O.U3(t, p, l)))
+            ops.append({"name": "U3", "wires": [q], "parameters": [t, p, l]})
+        elif r < 0.6 and n >= 2:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            gates.append(gate_u(b, O.X(), [a]))
+            ops.append({"name": "CNOT", "wires": [a, b]})
+        elif r < 0.7 and n >= 2:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            th = float(rng.uniform(0, math.pi))
+            d = np.diag(O.pauli_pair_phase("ZZPHASE", th))
+            gates.append(gate_diag2(a, b, d))
+            ops.append({"name": "ZZPHASE", "wires": [a, b], "parameters": [th]})
+        elif r < 0.8:
+            q = int(rng.integers(n))
+            th = float(rng.uniform(-3, 3))
+            d = np.diag(O.RZ(th))
+            gates.append(gate_diag1(q, d[0], d[1]))
+            ops.append({"name": "RZ", "wires": [q], "parameters": [th]})
+        elif r < 0.9 and n >= 3:
+            a, b, c = (int(x) for x in rng.choice(n, 3, replace=False))
+            gates.append(gate_u(c, O.X(), [a, b]))
+            ops.append({"name": "TOFFOLI", "wires": [a, b, c]})
+        elif n >= 2:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            gates.append(gate_swap(a, b))
+            ops.append({"name": "SWAP", "wires": [a, b]})
+    return gates, ops
+
+
+@pytest.mark.parametrize("n,count", [(3, 30), (7, 80), (12, 120), (15, 150), (19, 100)])
+def test_generic_gate_list_matches_dense_oracle(gpu, n, count):
+    rng = np.random.default_rng(1000 + n)
+    gates, ops = _random_gate_list(n, count, rng)
+    want = O.run_ops(n, ops)
+    with _state(n) as s:
+        s.apply_gates(gates)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < 2e-5
+
+
+def test_generic_gates_complex128(gpu):
+    n = 11
+    rng = np.random.default_rng(77)
+    gates, ops = _random_gate_list(n, 120, rng)
+    want = O.run_ops(n, ops)
+    with _state(n, "complex128") as s:
+        s.apply_gates(gates)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL128
+
+
+@pytest.mark.parametrize("n,wires", [(5, None), (8, [0, 3, 7]), (12, [11, 0, 5, 6]),
+                                     (17, list(range(10))), (15, [14]), (14, [3, 2, 1, 0])])
+def test_probabilities_and_marginals(gpu, n, wires):
+    psi = _rand_state(n, 21)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        got = s.probabilities(wires)
+        amps = s.to_numpy()
+    want = O.probabilities(amps.astype(np.complex128), n, wires)
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 4, 12, 13, 18])
+def test_sampling_bit_exact_vs_cpu_restatement(gpu, n):
+    psi = _rand_state(n, 31 + n)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        amps = s.to_numpy()
+        got = s.sample(700, None, seed=20261204, stream_id=5)
+    p = O.amp_probs_c64(amps)
+    want = O.sample_indices(p, 700, 20261204, 5)
+    assert np.array_equal(got, want)
+
+
+def test_sampling_never_hits_zero_probability(gpu):
+    """Bell state: only 00 and 11 (tests/mlxQCoreTest.py:1244-1250)."""
+    n = 2
+    psi = np.array([1, 0, 0, 1], dtype=np.complex128) / math.sqrt(2)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        got = s.sample(4000, [0, 1], seed=1)
+    assert set(np.unique(got).tolist()) <= {0, 3}
+    frac = np.mean(got == 3)
+    assert abs(frac - 0.5) < 0.05
+
+
+def test_sampling_wire_subset_packing(gpu):
+    n = 6
+    psi = _rand_state(n, 8)
+    wires = [4, 0, 2]
+    with _state(n) as s:
+        s.from_numpy(psi)
+        amps = s.to_numpy()
+        full = s.sample(300, None, seed=9)
+        sub = s.sample(300, wires, seed=9)
+    assert np.array_equal(sub, O.pack_wires(full, n, wires))
+    want = O.sample_indices(O.amp_probs_c64(amps), 300, 9, 0)
+    assert np.array_equal(full, want)
+
+
+@pytest.mark.parametrize("n", [2, 7, 15])
+def test_pauli_expectations(gpu, n):
+    rng = np.random.default_rng(n)
+    psi = _rand_state(n, 44)
+    words = []
+    for _ in range(20):
+        w = {}
+        for q in rng.choice(n, size=min(n, 3), replace=False):
+            w[int(q)] = str(rng.choice(["X", "Y", "Z"]))
+        words.append(w)
+    words.append({})
+    with _state(n) as s:
+        s.from_numpy(psi)
+        amps = s.to_numpy().astype(np.complex128)
+        got = s.expval_pauli(words)
+    want = np.array([O.pauli_expectation(amps, n, w) for w in words])
+    assert np.max(np.abs(got - want)) < 1e-10
+
+
+def test_bell_pauli_algebra(gpu):
+    """<XX> = <ZZ> = 1, <YY> = -1 on the Bell state (tests/mlxQCoreTest.py:817-841)."""
+    psi = np.array([1, 0, 0, 1], dtype=np.complex128) / math.sqrt(2)
+    with _state(2) as s:
+        s.from_numpy(psi)
+        v = s.expval_pauli([{0: "X", 1: "X"}, {0: "Z", 1: "Z"}, {0: "Y", 1: "Y"}])
+    assert np.allclose(v, [1, 1, -1], atol=1e-7)
+
+
+def test_inner_norm_copy_scale(gpu):
+    n = 16
+    psi = _rand_state(n, 1)
+    with _state(n) as a, _state(n, "complex128") as b:
+        a.from_numpy(psi)
+        b.copy_from(a)
+        assert abs(a.norm2() - 1.0) < 1e-6
+        ip = a.inner(b)
+        assert abs(ip - 1.0) < 1e-6
+        a.scale(1j)
+        assert abs(a.inner(b) - (-1j)) < 1e-6
+        assert np.max(np.abs(b.to_numpy() - psi)) < 1e-7
+
+
+def test_reset_and_basis(gpu):
+    with _state(10) as s:
+        s.set_basis(777)
+        v = s.to_numpy()
+        assert v[777] == 1 and np.count_nonzero(v) == 1
+        s.reset()
+        v = s.to_numpy()
+        assert v[0] == 1 and np.count_nonzero(v) == 1
+
+
+def test_invalid_arguments_raise_value_error(gpu):
+    with _state(4) as s:
+        with pytest.raises(ValueError):
+            s.apply_phase_layer([0], [(1, 1)], [0.1])
+        with pytest.raises(ValueError):
+            s.apply_qft(3, 2)
+        with pytest.raises(ValueError):
+            s.probabilities([0, 0])
+        with pytest.raises(ValueError):
+            s.apply_xor_affine([1, 1, 1, 1], 0)  # singular over GF(2)
