@@ -39,6 +39,8 @@ MSV_TUNE_TILE_LOCAL_BITS = 1
 MSV_TUNE_TILE_GATE_BITS = 2
 MSV_TUNE_PHASE_BLOCKS = 3
 MSV_TUNE_SWEEP_SPLIT = 4
+MSV_TUNE_TILE_BLOCKS_PER_CU = 5
+MSV_TUNE_ELEM_BLOCKS_PER_CU = 6
 
 
 class NativeUnavailableError(RuntimeError):

@@ -315,8 +315,12 @@ __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict
             a[r].y *= A.scale;
           }
         }
+        // Recompute the store addresses after the ops instead of keeping the
+        // 32 load addresses (64 VGPRs) live across them.
+        uint64_t gbs = gb;
+        asm volatile("" : "+v"(gbs));
 #pragma unroll
-        for (int r = 0; r < R; ++r) st[gb + reg_goff<RB>(r, gstr)] = a[r];
+        for (int r = 0; r < R; ++r) st[gbs + reg_goff<RB>(r, gstr)] = a[r];
       } else {
         __syncthreads();
 #pragma unroll

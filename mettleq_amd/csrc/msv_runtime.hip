@@ -53,6 +53,7 @@ struct Tuning {
   int phase_blocks = 0;     // 0 = CUs * 8
   int sweep_split = 0;
   int tile_blocks_per_cu = 8;
+  int elem_blocks_per_cu = 16;
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -178,7 +179,7 @@ int ensure_alt(msv_state *s) {
 
 uint32_t elem_grid(const msv_state *s, uint64_t work) {
   const uint64_t want = (work + 255) / 256;
-  const uint64_t cap = static_cast<uint64_t>(s->cu) * 16;
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.elem_blocks_per_cu;
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
 }
 
@@ -839,7 +840,7 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
     void *d_lut = nullptr;
     MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &d_lut));
     const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
-                                                  : static_cast<uint32_t>(s->cu * 8);
+                                                  : elem_grid(s, s->amps / 2);
     MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A,
                             grid, s->stream));
     s->sweeps += 1;
@@ -1588,6 +1589,14 @@ int msv_set_tuning(int key, int64_t value) {
     case MSV_TUNE_SWEEP_SPLIT:
       g_tune.sweep_split = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_TILE_BLOCKS_PER_CU:
+      if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "tile blocks per CU out of range");
+      g_tune.tile_blocks_per_cu = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_ELEM_BLOCKS_PER_CU:
+      if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "elementwise blocks per CU out of range");
+      g_tune.elem_blocks_per_cu = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -1600,6 +1609,8 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_TILE_GATE_BITS: *value = g_tune.gate_bits; return MSV_OK;
     case MSV_TUNE_PHASE_BLOCKS: *value = g_tune.phase_blocks; return MSV_OK;
     case MSV_TUNE_SWEEP_SPLIT: *value = g_tune.sweep_split; return MSV_OK;
+    case MSV_TUNE_TILE_BLOCKS_PER_CU: *value = g_tune.tile_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune.elem_blocks_per_cu; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

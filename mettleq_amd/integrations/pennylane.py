@@ -1,0 +1,268 @@
+"""PennyLane device ``mettleq`` over the HIP statevector engine.
+
+Mirrors the reference's MettleQDevice (integrations/pennylane.py:95-643):
+wire k of the device is core wire k (MSB first, pennylane.py:397), Ising
+angles are halved on entry (pennylane.py:394-396), GlobalPhase(phi) is
+exp(-i phi) I, and State / Probability / Expectation / Variance / Sample /
+Counts measurements are served from the device-resident statevector.
+Gradients stay with PennyLane's parameter shift (use_device_gradient=False,
+pennylane.py:216-223); the engine's own adjoint kernel is exposed through
+mettleq_amd.adjoint instead.
+"""
+from __future__ import annotations
+
+from collections import Counter
+from dataclasses import replace
+from os import path
+from typing import Optional
+
+import numpy as np
+
+try:
+    import pennylane as qml
+    from pennylane.devices import Device as PennyLaneDevice
+    from pennylane.devices import ExecutionConfig
+    from pennylane.devices.modifiers import simulator_tracking, single_tape_support
+    from pennylane.devices.preprocess import (decompose, validate_device_wires,
+                                              validate_measurements, validate_observables)
+    try:
+        from pennylane.exceptions import DeviceError
+    except ImportError:  # older PennyLane
+        from pennylane import DeviceError
+    from pennylane.measurements import (CountsMP, ExpectationMP, ProbabilityMP, SampleMP, StateMP,
+                                        VarianceMP)
+    from pennylane.transforms import broadcast_expand, convert_to_numpy_parameters
+    try:
+        from pennylane.transforms.core import CompilePipeline as _Program
+    except ImportError:  # older PennyLane
+        from pennylane.transforms.core import TransformProgram as _Program
+except ImportError as exc:  # pragma: no cover - exercised without the extra
+    raise ImportError("The MettleQ PennyLane device requires the optional PennyLane dependency.") from exc
+
+from ._common import (apply_global_phase, bit_counts, core_operation, execute_operations,
+                      local_expectation, marginal_probabilities, pauli_product_expectation,
+                      pauli_product_expectations, sample_bits, statevector_numpy)
+
+_PENNYLANE_TO_CORE = {
+    "Hadamard": "H", "PauliX": "X", "PauliY": "Y", "PauliZ": "Z", "S": "S", "T": "T", "SX": "SX",
+    "RX": "RX", "RY": "RY", "RZ": "RZ", "PhaseShift": "U1", "CNOT": "CNOT", "CH": "CH", "CZ": "CZ",
+    "ControlledPhaseShift": "CPHASE", "CRX": "CRX", "CRY": "CRY", "CRZ": "CRZ", "SWAP": "SWAP",
+    "ISWAP": "ISWAP", "Toffoli": "TOFFOLI", "CSWAP": "FREDKIN", "IsingXX": "XXPHASE",
+    "IsingYY": "YYPHASE", "IsingZZ": "ZZPHASE",
+}
+_ANALYTIC = (StateMP, ProbabilityMP, ExpectationMP, VarianceMP)
+_SHOT = (ProbabilityMP, SampleMP, CountsMP, ExpectationMP, VarianceMP)
+
+
+@simulator_tracking
+@single_tape_support
+class MettleQDevice(PennyLaneDevice):
+    """PennyLane device running the exact statevector on one MI355X."""
+
+    config_filepath = path.join(path.dirname(__file__), "pennylane_capabilities.toml")
+
+    def __init__(self, wires, shots=None, *, seed=None, method: str = "automatic", device: str = "auto",
+                 execution_report: bool = False, allow_unsafe_statevector: Optional[bool] = None,
+                 dtype: str = "complex64", **_ignored) -> None:
+        if wires is None:
+            raise DeviceError("MettleQDevice requires an explicit wire set")
+        super().__init__(wires=wires, shots=shots)
+        if self.wires is None or len(self.wires) <= 0:
+            raise DeviceError("MettleQDevice requires at least one wire")
+        if str(method).lower() not in ("automatic", "statevector"):
+            raise DeviceError("this engine provides method='statevector' (or 'automatic') only")
+        if str(device).lower() not in ("auto", "gpu"):
+            raise DeviceError("device must be 'auto' or 'gpu' for the HIP engine")
+        self._rng = np.random.default_rng(seed)
+        self._execution_report = bool(execution_report)
+        self._allow_unsafe_statevector = allow_unsafe_statevector
+        self._dtype = dtype
+        self.last_execution_plan = None
+        self.statevector_preflight = None
+        self.last_execution_selection = None
+
+    @property
+    def name(self) -> str:
+        return "mettleq"
+
+    def setup_execution_config(self, config: Optional[ExecutionConfig] = None, circuit=None):
+        config = ExecutionConfig() if config is None else config
+        return replace(config, convert_to_numpy=True, use_device_gradient=False,
+                       use_device_jacobian_product=False, grad_on_execution=False)
+
+    def _supports_operation(self, op) -> bool:
+        return op.name in _PENNYLANE_TO_CORE or op.name in {"Identity", "GlobalPhase"}
+
+    def preprocess_transforms(self, execution_config: Optional[ExecutionConfig] = None):
+        program = _Program()
+        program.add_transform(convert_to_numpy_parameters)
+        program.add_transform(decompose, stopping_condition=self._supports_operation,
+                              device_wires=self.wires, name=self.name)
+        program.add_transform(broadcast_expand)
+        program.add_transform(validate_device_wires, self.wires, name=self.name)
+        program.add_transform(validate_measurements,
+                              analytic_measurements=lambda m: isinstance(m, _ANALYTIC),
+                              sample_measurements=lambda m: isinstance(m, _SHOT), name=self.name)
+        program.add_transform(validate_observables,
+                              stopping_condition=lambda o: bool(o.has_matrix), name=self.name)
+        return program
+
+    def execute(self, circuits, execution_config=None):
+        cache: dict = {}
+        try:
+            return tuple(self._execute_circuit(c, cache) for c in circuits)
+        finally:
+            for dev in cache.values():
+                dev.close()
+
+    def _execute_circuit(self, circuit, cache):
+        operations, phase = self._canonical_operations(circuit)
+        shots = circuit.shots if circuit.shots else self.shots
+        default_shots = shots.total_shots or 1000
+        device = execute_operations(len(self.wires), operations, shots=default_shots,
+                                    report=self._execution_report,
+                                    allow_unsafe_statevector=self._allow_unsafe_statevector,
+                                    execution_cache=cache, dtype=self._dtype)
+        apply_global_phase(device, phase)
+        self.last_execution_plan = device.last_execution_plan
+        self.statevector_preflight = device.statevector_preflight
+        self.last_execution_selection = device.execution_selection
+        if shots.has_partitioned_shots:
+            return tuple(self._measure_all(device, circuit.measurements, int(s)) for s in shots)
+        return self._measure_all(device, circuit.measurements, shots.total_shots)
+
+    def _canonical_operations(self, circuit):
+        operations, phase = [], 0.0
+        for op in circuit.operations:
+            if op.name == "Identity":
+                continue
+            if op.name == "GlobalPhase":
+                phase -= self._parameter(op.parameters[0], op.name)
+                continue
+            if op.name not in _PENNYLANE_TO_CORE:
+                raise DeviceError(f"Operation {op.name!r} reached MettleQ without a supported decomposition")
+            params = [self._parameter(v, op.name) for v in op.parameters]
+            if op.name in ("IsingXX", "IsingYY", "IsingZZ"):
+                params[0] /= 2.0
+            wires = [self.wires.index(w) for w in op.wires]
+            try:
+                operations.append(core_operation(_PENNYLANE_TO_CORE[op.name], wires, params))
+            except ValueError as exc:
+                raise DeviceError(str(exc)) from exc
+        return operations, phase
+
+    @staticmethod
+    def _parameter(value, name: str) -> float:
+        arr = np.asarray(value)
+        if arr.ndim != 0:
+            raise DeviceError(f"Operation {name!r} still has broadcast parameters")
+        try:
+            return float(arr)
+        except (TypeError, ValueError) as exc:
+            raise DeviceError(f"Operation {name!r} has a non-numeric parameter") from exc
+
+    def _measure_all(self, device, measurements, shots):
+        device.synchronize()
+        vals = tuple(self._measure(device, m, shots) for m in measurements)
+        return vals[0] if len(vals) == 1 else vals
+
+    def _measure(self, device, m, shots):
+        if isinstance(m, StateMP):
+            if shots is not None:
+                raise DeviceError("State measurements require analytic execution")
+            return statevector_numpy(device)
+        if m.obs is None:
+            wires = ([self.wires.index(w) for w in m.wires] if len(m.wires)
+                     else list(range(len(self.wires))))
+            if isinstance(m, ProbabilityMP):
+                if shots is None:
+                    return marginal_probabilities(device, wires)
+                samples = sample_bits(device, shots, wires, self._rng)
+                idx = samples.dot(1 << np.arange(len(wires) - 1, -1, -1))
+                return np.bincount(idx, minlength=1 << len(wires)) / float(shots)
+            if isinstance(m, SampleMP):
+                if shots is None:
+                    raise DeviceError("Sample measurements require finite shots")
+                return sample_bits(device, shots, wires, self._rng)
+            if isinstance(m, CountsMP):
+                if shots is None:
+                    raise DeviceError("Counts measurements require finite shots")
+                counts = bit_counts(sample_bits(device, shots, wires, self._rng))
+                if m.all_outcomes:
+                    for v in range(1 << len(wires)):
+                        counts.setdefault(format(v, f"0{len(wires)}b"), 0)
+                return counts
+        if isinstance(m, (ExpectationMP, VarianceMP)):
+            if shots is None:
+                e, e2 = self._moments(device, m.obs)
+                if isinstance(m, ExpectationMP):
+                    return self._real(e, "expectation")
+                return self._real(e2 - e * e, "variance")
+            samples = self._sample_observable(device, m.obs, shots)
+            if isinstance(m, ExpectationMP):
+                return self._real(np.mean(samples), "expectation")
+            return self._real(np.var(samples), "variance")
+        if isinstance(m, (SampleMP, CountsMP)):
+            if shots is None:
+                raise DeviceError("Observable sampling requires finite shots")
+            samples = self._sample_observable(device, m.obs, shots)
+            if isinstance(m, SampleMP):
+                return samples
+            counts = dict(Counter(np.real_if_close(v).item() for v in samples))
+            if m.all_outcomes:
+                for v in np.unique(np.real_if_close(m.eigvals())):
+                    counts.setdefault(v.item(), 0)
+            return counts
+        raise DeviceError(f"Measurement {type(m).__name__!r} is not supported")
+
+    def _moments(self, device, obs):
+        sentence = self._sentence(obs)
+        if sentence is not None:
+            return self._sentence_value(device, sentence), self._sentence_value(device, sentence @ sentence)
+        wires = [self.wires.index(w) for w in obs.wires]
+        if len(wires) > 8:
+            raise DeviceError("Non-Pauli observables spanning more than 8 wires are not supported")
+        mat = np.asarray(qml.matrix(obs, wire_order=obs.wires))
+        return local_expectation(device, wires, mat), local_expectation(device, wires, mat @ mat)
+
+    def _sample_observable(self, device, obs, shots):
+        sentence = self._sentence(obs)
+        if sentence is not None and len(sentence) == 1:
+            word, coeff = next(iter(sentence.items()))
+            if not word:
+                return np.full(int(shots), np.real_if_close(coeff))
+            e = pauli_product_expectation(device, {self.wires.index(w): p for w, p in word.items()})
+            e = float(np.clip(e.real, -1.0, 1.0))
+            signs = self._rng.choice(np.array([-1.0, 1.0]), size=int(shots),
+                                     p=((1.0 - e) / 2.0, (1.0 + e) / 2.0))
+            return np.real_if_close(complex(coeff) * signs)
+        wires = [self.wires.index(w) for w in obs.wires]
+        mat = np.asarray(qml.matrix(obs, wire_order=obs.wires))
+        evals, evecs = np.linalg.eigh(mat)
+        probs = np.array([local_expectation(device, wires, np.outer(v, v.conj())).real for v in evecs.T])
+        probs = np.clip(probs, 0.0, None)
+        probs /= probs.sum()
+        return np.real_if_close(evals[self._rng.choice(len(evals), size=int(shots), p=probs)])
+
+    def _sentence_value(self, device, sentence) -> complex:
+        coeffs, words = [], []
+        for word, c in sentence.items():
+            coeffs.append(complex(c))
+            words.append({self.wires.index(w): p for w, p in word.items()})
+        vals = pauli_product_expectations(device, words)
+        return sum(c * v for c, v in zip(coeffs, vals))
+
+    @staticmethod
+    def _sentence(obs):
+        try:
+            return qml.pauli.pauli_sentence(obs)
+        except (TypeError, ValueError):
+            return None
+
+    @staticmethod
+    def _real(value, label: str) -> float:
+        value = complex(value)
+        tol = 5e-5 * max(1.0, abs(value.real))
+        if abs(value.imag) > tol:
+            raise DeviceError(f"Hermitian {label} developed an imaginary component {value.imag:.3e}")
+        return float(value.real)

@@ -1,0 +1,317 @@
+"""Qiskit ``BackendV2`` adapter over the HIP statevector engine.
+
+Mirrors the reference's MettleQBackend (integrations/qiskit.py:319-744):
+same constructor keywords and Options, dynamic-width Target, synchronous
+already-DONE jobs, final-measurement-only translation with wire reversal
+(core wire = n-1-qiskit qubit, qiskit.py:576-581), pair-phase angle halving
+for rxx/ryy/rzz (qiskit.py:572-575), hex-keyed counts packed by classical bit
+(qiskit.py:683-695) and the optional statevector / execution-report fields.
+Only the exact statevector method is provided; MPS and midpoint-MPO options
+are accepted for signature compatibility and rejected if selected.
+"""
+from __future__ import annotations
+
+import time
+import uuid
+from collections import Counter
+from collections.abc import Sequence
+from typing import Optional
+
+import numpy as np
+
+try:
+    from qiskit.circuit import QuantumCircuit
+    from qiskit.circuit.library.standard_gates import get_standard_gate_name_mapping
+    from qiskit.exceptions import QiskitError
+    from qiskit.primitives import BackendSamplerV2, BaseEstimatorV2, PrimitiveJob
+    from qiskit.primitives.containers import DataBin, EstimatorPub, PrimitiveResult, PubResult
+    from qiskit.providers import BackendV2, JobStatus, JobV1, Options
+    from qiskit.result import Result
+    from qiskit.transpiler import Target
+except ImportError as exc:  # pragma: no cover - exercised without the extra
+    raise ImportError("The MettleQ Qiskit backend requires the optional Qiskit dependency.") from exc
+
+from .. import __version__
+from ..execution import StatevectorMemoryError  # noqa: F401  (re-export for callers)
+from ._common import (apply_global_phase, core_operation, execute_operations,
+                      pauli_product_expectations, sample_bits, statevector_numpy)
+
+_QISKIT_TO_CORE = {
+    "h": "H", "x": "X", "y": "Y", "z": "Z", "s": "S", "sdg": "SDG", "t": "T", "tdg": "TDG",
+    "sx": "SX", "rx": "RX", "ry": "RY", "rz": "RZ", "p": "U1", "u1": "U1", "u2": "U2", "u": "U3",
+    "u3": "U3", "cx": "CNOT", "ch": "CH", "cz": "CZ", "cp": "CPHASE", "crx": "CRX", "cry": "CRY",
+    "crz": "CRZ", "swap": "SWAP", "iswap": "ISWAP", "ccx": "TOFFOLI", "cswap": "FREDKIN",
+    "rxx": "XXPHASE", "ryy": "YYPHASE", "rzz": "ZZPHASE",
+}
+_TARGET_GATES = tuple(n for n in _QISKIT_TO_CORE if n not in ("ccx", "cswap")) + ("id", "measure")
+_METHODS = {"automatic", "statevector"}
+
+
+def _normalize_method(method: str) -> str:
+    m = str(method).strip().lower().replace("-", "_")
+    if m in ("sv", "state_vector"):
+        m = "statevector"
+    if m not in _METHODS:
+        raise ValueError("this engine provides method='statevector' (or 'automatic') only")
+    return m
+
+
+def _normalize_device(device: str) -> str:
+    d = str(device).strip().lower()
+    if d not in ("auto", "gpu"):
+        raise ValueError("device must be 'auto' or 'gpu' for the HIP engine")
+    return d
+
+
+class _CompletedJob(JobV1):
+    """Synchronous job returned after device execution completes."""
+
+    def __init__(self, backend, job_id, result):
+        super().__init__(backend, job_id)
+        self._result = result
+
+    def submit(self):
+        return None
+
+    def result(self, timeout=None):
+        return self._result
+
+    def status(self):
+        return JobStatus.DONE
+
+
+class MettleQBackend(BackendV2):
+    """Run final-measurement circuits on the MI355X statevector engine."""
+
+    version = 2
+
+    def __init__(self, provider=None, target: Optional[Target] = None, *, method: str = "automatic",
+                 device: str = "auto", allow_unsafe_statevector: Optional[bool] = None,
+                 dtype: str = "complex64", metal_checkpoint_budget_bytes: Optional[int] = None,
+                 **fields) -> None:
+        mps_fields = {k for k in list(fields) if k.startswith("mps_") or k in (
+            "allow_approximation", "statevector_gpu_min_qubits", "automatic_mps_min_qubits")}
+        for k in mps_fields:
+            fields.pop(k)
+        super().__init__(provider=provider, name="mettleq",
+                         description="MettleQ statevector simulator on AMD Instinct MI355X (HIP)",
+                         backend_version=__version__, **fields)
+        self._target = target
+        self.set_options(method=_normalize_method(method), device=_normalize_device(device), dtype=dtype)
+        self._allow_unsafe_statevector = allow_unsafe_statevector
+        self.last_execution_plans = []
+        self.last_statevector_preflights = []
+        self.last_execution_selections = []
+
+    @classmethod
+    def _default_options(cls):
+        return Options(shots=1024, memory=False, seed_simulator=None, return_statevector=False,
+                       execution_report=False, method="automatic", device="auto", dtype="complex64")
+
+    @property
+    def target(self) -> Target:
+        if self._target is None:
+            target = Target(description="MettleQ dynamic-width simulator target", num_qubits=None)
+            standard = get_standard_gate_name_mapping()
+            for name in _TARGET_GATES:
+                target.add_instruction(standard[name], properties=None, name=name)
+            self._target = target
+        return self._target
+
+    @property
+    def max_circuits(self):
+        return None
+
+    def run(self, run_input, **run_options) -> _CompletedJob:
+        circuits = ([run_input] if isinstance(run_input, QuantumCircuit)
+                    else list(run_input) if isinstance(run_input, Sequence) else None)
+        if not circuits or not all(isinstance(c, QuantumCircuit) for c in circuits):
+            raise QiskitError("MettleQBackend.run expects a QuantumCircuit or a non-empty sequence "
+                              "of QuantumCircuit objects")
+        unknown = set(run_options) - set(self.options)
+        if unknown:
+            raise QiskitError(f"Unsupported MettleQ run option(s): {', '.join(sorted(unknown))}")
+        options = {name: getattr(self.options, name) for name in self.options}
+        options.update(run_options)
+        try:
+            options["method"] = _normalize_method(options["method"])
+            options["device"] = _normalize_device(options["device"])
+        except ValueError as exc:
+            raise QiskitError(str(exc)) from exc
+        shots = options["shots"]
+        if shots is None or int(shots) <= 0:
+            raise QiskitError("shots must be a positive integer")
+        shots = int(shots)
+        rng = np.random.default_rng(options["seed_simulator"])
+        job_id = str(uuid.uuid4())
+        self.last_execution_plans = []
+        self.last_statevector_preflights = []
+        self.last_execution_selections = []
+        cache: dict = {}
+        start = time.perf_counter()
+        try:
+            results = [self._run_circuit(c, shots=shots, rng=rng, options=options, execution_cache=cache)
+                       for c in circuits]
+        finally:
+            for dev in cache.values():
+                dev.close()
+        result = Result.from_dict({
+            "backend_name": self.name, "backend_version": self.backend_version, "qobj_id": None,
+            "job_id": job_id, "success": True, "results": results, "status": "COMPLETED",
+            "time_taken": time.perf_counter() - start,
+        })
+        return _CompletedJob(self, job_id, result)
+
+    @staticmethod
+    def _translate_circuit(circuit: QuantumCircuit):
+        """qiskit.py:530-594."""
+        if circuit.num_qubits <= 0:
+            raise QiskitError("MettleQ requires at least one circuit qubit")
+        operations, measured, saw_measure = [], {}, False
+        for instruction in circuit.data:
+            op = instruction.operation
+            name = op.name
+            if getattr(op, "condition", None) is not None:
+                raise QiskitError("Classically conditioned operations are not supported yet")
+            if name == "barrier":
+                continue
+            if name == "measure":
+                saw_measure = True
+                q = circuit.find_bit(instruction.qubits[0]).index
+                c = circuit.find_bit(instruction.clbits[0]).index
+                measured[c] = q
+                continue
+            if saw_measure:
+                raise QiskitError("MettleQ's Qiskit adapter currently supports final measurements only")
+            if name == "id":
+                continue
+            if name not in _QISKIT_TO_CORE:
+                raise QiskitError(f"Unsupported instruction {name!r}; transpile the circuit against "
+                                  "MettleQBackend before running it")
+            try:
+                params = [float(v) for v in op.params]
+            except (TypeError, ValueError) as exc:
+                raise QiskitError(f"Instruction {name!r} contains unbound parameters") from exc
+            if name in ("rxx", "ryy", "rzz"):
+                params[0] /= 2.0
+            qwires = [circuit.find_bit(q).index for q in instruction.qubits]
+            wires = [circuit.num_qubits - 1 - q for q in qwires]
+            try:
+                operations.append(core_operation(_QISKIT_TO_CORE[name], wires, params))
+            except ValueError as exc:
+                raise QiskitError(str(exc)) from exc
+        try:
+            phase = float(circuit.global_phase)
+        except (TypeError, ValueError) as exc:
+            raise QiskitError("Circuit global phase is unbound") from exc
+        return operations, measured, phase
+
+    def _execute_bound_circuit(self, circuit, *, shots, options, execution_cache=None):
+        operations, measured, phase = self._translate_circuit(circuit)
+        device = execute_operations(circuit.num_qubits, operations, shots=shots,
+                                    report=bool(options["execution_report"]),
+                                    allow_unsafe_statevector=self._allow_unsafe_statevector,
+                                    method=options["method"], execution_device=options["device"],
+                                    execution_cache=execution_cache, dtype=options.get("dtype", "complex64"))
+        apply_global_phase(device, phase)
+        return device, measured, phase
+
+    def _run_circuit(self, circuit, *, shots, rng, options, execution_cache=None) -> dict:
+        device, measured, phase = self._execute_bound_circuit(circuit, shots=shots, options=options,
+                                                              execution_cache=execution_cache)
+        self.last_execution_plans.append(device.last_execution_plan)
+        self.last_statevector_preflights.append(device.statevector_preflight)
+        self.last_execution_selections.append(device.execution_selection)
+        data, memory = {}, []
+        if measured:
+            clbits = sorted(measured)
+            wires = [circuit.num_qubits - 1 - measured[c] for c in clbits]
+            for row in sample_bits(device, shots, wires, rng):
+                value = 0
+                for bit, clbit in zip(row, clbits):
+                    value |= int(bit) << clbit
+                memory.append(hex(value))
+            data["counts"] = dict(Counter(memory))
+        else:
+            data["counts"] = {}
+        if options["memory"]:
+            data["memory"] = memory
+        if options["return_statevector"]:
+            data["statevector"] = statevector_numpy(device)
+        if options["execution_report"]:
+            data["mettleq_execution_plan"] = device.last_execution_plan
+            data["mettleq_statevector_preflight"] = device.statevector_preflight
+            data["mettleq_execution_selection"] = device.execution_selection
+        header = {
+            "name": circuit.name, "n_qubits": circuit.num_qubits,
+            "qreg_sizes": [[r.name, r.size] for r in circuit.qregs],
+            "creg_sizes": [[r.name, r.size] for r in circuit.cregs],
+            "qubit_labels": [[r.name, i] for r in circuit.qregs for i in range(r.size)],
+            "clbit_labels": [[r.name, i] for r in circuit.cregs for i in range(r.size)],
+            "memory_slots": circuit.num_clbits, "global_phase": phase,
+            "metadata": circuit.metadata or {},
+        }
+        return {"name": circuit.name, "seed_simulator": options["seed_simulator"], "shots": shots,
+                "data": data, "status": "DONE", "success": True, "header": header}
+
+
+class MettleQSamplerV2(BackendSamplerV2):
+    """Sampler primitive over MettleQBackend (qiskit.py:883-903)."""
+
+    def __init__(self, *, backend: Optional[MettleQBackend] = None, options=None, **backend_options):
+        super().__init__(backend=backend or MettleQBackend(**backend_options), options=options)
+
+
+class MettleQEstimatorV2(BaseEstimatorV2):
+    """Exact Pauli-sum expectation values from the device statevector
+    (qiskit.py:906-1090): each Pauli term is one on-device parity
+    reduction, evaluated 16 terms per state pass."""
+
+    def __init__(self, *, backend: Optional[MettleQBackend] = None, default_precision: float = 0.0,
+                 **backend_options):
+        self._backend = backend or MettleQBackend(**backend_options)
+        self._default_precision = float(default_precision)
+
+    def run(self, pubs, *, precision=None):
+        coerced = [EstimatorPub.coerce(pub, precision) for pub in pubs]
+        job = PrimitiveJob(self._run, coerced)
+        job._submit()
+        return job
+
+    def _run(self, pubs):
+        results = []
+        for pub in pubs:
+            circuit, observables, params = pub.circuit, pub.observables, pub.parameter_values
+            bound = params.bind_all(circuit)
+            shape = np.broadcast_shapes(observables.shape, bound.shape)
+            evs = np.zeros(shape, dtype=np.float64)
+            stds = np.zeros(shape, dtype=np.float64)
+            cache: dict = {}
+            try:
+                for index in np.ndindex(*shape):
+                    bc = bound[index if bound.shape else ()]
+                    obs = observables[index if observables.shape else ()]
+                    if getattr(bc, "num_clbits", 0):
+                        bc = bc.remove_final_measurements(inplace=False)
+                    device, _, _ = self._backend._execute_bound_circuit(
+                        bc, shots=1, options={"execution_report": False, "method": "statevector",
+                                              "device": "gpu", "dtype": "complex64"},
+                        execution_cache=cache)
+                    n = bc.num_qubits
+                    words, coeffs = [], []
+                    for label, coeff in obs.items():
+                        word = {n - 1 - q: ch for q, ch in enumerate(reversed(label)) if ch != "I"}
+                        words.append(word)
+                        coeffs.append(complex(coeff))
+                    vals = pauli_product_expectations(device, words)
+                    evs[index] = float(np.real(sum(c * v for c, v in zip(coeffs, vals))))
+            finally:
+                for dev in cache.values():
+                    dev.close()
+            results.append(PubResult(DataBin(evs=evs, stds=stds, shape=shape),
+                                     metadata={"target_precision": pub.precision}))
+        return PrimitiveResult(results, metadata={"version": 2})
+
+
+# legacy alias names (qiskit.py:1095-1097)
+MlxQBackend = MettleQBackend
