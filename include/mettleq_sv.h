@@ -180,6 +180,7 @@ int msv_norm2(msv_state *s, double *out);
 #define MSV_TUNE_SWEEP_SPLIT 4       /* 1q-layer split policy (0 = auto)        */
 #define MSV_TUNE_TILE_BLOCKS_PER_CU 5 /* persistent tile-sweep blocks per CU     */
 #define MSV_TUNE_ELEM_BLOCKS_PER_CU 6 /* elementwise/gather blocks per CU        */
+#define MSV_TUNE_COL_BITS 7           /* contiguous index bits per tile row (c64) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

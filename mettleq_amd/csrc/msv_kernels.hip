@@ -346,6 +346,33 @@ __device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) {
 }
 
 template <typename T>
+__device__ __forceinline__ void load_pack(const typename Vec2<T>::type *st, uint64_t pk,
+                                          typename Vec2<T>::type *amp) {
+  if constexpr (Pack<T>::kAmps == 2) {
+    const float4 p = reinterpret_cast<const float4 *>(st)[pk];
+    amp[0] = make_float2(p.x, p.y);
+    amp[1] = make_float2(p.z, p.w);
+  } else {
+    amp[0] = st[pk];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_pack(typename Vec2<T>::type *st, uint64_t pk,
+                                           const typename Vec2<T>::type *amp) {
+  if constexpr (Pack<T>::kAmps == 2) {
+    reinterpret_cast<float4 *>(st)[pk] = make_float4(amp[0].x, amp[0].y, amp[1].x, amp[1].y);
+  } else {
+    st[pk] = amp[0];
+  }
+}
+
+// Each thread keeps kUnroll packs in flight: iteration t covers packs
+// base + u * nthreads (u < kUnroll), so every load instruction of a wave
+// still reads consecutive addresses.
+constexpr int kUnroll = 4;
+
+template <typename T>
 __global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restrict__ st,
                                                const typename Vec2<T>::type *__restrict__ lut_g,
                                                const PhaseArgs A) {
@@ -356,35 +383,34 @@ __global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restric
   __syncthreads();
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = A.n_amps / KA;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t pk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; pk < npack;
-       pk += stride) {
-    const uint64_t i0 = pk * KA;
-    V ph;
-    ph.x = T(1);
-    ph.y = T(0);
-    for (uint32_t w = A.nlow; w < A.nwin; ++w) ph = cmul(ph, lut[win_index(A.win[w], i0)]);
-    V amp[KA];
-    if constexpr (KA == 2) {
-      const float4 p = reinterpret_cast<const float4 *>(st)[pk];
-      amp[0] = make_float2(p.x, p.y);
-      amp[1] = make_float2(p.z, p.w);
-    } else {
-      amp[0] = st[i0];
+  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
+       base += nthreads * kUnroll) {
+    V amp[kUnroll][KA];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t pk = base + u * nthreads;
+      if (pk < npack) load_pack<T>(st, pk, amp[u]);
     }
 #pragma unroll
-    for (int d = 0; d < KA; ++d) {
-      V p = ph;
-      for (uint32_t w = 0; w < A.nlow; ++w) p = cmul(p, lut[win_index(A.win[w], i0 + d)]);
-      amp[d] = cmul(amp[d], p);
-    }
-    if constexpr (KA == 2) {
-      reinterpret_cast<float4 *>(st)[pk] = make_float4(amp[0].x, amp[0].y, amp[1].x, amp[1].y);
-    } else {
-      st[i0] = amp[0];
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t pk = base + u * nthreads;
+      if (pk < npack) {
+        const uint64_t i0 = pk * KA;
+        V ph;
+        ph.x = T(1);
+        ph.y = T(0);
+        for (uint32_t w = A.nlow; w < A.nwin; ++w) ph = cmul(ph, lut[win_index(A.win[w], i0)]);
+#pragma unroll
+        for (int d = 0; d < KA; ++d) {
+          V p = ph;
+          for (uint32_t w = 0; w < A.nlow; ++w) p = cmul(p, lut[win_index(A.win[w], i0 + d)]);
+          amp[u][d] = cmul(amp[u][d], p);
+        }
+        store_pack<T>(st, pk, amp[u]);
+      }
     }
   }
-  // a 1-amplitude register (n = 0 never occurs; n = 1 with KA = 2 is even)
 }
 
 // ---------------------------------------------------------------------------
@@ -403,20 +429,26 @@ __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type
   __syncthreads();
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = n_amps / KA;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t pk = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; pk < npack;
-       pk += stride) {
-    const uint64_t y0 = pk * KA;
-    uint64_t s0 = c;
-    for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
-    if constexpr (KA == 2) {
-      // y0 is even: src(y0 + 1) = src(y0) ^ column of index bit 0
-      const uint64_t s1 = s0 ^ tab[1];
-      const V a0 = src[s0];
-      const V a1 = src[s1];
-      reinterpret_cast<float4 *>(dst)[pk] = make_float4(a0.x, a0.y, a1.x, a1.y);
-    } else {
-      dst[y0] = src[s0];
+  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
+       base += nthreads * kUnroll) {
+    V amp[kUnroll][KA];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t pk = base + u * nthreads;
+      if (pk < npack) {
+        const uint64_t y0 = pk * KA;
+        uint64_t s0 = c;
+        for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
+        amp[u][0] = src[s0];
+        // y0 is even when KA = 2: src(y0 + 1) = src(y0) ^ column of index bit 0
+        if constexpr (KA == 2) amp[u][1] = src[s0 ^ tab[1]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t pk = base + u * nthreads;
+      if (pk < npack) store_pack<T>(dst, pk, amp[u]);
     }
   }
 }
@@ -446,10 +478,24 @@ __global__ void __launch_bounds__(256) k_scale(typename Vec2<T>::type *__restric
   V f;
   f.x = re;
   f.y = im;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_amps;
-       i += stride)
-    st[i] = cmul(st[i], f);
+  constexpr int KA = Pack<T>::kAmps;
+  const uint64_t npack = n_amps / KA;
+  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
+       base += nthreads * kUnroll) {
+    V amp[kUnroll][KA];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (base + u * nthreads < npack) load_pack<T>(st, base + u * nthreads, amp[u]);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if (base + u * nthreads < npack) {
+#pragma unroll
+        for (int d = 0; d < KA; ++d) amp[u][d] = cmul(amp[u][d], f);
+        store_pack<T>(st, base + u * nthreads, amp[u]);
+      }
+    }
+  }
 }
 
 template <typename Ts, typename Td>
@@ -468,10 +514,17 @@ __global__ void __launch_bounds__(256) k_convert(const typename Vec2<Ts>::type *
 
 __global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ src,
                                                     float4 *__restrict__ dst, uint64_t n_vec) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_vec;
-       i += stride)
-    dst[i] = src[i];
+  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < n_vec;
+       base += nthreads * kUnroll) {
+    float4 v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (base + u * nthreads < n_vec) v[u] = src[base + u * nthreads];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (base + u * nthreads < n_vec) dst[base + u * nthreads] = v[u];
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -53,7 +53,8 @@ struct Tuning {
   int phase_blocks = 0;     // 0 = CUs * 8
   int sweep_split = 0;
   int tile_blocks_per_cu = 8;
-  int elem_blocks_per_cu = 16;
+  int elem_blocks_per_cu = 32;
+  int col_bits_c64 = 4;     // contiguous low index bits per tile row in high sweeps
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -220,7 +221,7 @@ template <typename T> int rb_of() { return TileCfg<T>::kRB; }
 template <typename T> int local_max() {
   return std::is_same<T, float>::value ? g_tune.local_bits_c64 : g_tune.local_bits_c128;
 }
-template <typename T> int col_bits() { return std::is_same<T, float>::value ? 4 : 3; }
+template <typename T> int col_bits() { return std::is_same<T, float>::value ? g_tune.col_bits_c64 : 3; }
 
 // Partition of index bits [0, n) into a low sweep [0, L) and high chunks of
 // between rb and gate_bits bits each (ascending).
@@ -1597,6 +1598,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "elementwise blocks per CU out of range");
       g_tune.elem_blocks_per_cu = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_COL_BITS:
+      if (value < 0 || value > 8) return fail(MSV_E_INVALID, "column bits out of range");
+      g_tune.col_bits_c64 = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -1611,6 +1616,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_SWEEP_SPLIT: *value = g_tune.sweep_split; return MSV_OK;
     case MSV_TUNE_TILE_BLOCKS_PER_CU: *value = g_tune.tile_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune.elem_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_COL_BITS: *value = g_tune.col_bits_c64; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
