@@ -16,10 +16,19 @@ template <> struct Vec2<double> { using type = double2; };
 // Register bits per thread in a tile sweep: 32 complex64 or 16 complex128
 // amplitudes live in registers (64 VGPRs either way).
 template <typename T> struct TileCfg;
-template <> struct TileCfg<float> { static constexpr int kRB = 5; static constexpr int kMaxLocal = 13; };
-template <> struct TileCfg<double> { static constexpr int kRB = 4; static constexpr int kMaxLocal = 12; };
+// kMaxOps bounds a tile program so that TileArgs stays under 4 KiB of
+// kernel arguments.
+template <> struct TileCfg<float> {
+  static constexpr int kRB = 5;
+  static constexpr int kMaxLocal = 13;
+  static constexpr int kMaxOps = 24;
+};
+template <> struct TileCfg<double> {
+  static constexpr int kRB = 4;
+  static constexpr int kMaxLocal = 12;
+  static constexpr int kMaxOps = 16;
+};
 
-constexpr int kMaxTileOps = 24;
 constexpr int kMaxSegs = 6;
 constexpr int kMaxPhaseWins = 32;
 constexpr int kPhaseLutMax = 4096;   // entries of all windows together
@@ -36,18 +45,20 @@ enum TileOpKind : uint8_t {
 
 // One operation of a tile program. Matrix entries are stored in the
 // amplitude precision; the host rounds its double-precision values once.
+// Every field is a whole dword: the kernel indexes ops with a run-time
+// (wave-uniform) op number, and dword fields load with scalar loads, while
+// byte fields at a run-time offset would need vector loads.
 template <typename T>
 struct TileOp {
-  uint8_t kind;
-  int8_t rpos;      // target register bit (U2, QFT)
-  int8_t ra;        // DIAG: register bit of index bit a, or -1
-  int8_t rb;        // DIAG: register bit of index bit b, or -1
-  uint8_t jl;       // QFT: ladder register bits directly below rpos
-  uint8_t nbits;    // DIAG: 1 or 2
-  uint8_t has_ctrl; // U2: controls present
-  uint8_t pbit;     // QFT: index bit p of the stage
-  uint8_t lb;       // QFT: lowest index bit of the ladder
-  uint8_t pad[3];
+  uint32_t kind;    // TileOpKind
+  int32_t rpos;     // target register bit (U2, QFT)
+  int32_t ra;       // DIAG: register bit of index bit a, or -1
+  int32_t rb;       // DIAG: register bit of index bit b, or -1
+  uint32_t jl;      // QFT: ladder register bits directly below rpos
+  uint32_t nbits;   // DIAG: 1 or 2
+  uint32_t has_ctrl;// U2: controls present
+  uint32_t pbit;    // QFT: index bit p of the stage
+  uint32_t lb;      // QFT: lowest index bit of the ladder
   uint32_t cmask_r; // U2: control bits that are register bits (register positions)
   uint64_t cmask_t; // U2: control bits that are not register bits (index mask)
   uint64_t amask;   // DIAG: index mask of bit a (when ra < 0)
@@ -55,26 +66,44 @@ struct TileOp {
   T m[8];
 };
 
+constexpr int kTileThreadBits = 8;  // 256 threads per tile
+
+// One segment: registers hold the local bits [reg0, reg0 + RB). The host
+// precomputes where each thread-id bit and each register bit lands, in the
+// tile (local bit) and in the amplitude index, so the kernel never looks
+// up a table with a run-time index.
 struct TileSeg {
-  uint8_t reg[5];   // local bit held by register bit j
-  uint8_t op_begin; // ops [op_begin, op_end) run in this segment
-  uint8_t op_end;
-  uint8_t pad;
+  uint32_t reg0;
+  uint32_t op_begin;  // ops [op_begin, op_end) run in this segment
+  uint32_t op_end;
+  uint32_t tmask;     // thread-id bits in use (tiles smaller than 256 x 2^RB
+                      // amplitudes run duplicate threads with equal values)
+  uint32_t tl_bit[kTileThreadBits];  // local bit of thread-id bit k
+  uint32_t gi_bit[kTileThreadBits];  // index bit of thread-id bit k
+  uint64_t gstr[5];                  // index stride of register bit j
 };
+
+// The tile number's bits are scattered onto the non-local index bits in
+// at most kMaxRuns runs: base |= ((tile >> src) & (2^width - 1)) << dst.
+constexpr int kMaxRuns = 16;
 
 template <typename T>
 struct TileArgs {
   uint64_t n_tiles;        // tiles = 2^(n - nlocal)
-  uint64_t nonlocal_mask;  // index bits enumerated by the tile number
   uint32_t nlocal;         // local bits per tile
   uint32_t nseg;
-  uint8_t lpos[16];        // index bit of local bit k (increasing)
+  uint32_t nruns;
+  uint32_t run_src[kMaxRuns];
+  uint32_t run_width[kMaxRuns];
+  uint32_t run_dst[kMaxRuns];
   TileSeg seg[kMaxSegs];
   T scale;                 // applied at the final store (QFT 1/sqrt2 factors)
   uint32_t pad;
   T w32[64];               // exp(2*pi*i*m/32), m = 0..31, (re, im)
-  TileOp<T> ops[kMaxTileOps];
+  TileOp<T> ops[TileCfg<T>::kMaxOps];
 };
+static_assert(sizeof(TileArgs<float>) <= 4096, "tile kernel arguments exceed 4 KiB");
+static_assert(sizeof(TileArgs<double>) <= 4096, "tile kernel arguments exceed 4 KiB");
 
 struct PhaseWin {
   uint8_t s1, w1, s2, w2;  // lut index = ((i>>s1)&(2^w1-1)) | (((i>>s2)&(2^w2-1))<<w1)

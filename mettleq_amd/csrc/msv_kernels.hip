@@ -45,15 +45,6 @@ __device__ __forceinline__ V csub(V a, V b) {
   return r;
 }
 
-__device__ __forceinline__ uint32_t pdep32(uint32_t v, uint32_t mask) {
-  uint32_t r = 0;
-  for (uint32_t m = mask; m; m &= m - 1u) {
-    if (v & 1u) r |= m & (0u - m);
-    v >>= 1;
-  }
-  return r;
-}
-
 template <typename T> struct CisPi;
 template <> struct CisPi<float> {
   // exp(i*pi*x/2^w) for an integer x < 2^w; x/2^w is formed exactly up to the
@@ -215,7 +206,7 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
   using V = typename Vec2<T>::type;
   for (uint32_t o = ob; o < oe; ++o) {
     const TileOp<T> &op = A.ops[o];
-    const uint8_t kind = op.kind;
+    const uint32_t kind = op.kind;
     if (kind == OP_U2) {
       T m[8];
 #pragma unroll
@@ -245,18 +236,12 @@ __device__ __forceinline__ uint64_t reg_goff(int r, const uint64_t *gstr) {
     if ((r >> j) & 1) o += gstr[j];
   return o;
 }
-template <int RB>
-__device__ __forceinline__ uint32_t reg_loff(int r, const uint32_t *lstr) {
-  uint32_t o = 0;
-#pragma unroll
-  for (int j = 0; j < RB; ++j)
-    if ((r >> j) & 1) o |= lstr[j];
-  return o;
-}
 
 // XOR swizzle of LDS slots: when lanes step by 2^RB slots (registers on the
 // lowest local bits) they would all hit one bank pair; folding bits above
-// RB into the low bits spreads them while staying a bijection.
+// RB into the low bits spreads them while staying a bijection. The map is
+// linear over GF(2), so swz(tl | roff) = swz(tl) ^ swz(roff) for disjoint
+// thread and register bits: the register part is wave-uniform.
 template <int RB>
 __device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
   constexpr uint32_t kMask = (RB >= 5) ? 15u : ((RB == 4) ? 7u : 0u);
@@ -271,40 +256,36 @@ __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict
   extern __shared__ unsigned char smem_raw[];
   V *lds = reinterpret_cast<V *>(smem_raw);
   const uint32_t tid = threadIdx.x;
-  const uint32_t nlocal = A.nlocal;
-  const uint32_t lmask = (nlocal >= 32) ? 0xffffffffu : ((1u << nlocal) - 1u);
   for (uint64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
     uint64_t base = 0;
-    {
-      uint64_t m = A.nonlocal_mask, t = tile;
-      while (m) {
-        if (t & 1ull) base |= m & (0ull - m);
-        t >>= 1;
-        m &= m - 1ull;
-      }
-    }
+#pragma unroll
+    for (int k = 0; k < kMaxRuns; ++k)
+      if (k < static_cast<int>(A.nruns))
+        base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
     V a[R];
     for (uint32_t s = 0; s < A.nseg; ++s) {
-      const TileSeg sg = A.seg[s];
-      uint32_t regmask = 0;
+      const TileSeg &sg = A.seg[s];
       uint64_t gstr[RB];
-      uint32_t lstr[RB];
 #pragma unroll
-      for (int j = 0; j < RB; ++j) {
-        regmask |= 1u << sg.reg[j];
-        gstr[j] = 1ull << A.lpos[sg.reg[j]];
-        lstr[j] = 1u << sg.reg[j];
-      }
-      const uint32_t tl = pdep32(tid, lmask & ~regmask);
+      for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+      uint32_t tl = 0;
       uint64_t gb = base;
-      for (uint32_t k = 0; k < nlocal; ++k)
-        if ((tl >> k) & 1u) gb |= 1ull << A.lpos[k];
+      const uint32_t tm = tid & sg.tmask;
+#pragma unroll
+      for (int k = 0; k < kTileThreadBits; ++k) {
+        const uint32_t b = (tm >> k) & 1u;
+        tl |= b << sg.tl_bit[k];
+        gb |= static_cast<uint64_t>(b) << sg.gi_bit[k];
+      }
+      const uint32_t reg0 = sg.reg0;
+      const uint32_t tsw = lds_swz<RB>(tl);
       if (s == 0) {
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = st[gb + reg_goff<RB>(r, gstr)];
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = lds[lds_swz<RB>(tl | reg_loff<RB>(r, lstr))];
+        for (int r = 0; r < R; ++r)
+          a[r] = lds[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
       }
       run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, gb);
       if (s + 1 == A.nseg) {
@@ -324,7 +305,7 @@ __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict
       } else {
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lds_swz<RB>(tl | reg_loff<RB>(r, lstr))] = a[r];
+        for (int r = 0; r < R; ++r) lds[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)] = a[r];
         __syncthreads();
       }
     }

@@ -306,28 +306,56 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
   for (int i = 0; i < 64; ++i) loc_of[i] = -1;
   uint64_t lmask = 0;
   for (int k = 0; k < nlocal; ++k) {
-    A.lpos[k] = static_cast<uint8_t>(plan.local[k]);
     loc_of[plan.local[k]] = k;
     lmask |= 1ull << plan.local[k];
   }
-  const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
-  A.nonlocal_mask = all & ~lmask;
+  // runs of non-local index bits, numbered by the tile index from bit 0 up
+  {
+    int src = 0, nr = 0;
+    for (int b = 0; b < n;) {
+      if (lmask >> b & 1ull) {
+        ++b;
+        continue;
+      }
+      int e = b;
+      while (e < n && !(lmask >> e & 1ull)) ++e;
+      if (nr >= kMaxRuns) return fail(MSV_E_INVALID, "internal: too many non-local bit runs");
+      A.run_src[nr] = static_cast<uint32_t>(src);
+      A.run_width[nr] = static_cast<uint32_t>(e - b);
+      A.run_dst[nr] = static_cast<uint32_t>(b);
+      src += e - b;
+      ++nr;
+      b = e;
+    }
+    A.nruns = static_cast<uint32_t>(nr);
+  }
   A.n_tiles = 1ull << (n - nlocal);
   A.scale = static_cast<T>(plan.scale);
   for (int m = 0; m < 32; ++m) {
     A.w32[2 * m] = static_cast<T>(std::cos(2.0 * M_PI * m / 32.0));
     A.w32[2 * m + 1] = static_cast<T>(std::sin(2.0 * M_PI * m / 32.0));
   }
+  if (nlocal - rb > kTileThreadBits)
+    return fail(MSV_E_INVALID, "internal: tile larger than one block");
   int nops = 0;
   for (size_t si = 0; si < plan.segs.size(); ++si) {
     const SegPlan &sp = plan.segs[si];
     TileSeg &sg = A.seg[si];
     if (sp.reg0 < 0 || sp.reg0 + rb > nlocal)
       return fail(MSV_E_INVALID, "internal: register window outside the tile");
-    for (int j = 0; j < 5; ++j) sg.reg[j] = static_cast<uint8_t>(j < rb ? sp.reg0 + j : 0);
-    sg.op_begin = static_cast<uint8_t>(nops);
+    sg.reg0 = static_cast<uint32_t>(sp.reg0);
+    for (int j = 0; j < rb; ++j) sg.gstr[j] = 1ull << plan.local[sp.reg0 + j];
+    // thread-id bits take the non-register local bits in increasing order
+    sg.tmask = (1u << (nlocal - rb)) - 1u;
+    for (int k = 0, t = 0; k < nlocal; ++k) {
+      if (k >= sp.reg0 && k < sp.reg0 + rb) continue;
+      sg.tl_bit[t] = static_cast<uint32_t>(k);
+      sg.gi_bit[t] = static_cast<uint32_t>(plan.local[k]);
+      ++t;
+    }
+    sg.op_begin = static_cast<uint32_t>(nops);
     for (const PlanOp &po : sp.ops) {
-      if (nops >= kMaxTileOps) return fail(MSV_E_INVALID, "internal: too many tile ops");
+      if (nops >= TileCfg<T>::kMaxOps) return fail(MSV_E_INVALID, "internal: too many tile ops");
       TileOp<T> &op = A.ops[nops++];
       op.kind = po.kind;
       op.rpos = -1;
@@ -354,7 +382,7 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
           if (j >= 0) op.cmask_r |= 1u << j;
           else op.cmask_t |= 1ull << bit;
         }
-        op.has_ctrl = po.cmask != 0;
+        op.has_ctrl = po.cmask != 0 ? 1u : 0u;
       } else if (po.kind == OP_QFT_F || po.kind == OP_QFT_I) {
         const int j = op.rpos;
         const int p = po.tbit;
@@ -368,11 +396,11 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
           if (plan.local[sp.reg0 + j - d] >= po.lb)
             return fail(MSV_E_INVALID, "internal: QFT register window not contiguous");
         }
-        op.jl = static_cast<uint8_t>(jl);
-        op.pbit = static_cast<uint8_t>(p);
-        op.lb = static_cast<uint8_t>(po.lb);
+        op.jl = static_cast<uint32_t>(jl);
+        op.pbit = static_cast<uint32_t>(p);
+        op.lb = static_cast<uint32_t>(po.lb);
       } else if (po.kind == OP_DIAG) {
-        op.nbits = po.bbit >= 0 ? 2 : 1;
+        op.nbits = po.bbit >= 0 ? 2u : 1u;
         const int ja = reg_pos(po.abit);
         op.ra = static_cast<int8_t>(ja);
         op.amask = ja >= 0 ? 0ull : (1ull << po.abit);
@@ -383,7 +411,7 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
         }
       }
     }
-    sg.op_end = static_cast<uint8_t>(nops);
+    sg.op_end = static_cast<uint32_t>(nops);
   }
   const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
@@ -392,33 +420,44 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
   return MSV_OK;
 }
 
-// Commuting per-bit ops (one op per gate local bit, any order): windows from
-// the top, then the bottom window moved inside so that the first and last
-// segments keep coalesced lanes.
+// Shortest sequence of register windows (window = local bits [w, w + rb))
+// that covers every bit of `need` and whose first and last windows start at
+// or above `gf`, so that the global loads and stores stay coalesced. For a
+// 13-bit tile with 5 register bits this is 3 windows, e.g. [8,13) [0,5)
+// [4,9): two LDS exchanges per tile.
+std::vector<int> commuting_windows(uint32_t need, int nlocal, int rb, int gf) {
+  const int nw = nlocal - rb + 1;
+  const uint32_t wmask = (1u << rb) - 1u;
+  for (int m = 1; m <= kMaxSegs; ++m) {
+    std::vector<int> w(m, gf);
+    for (int i = 1; i + 1 < m; ++i) w[i] = 0;
+    while (true) {
+      uint32_t cov = 0;
+      for (int x : w) cov |= wmask << x;
+      if ((cov & need) == need) return w;
+      // next sequence: first and last in [gf, nw), middle in [0, nw)
+      int i = 0;
+      for (; i < m; ++i) {
+        const int lo = (i == 0 || i == m - 1) ? gf : 0;
+        if (++w[i] < nw) break;
+        w[i] = lo;
+      }
+      if (i == m) break;
+    }
+  }
+  return {};
+}
+
+// Commuting per-bit ops (one op per gate local bit, any order).
 template <typename T>
 int run_commuting_sweep(msv_state *s, const std::vector<int> &local,
                         const std::vector<std::pair<int, PlanOp>> &ops_by_local) {
   const int nlocal = static_cast<int>(local.size());
   const int rb = std::min(rb_of<T>(), nlocal);
-  std::vector<int> gl;
-  for (auto &p : ops_by_local) gl.push_back(p.first);
-  std::vector<int> wins = cover_top_down(gl, nlocal, rb, 0);
-  const int gf = good_from(nlocal, rb);
-  std::vector<int> good, bad;
-  for (int w : wins) (w >= gf ? good : bad).push_back(w);
-  std::vector<int> order;
-  std::vector<bool> has_ops;
-  if (good.empty()) {
-    const int top = nlocal - rb;
-    order.push_back(top);
-    for (int w : bad) order.push_back(w);
-    order.push_back(top);
-  } else {
-    order.push_back(good[0]);
-    for (int w : bad) order.push_back(w);
-    for (size_t i = 1; i < good.size(); ++i) order.push_back(good[i]);
-    if (!bad.empty() && good.size() == 1) order.push_back(good[0]);
-  }
+  uint32_t need = 0;
+  for (auto &p : ops_by_local) need |= 1u << p.first;
+  const std::vector<int> order = commuting_windows(need, nlocal, rb, good_from(nlocal, rb));
+  if (order.empty()) return fail(MSV_E_INVALID, "internal: no register window cover");
   SweepPlan plan;
   plan.local = local;
   std::vector<bool> used(ops_by_local.size(), false);
@@ -694,7 +733,7 @@ int apply_plan_ops(msv_state *s, const std::vector<PlanOp> &all) {
     trial.push_back(po);
     std::vector<int> t2 = targets;
     if (po.tbit >= 0 && std::find(t2.begin(), t2.end(), po.tbit) == t2.end()) t2.push_back(po.tbit);
-    const bool fits = static_cast<int>(trial.size()) <= kMaxTileOps &&
+    const bool fits = static_cast<int>(trial.size()) <= TileCfg<T>::kMaxOps &&
                       static_cast<int>(t2.size()) <= max_program_targets() &&
                       count_segments<T>(s, trial) <= kMaxSegs;
     if (!fits) {
