@@ -177,10 +177,13 @@ int msv_norm2(msv_state *s, double *out);
 #define MSV_TUNE_TILE_LOCAL_BITS 1   /* local bits per tile sweep (c64)        */
 #define MSV_TUNE_TILE_GATE_BITS 2    /* max non-column bits per high sweep     */
 #define MSV_TUNE_PHASE_BLOCKS 3      /* persistent grid size of phase kernels   */
-#define MSV_TUNE_SWEEP_SPLIT 4       /* 1q-layer split policy (0 = auto)        */
+#define MSV_TUNE_SWEEP_SPLIT 4       /* 1q-layer high sweeps: 0 contiguous bits, 1 page-balanced */
 #define MSV_TUNE_TILE_BLOCKS_PER_CU 5 /* persistent tile-sweep blocks per CU     */
 #define MSV_TUNE_ELEM_BLOCKS_PER_CU 6 /* elementwise/gather blocks per CU        */
 #define MSV_TUNE_COL_BITS 7           /* contiguous index bits per tile row (c64) */
+#define MSV_TUNE_ROT_SPLIT_MIN 8      /* gates per sweep from which 1q gates run as
+                                         diagonal x real rotation x diagonal (c64;
+                                         0 = never) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

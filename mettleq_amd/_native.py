@@ -42,6 +42,7 @@ MSV_TUNE_SWEEP_SPLIT = 4
 MSV_TUNE_TILE_BLOCKS_PER_CU = 5
 MSV_TUNE_ELEM_BLOCKS_PER_CU = 6
 MSV_TUNE_COL_BITS = 7
+MSV_TUNE_ROT_SPLIT_MIN = 8
 
 
 class NativeUnavailableError(RuntimeError):

@@ -18,16 +18,21 @@ template <> struct Vec2<double> { using type = double2; };
 template <typename T> struct TileCfg;
 // kMaxOps bounds a tile program so that TileArgs stays under 4 KiB of
 // kernel arguments.
+// kTabVals sizes the per-launch table pool: the QFT twiddles (32 complex)
+// or, for complex64, up to two segment diagonals (32 + 8 complex each).
 template <> struct TileCfg<float> {
   static constexpr int kRB = 5;
   static constexpr int kMaxLocal = 13;
   static constexpr int kMaxOps = 24;
+  static constexpr int kTabVals = 160;
 };
 template <> struct TileCfg<double> {
   static constexpr int kRB = 4;
   static constexpr int kMaxLocal = 12;
   static constexpr int kMaxOps = 16;
+  static constexpr int kTabVals = 64;
 };
+constexpr int kDtabVals = 80;  // one segment diagonal: 32 register + 8 thread-bit complex
 
 constexpr int kMaxSegs = 6;
 constexpr int kMaxPhaseWins = 32;
@@ -41,6 +46,8 @@ enum TileOpKind : uint8_t {
   OP_QFT_F = 2,  // forward QFT stage: butterfly then ladder phase on the 1 branch
   OP_QFT_I = 3,  // inverse QFT stage: conjugate ladder phase then butterfly
   OP_DIAG = 4,   // diagonal over 1 or 2 index bits (anywhere in the index)
+  OP_DTAB = 5,   // diagonal over the tile's local bits: register table x thread factors
+  OP_RU2 = 6,    // real 2x2 on a register bit (m[0..3] = r00 r01 r10 r11)
 };
 
 // One operation of a tile program. Matrix entries are stored in the
@@ -58,7 +65,7 @@ struct TileOp {
   uint32_t nbits;   // DIAG: 1 or 2
   uint32_t has_ctrl;// U2: controls present
   uint32_t pbit;    // QFT: index bit p of the stage
-  uint32_t lb;      // QFT: lowest index bit of the ladder
+  uint32_t lb;      // QFT: lowest index bit of the ladder; DTAB: table offset
   uint32_t cmask_r; // U2: control bits that are register bits (register positions)
   uint64_t cmask_t; // U2: control bits that are not register bits (index mask)
   uint64_t amask;   // DIAG: index mask of bit a (when ra < 0)
@@ -99,7 +106,8 @@ struct TileArgs {
   TileSeg seg[kMaxSegs];
   T scale;                 // applied at the final store (QFT 1/sqrt2 factors)
   uint32_t pad;
-  T w32[64];               // exp(2*pi*i*m/32), m = 0..31, (re, im)
+  T tab[TileCfg<T>::kTabVals];  // QFT: exp(2*pi*i*m/32), m = 0..31, (re, im) at 0;
+                                // DTAB: kDtabVals values per table at op.lb
   TileOp<T> ops[TileCfg<T>::kMaxOps];
 };
 static_assert(sizeof(TileArgs<float>) <= 4096, "tile kernel arguments exceed 4 KiB");

@@ -109,6 +109,64 @@ __device__ __forceinline__ void u2_dispatch(typename Vec2<T>::type *a, const T *
   }
 }
 
+// Real 2x2 [[r0, r1], [r2, r3]] on register bit J: half the FMAs of a
+// complex 2x2 (used for the rotation factor of a ZYZ-split 1q gate).
+template <typename T, int RB, int J>
+__device__ __forceinline__ void ru2_pairs(typename Vec2<T>::type *a, const T *m) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    const V v0 = a[r], v1 = a[r1];
+    a[r].x = m[0] * v0.x + m[1] * v1.x;
+    a[r].y = m[0] * v0.y + m[1] * v1.y;
+    a[r1].x = m[2] * v0.x + m[3] * v1.x;
+    a[r1].y = m[2] * v0.y + m[3] * v1.y;
+  }
+}
+
+template <typename T, int RB>
+__device__ __forceinline__ void ru2_dispatch(typename Vec2<T>::type *a, const T *m, int j) {
+  switch (j) {
+    case 0: ru2_pairs<T, RB, 0>(a, m); break;
+    case 1: if constexpr (RB > 1) ru2_pairs<T, RB, 1>(a, m); break;
+    case 2: if constexpr (RB > 2) ru2_pairs<T, RB, 2>(a, m); break;
+    case 3: if constexpr (RB > 3) ru2_pairs<T, RB, 3>(a, m); break;
+    case 4: if constexpr (RB > 4) ru2_pairs<T, RB, 4>(a, m); break;
+    default: break;
+  }
+}
+
+// Diagonal over the tile's local bits. tab[0..63] holds the register part
+// d(r) for the segment's register bits (constant factor included),
+// tab[64..79] the factor of each thread-id bit when it is set; the thread
+// part is the product of the factors of its set bits.
+template <typename T, int RB>
+__device__ __forceinline__ void dtab_apply(typename Vec2<T>::type *a, const T *tab, uint32_t tm) {
+  using V = typename Vec2<T>::type;
+  V f;
+  f.x = T(1);
+  f.y = T(0);
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) {
+    V q;
+    q.x = tab[64 + 2 * k];
+    q.y = tab[65 + 2 * k];
+    const V g = cmul(f, q);
+    const bool on = (tm >> k) & 1u;
+    f.x = on ? g.x : f.x;
+    f.y = on ? g.y : f.y;
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    V d;
+    d.x = tab[2 * r];
+    d.y = tab[2 * r + 1];
+    a[r] = cmul(a[r], cmul(d, f));
+  }
+}
+
 // QFT stage on register bit J. The ladder phase of the amplitude with the
 // stage bit set is exp(i*pi*L/2^(p-lb)), L = index bits [lb, p). It factors
 // into the thread part wt (bits outside the registers) and the register
@@ -202,7 +260,7 @@ __device__ __forceinline__ void diag_apply(typename Vec2<T>::type *a, const Tile
 
 template <typename T, int RB>
 __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArgs<T> &A,
-                                        uint32_t ob, uint32_t oe, uint64_t gb) {
+                                        uint32_t ob, uint32_t oe, uint64_t gb, uint32_t tm) {
   using V = typename Vec2<T>::type;
   for (uint32_t o = ob; o < oe; ++o) {
     const TileOp<T> &op = A.ops[o];
@@ -221,9 +279,16 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
       const uint32_t w = static_cast<uint32_t>(op.pbit) - op.lb;
       const uint64_t x = (gb >> op.lb) & ((w >= 64) ? ~0ull : ((1ull << w) - 1ull));
       const V wt = CisPi<T>::eval(x, w);
-      qft_dispatch<T, RB>(a, wt, A.w32, op.rpos, op.jl, kind == OP_QFT_I);
+      qft_dispatch<T, RB>(a, wt, A.tab, op.rpos, op.jl, kind == OP_QFT_I);
     } else if (kind == OP_DIAG) {
       diag_apply<T, RB>(a, op, gb);
+    } else if (kind == OP_RU2) {
+      T m[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) m[k] = op.m[k];
+      ru2_dispatch<T, RB>(a, m, op.rpos);
+    } else if (kind == OP_DTAB) {
+      dtab_apply<T, RB>(a, A.tab + op.lb, tm);
     }
   }
 }
@@ -248,6 +313,33 @@ __device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
   return l ^ ((l >> RB) & kMask);
 }
 
+template <typename T>
+__device__ __forceinline__ uint64_t tile_base(const TileArgs<T> &A, uint64_t tile) {
+  uint64_t base = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxRuns; ++k)
+    if (k < static_cast<int>(A.nruns))
+      base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
+  return base;
+}
+
+// Position of this thread in segment `sg`: tl = its local bits, gb = its
+// amplitude index bits (tile base included), tm = its thread-id bits in use.
+__device__ __forceinline__ void seg_thread(const TileSeg &sg, uint32_t tid, uint64_t base,
+                                           uint32_t &tl, uint64_t &gb, uint32_t &tm) {
+  tl = 0;
+  gb = base;
+  tm = tid & sg.tmask;
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) {
+    const uint32_t b = (tm >> k) & 1u;
+    tl |= b << sg.tl_bit[k];
+    gb |= static_cast<uint64_t>(b) << sg.gi_bit[k];
+  }
+}
+
+// One tile per block and iteration (persistent grid): the first segment
+// loads from HBM, later segments exchange through LDS, the last one stores.
 template <typename T, int RB>
 __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict__ st,
                                               const TileArgs<T> A) {
@@ -257,26 +349,16 @@ __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict
   V *lds = reinterpret_cast<V *>(smem_raw);
   const uint32_t tid = threadIdx.x;
   for (uint64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
-    uint64_t base = 0;
-#pragma unroll
-    for (int k = 0; k < kMaxRuns; ++k)
-      if (k < static_cast<int>(A.nruns))
-        base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
+    const uint64_t base = tile_base(A, tile);
     V a[R];
     for (uint32_t s = 0; s < A.nseg; ++s) {
       const TileSeg &sg = A.seg[s];
       uint64_t gstr[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
-      uint32_t tl = 0;
-      uint64_t gb = base;
-      const uint32_t tm = tid & sg.tmask;
-#pragma unroll
-      for (int k = 0; k < kTileThreadBits; ++k) {
-        const uint32_t b = (tm >> k) & 1u;
-        tl |= b << sg.tl_bit[k];
-        gb |= static_cast<uint64_t>(b) << sg.gi_bit[k];
-      }
+      uint32_t tl, tm;
+      uint64_t gb;
+      seg_thread(sg, tid, base, tl, gb, tm);
       const uint32_t reg0 = sg.reg0;
       const uint32_t tsw = lds_swz<RB>(tl);
       if (s == 0) {
@@ -287,7 +369,7 @@ __global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict
         for (int r = 0; r < R; ++r)
           a[r] = lds[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
       }
-      run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, gb);
+      run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, gb, tm);
       if (s + 1 == A.nseg) {
         if (A.scale != T(1)) {
 #pragma unroll

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -51,10 +52,11 @@ struct Tuning {
   int local_bits_c128 = 12; // 2^12 complex128 = 64 KiB
   int gate_bits = 9;        // index bits above the columns per high sweep
   int phase_blocks = 0;     // 0 = CUs * 8
-  int sweep_split = 0;
-  int tile_blocks_per_cu = 8;
-  int elem_blocks_per_cu = 32;
+  int sweep_split = 0;      // high-sweep bit sets of 1q layers: 0 contiguous, 1 page-balanced
+  int tile_blocks_per_cu = 32;
+  int elem_blocks_per_cu = 128;
   int col_bits_c64 = 4;     // contiguous low index bits per tile row in high sweeps
+  int rot_split_min = 13;   // min gates per sweep for the rotation split (0 = never)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -198,12 +200,15 @@ int check_qubit(const msv_state *s, int q, const char *what) {
 // ---------------------------------------------------------------------------
 
 struct PlanOp {
-  uint8_t kind = 0;   // OP_U2 / OP_QFT_F / OP_QFT_I / OP_DIAG
-  int tbit = -1;      // target index bit (U2, QFT)
+  uint8_t kind = 0;   // OP_U2 / OP_QFT_F / OP_QFT_I / OP_DIAG / OP_DTAB / OP_RU2
+  int tbit = -1;      // target index bit (U2, QFT, RU2)
   uint64_t cmask = 0; // controls (U2), index bits
   int abit = -1, bbit = -1;  // DIAG bits (bbit < 0: one-bit diagonal)
   int lb = 0;         // QFT ladder low bit
   double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // DTAB: the diagonal is cst * prod over local bits k that are set of ratio[k]
+  std::vector<std::complex<double>> ratio;
+  std::complex<double> cst = 1.0;
 };
 
 struct SegPlan {
@@ -251,18 +256,64 @@ void partition(int n, int *low_len, std::vector<std::pair<int, int>> *chunks) {
   *low_len = L;
 }
 
+// Local bits of a high sweep whose gate bits are `gates` (ascending): the
+// lowest index bits as columns (at least col_bits(), more when the gates
+// leave room in the tile, for longer contiguous rows), then the gates.
+template <typename T>
+std::vector<int> high_sweep_local_bits(const std::vector<int> &gates) {
+  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int lowest = gates.empty() ? 0 : gates.front();
+  const int C = std::min(std::max(col_bits<T>(), NL - static_cast<int>(gates.size())), lowest);
+  std::vector<int> local;
+  for (int b = 0; b < C; ++b) local.push_back(b);
+  for (int b : gates) local.push_back(b);
+  return local;
+}
+
 template <typename T>
 std::vector<int> sweep_local_bits(int n, int low_len, bool is_low, std::pair<int, int> chunk) {
-  std::vector<int> local;
-  if (is_low) {
-    for (int b = 0; b < low_len; ++b) local.push_back(b);
-  } else {
-    const int C = std::min(col_bits<T>(), chunk.first);
-    for (int b = 0; b < C; ++b) local.push_back(b);
-    for (int b = chunk.first; b < chunk.first + chunk.second; ++b) local.push_back(b);
-  }
   (void)n;
-  return local;
+  if (is_low) {
+    std::vector<int> local;
+    for (int b = 0; b < low_len; ++b) local.push_back(b);
+    return local;
+  }
+  std::vector<int> gates;
+  for (int b = chunk.first; b < chunk.first + chunk.second; ++b) gates.push_back(b);
+  return high_sweep_local_bits<T>(gates);
+}
+
+// Gate-bit sets of the high sweeps of a commuting layer. Policy 0: the
+// contiguous chunks of partition(). Policy 1: same sizes, but the bits at
+// or above the 2 MiB page boundary are spread evenly over the sweeps, so
+// that no sweep's tile touches many more pages than another's.
+template <typename T>
+std::vector<std::vector<int>> high_gate_sets(const std::vector<std::pair<int, int>> &chunks) {
+  std::vector<std::vector<int>> sets;
+  if (g_tune.sweep_split != 1 || chunks.size() < 2) {
+    for (auto &ch : chunks) {
+      std::vector<int> g;
+      for (int b = ch.first; b < ch.first + ch.second; ++b) g.push_back(b);
+      sets.push_back(g);
+    }
+    return sets;
+  }
+  const int page_bit = std::is_same<T, float>::value ? 18 : 17;  // 2 MiB / amplitude size
+  std::vector<int> below, above;
+  for (auto &ch : chunks)
+    for (int b = ch.first; b < ch.first + ch.second; ++b) (b < page_bit ? below : above).push_back(b);
+  const size_t k = chunks.size();
+  size_t ib = 0, ia = 0;
+  for (size_t c = 0; c < k; ++c) {
+    const size_t want_below = below.size() / k + (c < below.size() % k ? 1 : 0);
+    std::vector<int> g;
+    for (size_t t = 0; t < want_below && ib < below.size(); ++t) g.push_back(below[ib++]);
+    while (static_cast<int>(g.size()) < chunks[c].second && ia < above.size()) g.push_back(above[ia++]);
+    while (static_cast<int>(g.size()) < chunks[c].second && ib < below.size()) g.push_back(below[ib++]);
+    std::sort(g.begin(), g.end());
+    sets.push_back(g);
+  }
+  return sets;
 }
 
 // Windows of rb consecutive local bits covering the given local bits,
@@ -331,9 +382,17 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
   }
   A.n_tiles = 1ull << (n - nlocal);
   A.scale = static_cast<T>(plan.scale);
-  for (int m = 0; m < 32; ++m) {
-    A.w32[2 * m] = static_cast<T>(std::cos(2.0 * M_PI * m / 32.0));
-    A.w32[2 * m + 1] = static_cast<T>(std::sin(2.0 * M_PI * m / 32.0));
+  // table pool: QFT twiddles at 0, or the segment diagonals
+  bool has_qft = false;
+  for (const SegPlan &sp : plan.segs)
+    for (const PlanOp &po : sp.ops) has_qft |= po.kind == OP_QFT_F || po.kind == OP_QFT_I;
+  int tab_used = 0;
+  if (has_qft) {
+    for (int m = 0; m < 32; ++m) {
+      A.tab[2 * m] = static_cast<T>(std::cos(2.0 * M_PI * m / 32.0));
+      A.tab[2 * m + 1] = static_cast<T>(std::sin(2.0 * M_PI * m / 32.0));
+    }
+    tab_used = 64;
   }
   if (nlocal - rb > kTileThreadBits)
     return fail(MSV_E_INVALID, "internal: tile larger than one block");
@@ -399,6 +458,32 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
         op.jl = static_cast<uint32_t>(jl);
         op.pbit = static_cast<uint32_t>(p);
         op.lb = static_cast<uint32_t>(po.lb);
+      } else if (po.kind == OP_RU2) {
+        const int j = reg_pos(po.tbit);
+        if (j < 0) return fail(MSV_E_INVALID, "internal: op target is not a register bit");
+        op.rpos = static_cast<int8_t>(j);
+      } else if (po.kind == OP_DTAB) {
+        if (tab_used + kDtabVals > TileCfg<T>::kTabVals)
+          return fail(MSV_E_INVALID, "internal: tile table pool exhausted");
+        if (static_cast<int>(po.ratio.size()) != nlocal)
+          return fail(MSV_E_INVALID, "internal: diagonal table size");
+        op.lb = static_cast<uint32_t>(tab_used);
+        T *tab = A.tab + tab_used;
+        for (int r = 0; r < (1 << rb); ++r) {
+          std::complex<double> d = po.cst;
+          for (int j = 0; j < rb; ++j)
+            if ((r >> j) & 1) d *= po.ratio[sp.reg0 + j];
+          tab[2 * r] = static_cast<T>(d.real());
+          tab[2 * r + 1] = static_cast<T>(d.imag());
+        }
+        for (int r = 1 << rb; r < 32; ++r) tab[2 * r] = tab[2 * r + 1] = T(0);
+        for (int k = 0; k < kTileThreadBits; ++k) {
+          std::complex<double> q = 1.0;
+          if (k < nlocal - rb) q = po.ratio[sg.tl_bit[k]];
+          tab[64 + 2 * k] = static_cast<T>(q.real());
+          tab[65 + 2 * k] = static_cast<T>(q.imag());
+        }
+        tab_used += kDtabVals;
       } else if (po.kind == OP_DIAG) {
         op.nbits = po.bbit >= 0 ? 2u : 1u;
         const int ja = reg_pos(po.abit);
@@ -449,9 +534,12 @@ std::vector<int> commuting_windows(uint32_t need, int nlocal, int rb, int gf) {
 }
 
 // Commuting per-bit ops (one op per gate local bit, any order).
+// `pre` runs before every op of the sweep (first segment), `post` after
+// every op (last segment); both are diagonals (or absent: kind 0).
 template <typename T>
 int run_commuting_sweep(msv_state *s, const std::vector<int> &local,
-                        const std::vector<std::pair<int, PlanOp>> &ops_by_local) {
+                        const std::vector<std::pair<int, PlanOp>> &ops_by_local,
+                        const PlanOp *pre = nullptr, const PlanOp *post = nullptr) {
   const int nlocal = static_cast<int>(local.size());
   const int rb = std::min(rb_of<T>(), nlocal);
   uint32_t need = 0;
@@ -464,6 +552,7 @@ int run_commuting_sweep(msv_state *s, const std::vector<int> &local,
   for (int w : order) {
     SegPlan sp;
     sp.reg0 = w;
+    if (plan.segs.empty() && pre) sp.ops.push_back(*pre);
     for (size_t i = 0; i < ops_by_local.size(); ++i) {
       const int l = ops_by_local[i].first;
       if (!used[i] && l >= w && l < w + rb) {
@@ -473,7 +562,100 @@ int run_commuting_sweep(msv_state *s, const std::vector<int> &local,
     }
     plan.segs.push_back(sp);
   }
+  if (post) plan.segs.back().ops.push_back(*post);
   return launch_sweep<T>(s, plan);
+}
+
+// U = diag(a1, b1) * [[c, -s], [s, c]] * diag(1, b2) for a unitary 2x2 U
+// (the Z-Y-Z Euler form with the phases gathered into two diagonals).
+struct RotSplit {
+  std::complex<double> a1, b1, b2;
+  double c, s;
+};
+
+bool split_rotation(const double *m, RotSplit *z) {
+  using C = std::complex<double>;
+  const C u00(m[0], m[1]), u01(m[2], m[3]), u10(m[4], m[5]), u11(m[6], m[7]);
+  double c = 0.5 * (std::abs(u00) + std::abs(u11));
+  double sn = 0.5 * (std::abs(u01) + std::abs(u10));
+  const double nrm = std::hypot(c, sn);
+  if (!(nrm > 0.5 && nrm < 2.0)) return false;
+  c /= nrm;
+  sn /= nrm;
+  C a1 = 1.0, b1 = 1.0, b2 = 1.0;
+  if (std::abs(u00) > 0.0) a1 = u00 / std::abs(u00);
+  if (std::abs(u10) > 0.0) {
+    b1 = u10 / std::abs(u10);
+    if (std::abs(u01) > 0.0) b2 = -u01 / std::abs(u01) / a1;
+    else if (std::abs(u11) > 0.0) b2 = u11 / std::abs(u11) / b1;
+  } else if (std::abs(u11) > 0.0) {
+    b2 = u11 / std::abs(u11);
+  }
+  // the split must reproduce U (it does for unitaries)
+  const C r00 = a1 * c, r01 = -a1 * sn * b2, r10 = b1 * sn, r11 = b1 * c * b2;
+  const double err = std::max(std::max(std::abs(r00 - u00), std::abs(r01 - u01)),
+                              std::max(std::abs(r10 - u10), std::abs(r11 - u11)));
+  if (!(err < 1e-9)) return false;
+  z->a1 = a1;
+  z->b1 = b1;
+  z->b2 = b2;
+  z->c = c;
+  z->s = sn;
+  return true;
+}
+
+// One commuting 1q sweep over `local` with ops on the active gate bits.
+// For complex64 with enough gates the ops are split into real rotations
+// between two segment diagonals (4 FMAs per amplitude per gate instead of
+// 8, plus two diagonal passes); otherwise general 2x2 ops.
+template <typename T>
+int run_1q_sweep(msv_state *s, const std::vector<int> &local, const std::vector<int> &gate_locals,
+                 const double *mats) {
+  const int n = s->n;
+  const int nlocal = static_cast<int>(local.size());
+  const int k = static_cast<int>(gate_locals.size());
+  if (k == 0) return MSV_OK;
+  std::vector<RotSplit> zs(k);
+  bool split = TileCfg<T>::kTabVals >= 2 * kDtabVals && g_tune.rot_split_min > 0 &&
+               k >= g_tune.rot_split_min;
+  for (int i = 0; split && i < k; ++i) split = split_rotation(mats + 8 * (n - 1 - local[gate_locals[i]]), &zs[i]);
+  if (split) {
+    PlanOp pre, post;
+    pre.kind = post.kind = OP_DTAB;
+    pre.ratio.assign(nlocal, 1.0);
+    post.ratio.assign(nlocal, 1.0);
+    bool pre_on = false;
+    for (int i = 0; i < k; ++i) {
+      pre.ratio[gate_locals[i]] = zs[i].b2;
+      post.ratio[gate_locals[i]] = zs[i].b1 / zs[i].a1;
+      post.cst *= zs[i].a1;
+      pre_on |= std::abs(zs[i].b2 - 1.0) > 0.0;
+    }
+    {
+      std::vector<std::pair<int, PlanOp>> ops;
+      for (int i = 0; i < k; ++i) {
+        PlanOp po;
+        po.kind = OP_RU2;
+        po.tbit = local[gate_locals[i]];
+        po.m[0] = zs[i].c;
+        po.m[1] = -zs[i].s;
+        po.m[2] = zs[i].s;
+        po.m[3] = zs[i].c;
+        ops.push_back({gate_locals[i], po});
+      }
+      return run_commuting_sweep<T>(s, local, ops, pre_on ? &pre : nullptr, &post);
+    }
+  }
+  std::vector<std::pair<int, PlanOp>> ops;
+  for (int i = 0; i < k; ++i) {
+    PlanOp po;
+    po.kind = OP_U2;
+    po.tbit = local[gate_locals[i]];
+    const int q = n - 1 - po.tbit;
+    for (int t = 0; t < 8; ++t) po.m[t] = mats[8 * q + t];
+    ops.push_back({gate_locals[i], po});
+  }
+  return run_commuting_sweep<T>(s, local, ops);
 }
 
 template <typename T>
@@ -485,28 +667,20 @@ int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   std::vector<bool> act_bit(n, false);
   for (int q = 0; q < n; ++q)
     if (!active || active[q]) act_bit[bit_of(s, q)] = true;
-  auto make_op = [&](int bit) {
-    PlanOp po;
-    po.kind = OP_U2;
-    po.tbit = bit;
-    const int q = n - 1 - bit;
-    for (int k = 0; k < 8; ++k) po.m[k] = mats[8 * q + k];
-    return po;
-  };
   // low sweep
   {
     std::vector<int> local = sweep_local_bits<T>(n, L, true, {0, 0});
-    std::vector<std::pair<int, PlanOp>> ops;
+    std::vector<int> gl;
     for (int k = 0; k < static_cast<int>(local.size()); ++k)
-      if (act_bit[local[k]]) ops.push_back({k, make_op(local[k])});
-    if (!ops.empty()) MSV_TRY(run_commuting_sweep<T>(s, local, ops));
+      if (act_bit[local[k]]) gl.push_back(k);
+    MSV_TRY(run_1q_sweep<T>(s, local, gl, mats));
   }
-  for (auto &ch : chunks) {
-    std::vector<int> local = sweep_local_bits<T>(n, L, false, ch);
-    std::vector<std::pair<int, PlanOp>> ops;
+  for (const std::vector<int> &gates : high_gate_sets<T>(chunks)) {
+    std::vector<int> local = high_sweep_local_bits<T>(gates);
+    std::vector<int> gl;
     for (int k = 0; k < static_cast<int>(local.size()); ++k)
-      if (local[k] >= ch.first && act_bit[local[k]]) ops.push_back({k, make_op(local[k])});
-    if (!ops.empty()) MSV_TRY(run_commuting_sweep<T>(s, local, ops));
+      if (local[k] >= L && act_bit[local[k]]) gl.push_back(k);
+    MSV_TRY(run_1q_sweep<T>(s, local, gl, mats));
   }
   return MSV_OK;
 }
@@ -547,7 +721,7 @@ int apply_qft_T(msv_state *s, int hb, int lb, bool inverse, int only_stage_bit) 
     for (int p : stage_bits) gl.push_back(loc_of[p]);
     // windows stay inside the gate bits so that register bits are
     // contiguous index bits
-    const int floor = (si == 0) ? 0 : std::min(col_bits<T>(), sw.gate_lo);
+    const int floor = (si == 0) ? 0 : nlocal - (sw.gate_hi - sw.gate_lo + 1);
     std::vector<int> wins = cover_top_down(gl, nlocal, rb, floor);
     if (inverse) std::reverse(wins.begin(), wins.end());
     SweepPlan plan;
@@ -1627,6 +1801,7 @@ int msv_set_tuning(int key, int64_t value) {
       g_tune.phase_blocks = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_SWEEP_SPLIT:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "sweep split policy out of range");
       g_tune.sweep_split = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_TILE_BLOCKS_PER_CU:
@@ -1640,6 +1815,10 @@ int msv_set_tuning(int key, int64_t value) {
     case MSV_TUNE_COL_BITS:
       if (value < 0 || value > 8) return fail(MSV_E_INVALID, "column bits out of range");
       g_tune.col_bits_c64 = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_ROT_SPLIT_MIN:
+      if (value < 0 || value > 64) return fail(MSV_E_INVALID, "rotation split threshold out of range");
+      g_tune.rot_split_min = static_cast<int>(value);
       return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
@@ -1656,6 +1835,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_TILE_BLOCKS_PER_CU: *value = g_tune.tile_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune.elem_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_COL_BITS: *value = g_tune.col_bits_c64; return MSV_OK;
+    case MSV_TUNE_ROT_SPLIT_MIN: *value = g_tune.rot_split_min; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

@@ -48,6 +48,49 @@ def test_1q_layer_matches_oracle(gpu, n, dtype):
     assert np.max(np.abs(got - want)) < tol
 
 
+@pytest.mark.parametrize("kind", ["special", "nonunitary"])
+def test_1q_layer_special_and_nonunitary_matrices(gpu, kind):
+    """Diagonal, anti-diagonal and real rotations (edge cases of the
+    rotation split), and non-unitary matrices (which must not be split)."""
+    n = 15
+    rng = np.random.default_rng(77)
+    psi = _rand_state(n, 9)
+    if kind == "special":
+        pool = [O.X(), O.Y(), O.Z(), O.S(), O.T(), O.H(), O.RY(0.3), O.RZ(1.1), O.I(),
+                O.RX(math.pi), O.PhaseShift(2.0)]
+    else:
+        pool = [np.array([[1, 0], [0, 0]], complex), 0.5 * O.H(),
+                np.array([[1, 2j], [0.5, -1]], complex), _rand_u(rng)]
+    mats = [pool[i % len(pool)] for i in range(n)]
+    want = O.apply_1q_layer(psi, n, mats)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_1q_layer(mats)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64 * max(1.0, np.max(np.abs(want)) * 4)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_1q_layer_page_balanced_split(gpu, dtype):
+    """High sweeps with non-contiguous gate-bit sets (split policy 1)."""
+    from mettleq_amd import _native as nat
+    n = 24
+    rng = np.random.default_rng(41)
+    psi = _rand_state(n, 5)
+    mats = [_rand_u(rng) for _ in range(n)]
+    want = O.apply_1q_layer(psi, n, mats)
+    nat.set_tuning(nat.MSV_TUNE_SWEEP_SPLIT, 1)
+    try:
+        with _state(n, dtype) as s:
+            s.from_numpy(psi)
+            s.apply_1q_layer(mats)
+            got = s.to_numpy()
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_SWEEP_SPLIT, 0)
+    tol = TOL64 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
 @pytest.mark.parametrize("n", [6, 13, 17, 21])
 def test_1q_layer_active_subset(gpu, n):
     rng = np.random.default_rng(7)
