@@ -45,10 +45,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--layers", type=int, default=40)
-    ap.add_argument("--cpu-qubits", type=int, default=24)
+    ap.add_argument("--cpu-qubits", type=int, default=27)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true", help="skip the complex128 fidelity check")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--traffic-from", default=os.path.join(ROOT, "profiles", "k_tile_30q_layered_circuit_pmc.md"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the tile kernel (HBM bytes per launch)")
     return ap.parse_args()
 
 
@@ -82,7 +85,21 @@ def run_circuit(state, layers, n, lay):
         apply_layer(state, layer, n, lay)
 
 
-def cpu_baseline(n_cpu: int, layers):
+def read_traffic(path):
+    """HBM bytes per k_tile launch from a committed rocprofv3 PMC summary: the
+    line `traffic_bytes_per_launch: N` (FETCH_SIZE and WRITE_SIZE of the same
+    bench command, each scaled by what the counter reads for the copy probe)."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1].strip())), os.path.basename(path)
+    except OSError:
+        pass
+    return None, None
+
+
+def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
     """Gate-by-gate CPU restatement on one layer of each type at n_cpu qubits."""
     from oracle.sv_cpu import CpuEngine, lib
     from mettleq_amd.workloads import layer_ops
@@ -107,9 +124,10 @@ def cpu_baseline(n_cpu: int, layers):
         ops = layer_ops(src, n_cpu)
         p0 = eng.passes
         t0 = time.perf_counter()
-        eng.run(ops)
-        dt = time.perf_counter() - t0
-        passes = eng.passes - p0
+        for _ in range(repeats):
+            eng.run(ops)
+        dt = (time.perf_counter() - t0) / repeats
+        passes = (eng.passes - p0) // repeats
         per[t] = {"ms": dt * 1e3, "passes": passes}
         total_t += dt
         total_passes += passes
@@ -121,7 +139,8 @@ def cpu_baseline(n_cpu: int, layers):
         "cores": threads,
         "kind": "port",
         "sample": (f"one layer of each type ({', '.join(sorted(per))}) at {n_cpu} qubits complex64, "
-                   f"gate by gate ({total_passes} full-state passes, {total_t:.1f} s)"),
+                   f"gate by gate, {repeats}x ({total_passes} full-state passes per repeat, "
+                   f"{total_t * repeats:.1f} s in all)"),
         "layer_ms": {k: round(v["ms"], 2) for k, v in per.items()},
         "layer_ms_scaled_to_30q": {k: round(v["ms"] * (1 << (30 - n_cpu)), 1) for k, v in per.items()},
     }
@@ -220,9 +239,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline at {args.cpu_qubits} qubits ...")
-        cpu = cpu_baseline(args.cpu_qubits, layers)
+        cpu = cpu_baseline(args.cpu_qubits, layers, args.cpu_repeats)
         log(f"cpu baseline: {cpu['value']} GB/s with {cpu['cores']} threads")
 
+    traffic, traffic_src = read_traffic(args.traffic_from) if n == 30 else (None, None)
     out = {
         "metric": "layer GB/s & %HBM peak at 30q c64; served circuits/s + p50/p99 latency",
         "value": round(value, 1),
@@ -256,7 +276,8 @@ def main():
             "peak": PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "avg_launch_ms": round(tile_avg_ms, 4),
             "algorithmic_bytes_per_launch": sweep_bytes,
         },
