@@ -171,6 +171,37 @@ int msv_inner(msv_state *a, msv_state *b, double *re, double *im);
 /* FP64 sum |a_i|^2. */
 int msv_norm2(msv_state *s, double *out);
 
+/* ---- batched bindings of a layered rotation ansatz, adjoint gradients ---- */
+/* An ansatz is an ordered gate list of parameterized rotations
+ * R_P(theta) = exp(-i theta P / 2), P = X, Y, Z (theta = params[b][param_index]
+ * for binding b) and fixed two-qubit phases (MSV_PHASE_ZZ / MSV_PHASE_CPHASE,
+ * e.g. CZ = CPHASE(pi)), with a diagonal observable H = sum_t w_t Z^{zmask_t}
+ * (masks over qubits: bit q = qubit q). msv_ansatz_evaluate runs every
+ * binding from |0...0> in one graph launch and returns the energies
+ * <psi_b|H|psi_b> and, when grads != NULL, d energy_b / d params[b][p] by
+ * the adjoint method. The first call for a batch size autotunes the tile
+ * shape and captures the launch sequence as a HIP graph. Replaces the
+ * per-binding estimator loop (integrations/qiskit.py:906-1090) and
+ * parameter-shift differentiation of the PennyLane device
+ * (integrations/pennylane.py:558-625). */
+#define MSV_ROT_X 0
+#define MSV_ROT_Y 1
+#define MSV_ROT_Z 2
+typedef struct msv_ansatz msv_ansatz;
+int msv_ansatz_create(int n_qubits, int device, msv_ansatz **out);
+int msv_ansatz_destroy(msv_ansatz *a);
+int msv_ansatz_rotation(msv_ansatz *a, int kind, int qubit, int param_index);
+int msv_ansatz_phase(msv_ansatz *a, int kind, int q0, int q1, double theta);
+int msv_ansatz_observable(msv_ansatz *a, const uint64_t *zmasks, const double *weights, int n_terms);
+/* params: host [batch][n_params]; energies: host [batch]; grads: host
+ * [batch][n_params] or NULL. Synchronous. */
+int msv_ansatz_evaluate(msv_ansatz *a, const double *params, int batch, int n_params,
+                        double *energies, double *grads);
+/* Chosen plan of a batch size: out[0] tile local bits, out[1] blocks per CU,
+ * out[2] forward sweeps, out[3] backward sweeps, out[4] tile ops,
+ * out[5] autotuned time in microseconds. */
+int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n_out);
+
 /* ---- tuning knobs ------------------------------------------------------- */
 /* Sets an integer tuning parameter (MSV_TUNE_*); returns MSV_E_INVALID for
  * unknown keys. Values apply to subsequent launches from any state. */

@@ -30,6 +30,10 @@ MSV_C128 = 1
 MSV_PHASE_ZZ = 0
 MSV_PHASE_CPHASE = 1
 
+MSV_ROT_X = 0
+MSV_ROT_Y = 1
+MSV_ROT_Z = 2
+
 MSV_G_U = 0
 MSV_G_DIAG1 = 1
 MSV_G_DIAG2 = 2
@@ -119,6 +123,13 @@ SIGNATURES = {
     "msv_state_timer_start": (_I, [_P]),
     "msv_state_timer_stop": (_I, [_P, _DP]),
     "msv_probe_copy_bandwidth": (_I, [_I, _U64, _I, _DP]),
+    "msv_ansatz_create": (_I, [_I, _I, ctypes.POINTER(_P)]),
+    "msv_ansatz_destroy": (_I, [_P]),
+    "msv_ansatz_rotation": (_I, [_P, _I, _I, _I]),
+    "msv_ansatz_phase": (_I, [_P, _I, _I, _I, ctypes.c_double]),
+    "msv_ansatz_observable": (_I, [_P, _U64P, _DP, _I]),
+    "msv_ansatz_evaluate": (_I, [_P, _DP, _I, _I, _DP, _DP]),
+    "msv_ansatz_info": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_int64), _I]),
 }
 
 _lib = None

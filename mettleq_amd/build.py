@@ -20,8 +20,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libmettleq_sv.so"
-SOURCES = ["msv_kernels.hip", "msv_runtime.hip"]
-HEADERS = ["msv_internal.h", "msv_philox.h"]
+SOURCES = ["msv_kernels.hip", "msv_runtime.hip", "msv_batch.hip"]
+HEADERS = ["msv_internal.h", "msv_device.h", "msv_philox.h"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 namespace msv {
 
@@ -172,5 +173,9 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
                              hipStream_t stream);
 
 constexpr uint32_t kReduceThreads = 256;
+
+// Host helpers of the runtime shared with the batched ansatz engine.
+int set_error(int code, const std::string &msg);  // records the thread's last error, returns code
+int device_cu_count(int device);
 
 }  // namespace msv

@@ -17,33 +17,12 @@
 //               one/two-bit diagonals) executed segment by segment.
 #include <hip/hip_runtime.h>
 
+#include "msv_device.h"
 #include "msv_internal.h"
 
 namespace msv {
 
 namespace {
-
-template <typename V>
-__device__ __forceinline__ V cmul(V a, V b) {
-  V r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
-  return r;
-}
-template <typename V>
-__device__ __forceinline__ V cadd(V a, V b) {
-  V r;
-  r.x = a.x + b.x;
-  r.y = a.y + b.y;
-  return r;
-}
-template <typename V>
-__device__ __forceinline__ V csub(V a, V b) {
-  V r;
-  r.x = a.x - b.x;
-  r.y = a.y - b.y;
-  return r;
-}
 
 template <typename T> struct CisPi;
 template <> struct CisPi<float> {
@@ -290,51 +269,6 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
     } else if (kind == OP_DTAB) {
       dtab_apply<T, RB>(a, A.tab + op.lb, tm);
     }
-  }
-}
-
-template <int RB>
-__device__ __forceinline__ uint64_t reg_goff(int r, const uint64_t *gstr) {
-  uint64_t o = 0;
-#pragma unroll
-  for (int j = 0; j < RB; ++j)
-    if ((r >> j) & 1) o += gstr[j];
-  return o;
-}
-
-// XOR swizzle of LDS slots: when lanes step by 2^RB slots (registers on the
-// lowest local bits) they would all hit one bank pair; folding bits above
-// RB into the low bits spreads them while staying a bijection. The map is
-// linear over GF(2), so swz(tl | roff) = swz(tl) ^ swz(roff) for disjoint
-// thread and register bits: the register part is wave-uniform.
-template <int RB>
-__device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
-  constexpr uint32_t kMask = (RB >= 5) ? 15u : ((RB == 4) ? 7u : 0u);
-  return l ^ ((l >> RB) & kMask);
-}
-
-template <typename T>
-__device__ __forceinline__ uint64_t tile_base(const TileArgs<T> &A, uint64_t tile) {
-  uint64_t base = 0;
-#pragma unroll
-  for (int k = 0; k < kMaxRuns; ++k)
-    if (k < static_cast<int>(A.nruns))
-      base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
-  return base;
-}
-
-// Position of this thread in segment `sg`: tl = its local bits, gb = its
-// amplitude index bits (tile base included), tm = its thread-id bits in use.
-__device__ __forceinline__ void seg_thread(const TileSeg &sg, uint32_t tid, uint64_t base,
-                                           uint32_t &tl, uint64_t &gb, uint32_t &tm) {
-  tl = 0;
-  gb = base;
-  tm = tid & sg.tmask;
-#pragma unroll
-  for (int k = 0; k < kTileThreadBits; ++k) {
-    const uint32_t b = (tm >> k) & 1u;
-    tl |= b << sg.tl_bit[k];
-    gb |= static_cast<uint64_t>(b) << sg.gi_bit[k];
   }
 }
 

@@ -84,6 +84,11 @@ int dev_cu(int device) {
 
 }  // namespace
 
+namespace msv {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+int device_cu_count(int device) { return dev_cu(device); }
+}  // namespace msv
+
 // ---------------------------------------------------------------------------
 // state
 // ---------------------------------------------------------------------------

@@ -16,6 +16,12 @@ reference checkout's ``src/mettleq``):
 - phase-layer LUT semantics .... shaders/zz.py:78-84, shaders/diag.py:89-98
 - probabilities / marginals .... sim.py:322-343
 - Pauli-word expectation ....... integrations/_common.py:384-427
+- layered-ansatz energies ...... integrations/qiskit.py:952-1018 (EstimatorV2:
+                                 one circuit per binding, <psi|H|psi>);
+                                 gradients by parameter shift (the PennyLane
+                                 device's default differentiation,
+                                 integrations/pennylane.py:558-625) and by
+                                 FP64 central differences
 - sampling ..................... this engine's own counter-based algorithm
                                  (Philox4x32-10 + fixed-order FP64 block CDF);
                                  the reference draws with mx.random.categorical
@@ -581,3 +587,73 @@ def pack_wires(indices: np.ndarray, n: int, wires: Sequence[int]) -> np.ndarray:
     for w in wires:
         out = (out << np.uint64(1)) | ((indices >> np.uint64(n - 1 - w)) & np.uint64(1))
     return out
+
+
+# ---------------------------------------------------------------------------
+# layered rotation ansatz: energies of a diagonal observable and gradients
+# (the batched estimator / adjoint path of mettleq_amd.ansatz)
+# ---------------------------------------------------------------------------
+
+
+def ansatz_state(n: int, gates: Sequence[tuple], params: Sequence[float]) -> np.ndarray:
+    """gates: ("RX"|"RY"|"RZ", qubit, param_index) or
+    ("CPHASE"|"ZZPHASE", q0, q1, theta), applied in order from |0...0>."""
+    s = zero_state(n)
+    for g in gates:
+        if g[0] in ("RX", "RY", "RZ"):
+            s = apply_dense(s, n, gate_matrix(g[0], [params[g[2]]]), [g[1]])
+        else:
+            s = apply_dense(s, n, gate_matrix(g[0], [g[3]]), [g[1], g[2]])
+    return s
+
+
+def z_observable_diag(n: int, terms: Sequence[tuple]) -> np.ndarray:
+    """h(x) = sum_t w_t (-1)^{sum of qubit bits of x in term t}; terms are
+    (qubits, weight), qubit q = index bit n-1-q."""
+    x = np.arange(1 << n, dtype=np.int64)
+    h = np.zeros(1 << n)
+    for qubits, w in terms:
+        par = np.zeros(1 << n, dtype=np.int64)
+        for q in qubits:
+            par ^= (x >> (n - 1 - q)) & 1
+        h += w * (1 - 2 * par)
+    return h
+
+
+def ansatz_energy(n: int, gates, params, terms) -> float:
+    s = ansatz_state(n, gates, params)
+    return float(np.sum(np.abs(s) ** 2 * z_observable_diag(n, terms)))
+
+
+def ansatz_param_shift(n: int, gates, params, terms) -> np.ndarray:
+    """Exact gradient of Pauli rotations: per gate occurrence,
+    (E(theta + pi/2) - E(theta - pi/2)) / 2 with only that occurrence shifted."""
+    params = np.asarray(params, dtype=np.float64)
+    grad = np.zeros_like(params)
+    for gi, g in enumerate(gates):
+        if g[0] not in ("RX", "RY", "RZ"):
+            continue
+        vals = []
+        for sh in (math.pi / 2, -math.pi / 2):
+            s = zero_state(n)
+            for gj, h in enumerate(gates):
+                if h[0] in ("RX", "RY", "RZ"):
+                    th = params[h[2]] + (sh if gj == gi else 0.0)
+                    s = apply_dense(s, n, gate_matrix(h[0], [th]), [h[1]])
+                else:
+                    s = apply_dense(s, n, gate_matrix(h[0], [h[3]]), [h[1], h[2]])
+            vals.append(float(np.sum(np.abs(s) ** 2 * z_observable_diag(n, terms))))
+        grad[g[2]] += 0.5 * (vals[0] - vals[1])
+    return grad
+
+
+def ansatz_central_diff(n: int, gates, params, terms, h: float = 1e-4) -> np.ndarray:
+    params = np.asarray(params, dtype=np.float64)
+    grad = np.zeros_like(params)
+    for p in range(len(params)):
+        hi = params.copy()
+        lo = params.copy()
+        hi[p] += h
+        lo[p] -= h
+        grad[p] = (ansatz_energy(n, gates, hi, terms) - ansatz_energy(n, gates, lo, terms)) / (2 * h)
+    return grad
