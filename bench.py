@@ -50,6 +50,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true", help="skip the complex128 fidelity check")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--workload", default="layered30",
+                    choices=["layered30", "brick20", "memlimit", "vqe16", "serve"],
+                    help="layered30 (default): the 30-qubit layered circuit; brick20: 20-qubit brick "
+                         "circuit with probabilities and shots vs the CPU restatement; memlimit: 34q "
+                         "complex64 and 33q complex128 through the preflight; vqe16: 1024 bindings of a "
+                         "16-qubit ansatz with adjoint gradients; serve: Poisson request stream")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="layered30 only: skip the short brick20 / vqe16 / serve summaries")
+    ap.add_argument("--rates", default="50,100,200,500,1000,2000")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--traffic-from", default=os.path.join(ROOT, "profiles", "k_tile_30q_layered_circuit_pmc.md"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the tile kernel (HBM bytes per launch)")
     return ap.parse_args()
@@ -146,8 +157,251 @@ def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
     }
 
 
+# ---------------------------------------------------------------------------
+# other configurations (one JSON line each with --workload)
+# ---------------------------------------------------------------------------
+
+
+def bench_brick20(args, quick=False):
+    """20 qubits complex64, 10 layers of U3-all + CZ brick: full FP64
+    probability vector and 1000 shots (seed base + 1), GPU engine vs the
+    gate-by-gate CPU restatement of the reference's CPU path."""
+    from mettleq_amd.device import Device
+    from mettleq_amd.workloads import BASE_SEED, brick_20q, circuit_ops
+    from oracle.sv_cpu import CpuEngine, lib
+    import oracle.sv_oracle as O
+    n = 20
+    ops = circuit_ops(brick_20q(n, 10), n)
+    dev = Device(n)
+    for _ in range(2):
+        dev.reset()
+        dev.execute(ops)
+        dev.synchronize()
+    st = dev.native
+    reps = 3 if quick else 10
+    ts = []
+    for _ in range(reps):
+        dev.reset()
+        st.sync()
+        t0 = time.perf_counter()
+        dev.execute(ops)
+        probs = st.probabilities()
+        shots = st.sample(1000, seed=BASE_SEED + 1)
+        ts.append(time.perf_counter() - t0)
+    gpu_state = st.to_numpy()
+    lib()
+    cpu = CpuEngine(n)
+    t0 = time.perf_counter()
+    cpu.run(ops)
+    cpu_probs = O.amp_probs_c64(cpu.state)
+    cpu_shots = cpu.sample_indices(1000, BASE_SEED + 1)
+    cpu_s = time.perf_counter() - t0
+    # bit-exact sampling pin: the C restatement sampling the GPU's amplitudes
+    cross = CpuEngine(n, gpu_state).sample_indices(1000, BASE_SEED + 1)
+    dev.close()
+    return {
+        "metric": "20q brick circuit: probabilities + 1000 shots, ms per circuit",
+        "workload": "20-qubit complex64, 10 layers U3-all + CZ brick, full probabilities + 1000 shots",
+        "gpu_ms": round(1e3 * float(np.median(ts)), 3),
+        "cpu_ms": round(1e3 * cpu_s, 1),
+        "cpu_threads": int(lib().cpu_num_threads()),
+        "probabilities_max_abs_diff_vs_cpu": float(np.max(np.abs(probs - cpu_probs))),
+        "shots_bit_exact_vs_cpu_sampler_on_gpu_state": bool(np.array_equal(shots, cross)),
+        "shots_equal_to_cpu_run": float(np.mean(shots == cpu_shots)),
+        "norm": float(probs.sum()),
+    }
+
+
+def bench_memlimit(args):
+    """34q complex64 (128 GiB) and 33q complex128 (128 GiB) through the
+    preflight, in place; 1-F between 33q complex64 and the 33q complex128
+    oracle, both resident. Layer mix of the 20q brick circuit, depth 20."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.execution import statevector_preflight
+    from mettleq_amd.native_state import NativeState
+    from mettleq_amd.workloads import brick_20q
+    out = {"metric": "memory-limit run: seconds per circuit and 1-F (33q c64 vs c128)",
+           "workload": "brick layers (U3-all + CZ brick), depth 20: 34q complex64, 33q complex128 and 33q complex64"}
+    pf = {}
+    for n, dt, mode in ((34, "complex64", "in_place"), (34, "complex64", "out_of_place"),
+                        (33, "complex128", "in_place"), (35, "complex64", "out_of_place"),
+                        (36, "complex64", "in_place")):
+        r = statevector_preflight(n, dtype=dt, mode=mode)
+        pf[f"{n}q_{dt}_{mode}"] = r["decision"]
+    out["preflight"] = pf
+
+    def run(n, dtype, depth=20):
+        layers = brick_20q(n, depth)
+        st = NativeState(n, dtype)
+        st.sync()
+        s0 = st.sweep_count()
+        st.timer_start()
+        for layer in layers:
+            if layer["type"] == "u3":
+                from mettleq_amd import gates as G
+                st.apply_1q_layer([G.single_qubit_matrix("U3", a) for a in layer["angles"]])
+            else:
+                st.apply_phase_layer([nat.MSV_PHASE_CPHASE] * len(layer["bonds"]), layer["bonds"],
+                                     [math.pi] * len(layer["bonds"]))
+        ms = st.timer_stop()
+        sweeps = st.sweep_count() - s0
+        return st, ms, sweeps
+
+    st34, ms34, sw34 = run(34, "complex64")
+    gb34 = sw34 * 2 * st34.nbytes / (ms34 * 1e-3) / 1e9
+    out["34q_complex64"] = {"s": round(ms34 / 1e3, 3), "sweeps": sw34, "GBps": round(gb34, 1),
+                            "norm2": st34.norm2()}
+    st34.close()
+    st128, ms128, sw128 = run(33, "complex128")
+    st64, ms64, sw64 = run(33, "complex64")
+    ip = st64.inner(st128)
+    fid = abs(ip) ** 2 / (st64.norm2() * st128.norm2())
+    out["33q_complex128"] = {"s": round(ms128 / 1e3, 3), "sweeps": sw128,
+                             "GBps": round(sw128 * 2 * st128.nbytes / (ms128 * 1e-3) / 1e9, 1)}
+    out["33q_complex64"] = {"s": round(ms64 / 1e3, 3), "sweeps": sw64,
+                            "GBps": round(sw64 * 2 * st64.nbytes / (ms64 * 1e-3) / 1e9, 1)}
+    out["one_minus_fidelity_33q_c64_vs_c128"] = float(1.0 - fid)
+    st64.close()
+    st128.close()
+    return out
+
+
+def _vqe_plain_gradients(params_row, n, L):
+    """The plain per-circuit path on the same GPU: every gradient entry by
+    parameter shift, each shifted circuit executed alone through Device
+    (fusion + dispatch) and its <H> read with the Pauli-word reduction."""
+    from mettleq_amd.device import Device
+    words = [{i: "Z", (i + 1) % n: "Z"} for i in range(n)]
+
+    def ops_for(th):
+        ops = []
+        for l in range(L):
+            for q in range(n):
+                ops.append({"name": "RY", "wires": [q], "parameters": [float(th[2 * (l * n + q)])]})
+                ops.append({"name": "RZ", "wires": [q], "parameters": [float(th[2 * (l * n + q) + 1])]})
+            for q in range(n):
+                ops.append({"name": "CZ", "wires": [q, (q + 1) % n], "parameters": []})
+        return ops
+
+    dev = Device(n)
+    grads = np.zeros(len(params_row))
+    for p in range(len(params_row)):
+        vals = []
+        for sh in (math.pi / 2, -math.pi / 2):
+            th = params_row.copy()
+            th[p] += sh
+            dev.reset()
+            dev.execute(ops_for(th))
+            vals.append(float(np.real(dev.native.expval_pauli(words)).sum()))
+        grads[p] = 0.5 * (vals[0] - vals[1])
+    dev.close()
+    return grads
+
+
+def bench_vqe16(args, quick=False):
+    """1024 bindings of the 16-qubit hardware-efficient ansatz (8 layers of
+    RY, RZ per qubit + CZ ring, 256 parameters), H = sum Z_i Z_{i+1}:
+    energies + all gradients per graph launch, vs the plain per-circuit
+    parameter-shift path on the same GPU."""
+    from mettleq_amd.ansatz import hardware_efficient, zz_ring_terms
+    from mettleq_amd.workloads import BASE_SEED
+    n, L, B = 16, 8, 1024
+    rng = np.random.default_rng(BASE_SEED + 4)
+    params = rng.uniform(-math.pi, math.pi, (B, 2 * n * L))
+    with hardware_efficient(n, L) as a:
+        a.observable(zz_ring_terms(n))
+        t0 = time.perf_counter()
+        e, g = a.evaluate(params)
+        first = time.perf_counter() - t0
+        ts = []
+        for _ in range(3 if quick else 10):
+            t0 = time.perf_counter()
+            e, g = a.evaluate(params)
+            ts.append(time.perf_counter() - t0)
+        fwd = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            a.evaluate(params, grads=False)
+            fwd.append(time.perf_counter() - t0)
+        info = a.info(B)
+    t_adj = float(np.median(ts))
+    n_plain = 1 if quick else 2
+    t0 = time.perf_counter()
+    plain = [_vqe_plain_gradients(params[b].copy(), n, L) for b in range(n_plain)]
+    t_plain = (time.perf_counter() - t0) / n_plain
+    diff = max(float(np.max(np.abs(plain[b] - g[b]))) for b in range(n_plain))
+    return {
+        "metric": "gradients/s, 16q hardware-efficient ansatz, 1024 bindings x 256 parameters",
+        "value": round(B * params.shape[1] / t_adj, 1),
+        "unit": "gradients/s",
+        "ms_per_evaluation": round(1e3 * t_adj, 3),
+        "forward_only_ms": round(1e3 * float(np.median(fwd)), 3),
+        "first_call_s_autotune_and_graph_capture": round(first, 3),
+        "plan": info,
+        "plain_per_circuit_gradients_per_s": round(params.shape[1] / t_plain, 1),
+        "plain_bindings_timed": n_plain,
+        "speedup_vs_plain": round((B * params.shape[1] / t_adj) / (params.shape[1] / t_plain), 1),
+        "max_abs_diff_adjoint_vs_plain_parameter_shift": diff,
+        "mean_energy": float(np.mean(e)),
+    }
+
+
+def bench_serve(args, rates=None, n_requests=None, quick=False):
+    """Poisson request streams at each rate: the continuous-batching server
+    vs the plain FIFO one-at-a-time path, circuits/s and p50/p99 latency."""
+    from mettleq_amd.serving import Server, StatePool, poisson_stream, summarize
+    rates = rates or [float(x) for x in args.rates.split(",")]
+    n_requests = n_requests or args.requests
+    pool = StatePool()
+    # warm the pool and the planner once with a short stream of every width
+    warm = poisson_stream(17 * 2, 1e6, widths=(12, 28), gates=(10, 20), shots=(1000, 1000))
+    Server(args.workers, pool).serve(warm, realtime=False)
+    out = {"metric": "served circuits/s and p50/p99 latency (arrival to result)",
+           "workload": (f"Poisson arrivals, {n_requests} requests per rate, width U{{12..28}}, "
+                        "10-200 gates (50% U3, 25% CZ/RZZ, 25% CNOT), shots log-uniform 1e3-1e4"),
+           "workers": args.workers, "rates": {}}
+    for rate in rates:
+        reqs = poisson_stream(n_requests, rate)
+        cont = summarize(Server(args.workers, pool).serve(reqs, realtime=True))
+        fifo = summarize(Server(1, pool).serve(reqs, realtime=True))
+        out["rates"][str(int(rate))] = {"continuous": cont, "fifo": fifo}
+        log(f"serve rate {rate}: continuous {cont}, fifo {fifo}")
+    out["pool"] = {"peak_resident_GiB": round(pool.peak_resident / 2**30, 2), "allocations": pool.allocations,
+                   "evictions": pool.evictions}
+    pool.close()
+    return out
+
+
+def extra_summaries(args):
+    """Short runs of the other configurations for the default bench line."""
+    res = {}
+    for name, fn in (("brick20", lambda: bench_brick20(args, quick=True)),
+                     ("vqe16_adjoint", lambda: bench_vqe16(args, quick=True)),
+                     ("serve_1000_per_s", lambda: bench_serve(args, rates=[1000.0], n_requests=400, quick=True))):
+        try:
+            t0 = time.perf_counter()
+            res[name] = fn()
+            log(f"{name} summary done in {time.perf_counter() - t0:.1f} s")
+        except Exception as exc:  # keep the headline line even if a side leg fails
+            res[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"{name} summary failed: {exc}")
+    return res
+
+
 def main():
     args = parse()
+    if args.workload != "layered30":
+        if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+            print("bench.py measures one GPU; run with --gpus 1.", file=sys.stderr)
+            sys.exit(2)
+        out = {"brick20": bench_brick20, "memlimit": bench_memlimit, "vqe16": bench_vqe16,
+               "serve": bench_serve}[args.workload](args)
+        line = json.dumps(out)
+        if args.json_out:
+            with open(args.json_out, "w") as fh:
+                fh.write(line + "\n")
+        print(line, flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
               "engine's multi-GPU sharding is a design (see DESIGN.md), not a measured path. "
@@ -243,6 +497,9 @@ def main():
         log(f"cpu baseline: {cpu['value']} GB/s with {cpu['cores']} threads")
 
     traffic, traffic_src = read_traffic(args.traffic_from) if n == 30 else (None, None)
+    if not args.no_extra:
+        state.close()
+        extras["configs"] = extra_summaries(args)
     out = {
         "metric": "layer GB/s & %HBM peak at 30q c64; served circuits/s + p50/p99 latency",
         "value": round(value, 1),

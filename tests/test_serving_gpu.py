@@ -1,0 +1,64 @@
+"""Serving on the GPU: each request's result is the same alone and in the
+mix (bit-identical probabilities and counts: the kernels are deterministic
+and share nothing across requests), and a realtime Poisson stream
+completes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests(count, seed=7):
+    from mettleq_amd.serving import Request, random_circuit
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(rng.integers(8, 17))
+        reqs.append(Request(i, n, random_circuit(n, int(rng.integers(10, 81)), rng),
+                            int(rng.integers(100, 2001)), 5000 + i))
+    return reqs
+
+
+def test_same_alone_and_in_the_mix(gpu):
+    from mettleq_amd.native_state import NativeState
+    from mettleq_amd.serving import Server, StatePool, run_request
+    reqs = _requests(30)
+    pool = StatePool()
+    try:
+        mixed = Server(6, pool, keep_probabilities=True).serve(reqs, realtime=False)
+    finally:
+        pool.close()
+    for req, res in zip(reqs, mixed):
+        assert res.error is None, res.error
+        with NativeState(req.n) as st:
+            outcomes, counts, probs = run_request(st, req, keep_probabilities=True)
+        assert np.array_equal(probs, res.probabilities), req.id
+        assert np.array_equal(outcomes, res.outcomes) and np.array_equal(counts, res.counts), req.id
+        assert counts.sum() == req.shots
+
+
+def test_results_match_the_oracle(gpu):
+    from oracle import sv_oracle as O
+    from mettleq_amd.serving import Server, StatePool
+    reqs = _requests(6, seed=3)
+    pool = StatePool()
+    try:
+        res = Server(3, pool, keep_probabilities=True).serve(reqs, realtime=False)
+    finally:
+        pool.close()
+    for req, r in zip(reqs, res):
+        want = np.abs(O.run_ops(req.n, req.ops)) ** 2
+        assert np.max(np.abs(r.probabilities - want)) < 5e-6
+
+
+def test_realtime_poisson_stream_completes(gpu):
+    from mettleq_amd.serving import Server, StatePool, poisson_stream, summarize
+    reqs = poisson_stream(60, rate=300.0, widths=(12, 20), gates=(10, 120), shots=(1000, 3000))
+    pool = StatePool()
+    try:
+        res = Server(8, pool).serve(reqs, realtime=True)
+    finally:
+        pool.close()
+    s = summarize(res)
+    assert s["completed"] == 60 and s["failed"] == 0
+    assert all(int(r.counts.sum()) == q.shots for r, q in zip(res, reqs))
