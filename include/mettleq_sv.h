@@ -146,6 +146,50 @@ int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse);
  * (shaders/controlled_single.py:89-134). */
 int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates);
 
+/* Gate codes of msv_apply_circuit: names, wire order and angle conventions
+ * of the reference (gates.py:5-234, device.py:1138-1290): U3(t, p, l) =
+ * RZ(p) RY(t) RZ(l); RZ(t) = diag(e^{-it/2}, e^{it/2}); U1 = P = phase;
+ * ZZ/XX/YYPHASE(t) = exp(-i t P P); two-qubit gates take (control, target)
+ * or (MSB, LSB) wires. */
+#define MSV_OP_I 0
+#define MSV_OP_X 1
+#define MSV_OP_Y 2
+#define MSV_OP_Z 3
+#define MSV_OP_H 4
+#define MSV_OP_S 5
+#define MSV_OP_SDG 6
+#define MSV_OP_T 7
+#define MSV_OP_TDG 8
+#define MSV_OP_SX 9
+#define MSV_OP_RX 10
+#define MSV_OP_RY 11
+#define MSV_OP_RZ 12
+#define MSV_OP_U1 13
+#define MSV_OP_U2 14
+#define MSV_OP_U3 15
+#define MSV_OP_CNOT 16
+#define MSV_OP_CZ 17
+#define MSV_OP_CPHASE 18
+#define MSV_OP_CRX 19
+#define MSV_OP_CRY 20
+#define MSV_OP_CRZ 21
+#define MSV_OP_CH 22
+#define MSV_OP_SWAP 23
+#define MSV_OP_ISWAP 24
+#define MSV_OP_ZZPHASE 25
+#define MSV_OP_XXPHASE 26
+#define MSV_OP_YYPHASE 27
+#define MSV_OP_TOFFOLI 28
+#define MSV_OP_FREDKIN 29
+/* Ordered circuit in compact form: op k is codes[k] on wires[3k..3k+2]
+ * (unused = -1) with angles params[3k..3k+2]. The gate matrices are built
+ * natively and the gates fused into tile sweeps as by msv_apply_gates.
+ * Replaces the per-op translation of integrations/_common.py:29-115 plus
+ * the gate routing of device.py:1138-1290 for callers that submit many
+ * small circuits (the serving loop). */
+int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires,
+                      const double *params, int n_ops);
+
 /* ---- reductions ----------------------------------------------------------- */
 /* FP64 probabilities of the requested wires, first wire = MSB of the output
  * index (sim.py:322-343); wires == NULL means all qubits in order. `out`

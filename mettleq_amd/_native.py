@@ -112,6 +112,7 @@ SIGNATURES = {
     "msv_apply_qft": (_I, [_P, _I, _I, _I]),
     "msv_apply_qft_stage": (_I, [_P, _I, _I, _I]),
     "msv_apply_gates": (_I, [_P, ctypes.POINTER(msv_gate), _I]),
+    "msv_apply_circuit": (_I, [_P, _I32P, _I32P, _DP, _I]),
     "msv_probabilities": (_I, [_P, _I32P, _I, _DP]),
     "msv_sample": (_I, [_P, _I32P, _I, _U64, _U64, _U64, _U64P]),
     "msv_expval_pauli": (_I, [_P, _U64P, _U64P, _I, _DP, _DP]),

@@ -1345,6 +1345,152 @@ int init_basis_T(msv_state *s, uint64_t index) {
 #define MSV_DISPATCH(s, fn, ...) \
   ((s)->dtype == MSV_C128 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
+namespace {
+
+using cd = std::complex<double>;
+struct M2 {
+  cd a, b, c, d;  // [[a, b], [c, d]]
+};
+M2 mul(const M2 &x, const M2 &y) {
+  return {x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+}
+M2 dagger(const M2 &x) { return {std::conj(x.a), std::conj(x.c), std::conj(x.b), std::conj(x.d)}; }
+M2 rx(double t) {
+  const double c = std::cos(t / 2), s = std::sin(t / 2);
+  return {c, cd(0, -s), cd(0, -s), c};
+}
+M2 ry(double t) {
+  const double c = std::cos(t / 2), s = std::sin(t / 2);
+  return {c, -s, s, c};
+}
+M2 rz(double t) { return {std::polar(1.0, -t / 2), 0.0, 0.0, std::polar(1.0, t / 2)}; }
+
+msv_gate g_u(int t, const M2 &m, int c0 = -1, int c1 = -1) {
+  msv_gate g;
+  std::memset(&g, 0, sizeof(g));
+  g.kind = MSV_G_U;
+  g.q[0] = t;
+  if (c0 >= 0) g.q[1 + g.ncontrols++] = c0;
+  if (c1 >= 0) g.q[1 + g.ncontrols++] = c1;
+  const cd e[4] = {m.a, m.b, m.c, m.d};
+  for (int k = 0; k < 4; ++k) {
+    g.m[2 * k] = e[k].real();
+    g.m[2 * k + 1] = e[k].imag();
+  }
+  return g;
+}
+msv_gate g_d1(int q, cd d0, cd d1) {
+  msv_gate g;
+  std::memset(&g, 0, sizeof(g));
+  g.kind = MSV_G_DIAG1;
+  g.q[0] = q;
+  g.m[0] = d0.real(); g.m[1] = d0.imag(); g.m[2] = d1.real(); g.m[3] = d1.imag();
+  return g;
+}
+msv_gate g_d2(int a, int b, cd d0, cd d1, cd d2, cd d3) {
+  msv_gate g;
+  std::memset(&g, 0, sizeof(g));
+  g.kind = MSV_G_DIAG2;
+  g.q[0] = a;
+  g.q[1] = b;
+  const cd e[4] = {d0, d1, d2, d3};
+  for (int k = 0; k < 4; ++k) {
+    g.m[2 * k] = e[k].real();
+    g.m[2 * k + 1] = e[k].imag();
+  }
+  return g;
+}
+msv_gate g_swap(int a, int b, int c = -1) {
+  msv_gate g;
+  std::memset(&g, 0, sizeof(g));
+  g.kind = MSV_G_SWAP;
+  g.q[0] = a;
+  g.q[1] = b;
+  if (c >= 0) {
+    g.ncontrols = 1;
+    g.q[2] = c;
+  }
+  return g;
+}
+
+// Device gates of one op (the routing of mettleq_amd.fusion.gates_for_op).
+int circuit_op_gates(int code, const int32_t *w, const double *p, std::vector<msv_gate> *out) {
+  const double r2 = std::sqrt(0.5);
+  const M2 X{0.0, 1.0, 1.0, 0.0}, Y{0.0, cd(0, -1), cd(0, 1), 0.0}, H{r2, r2, r2, -r2};
+  const M2 S{1.0, 0.0, 0.0, cd(0, 1)};
+  switch (code) {
+    case MSV_OP_I: return MSV_OK;
+    case MSV_OP_X: out->push_back(g_u(w[0], X)); return MSV_OK;
+    case MSV_OP_Y: out->push_back(g_u(w[0], Y)); return MSV_OK;
+    case MSV_OP_H: out->push_back(g_u(w[0], H)); return MSV_OK;
+    case MSV_OP_SX:
+      out->push_back(g_u(w[0], {cd(0.5, 0.5), cd(0.5, -0.5), cd(0.5, -0.5), cd(0.5, 0.5)}));
+      return MSV_OK;
+    case MSV_OP_Z: out->push_back(g_d1(w[0], 1.0, -1.0)); return MSV_OK;
+    case MSV_OP_S: out->push_back(g_d1(w[0], 1.0, cd(0, 1))); return MSV_OK;
+    case MSV_OP_SDG: out->push_back(g_d1(w[0], 1.0, cd(0, -1))); return MSV_OK;
+    case MSV_OP_T: out->push_back(g_d1(w[0], 1.0, std::polar(1.0, M_PI / 4))); return MSV_OK;
+    case MSV_OP_TDG: out->push_back(g_d1(w[0], 1.0, std::polar(1.0, -M_PI / 4))); return MSV_OK;
+    case MSV_OP_RX: out->push_back(g_u(w[0], rx(p[0]))); return MSV_OK;
+    case MSV_OP_RY: out->push_back(g_u(w[0], ry(p[0]))); return MSV_OK;
+    case MSV_OP_RZ: {
+      const M2 m = rz(p[0]);
+      out->push_back(g_d1(w[0], m.a, m.d));
+      return MSV_OK;
+    }
+    case MSV_OP_U1: out->push_back(g_d1(w[0], 1.0, std::polar(1.0, p[0]))); return MSV_OK;
+    case MSV_OP_U2: out->push_back(g_u(w[0], mul(mul(rz(p[0]), ry(M_PI / 2)), rz(p[1])))); return MSV_OK;
+    case MSV_OP_U3: out->push_back(g_u(w[0], mul(mul(rz(p[1]), ry(p[0])), rz(p[2])))); return MSV_OK;
+    case MSV_OP_CNOT: out->push_back(g_u(w[1], X, w[0])); return MSV_OK;
+    case MSV_OP_CH: out->push_back(g_u(w[1], H, w[0])); return MSV_OK;
+    case MSV_OP_CRX: out->push_back(g_u(w[1], rx(p[0]), w[0])); return MSV_OK;
+    case MSV_OP_CRY: out->push_back(g_u(w[1], ry(p[0]), w[0])); return MSV_OK;
+    case MSV_OP_CZ: out->push_back(g_d2(w[0], w[1], 1.0, 1.0, 1.0, -1.0)); return MSV_OK;
+    case MSV_OP_CPHASE:
+      out->push_back(g_d2(w[0], w[1], 1.0, 1.0, 1.0, cd(std::cos(p[0]), std::sin(p[0]))));
+      return MSV_OK;
+    case MSV_OP_CRZ: {
+      const M2 m = rz(p[0]);
+      out->push_back(g_d2(w[0], w[1], 1.0, 1.0, m.a, m.d));
+      return MSV_OK;
+    }
+    case MSV_OP_ZZPHASE: {
+      const cd em(std::cos(p[0]), -std::sin(p[0])), ep(std::cos(p[0]), std::sin(p[0]));
+      out->push_back(g_d2(w[0], w[1], em, ep, ep, em));
+      return MSV_OK;
+    }
+    case MSV_OP_XXPHASE:
+    case MSV_OP_YYPHASE: {
+      // exp(-i t PP) = (V x V) exp(-i t ZZ) (V^dag x V^dag), V Z V^dag = P
+      const M2 v = code == MSV_OP_XXPHASE ? H : mul(S, H);
+      const M2 vd = dagger(v);
+      const cd em(std::cos(p[0]), -std::sin(p[0])), ep(std::cos(p[0]), std::sin(p[0]));
+      out->push_back(g_u(w[0], vd));
+      out->push_back(g_u(w[1], vd));
+      out->push_back(g_d2(w[0], w[1], em, ep, ep, em));
+      out->push_back(g_u(w[0], v));
+      out->push_back(g_u(w[1], v));
+      return MSV_OK;
+    }
+    case MSV_OP_SWAP: out->push_back(g_swap(w[0], w[1])); return MSV_OK;
+    case MSV_OP_ISWAP:
+      out->push_back(g_d2(w[0], w[1], 1.0, cd(0, 1), cd(0, 1), 1.0));
+      out->push_back(g_swap(w[0], w[1]));
+      return MSV_OK;
+    case MSV_OP_TOFFOLI: out->push_back(g_u(w[2], X, w[0], w[1])); return MSV_OK;
+    case MSV_OP_FREDKIN: out->push_back(g_swap(w[1], w[2], w[0])); return MSV_OK;
+    default: return fail(MSV_E_INVALID, "circuit: unknown gate code " + std::to_string(code));
+  }
+}
+
+int circuit_arity(int code) {
+  if (code <= MSV_OP_U3) return 1;
+  if (code <= MSV_OP_YYPHASE) return 2;
+  return 3;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *msv_last_error(void) { return g_err.c_str(); }
@@ -1714,6 +1860,32 @@ int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
   }
   MSV_TRY(set_dev(s));
   return MSV_DISPATCH(s, apply_plan_ops, s, ops);
+}
+
+int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires, const double *params,
+                      int n_ops) {
+  MSV_TRY(check_state(s));
+  if (n_ops < 0 || (n_ops > 0 && (!codes || !wires || !params)))
+    return fail(MSV_E_INVALID, "circuit: null arrays");
+  std::vector<msv_gate> gates;
+  gates.reserve(static_cast<size_t>(n_ops) + 8);
+  for (int k = 0; k < n_ops; ++k) {
+    const int code = codes[k];
+    if (code < MSV_OP_I || code > MSV_OP_FREDKIN)
+      return fail(MSV_E_INVALID, "circuit: unknown gate code " + std::to_string(code));
+    const int32_t *w = wires + 3 * k;
+    const int ar = circuit_arity(code);
+    for (int i = 0; i < ar; ++i) {
+      if (w[i] < 0 || w[i] >= s->n) return fail(MSV_E_INVALID, "circuit: qubit index out of range");
+      for (int j = 0; j < i; ++j)
+        if (w[i] == w[j]) return fail(MSV_E_INVALID, "circuit: repeated wire in one gate");
+    }
+    const double *p = params + 3 * k;
+    for (int i = 0; i < 3; ++i)
+      if (!std::isfinite(p[i])) return fail(MSV_E_INVALID, "circuit: non-finite angle");
+    MSV_TRY(circuit_op_gates(code, w, p, &gates));
+  }
+  return msv_apply_gates(s, gates.data(), static_cast<int>(gates.size()));
 }
 
 int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out) {

@@ -169,6 +169,18 @@ class NativeState:
         nat.check(nat.lib().msv_apply_qft_stage(self.handle, int(j), int(m), int(bool(inverse))),
                   "apply_qft_stage")
 
+    def apply_circuit(self, codes: np.ndarray, wires: np.ndarray, params: np.ndarray) -> None:
+        """Compact circuit (mettleq_amd.circuit.encode_ops): gate matrices are
+        built and fused natively."""
+        c = np.ascontiguousarray(codes, dtype=np.int32)
+        w = np.ascontiguousarray(wires, dtype=np.int32)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if w.size != 3 * c.size or p.size != 3 * c.size:
+            raise ValueError("circuit arrays must hold 3 wires and 3 angles per op")
+        nat.check(nat.lib().msv_apply_circuit(self.handle, nat.as_ptr(c, ctypes.c_int32),
+                                              nat.as_ptr(w, ctypes.c_int32), nat.as_ptr(p, ctypes.c_double),
+                                              int(c.size)), "apply_circuit")
+
     def apply_gates(self, gates: Sequence[nat.msv_gate]) -> None:
         if not gates:
             return

@@ -14,7 +14,8 @@ worker and a state of their width are free (continuous batching, not a
 queue in front of a single executor). `FIFO` mode (workers=1) is the plain
 one-circuit-at-a-time path.
 
-Every request's gates go through the native planner (msv_apply_gates), its
+Every request's gates go through the native translation and planner
+(msv_apply_circuit), its
 shots through the counter-based sampler keyed by the request's seed, so a
 result does not depend on what else ran beside it: the kernels involve no
 atomics or cross-request state.
@@ -210,12 +211,9 @@ def _preflight_in_place(n: int, reserved: int) -> dict:
 def run_request(state, req: Request, keep_probabilities: bool = False):
     """Reset, apply the request's gates, sample its shots (and optionally
     read the probability vector). Returns (outcomes, counts, probs)."""
-    from .fusion import gates_for_op
+    from .circuit import encode_ops
     state.reset()
-    gates = []
-    for op in req.ops:
-        gates.extend(gates_for_op(op, req.n))
-    state.apply_gates(gates)
+    state.apply_circuit(*encode_ops(req.ops))
     probs = state.probabilities() if keep_probabilities else None
     packed = state.sample(req.shots, seed=req.seed)
     outcomes, counts = np.unique(packed, return_counts=True)
