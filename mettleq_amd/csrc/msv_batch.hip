@@ -26,6 +26,8 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1043,8 +1045,17 @@ int get_plan(msv_ansatz *a, int batch, bool with_grad, msv_ansatz::Plan **out) {
   if (cands.empty()) cands.push_back({a->n_eff, 4});
   msv_ansatz::Plan best;
   double best_ms = 1e30;
+  // METTLEQ_SV_BATCH_TUNE=0 skips the search (first candidate);
+  // METTLEQ_SV_BATCH_PLAN="<tile bits>,<blocks per CU>" fixes the plan.
   const char *env = std::getenv("METTLEQ_SV_BATCH_TUNE");
-  const bool tune = !(env && env[0] == '0') && cands.size() > 1;
+  bool tune = !(env && env[0] == '0') && cands.size() > 1;
+  if (const char *fixed = std::getenv("METTLEQ_SV_BATCH_PLAN")) {
+    int nl = 0, bpc = 0;
+    if (std::sscanf(fixed, "%d,%d", &nl, &bpc) == 2 && nl >= kMinBits && nl <= 12 && bpc >= 1 && bpc <= 64) {
+      cands.assign(1, BTune{std::min(nl, a->n_eff), bpc});
+      tune = false;
+    }
+  }
   for (const BTune &t : tune ? cands : std::vector<BTune>{cands.front()}) {
     msv_ansatz::Plan P;
     BTRY(build_plan(a, batch, t, with_grad, &P));
