@@ -318,12 +318,10 @@ def _two_level(u: np.ndarray, i: int, j: int, a: int, b: int):
         return pre + [gate_u(a, _order(u, bi[0]), [b])] + pre
     # differ in both bits: conjugate by CNOT(a -> b) to make them differ in a only
     cx = gate_u(b, x, [a])
-    ii = i ^ (2 if False else 0)
     # |i>,|j> -> after CX: bit b ^= bit a
     ti = (bi[0], bi[1] ^ bi[0])
     tj = (bj[0], bj[1] ^ bj[0])
     inner = _two_level(u, ti[0] * 2 + ti[1], tj[0] * 2 + tj[1], a, b)
-    del ii
     return [cx] + inner + [cx]
 
 
