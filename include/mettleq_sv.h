@@ -259,6 +259,10 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
 #define MSV_TUNE_ROT_SPLIT_MIN 8      /* gates per sweep from which 1q gates run as
                                          diagonal x real rotation x diagonal (c64;
                                          0 = never) */
+#define MSV_TUNE_XCD_GROUPS 9         /* tile sweeps: blocks of one XCD take
+                                         consecutive tiles (0 = plain order) */
+#define MSV_TUNE_TILE_REG_BITS 10     /* c64 tile sweeps: amplitudes per thread
+                                         2^bits (5: 256-thread tiles, 4: 512) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

@@ -47,6 +47,8 @@ MSV_TUNE_TILE_BLOCKS_PER_CU = 5
 MSV_TUNE_ELEM_BLOCKS_PER_CU = 6
 MSV_TUNE_COL_BITS = 7
 MSV_TUNE_ROT_SPLIT_MIN = 8
+MSV_TUNE_XCD_GROUPS = 9
+MSV_TUNE_TILE_REG_BITS = 10
 
 
 class NativeUnavailableError(RuntimeError):

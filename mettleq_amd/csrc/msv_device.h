@@ -48,7 +48,7 @@ __device__ __forceinline__ uint64_t reg_goff(int r, const uint64_t *gstr) {
 // thread and register bits: the register part is wave-uniform.
 template <int RB>
 __device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
-  constexpr uint32_t kMask = (RB >= 5) ? 15u : ((RB == 4) ? 7u : 0u);
+  constexpr uint32_t kMask = (RB >= 4) ? 15u : 0u;
   return l ^ ((l >> RB) & kMask);
 }
 

@@ -25,7 +25,7 @@ template <> struct TileCfg<float> {
   static constexpr int kRB = 5;
   static constexpr int kMaxLocal = 13;
   static constexpr int kMaxOps = 24;
-  static constexpr int kTabVals = 160;
+  static constexpr int kTabVals = 168;
 };
 template <> struct TileCfg<double> {
   static constexpr int kRB = 4;
@@ -33,7 +33,7 @@ template <> struct TileCfg<double> {
   static constexpr int kMaxOps = 16;
   static constexpr int kTabVals = 64;
 };
-constexpr int kDtabVals = 80;  // one segment diagonal: 32 register + 8 thread-bit complex
+constexpr int kDtabVals = 84;  // one segment diagonal: 32 register + 9 thread-bit complex (+ pad)
 
 constexpr int kMaxSegs = 6;
 constexpr int kMaxPhaseWins = 32;
@@ -74,7 +74,7 @@ struct TileOp {
   T m[8];
 };
 
-constexpr int kTileThreadBits = 8;  // 256 threads per tile
+constexpr int kTileThreadBits = 9;  // up to 512 threads per tile
 
 // One segment: registers hold the local bits [reg0, reg0 + RB). The host
 // precomputes where each thread-id bit and each register bit lands, in the
@@ -101,6 +101,7 @@ struct TileArgs {
   uint32_t nlocal;         // local bits per tile
   uint32_t nseg;
   uint32_t nruns;
+  uint32_t xcd_groups;     // > 1: consecutive tiles go to blocks of one XCD
   uint32_t run_src[kMaxRuns];
   uint32_t run_width[kMaxRuns];
   uint32_t run_dst[kMaxRuns];

@@ -119,7 +119,7 @@ __device__ __forceinline__ void ru2_dispatch(typename Vec2<T>::type *a, const T 
 
 // Diagonal over the tile's local bits. tab[0..63] holds the register part
 // d(r) for the segment's register bits (constant factor included),
-// tab[64..79] the factor of each thread-id bit when it is set; the thread
+// tab[64..81] the factor of each thread-id bit when it is set; the thread
 // part is the product of the factors of its set bits.
 template <typename T, int RB>
 __device__ __forceinline__ void dtab_apply(typename Vec2<T>::type *a, const T *tab, uint32_t tm) {
@@ -275,14 +275,20 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
 // One tile per block and iteration (persistent grid): the first segment
 // loads from HBM, later segments exchange through LDS, the last one stores.
 template <typename T, int RB>
-__global__ void __launch_bounds__(256) k_tile(typename Vec2<T>::type *__restrict__ st,
+__global__ void __launch_bounds__(512) k_tile(typename Vec2<T>::type *__restrict__ st,
                                               const TileArgs<T> A) {
   using V = typename Vec2<T>::type;
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
   V *lds = reinterpret_cast<V *>(smem_raw);
   const uint32_t tid = threadIdx.x;
-  for (uint64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+  // Blocks are dispatched round-robin over the XCDs (block b on XCD
+  // b mod G). With xcd_groups = G the block's virtual rank makes the
+  // blocks of one XCD take consecutive tiles (adjacent rows in HBM).
+  uint64_t vrank = blockIdx.x;
+  if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
+    vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
     V a[R];
     for (uint32_t s = 0; s < A.nseg; ++s) {

@@ -53,10 +53,12 @@ struct Tuning {
   int gate_bits = 9;        // index bits above the columns per high sweep
   int phase_blocks = 0;     // 0 = CUs * 8
   int sweep_split = 0;      // high-sweep bit sets of 1q layers: 0 contiguous, 1 page-balanced
-  int tile_blocks_per_cu = 32;
+  int tile_blocks_per_cu = 64;
   int elem_blocks_per_cu = 128;
   int col_bits_c64 = 4;     // contiguous low index bits per tile row in high sweeps
   int rot_split_min = 13;   // min gates per sweep for the rotation split (0 = never)
+  int xcd_groups = 8;       // tile-to-block mapping groups (gfx950: 8 XCDs); 0/1 = plain
+  int reg_bits_c64 = 4;     // register bits per thread in c64 tile sweeps (4: 512-thread tiles)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -227,7 +229,9 @@ struct SweepPlan {
   double scale = 1.0;
 };
 
-template <typename T> int rb_of() { return TileCfg<T>::kRB; }
+template <typename T> int rb_of() {
+  return std::is_same<T, float>::value ? g_tune.reg_bits_c64 : TileCfg<T>::kRB;
+}
 template <typename T> int local_max() {
   return std::is_same<T, float>::value ? g_tune.local_bits_c64 : g_tune.local_bits_c128;
 }
@@ -386,6 +390,7 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     A.nruns = static_cast<uint32_t>(nr);
   }
   A.n_tiles = 1ull << (n - nlocal);
+  A.xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
   A.scale = static_cast<T>(plan.scale);
   // table pool: QFT twiddles at 0, or the segment diagonals
   bool has_qft = false;
@@ -1997,6 +2002,14 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 0 || value > 64) return fail(MSV_E_INVALID, "rotation split threshold out of range");
       g_tune.rot_split_min = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_XCD_GROUPS:
+      if (value < 0 || value > 64) return fail(MSV_E_INVALID, "XCD groups out of range");
+      g_tune.xcd_groups = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_TILE_REG_BITS:
+      if (value != 4 && value != 5) return fail(MSV_E_INVALID, "tile register bits must be 4 or 5");
+      g_tune.reg_bits_c64 = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2013,6 +2026,8 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune.elem_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_COL_BITS: *value = g_tune.col_bits_c64; return MSV_OK;
     case MSV_TUNE_ROT_SPLIT_MIN: *value = g_tune.rot_split_min; return MSV_OK;
+    case MSV_TUNE_XCD_GROUPS: *value = g_tune.xcd_groups; return MSV_OK;
+    case MSV_TUNE_TILE_REG_BITS: *value = g_tune.reg_bits_c64; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

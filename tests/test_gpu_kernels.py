@@ -48,6 +48,39 @@ def test_1q_layer_matches_oracle(gpu, n, dtype):
     assert np.max(np.abs(got - want)) < tol
 
 
+@pytest.mark.parametrize("reg_bits", [4, 5])
+@pytest.mark.parametrize("n", [13, 15, 19])
+def test_tile_register_bits_both_shapes(gpu, reg_bits, n):
+    """complex64 tile sweeps with 16 (512-thread tiles) or 32 (256-thread
+    tiles) amplitudes per thread: 1q layer, QFT and a generic gate list."""
+    from mettleq_amd import _native as nat
+    rng = np.random.default_rng(500 + n)
+    psi = _rand_state(n, n + 1)
+    mats = [_rand_u(rng) for _ in range(n)]
+    ops = []
+    for _ in range(60):
+        q = [int(x) for x in rng.choice(n, 2, replace=False)]
+        ops.append({"name": "CRY", "wires": q, "parameters": [float(rng.uniform(-3, 3))]})
+        ops.append({"name": "U3", "wires": [q[0]], "parameters": [float(x) for x in rng.uniform(0, 6, 3)]})
+    from mettleq_amd.fusion import gates_for_op
+    gates = [g for op in ops for g in gates_for_op(op, n)]
+    old = nat.get_tuning(nat.MSV_TUNE_TILE_REG_BITS)
+    nat.set_tuning(nat.MSV_TUNE_TILE_REG_BITS, reg_bits)
+    try:
+        with _state(n) as s:
+            s.from_numpy(psi)
+            s.apply_1q_layer(mats)
+            assert np.max(np.abs(s.to_numpy() - O.apply_1q_layer(psi, n, mats))) < TOL64
+            s.from_numpy(psi)
+            s.apply_qft(0, n)
+            assert np.max(np.abs(s.to_numpy() - O.qft_fft(psi, n))) < TOL64
+            s.from_numpy(psi)
+            s.apply_gates(gates)
+            assert np.max(np.abs(s.to_numpy() - O.run_ops(n, ops, psi))) < TOL64
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_TILE_REG_BITS, old)
+
+
 @pytest.mark.parametrize("kind", ["special", "nonunitary"])
 def test_1q_layer_special_and_nonunitary_matrices(gpu, kind):
     """Diagonal, anti-diagonal and real rotations (edge cases of the
