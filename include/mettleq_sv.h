@@ -263,6 +263,8 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
                                          consecutive tiles (0 = plain order) */
 #define MSV_TUNE_TILE_REG_BITS 10     /* c64 tile sweeps: amplitudes per thread
                                          2^bits (5: 256-thread tiles, 4: 512) */
+#define MSV_TUNE_TILE_PREFETCH 11     /* c64 512-thread tile sweeps: load the next
+                                         tile while the current one runs (0/1) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

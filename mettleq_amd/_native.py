@@ -49,6 +49,7 @@ MSV_TUNE_COL_BITS = 7
 MSV_TUNE_ROT_SPLIT_MIN = 8
 MSV_TUNE_XCD_GROUPS = 9
 MSV_TUNE_TILE_REG_BITS = 10
+MSV_TUNE_TILE_PREFETCH = 11
 
 
 class NativeUnavailableError(RuntimeError):

@@ -53,13 +53,15 @@ __device__ __forceinline__ uint32_t lds_swz(uint32_t l) {
 }
 
 // Args: any argument block with nruns / run_src / run_width / run_dst.
+// The run parameters are read in a rolled loop (scalar loads per tile):
+// hoisting all kMaxRuns triples into SGPRs would spill them.
 template <typename Args>
 __device__ __forceinline__ uint64_t tile_base(const Args &A, uint64_t tile) {
   uint64_t base = 0;
-#pragma unroll
-  for (int k = 0; k < kMaxRuns; ++k)
-    if (k < static_cast<int>(A.nruns))
-      base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
+  const int nr = static_cast<int>(A.nruns);
+#pragma unroll 1
+  for (int k = 0; k < nr; ++k)
+    base |= ((tile >> A.run_src[k]) & ((1ull << A.run_width[k]) - 1ull)) << A.run_dst[k];
   return base;
 }
 

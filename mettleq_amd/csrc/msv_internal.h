@@ -19,13 +19,12 @@ template <> struct Vec2<double> { using type = double2; };
 template <typename T> struct TileCfg;
 // kMaxOps bounds a tile program so that TileArgs stays under 4 KiB of
 // kernel arguments.
-// kTabVals sizes the per-launch table pool: the QFT twiddles (32 complex)
-// or, for complex64, up to two segment diagonals (32 + 8 complex each).
+// kTabVals sizes the per-launch table pool: the QFT twiddles (32 complex).
 template <> struct TileCfg<float> {
   static constexpr int kRB = 5;
   static constexpr int kMaxLocal = 13;
   static constexpr int kMaxOps = 24;
-  static constexpr int kTabVals = 168;
+  static constexpr int kTabVals = 64;
 };
 template <> struct TileCfg<double> {
   static constexpr int kRB = 4;
@@ -33,7 +32,7 @@ template <> struct TileCfg<double> {
   static constexpr int kMaxOps = 16;
   static constexpr int kTabVals = 64;
 };
-constexpr int kDtabVals = 84;  // one segment diagonal: 32 register + 9 thread-bit complex (+ pad)
+constexpr int kDtabVals = 84;  // one layer diagonal of k_rot: 32 register + 9 thread-bit complex (+ pad)
 
 constexpr int kMaxSegs = 6;
 constexpr int kMaxPhaseWins = 32;
@@ -47,8 +46,6 @@ enum TileOpKind : uint8_t {
   OP_QFT_F = 2,  // forward QFT stage: butterfly then ladder phase on the 1 branch
   OP_QFT_I = 3,  // inverse QFT stage: conjugate ladder phase then butterfly
   OP_DIAG = 4,   // diagonal over 1 or 2 index bits (anywhere in the index)
-  OP_DTAB = 5,   // diagonal over the tile's local bits: register table x thread factors
-  OP_RU2 = 6,    // real 2x2 on a register bit (m[0..3] = r00 r01 r10 r11)
 };
 
 // One operation of a tile program. Matrix entries are stored in the
@@ -66,7 +63,7 @@ struct TileOp {
   uint32_t nbits;   // DIAG: 1 or 2
   uint32_t has_ctrl;// U2: controls present
   uint32_t pbit;    // QFT: index bit p of the stage
-  uint32_t lb;      // QFT: lowest index bit of the ladder; DTAB: table offset
+  uint32_t lb;      // QFT: lowest index bit of the ladder
   uint32_t cmask_r; // U2: control bits that are register bits (register positions)
   uint64_t cmask_t; // U2: control bits that are not register bits (index mask)
   uint64_t amask;   // DIAG: index mask of bit a (when ra < 0)
@@ -107,13 +104,44 @@ struct TileArgs {
   uint32_t run_dst[kMaxRuns];
   TileSeg seg[kMaxSegs];
   T scale;                 // applied at the final store (QFT 1/sqrt2 factors)
-  uint32_t pad;
-  T tab[TileCfg<T>::kTabVals];  // QFT: exp(2*pi*i*m/32), m = 0..31, (re, im) at 0;
-                                // DTAB: kDtabVals values per table at op.lb
+  uint32_t prefetch;       // 1: load the next tile's first segment while this one runs
+  T tab[TileCfg<T>::kTabVals];  // QFT: exp(2*pi*i*m/32), m = 0..31, (re, im) at 0
   TileOp<T> ops[TileCfg<T>::kMaxOps];
 };
 static_assert(sizeof(TileArgs<float>) <= 4096, "tile kernel arguments exceed 4 KiB");
 static_assert(sizeof(TileArgs<double>) <= 4096, "tile kernel arguments exceed 4 KiB");
+
+// Commuting rotation sweep (1q layers in Z-Y-Z split form, k_rot): per
+// segment, a rotation on each register bit set in gmask, given by its three
+// shear factors (a, b); a diagonal over all n index bits before the first
+// segment (pre) and after the last one (post). Each diagonal is a table
+// over the segment's register bits (with the constant) x factors of the
+// segment's thread bits (table layout of OP_DTAB) x a per-tile factor, the
+// product over 4-bit groups of the tile number of tile_tab[g][16] entries.
+template <typename T>
+struct RotArgs {
+  uint64_t n_tiles;
+  uint32_t nlocal;
+  uint32_t nseg;
+  uint32_t nruns;
+  uint32_t xcd_groups;
+  uint32_t run_src[kMaxRuns];
+  uint32_t run_width[kMaxRuns];
+  uint32_t run_dst[kMaxRuns];
+  uint32_t gmask[kMaxSegs];
+  uint32_t has_pre;
+  uint32_t has_post;
+  uint32_t ntgrp;             // 4-bit groups of the tile number in tile_tab
+  uint32_t pad;
+  const typename Vec2<T>::type *tile_tab;  // [2][ntgrp][16]: pre, post (null: ntgrp = 0)
+  TileSeg seg[kMaxSegs];
+  T shear[kMaxSegs][5][2];
+  T pre[kDtabVals];
+  T post[kDtabVals];
+};
+static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4 KiB");
+static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");
+constexpr int kMaxTileGroups = 8;  // tile numbers up to 32 bits
 
 struct PhaseWin {
   uint8_t s1, w1, s2, w2;  // lut index = ((i>>s1)&(2^w1-1)) | (((i>>s2)&(2^w2-1))<<w1)
@@ -133,6 +161,9 @@ struct PhaseArgs {
 template <typename T>
 hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, int rb,
                        uint32_t grid, hipStream_t stream);
+template <typename T>
+hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
+                      hipStream_t stream);
 template <typename T>
 hipError_t launch_phase(typename Vec2<T>::type *state, const typename Vec2<T>::type *lut,
                         const PhaseArgs &args, uint32_t grid, hipStream_t stream);

@@ -17,6 +17,8 @@
 //               one/two-bit diagonals) executed segment by segment.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "msv_device.h"
 #include "msv_internal.h"
 
@@ -88,62 +90,10 @@ __device__ __forceinline__ void u2_dispatch(typename Vec2<T>::type *a, const T *
   }
 }
 
-// Real 2x2 [[r0, r1], [r2, r3]] on register bit J: half the FMAs of a
-// complex 2x2 (used for the rotation factor of a ZYZ-split 1q gate).
-template <typename T, int RB, int J>
-__device__ __forceinline__ void ru2_pairs(typename Vec2<T>::type *a, const T *m) {
-  using V = typename Vec2<T>::type;
-#pragma unroll
-  for (int r = 0; r < (1 << RB); ++r) {
-    if (r & (1 << J)) continue;
-    const int r1 = r | (1 << J);
-    const V v0 = a[r], v1 = a[r1];
-    a[r].x = m[0] * v0.x + m[1] * v1.x;
-    a[r].y = m[0] * v0.y + m[1] * v1.y;
-    a[r1].x = m[2] * v0.x + m[3] * v1.x;
-    a[r1].y = m[2] * v0.y + m[3] * v1.y;
-  }
-}
-
-template <typename T, int RB>
-__device__ __forceinline__ void ru2_dispatch(typename Vec2<T>::type *a, const T *m, int j) {
-  switch (j) {
-    case 0: ru2_pairs<T, RB, 0>(a, m); break;
-    case 1: if constexpr (RB > 1) ru2_pairs<T, RB, 1>(a, m); break;
-    case 2: if constexpr (RB > 2) ru2_pairs<T, RB, 2>(a, m); break;
-    case 3: if constexpr (RB > 3) ru2_pairs<T, RB, 3>(a, m); break;
-    case 4: if constexpr (RB > 4) ru2_pairs<T, RB, 4>(a, m); break;
-    default: break;
-  }
-}
-
-// Diagonal over the tile's local bits. tab[0..63] holds the register part
-// d(r) for the segment's register bits (constant factor included),
-// tab[64..81] the factor of each thread-id bit when it is set; the thread
-// part is the product of the factors of its set bits.
-template <typename T, int RB>
-__device__ __forceinline__ void dtab_apply(typename Vec2<T>::type *a, const T *tab, uint32_t tm) {
-  using V = typename Vec2<T>::type;
-  V f;
-  f.x = T(1);
-  f.y = T(0);
-#pragma unroll
-  for (int k = 0; k < kTileThreadBits; ++k) {
-    V q;
-    q.x = tab[64 + 2 * k];
-    q.y = tab[65 + 2 * k];
-    const V g = cmul(f, q);
-    const bool on = (tm >> k) & 1u;
-    f.x = on ? g.x : f.x;
-    f.y = on ? g.y : f.y;
-  }
-#pragma unroll
-  for (int r = 0; r < (1 << RB); ++r) {
-    V d;
-    d.x = tab[2 * r];
-    d.y = tab[2 * r + 1];
-    a[r] = cmul(a[r], cmul(d, f));
-  }
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+  if constexpr (std::is_same<T, float>::value) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
 }
 
 // QFT stage on register bit J. The ladder phase of the amplitude with the
@@ -261,22 +211,113 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
       qft_dispatch<T, RB>(a, wt, A.tab, op.rpos, op.jl, kind == OP_QFT_I);
     } else if (kind == OP_DIAG) {
       diag_apply<T, RB>(a, op, gb);
-    } else if (kind == OP_RU2) {
-      T m[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) m[k] = op.m[k];
-      ru2_dispatch<T, RB>(a, m, op.rpos);
-    } else if (kind == OP_DTAB) {
-      dtab_apply<T, RB>(a, A.tab + op.lb, tm);
     }
   }
 }
 
+// Amplitude loads and stores of the full-state sweeps. With MSV_NT (the
+// default) they carry the nontemporal hint: every amplitude is read and
+// written once per sweep, so keeping it in L2 only displaces other lines
+// (measured on MI355X: 3-4% faster in-place sweeps).
+#ifndef MSV_NT
+#define MSV_NT 1
+#endif
+template <typename V> struct NtVec;
+template <> struct NtVec<float2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<double2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct NtVec<float4> { typedef float type __attribute__((ext_vector_type(4))); };
+
+template <typename V>
+__device__ __forceinline__ V ld_amp(const V *p) {
+  if constexpr (MSV_NT) {
+    using N = typename NtVec<V>::type;
+    const N v = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+    V r;
+    r.x = v.x;
+    r.y = v.y;
+    return r;
+  } else {
+    return *p;
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ void st_amp(V *p, V a) {
+  if constexpr (MSV_NT) {
+    using N = typename NtVec<V>::type;
+    N v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<N *>(p));
+  } else {
+    *p = a;
+  }
+}
+
+__device__ __forceinline__ float4 ld_f4(const float4 *p) {
+  if constexpr (MSV_NT) {
+    using N = NtVec<float4>::type;
+    const N v = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+  } else {
+    return *p;
+  }
+}
+
+__device__ __forceinline__ void st_f4(float4 *p, float4 a) {
+  if constexpr (MSV_NT) {
+    using N = NtVec<float4>::type;
+    N v = {a.x, a.y, a.z, a.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<N *>(p));
+  } else {
+    *p = a;
+  }
+}
+
+// First-segment amplitudes of a tile straight from HBM.
+template <typename T, int RB>
+__device__ __forceinline__ void load_first(const typename Vec2<T>::type *__restrict__ st,
+                                           const TileArgs<T> &A, uint32_t tid, uint64_t tile,
+                                           typename Vec2<T>::type *a) {
+  const TileSeg &sg = A.seg[0];
+  uint64_t gstr[RB];
+#pragma unroll
+  for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+  uint32_t tl, tm;
+  uint64_t gb;
+  seg_thread(sg, tid, tile_base(A, tile), tl, gb, tm);
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) a[r] = ld_amp(st + gb + reg_goff<RB>(r, gstr));
+}
+
+// Segment `s` of a tile: its thread's local and index bits.
+struct SegPos {
+  uint32_t tl, tm, tsw;
+  uint64_t gb;
+};
+
+template <int RB>
+__device__ __forceinline__ SegPos seg_pos(const TileSeg &sg, uint32_t tid, uint64_t base) {
+  SegPos p;
+  seg_thread(sg, tid, base, p.tl, p.gb, p.tm);
+  p.tsw = lds_swz<RB>(p.tl);
+  return p;
+}
+
 // One tile per block and iteration (persistent grid): the first segment
 // loads from HBM, later segments exchange through LDS, the last one stores.
-template <typename T, int RB>
-__global__ void __launch_bounds__(512) k_tile(typename Vec2<T>::type *__restrict__ st,
-                                              const TileArgs<T> A) {
+// PF: the first segment of the block's next tile is loaded into a second
+// register set at the top of each iteration, so its HBM reads overlap this
+// tile's ops, LDS exchanges and stores. Each tile belongs to one block, so
+// reading it early is safe. The loads are issued before this tile's stores
+// and consumed after them, and the last segment (the one that stores) is
+// straight-line code after the segment loop, so that the wait for the
+// prefetched amplitudes does not also wait for the stores. On the block's
+// last tile the "next" load re-reads the current tile and is discarded:
+// the host uses this form only with many tiles per block.
+template <typename T, int RB, bool PF>
+__device__ __forceinline__ void tile_sweep(typename Vec2<T>::type *__restrict__ st,
+                                           const TileArgs<T> &A) {
   using V = typename Vec2<T>::type;
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
@@ -288,48 +329,214 @@ __global__ void __launch_bounds__(512) k_tile(typename Vec2<T>::type *__restrict
   uint64_t vrank = blockIdx.x;
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  V a[R];
+  V nx[PF ? R : 1];
+  if constexpr (PF) {
+    if (vrank < A.n_tiles) {
+      load_first<T, RB>(st, A, tid, vrank, a);
+      // complete these loads here, so that the loop never sees them pending
+#pragma unroll
+      for (int r = 0; r < R; ++r) asm volatile("" : "+v"(a[r].x), "+v"(a[r].y));
+    }
+  }
+  const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
-    V a[R];
-    for (uint32_t s = 0; s < A.nseg; ++s) {
+    if constexpr (PF) {
+      const uint64_t next = tile + gridDim.x;
+      load_first<T, RB>(st, A, tid, next < A.n_tiles ? next : tile, nx);
+    }
+    // segments 0 .. last; all but the last end with an LDS exchange
+    for (uint32_t s = 0;; ++s) {
       const TileSeg &sg = A.seg[s];
-      uint64_t gstr[RB];
-#pragma unroll
-      for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
-      uint32_t tl, tm;
-      uint64_t gb;
-      seg_thread(sg, tid, base, tl, gb, tm);
-      const uint32_t reg0 = sg.reg0;
-      const uint32_t tsw = lds_swz<RB>(tl);
+      const SegPos p = seg_pos<RB>(sg, tid, base);
       if (s == 0) {
+        if constexpr (!PF) {
+          uint64_t gstr[RB];
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = st[gb + reg_goff<RB>(r, gstr)];
+          for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+        }
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          a[r] = lds[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
+          a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
       }
-      run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, gb, tm);
-      if (s + 1 == A.nseg) {
-        if (A.scale != T(1)) {
+      run_ops<T, RB>(a, A, sg.op_begin, sg.op_end, p.gb, p.tm);
+      if (s == last) break;
+      __syncthreads();
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            a[r].x *= A.scale;
-            a[r].y *= A.scale;
+      for (int r = 0; r < R; ++r) lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)] = a[r];
+      __syncthreads();
+    }
+    // the last segment's stores (addresses recomputed here instead of being
+    // kept live across the ops)
+    {
+      const TileSeg &sg = A.seg[last];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+      uint64_t gstr[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+      if (A.scale != T(1)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          a[r].x *= A.scale;
+          a[r].y *= A.scale;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) st_amp(st + p.gb + reg_goff<RB>(r, gstr), a[r]);
+    }
+    if constexpr (PF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = nx[r];
+    }
+  }
+}
+
+template <typename T, int RB>
+__global__ void __launch_bounds__(512) k_tile(typename Vec2<T>::type *__restrict__ st,
+                                              const TileArgs<T> A) {
+  tile_sweep<T, RB, false>(st, A);
+}
+
+// The prefetching form holds two register sets; it is limited to 128 VGPRs
+// so that two 512-thread blocks (the LDS limit) stay resident per CU.
+template <typename T, int RB>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_tile_pf(typename Vec2<T>::type *__restrict__ st, const TileArgs<T> A) {
+  tile_sweep<T, RB, true>(st, A);
+}
+
+// ---------------------------------------------------------------------------
+// commuting rotation sweep (1q layers in Z-Y-Z split form)
+// ---------------------------------------------------------------------------
+
+// Rotation [[c, -s], [s, c]] on register bit J as three shears (Paeth):
+//   v0 += a v1;  v1 += b v0;  v0 += a v1,   a = -tan(theta/2), b = sin(theta).
+// Every step is one in-place FMA per component, so no temporaries (and no
+// register moves) are needed: 3 FMAs per amplitude component per gate.
+// a = b = 0 is the exact identity.
+template <typename T, int RB, int J>
+__device__ __forceinline__ void shear_pairs(typename Vec2<T>::type *a, T sa, T sb) {
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    a[r].x = fma_t(sa, a[r1].x, a[r].x);
+    a[r].y = fma_t(sa, a[r1].y, a[r].y);
+    a[r1].x = fma_t(sb, a[r].x, a[r1].x);
+    a[r1].y = fma_t(sb, a[r].y, a[r1].y);
+    a[r].x = fma_t(sa, a[r1].x, a[r].x);
+    a[r].y = fma_t(sa, a[r1].y, a[r].y);
+  }
+}
+
+// Diagonal over all index bits, applied to a segment's amplitudes: tab[0..63]
+// holds the register part d(r) for the segment's register bits (constant
+// included), tab[64..81] the factor of each thread-id bit when it is set; the
+// thread part is the product of the factors of its set bits, times `g` (the
+// tile's factor).
+template <typename T, int RB>
+__device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T *tab, uint32_t tm,
+                                             typename Vec2<T>::type g) {
+  using V = typename Vec2<T>::type;
+  V f = g;
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) {
+    V q;
+    q.x = tab[64 + 2 * k];
+    q.y = tab[65 + 2 * k];
+    const V h = cmul(f, q);
+    const bool on = (tm >> k) & 1u;
+    f.x = on ? h.x : f.x;
+    f.y = on ? h.y : f.y;
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    V d;
+    d.x = tab[2 * r];
+    d.y = tab[2 * r + 1];
+    a[r] = cmul(a[r], cmul(d, f));
+  }
+}
+
+// Per-tile factor of a diagonal: product over the 4-bit groups of the tile
+// number (wave-uniform; the table is read with scalar loads).
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec2<T>::type *tab,
+                                                              uint32_t ngrp, uint64_t tile) {
+  using V = typename Vec2<T>::type;
+  V f;
+  f.x = T(1);
+  f.y = T(0);
+  for (uint32_t g = 0; g < ngrp; ++g) f = cmul(f, tab[g * 16 + ((tile >> (4 * g)) & 15u)]);
+  return f;
+}
+
+template <typename T, int RB>
+__global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict__ st,
+                                             const RotArgs<T> A) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const uint32_t tid = threadIdx.x;
+  uint64_t vrank = blockIdx.x;
+  if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
+    vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  const uint32_t last = A.nseg - 1;
+  for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
+    const uint64_t base = tile_base(A, tile);
+    V a[R];
+    for (uint32_t s = 0;; ++s) {
+      const TileSeg &sg = A.seg[s];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+      if (s == 0) {
+        uint64_t gstr[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+        if (A.has_pre)
+          dtab_apply_g<T, RB>(a, A.pre, p.tm, tile_factor<T>(A.tile_tab, A.ntgrp, tile));
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
+      }
+      const uint32_t gm = A.gmask[s];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        if ((gm >> j) & 1u) {
+          const T sa = A.shear[s][j][0], sb = A.shear[s][j][1];
+          switch (j) {
+            case 0: shear_pairs<T, RB, 0>(a, sa, sb); break;
+            case 1: if constexpr (RB > 1) shear_pairs<T, RB, 1>(a, sa, sb); break;
+            case 2: if constexpr (RB > 2) shear_pairs<T, RB, 2>(a, sa, sb); break;
+            case 3: if constexpr (RB > 3) shear_pairs<T, RB, 3>(a, sa, sb); break;
+            case 4: if constexpr (RB > 4) shear_pairs<T, RB, 4>(a, sa, sb); break;
+            default: break;
           }
         }
-        // Recompute the store addresses after the ops instead of keeping the
-        // 32 load addresses (64 VGPRs) live across them.
-        uint64_t gbs = gb;
-        asm volatile("" : "+v"(gbs));
-#pragma unroll
-        for (int r = 0; r < R; ++r) st[gbs + reg_goff<RB>(r, gstr)] = a[r];
-      } else {
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)] = a[r];
-        __syncthreads();
       }
+      if (s == last) break;
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)] = a[r];
+      __syncthreads();
+    }
+    {
+      const TileSeg &sg = A.seg[last];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+      if (A.has_post)
+        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(A.tile_tab + A.ntgrp * 16, A.ntgrp, tile));
+      uint64_t gstr[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) st_amp(st + p.gb + reg_goff<RB>(r, gstr), a[r]);
     }
   }
 }
@@ -352,11 +559,11 @@ template <typename T>
 __device__ __forceinline__ void load_pack(const typename Vec2<T>::type *st, uint64_t pk,
                                           typename Vec2<T>::type *amp) {
   if constexpr (Pack<T>::kAmps == 2) {
-    const float4 p = reinterpret_cast<const float4 *>(st)[pk];
+    const float4 p = ld_f4(reinterpret_cast<const float4 *>(st) + pk);
     amp[0] = make_float2(p.x, p.y);
     amp[1] = make_float2(p.z, p.w);
   } else {
-    amp[0] = st[pk];
+    amp[0] = ld_amp(st + pk);
   }
 }
 
@@ -364,9 +571,9 @@ template <typename T>
 __device__ __forceinline__ void store_pack(typename Vec2<T>::type *st, uint64_t pk,
                                            const typename Vec2<T>::type *amp) {
   if constexpr (Pack<T>::kAmps == 2) {
-    reinterpret_cast<float4 *>(st)[pk] = make_float4(amp[0].x, amp[0].y, amp[1].x, amp[1].y);
+    st_f4(reinterpret_cast<float4 *>(st) + pk, make_float4(amp[0].x, amp[0].y, amp[1].x, amp[1].y));
   } else {
-    st[pk] = amp[0];
+    st_amp(st + pk, amp[0]);
   }
 }
 
@@ -443,9 +650,9 @@ __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type
         const uint64_t y0 = pk * KA;
         uint64_t s0 = c;
         for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
-        amp[u][0] = src[s0];
+        amp[u][0] = ld_amp(src + s0);
         // y0 is even when KA = 2: src(y0 + 1) = src(y0) ^ column of index bit 0
-        if constexpr (KA == 2) amp[u][1] = src[s0 ^ tab[1]];
+        if constexpr (KA == 2) amp[u][1] = ld_amp(src + (s0 ^ tab[1]));
       }
     }
 #pragma unroll
@@ -775,14 +982,47 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
   using V = typename Vec2<T>::type;
   const uint32_t threads = 1u << (args.nlocal - rb);
   const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  // The prefetching form keeps two register sets: built for 16 complex64
+  // amplitudes per thread only (the 512-thread tile).
+  const bool pf = args.prefetch != 0 && std::is_same<T, float>::value && rb == 4;
   switch (rb) {
     case 1: hipLaunchKernelGGL((k_tile<T, 1>), dim3(grid), dim3(threads), lds, stream, state, args); break;
     case 2: hipLaunchKernelGGL((k_tile<T, 2>), dim3(grid), dim3(threads), lds, stream, state, args); break;
     case 3: hipLaunchKernelGGL((k_tile<T, 3>), dim3(grid), dim3(threads), lds, stream, state, args); break;
-    case 4: hipLaunchKernelGGL((k_tile<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 4:
+      if constexpr (std::is_same<T, float>::value) {
+        if (pf) {
+          hipLaunchKernelGGL((k_tile_pf<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args);
+          break;
+        }
+      }
+      hipLaunchKernelGGL((k_tile<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args);
+      break;
     case 5:
       if constexpr (TileCfg<T>::kRB >= 5) {
         hipLaunchKernelGGL((k_tile<T, 5>), dim3(grid), dim3(threads), lds, stream, state, args);
+        break;
+      }
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
+                      hipStream_t stream) {
+  using V = typename Vec2<T>::type;
+  const uint32_t threads = 1u << (args.nlocal - rb);
+  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  switch (rb) {
+    case 1: hipLaunchKernelGGL((k_rot<T, 1>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 2: hipLaunchKernelGGL((k_rot<T, 2>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 3: hipLaunchKernelGGL((k_rot<T, 3>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 4: hipLaunchKernelGGL((k_rot<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 5:
+      if constexpr (TileCfg<T>::kRB >= 5) {
+        hipLaunchKernelGGL((k_rot<T, 5>), dim3(grid), dim3(threads), lds, stream, state, args);
         break;
       }
       return hipErrorInvalidValue;
@@ -893,6 +1133,8 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
 #define MSV_INST_T(T)                                                                            \
   template hipError_t launch_tile<T>(Vec2<T>::type *, const TileArgs<T> &, int, uint32_t,        \
                                      hipStream_t);                                               \
+  template hipError_t launch_rot<T>(Vec2<T>::type *, const RotArgs<T> &, int, uint32_t,          \
+                                    hipStream_t);                                                \
   template hipError_t launch_phase<T>(Vec2<T>::type *, const Vec2<T>::type *, const PhaseArgs &, \
                                       uint32_t, hipStream_t);                                    \
   template hipError_t launch_xor_gather<T>(const Vec2<T>::type *, Vec2<T>::type *, uint64_t,     \

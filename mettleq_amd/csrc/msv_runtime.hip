@@ -56,9 +56,10 @@ struct Tuning {
   int tile_blocks_per_cu = 64;
   int elem_blocks_per_cu = 128;
   int col_bits_c64 = 4;     // contiguous low index bits per tile row in high sweeps
-  int rot_split_min = 13;   // min gates per sweep for the rotation split (0 = never)
+  int rot_split_min = 2;    // min active gates of a 1q layer for the rotation-split form (0 = never)
   int xcd_groups = 8;       // tile-to-block mapping groups (gfx950: 8 XCDs); 0/1 = plain
   int reg_bits_c64 = 4;     // register bits per thread in c64 tile sweeps (4: 512-thread tiles)
+  int tile_prefetch = 0;    // c64 512-thread tiles: next tile's loads overlap the current tile
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -207,15 +208,12 @@ int check_qubit(const msv_state *s, int q, const char *what) {
 // ---------------------------------------------------------------------------
 
 struct PlanOp {
-  uint8_t kind = 0;   // OP_U2 / OP_QFT_F / OP_QFT_I / OP_DIAG / OP_DTAB / OP_RU2
-  int tbit = -1;      // target index bit (U2, QFT, RU2)
+  uint8_t kind = 0;   // OP_U2 / OP_QFT_F / OP_QFT_I / OP_DIAG
+  int tbit = -1;      // target index bit (U2, QFT)
   uint64_t cmask = 0; // controls (U2), index bits
   int abit = -1, bbit = -1;  // DIAG bits (bbit < 0: one-bit diagonal)
   int lb = 0;         // QFT ladder low bit
   double m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // DTAB: the diagonal is cst * prod over local bits k that are set of ratio[k]
-  std::vector<std::complex<double>> ratio;
-  std::complex<double> cst = 1.0;
 };
 
 struct SegPlan {
@@ -396,13 +394,11 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
   bool has_qft = false;
   for (const SegPlan &sp : plan.segs)
     for (const PlanOp &po : sp.ops) has_qft |= po.kind == OP_QFT_F || po.kind == OP_QFT_I;
-  int tab_used = 0;
   if (has_qft) {
     for (int m = 0; m < 32; ++m) {
       A.tab[2 * m] = static_cast<T>(std::cos(2.0 * M_PI * m / 32.0));
       A.tab[2 * m + 1] = static_cast<T>(std::sin(2.0 * M_PI * m / 32.0));
     }
-    tab_used = 64;
   }
   if (nlocal - rb > kTileThreadBits)
     return fail(MSV_E_INVALID, "internal: tile larger than one block");
@@ -468,32 +464,6 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
         op.jl = static_cast<uint32_t>(jl);
         op.pbit = static_cast<uint32_t>(p);
         op.lb = static_cast<uint32_t>(po.lb);
-      } else if (po.kind == OP_RU2) {
-        const int j = reg_pos(po.tbit);
-        if (j < 0) return fail(MSV_E_INVALID, "internal: op target is not a register bit");
-        op.rpos = static_cast<int8_t>(j);
-      } else if (po.kind == OP_DTAB) {
-        if (tab_used + kDtabVals > TileCfg<T>::kTabVals)
-          return fail(MSV_E_INVALID, "internal: tile table pool exhausted");
-        if (static_cast<int>(po.ratio.size()) != nlocal)
-          return fail(MSV_E_INVALID, "internal: diagonal table size");
-        op.lb = static_cast<uint32_t>(tab_used);
-        T *tab = A.tab + tab_used;
-        for (int r = 0; r < (1 << rb); ++r) {
-          std::complex<double> d = po.cst;
-          for (int j = 0; j < rb; ++j)
-            if ((r >> j) & 1) d *= po.ratio[sp.reg0 + j];
-          tab[2 * r] = static_cast<T>(d.real());
-          tab[2 * r + 1] = static_cast<T>(d.imag());
-        }
-        for (int r = 1 << rb; r < 32; ++r) tab[2 * r] = tab[2 * r + 1] = T(0);
-        for (int k = 0; k < kTileThreadBits; ++k) {
-          std::complex<double> q = 1.0;
-          if (k < nlocal - rb) q = po.ratio[sg.tl_bit[k]];
-          tab[64 + 2 * k] = static_cast<T>(q.real());
-          tab[65 + 2 * k] = static_cast<T>(q.imag());
-        }
-        tab_used += kDtabVals;
       } else if (po.kind == OP_DIAG) {
         op.nbits = po.bbit >= 0 ? 2u : 1u;
         const int ja = reg_pos(po.abit);
@@ -508,7 +478,18 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     }
     sg.op_end = static_cast<uint32_t>(nops);
   }
-  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+  uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+  // The prefetching form runs a resident grid (two 512-thread blocks per CU,
+  // the LDS limit) and pays one discarded tile load per block, so it is used
+  // only when every block gets at least 32 tiles.
+  A.prefetch = 0;
+  if (g_tune.tile_prefetch && std::is_same<T, float>::value && rb == 4) {
+    const uint64_t resident = static_cast<uint64_t>(s->cu) * 2;
+    if (A.n_tiles >= 32 * resident) {
+      A.prefetch = 1;
+      cap = resident;
+    }
+  }
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
   MSV_HIP(launch_tile<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
   s->sweeps += 1;
@@ -614,48 +595,15 @@ bool split_rotation(const double *m, RotSplit *z) {
   return true;
 }
 
-// One commuting 1q sweep over `local` with ops on the active gate bits.
-// For complex64 with enough gates the ops are split into real rotations
-// between two segment diagonals (4 FMAs per amplitude per gate instead of
-// 8, plus two diagonal passes); otherwise general 2x2 ops.
+// One commuting 1q sweep over `local` with general 2x2 ops on the active
+// gate bits (the form for matrices that do not split into diagonals and a
+// real rotation, e.g. non-unitary ones).
 template <typename T>
 int run_1q_sweep(msv_state *s, const std::vector<int> &local, const std::vector<int> &gate_locals,
                  const double *mats) {
   const int n = s->n;
-  const int nlocal = static_cast<int>(local.size());
   const int k = static_cast<int>(gate_locals.size());
   if (k == 0) return MSV_OK;
-  std::vector<RotSplit> zs(k);
-  bool split = TileCfg<T>::kTabVals >= 2 * kDtabVals && g_tune.rot_split_min > 0 &&
-               k >= g_tune.rot_split_min;
-  for (int i = 0; split && i < k; ++i) split = split_rotation(mats + 8 * (n - 1 - local[gate_locals[i]]), &zs[i]);
-  if (split) {
-    PlanOp pre, post;
-    pre.kind = post.kind = OP_DTAB;
-    pre.ratio.assign(nlocal, 1.0);
-    post.ratio.assign(nlocal, 1.0);
-    bool pre_on = false;
-    for (int i = 0; i < k; ++i) {
-      pre.ratio[gate_locals[i]] = zs[i].b2;
-      post.ratio[gate_locals[i]] = zs[i].b1 / zs[i].a1;
-      post.cst *= zs[i].a1;
-      pre_on |= std::abs(zs[i].b2 - 1.0) > 0.0;
-    }
-    {
-      std::vector<std::pair<int, PlanOp>> ops;
-      for (int i = 0; i < k; ++i) {
-        PlanOp po;
-        po.kind = OP_RU2;
-        po.tbit = local[gate_locals[i]];
-        po.m[0] = zs[i].c;
-        po.m[1] = -zs[i].s;
-        po.m[2] = zs[i].s;
-        po.m[3] = zs[i].c;
-        ops.push_back({gate_locals[i], po});
-      }
-      return run_commuting_sweep<T>(s, local, ops, pre_on ? &pre : nullptr, &post);
-    }
-  }
   std::vector<std::pair<int, PlanOp>> ops;
   for (int i = 0; i < k; ++i) {
     PlanOp po;
@@ -668,6 +616,200 @@ int run_1q_sweep(msv_state *s, const std::vector<int> &local, const std::vector<
   return run_commuting_sweep<T>(s, local, ops);
 }
 
+// Tile geometry shared by every tile-sweep argument block: the runs that
+// scatter the tile number onto the non-local index bits, and, per register
+// window, where each thread-id bit and register bit lands.
+template <typename Args>
+int fill_tile_geometry(int n, const std::vector<int> &local, const std::vector<int> &windows, int rb,
+                       Args *A) {
+  const int nlocal = static_cast<int>(local.size());
+  if (nlocal - rb > kTileThreadBits)
+    return fail(MSV_E_INVALID, "internal: tile larger than one block");
+  if (static_cast<int>(windows.size()) > kMaxSegs)
+    return fail(MSV_E_INVALID, "internal: too many tile segments");
+  uint64_t lmask = 0;
+  for (int b : local) lmask |= 1ull << b;
+  int src = 0, nr = 0;
+  for (int b = 0; b < n;) {
+    if (lmask >> b & 1ull) {
+      ++b;
+      continue;
+    }
+    int e = b;
+    while (e < n && !(lmask >> e & 1ull)) ++e;
+    if (nr >= kMaxRuns) return fail(MSV_E_INVALID, "internal: too many non-local bit runs");
+    A->run_src[nr] = static_cast<uint32_t>(src);
+    A->run_width[nr] = static_cast<uint32_t>(e - b);
+    A->run_dst[nr] = static_cast<uint32_t>(b);
+    src += e - b;
+    ++nr;
+    b = e;
+  }
+  A->nruns = static_cast<uint32_t>(nr);
+  A->nlocal = static_cast<uint32_t>(nlocal);
+  A->nseg = static_cast<uint32_t>(windows.size());
+  A->n_tiles = 1ull << (n - nlocal);
+  A->xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
+  for (size_t si = 0; si < windows.size(); ++si) {
+    const int w = windows[si];
+    if (w < 0 || w + rb > nlocal) return fail(MSV_E_INVALID, "internal: register window outside the tile");
+    TileSeg &sg = A->seg[si];
+    sg.reg0 = static_cast<uint32_t>(w);
+    for (int j = 0; j < rb; ++j) sg.gstr[j] = 1ull << local[w + j];
+    sg.tmask = (1u << (nlocal - rb)) - 1u;
+    for (int k = 0, t = 0; k < nlocal; ++k) {
+      if (k >= w && k < w + rb) continue;
+      sg.tl_bit[t] = static_cast<uint32_t>(k);
+      sg.gi_bit[t] = static_cast<uint32_t>(local[k]);
+      ++t;
+    }
+  }
+  return MSV_OK;
+}
+
+// Diagonal prod_b ratio[b]^bit_b(i) * cst over all n index bits, laid out for
+// segment `sg` of a tile: register table (constant included) and thread-bit
+// factors in `tab` (register entries at 0..63, thread-bit factors at
+// 64..81, as dtab_apply_g reads them), tile-number groups in `tgrp`
+// (ntgrp x 16 entries).
+template <typename T, typename Args>
+void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &local, int rb,
+                      const std::vector<std::complex<double>> &ratio, std::complex<double> cst,
+                      T *tab, int ntgrp, std::vector<typename Vec2<T>::type> *tgrp) {
+  using C = std::complex<double>;
+  using V = typename Vec2<T>::type;
+  const int nlocal = static_cast<int>(local.size());
+  for (int r = 0; r < 32; ++r) {
+    C d = 0.0;
+    if (r < (1 << rb)) {
+      d = cst;
+      for (int j = 0; j < rb; ++j)
+        if ((r >> j) & 1) d *= ratio[local[sg.reg0 + j]];
+    }
+    tab[2 * r] = static_cast<T>(d.real());
+    tab[2 * r + 1] = static_cast<T>(d.imag());
+  }
+  for (int k = 0; k < kTileThreadBits; ++k) {
+    C q = 1.0;
+    if (k < nlocal - rb) q = ratio[local[sg.tl_bit[k]]];
+    tab[64 + 2 * k] = static_cast<T>(q.real());
+    tab[65 + 2 * k] = static_cast<T>(q.imag());
+  }
+  // tile-number bit t lands on index bit run_dst + (t - run_src)
+  std::vector<int> tbit_index;
+  for (uint32_t r = 0; r < A.nruns; ++r)
+    for (uint32_t w = 0; w < A.run_width[r]; ++w) tbit_index.push_back(static_cast<int>(A.run_dst[r] + w));
+  for (int g = 0; g < ntgrp; ++g) {
+    for (int v = 0; v < 16; ++v) {
+      C f = 1.0;
+      for (int i = 0; i < 4; ++i) {
+        const int t = 4 * g + i;
+        if (((v >> i) & 1) && t < static_cast<int>(tbit_index.size())) f *= ratio[tbit_index[t]];
+      }
+      V e;
+      e.x = static_cast<T>(f.real());
+      e.y = static_cast<T>(f.imag());
+      tgrp->push_back(e);
+    }
+  }
+}
+
+// A 1q layer whose active gates all split as U = diag(a1, b1) R diag(1, b2)
+// (Z-Y-Z form, R a real rotation): every sweep applies only the rotations,
+// as three shears each (k_rot), and the layer's two diagonals, over all n
+// index bits, are applied once: diag(1, b2) on every gate bit before the
+// first sweep's rotations, diag(a1, b1) after the last sweep's. The
+// diagonals commute with the rotations on other bits, so this equals the
+// gate-by-gate layer.
+template <typename T>
+int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vector<RotSplit> &zb) {
+  using V = typename Vec2<T>::type;
+  using C = std::complex<double>;
+  const int n = s->n;
+  int L = 0;
+  std::vector<std::pair<int, int>> chunks;
+  partition<T>(n, &L, &chunks);
+  struct Sw {
+    std::vector<int> local;
+    std::vector<int> gl;  // local positions of the gate bits
+  };
+  std::vector<Sw> sweeps;
+  {
+    Sw w;
+    w.local = sweep_local_bits<T>(n, L, true, {0, 0});
+    for (int k = 0; k < static_cast<int>(w.local.size()); ++k)
+      if (act_bit[w.local[k]]) w.gl.push_back(k);
+    if (!w.gl.empty()) sweeps.push_back(w);
+  }
+  for (const std::vector<int> &gates : high_gate_sets<T>(chunks)) {
+    Sw w;
+    w.local = high_sweep_local_bits<T>(gates);
+    for (int k = 0; k < static_cast<int>(w.local.size()); ++k)
+      if (w.local[k] >= L && act_bit[w.local[k]]) w.gl.push_back(k);
+    if (!w.gl.empty()) sweeps.push_back(w);
+  }
+  if (sweeps.empty()) return MSV_OK;
+  std::vector<C> pre(n, 1.0), post(n, 1.0);
+  C cst = 1.0;
+  bool pre_on = false;
+  for (int b = 0; b < n; ++b) {
+    if (!act_bit[b]) continue;
+    pre[b] = zb[b].b2;
+    post[b] = zb[b].b1 / zb[b].a1;
+    cst *= zb[b].a1;
+    pre_on |= std::abs(zb[b].b2 - 1.0) > 0.0;
+  }
+  for (size_t i = 0; i < sweeps.size(); ++i) {
+    const Sw &sw = sweeps[i];
+    const int nlocal = static_cast<int>(sw.local.size());
+    const int rb = std::min(rb_of<T>(), nlocal);
+    uint32_t need = 0;
+    for (int k : sw.gl) need |= 1u << k;
+    const std::vector<int> wins = commuting_windows(need, nlocal, rb, good_from(nlocal, rb));
+    if (wins.empty()) return fail(MSV_E_INVALID, "internal: no register window cover");
+    RotArgs<T> A;
+    std::memset(&A, 0, sizeof(A));
+    MSV_TRY(fill_tile_geometry(n, sw.local, wins, rb, &A));
+    // each gate bit rotates in the first window that holds it
+    std::vector<bool> done(nlocal, false);
+    for (size_t si = 0; si < wins.size(); ++si) {
+      for (int k : sw.gl) {
+        if (done[k] || k < wins[si] || k >= wins[si] + rb) continue;
+        done[k] = true;
+        const int j = k - wins[si];
+        const RotSplit &z = zb[sw.local[k]];
+        // [[c, -s], [s, c]] = X(a) Y(b) X(a): a = -tan(theta/2) = -s/(1+c), b = s
+        A.gmask[si] |= 1u << j;
+        A.shear[si][j][0] = static_cast<T>(-z.s / (1.0 + z.c));
+        A.shear[si][j][1] = static_cast<T>(z.s);
+      }
+    }
+    const int ntgrp = (n - nlocal + 3) / 4;
+    if (ntgrp > kMaxTileGroups) return fail(MSV_E_INVALID, "internal: too many tile-number bits");
+    A.ntgrp = static_cast<uint32_t>(ntgrp);
+    std::vector<V> tgrp;
+    tgrp.reserve(2 * 16 * ntgrp);
+    std::vector<C> one(n, 1.0);
+    A.has_pre = (i == 0 && pre_on) ? 1u : 0u;
+    build_layer_diag<T>(A, A.seg[0], sw.local, rb, A.has_pre ? pre : one, 1.0, A.pre, ntgrp, &tgrp);
+    A.has_post = (i + 1 == sweeps.size()) ? 1u : 0u;
+    build_layer_diag<T>(A, A.seg[wins.size() - 1], sw.local, rb, A.has_post ? post : one,
+                        A.has_post ? cst : C(1.0), A.post, ntgrp, &tgrp);
+    if (ntgrp > 0 && (A.has_pre || A.has_post)) {
+      void *d = nullptr;
+      MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &d));
+      A.tile_tab = reinterpret_cast<const V *>(d);
+    } else {
+      A.ntgrp = 0;
+    }
+    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+    s->sweeps += 1;
+  }
+  return MSV_OK;
+}
+
 template <typename T>
 int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   const int n = s->n;
@@ -675,8 +817,20 @@ int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   std::vector<std::pair<int, int>> chunks;
   partition<T>(n, &L, &chunks);
   std::vector<bool> act_bit(n, false);
+  int n_act = 0;
   for (int q = 0; q < n; ++q)
-    if (!active || active[q]) act_bit[bit_of(s, q)] = true;
+    if (!active || active[q]) {
+      act_bit[bit_of(s, q)] = true;
+      ++n_act;
+    }
+  // rotation-split form when every active gate is a (scaled) unitary
+  if (g_tune.rot_split_min > 0 && n_act >= g_tune.rot_split_min) {
+    std::vector<RotSplit> zb(n);
+    bool ok = true;
+    for (int b = 0; ok && b < n; ++b)
+      if (act_bit[b]) ok = split_rotation(mats + 8 * (n - 1 - b), &zb[b]);
+    if (ok) return run_rot_layer<T>(s, act_bit, zb);
+  }
   // low sweep
   {
     std::vector<int> local = sweep_local_bits<T>(n, L, true, {0, 0});
@@ -2010,6 +2164,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value != 4 && value != 5) return fail(MSV_E_INVALID, "tile register bits must be 4 or 5");
       g_tune.reg_bits_c64 = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_TILE_PREFETCH:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "tile prefetch must be 0 or 1");
+      g_tune.tile_prefetch = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2028,6 +2186,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_ROT_SPLIT_MIN: *value = g_tune.rot_split_min; return MSV_OK;
     case MSV_TUNE_XCD_GROUPS: *value = g_tune.xcd_groups; return MSV_OK;
     case MSV_TUNE_TILE_REG_BITS: *value = g_tune.reg_bits_c64; return MSV_OK;
+    case MSV_TUNE_TILE_PREFETCH: *value = g_tune.tile_prefetch; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

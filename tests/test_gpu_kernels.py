@@ -465,3 +465,64 @@ def test_invalid_arguments_raise_value_error(gpu):
             s.probabilities([0, 0])
         with pytest.raises(ValueError):
             s.apply_xor_affine([1, 1, 1, 1], 0)  # singular over GF(2)
+
+
+def test_tile_prefetch_bit_identical_at_28q(gpu):
+    """The prefetching tile sweep (next tile's loads overlap the current
+    tile; used from 2^15 tiles up) only reorders HBM traffic: a 28-qubit
+    U3 layer + QFT + U3 layer + generic gate list must give bit-identical
+    amplitudes with prefetch on and off, with the norm kept."""
+    from mettleq_amd import _native as nat
+    n = 28
+    rng = np.random.default_rng(2828)
+    m1 = [_rand_u(rng) for _ in range(n)]
+    m2 = [_rand_u(rng) for _ in range(n)]
+    from mettleq_amd.native_state import gate_u
+    gates = [gate_u(int(rng.integers(n)), _rand_u(rng)) for _ in range(20)]
+    old_pf = nat.get_tuning(nat.MSV_TUNE_TILE_PREFETCH)
+    old_rb = nat.get_tuning(nat.MSV_TUNE_TILE_REG_BITS)
+    outs = []
+    try:
+        nat.set_tuning(nat.MSV_TUNE_TILE_REG_BITS, 4)
+        for pf in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_TILE_PREFETCH, pf)
+            with _state(n) as s:
+                s.apply_1q_layer(m1)
+                s.apply_qft(0, n)
+                s.apply_1q_layer(m2)
+                s.apply_gates(gates)
+                outs.append(s.to_numpy())
+                assert abs(s.norm2() - 1.0) < 1e-5
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_TILE_PREFETCH, old_pf)
+        nat.set_tuning(nat.MSV_TUNE_TILE_REG_BITS, old_rb)
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("split_min", [0, 1, 4, 13])
+@pytest.mark.parametrize("n", [9, 15, 21])
+def test_1q_layer_rotation_split_thresholds(gpu, split_min, n):
+    """1q layers with the Z-Y-Z split (two table diagonals around fast
+    Givens rotations, both forms |tan| <= 1 and |cot| < 1) from `split_min`
+    gates per sweep up, against the dense oracle; includes angles near 0,
+    pi/2 and pi where the form switches."""
+    from mettleq_amd import _native as nat
+    rng = np.random.default_rng(900 + n + split_min)
+    psi = _rand_state(n, 3 * n)
+    special = [0.0, 1e-7, math.pi / 2 - 1e-7, math.pi / 2, math.pi / 2 + 1e-7, math.pi - 1e-7, math.pi]
+    mats = []
+    for q in range(n):
+        t = special[q % len(special)] if q % 2 == 0 else rng.uniform(0, 2 * math.pi)
+        p, l = rng.uniform(0, 2 * math.pi, 2)
+        mats.append(O.U3(t, p, l))
+    want = O.apply_1q_layer(psi, n, mats)
+    old = nat.get_tuning(nat.MSV_TUNE_ROT_SPLIT_MIN)
+    nat.set_tuning(nat.MSV_TUNE_ROT_SPLIT_MIN, split_min)
+    try:
+        with _state(n) as s:
+            s.from_numpy(psi)
+            s.apply_1q_layer(mats)
+            got = s.to_numpy()
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_ROT_SPLIT_MIN, old)
+    assert np.max(np.abs(got - want)) < TOL64
