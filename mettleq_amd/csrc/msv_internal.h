@@ -143,6 +143,33 @@ static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4
 static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");
 constexpr int kMaxTileGroups = 8;  // tile numbers up to 32 bits
 
+// QFT sweep (k_qft): per segment, QFT stages on the register bits set in
+// smask, the window holding consecutive index bits p0 .. p0 + RB - 1 with
+// p0 >= lb. The stage at register bit J (index bit p = p0 + J) puts the
+// phase exp(i pi L / 2^(p - lb)) on its |1> branch, L = index bits [lb, p):
+// a register part exp(i pi v / 2^J) (v = the J register bits below) and a
+// thread part exp(i pi x / 2^wbits[J]), x = (gb >> lb) & (2^xbits - 1), the
+// thread and tile bits below the window.
+template <typename T>
+struct QftArgs {
+  uint64_t n_tiles;
+  uint32_t nlocal;
+  uint32_t nseg;
+  uint32_t nruns;
+  uint32_t xcd_groups;
+  uint32_t run_src[kMaxRuns];
+  uint32_t run_width[kMaxRuns];
+  uint32_t run_dst[kMaxRuns];
+  uint32_t smask[kMaxSegs];
+  uint32_t xbits[kMaxSegs];
+  uint32_t wbits[kMaxSegs][5];
+  uint32_t lb;
+  uint32_t pad;
+  T scale;
+  TileSeg seg[kMaxSegs];
+};
+static_assert(sizeof(QftArgs<double>) <= 4096, "QFT sweep arguments exceed 4 KiB");
+
 struct PhaseWin {
   uint8_t s1, w1, s2, w2;  // lut index = ((i>>s1)&(2^w1-1)) | (((i>>s2)&(2^w2-1))<<w1)
   uint32_t off;            // offset of this window's table
@@ -164,6 +191,9 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
                       hipStream_t stream);
+template <typename T>
+hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
+                      uint32_t grid, hipStream_t stream);
 template <typename T>
 hipError_t launch_phase(typename Vec2<T>::type *state, const typename Vec2<T>::type *lut,
                         const PhaseArgs &args, uint32_t grid, hipStream_t stream);

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "msv_device.h"
 #include "msv_internal.h"
@@ -463,7 +464,7 @@ __device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T 
 }
 
 // Per-tile factor of a diagonal: product over the 4-bit groups of the tile
-// number (wave-uniform; the table is read with scalar loads).
+// number (wave-uniform reads of a table staged in LDS).
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec2<T>::type *tab,
                                                               uint32_t ngrp, uint64_t tile) {
@@ -486,6 +487,10 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
   uint64_t vrank = blockIdx.x;
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  // tile-factor tables (pre, then post) behind the exchange buffer
+  V *ttab = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
+  for (uint32_t i = tid; i < 32u * A.ntgrp; i += blockDim.x) ttab[i] = A.tile_tab[i];
+  __syncthreads();
   const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
@@ -500,7 +505,7 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
         if (A.has_pre)
-          dtab_apply_g<T, RB>(a, A.pre, p.tm, tile_factor<T>(A.tile_tab, A.ntgrp, tile));
+          dtab_apply_g<T, RB>(a, A.pre, p.tm, tile_factor<T>(ttab, A.ntgrp, tile));
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -531,7 +536,174 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
       const TileSeg &sg = A.seg[last];
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (A.has_post)
-        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(A.tile_tab + A.ntgrp * 16, A.ntgrp, tile));
+        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(ttab + A.ntgrp * 16, A.ntgrp, tile));
+      uint64_t gstr[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) st_amp(st + p.gb + reg_goff<RB>(r, gstr), a[r]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// QFT sweep
+// ---------------------------------------------------------------------------
+
+// exp(i pi k / 16), k = 0 .. 15, as compile-time constants.
+__host__ __device__ constexpr double cos16(int k) {
+  constexpr double c[9] = {1.0,
+                           0.98078528040323044912618223613424,
+                           0.92387953251128675612818318939679,
+                           0.83146961230254523707878837761791,
+                           0.70710678118654752440084436210485,
+                           0.55557023301960222474283081394853,
+                           0.38268343236508977172845998403040,
+                           0.19509032201612826784828486847702,
+                           0.0};
+  return k <= 8 ? c[k] : -c[16 - k];
+}
+__host__ __device__ constexpr double sin16(int k) { return k <= 8 ? cos16(8 - k) : cos16(k - 8); }
+
+// In-place complex product a *= (c, s) with constant or wave-uniform factors.
+template <typename V, typename T>
+__device__ __forceinline__ void cmul_in(V &a, T c, T s) {
+  const T t = a.x * s;
+  a.x = fma_t(a.x, c, -(a.y * s));
+  a.y = fma_t(a.y, c, t);
+}
+
+// QFT stage on register bit J with thread twiddle wt: forward: butterfly,
+// then the |1> branch times exp(i pi v / 2^J) * wt (v = register bits below
+// J); inverse: the conjugate phase first, then the butterfly. All steps are
+// in place (a0 + a1, then a1 <- a0' - 2 a1), so the stage needs no register
+// moves; the 1/sqrt(2) factors are applied at the store.
+template <typename T, int RB, int J, bool INV>
+__device__ __forceinline__ void qft_stage(typename Vec2<T>::type *a, typename Vec2<T>::type wt) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    constexpr int kShift = 4 - J;
+    const int k = (r & ((1 << J) - 1)) << kShift;  // exp(i pi v / 2^J) = cis16(k)
+    V &a0 = a[r];
+    V &a1 = a[r1];
+    if (!INV) {
+      a0.x = a0.x + a1.x;
+      a0.y = a0.y + a1.y;
+      a1.x = fma_t(T(-2), a1.x, a0.x);
+      a1.y = fma_t(T(-2), a1.y, a0.y);
+      if (k != 0) cmul_in(a1, static_cast<T>(cos16(k)), static_cast<T>(sin16(k)));
+      cmul_in(a1, wt.x, wt.y);
+    } else {
+      cmul_in(a1, wt.x, -wt.y);
+      if (k != 0) cmul_in(a1, static_cast<T>(cos16(k)), static_cast<T>(-sin16(k)));
+      a0.x = a0.x + a1.x;
+      a0.y = a0.y + a1.y;
+      a1.x = fma_t(T(-2), a1.x, a0.x);
+      a1.y = fma_t(T(-2), a1.y, a0.y);
+    }
+  }
+}
+
+template <typename T, int RB, bool INV>
+__device__ __forceinline__ void qft_segment(typename Vec2<T>::type *a, const QftArgs<T> &A, uint32_t s,
+                                            uint64_t gb) {
+  using V = typename Vec2<T>::type;
+  const uint32_t sm = A.smask[s];
+  const uint32_t xb = A.xbits[s];
+  const uint64_t x = (gb >> A.lb) & ((xb >= 64) ? ~0ull : ((1ull << xb) - 1ull));
+  // thread twiddles, top stage first; a stage directly below an active one
+  // squares its twiddle (exp(i pi x / 2^(w-1)) = exp(i pi x / 2^w)^2)
+  V wt[RB];
+#pragma unroll
+  for (int j = RB - 1; j >= 0; --j) {
+    if ((sm >> j) & 1u) {
+      if (j + 1 < RB && ((sm >> (j + 1)) & 1u)) {
+        const V h = wt[j + 1 < RB ? j + 1 : j];
+        wt[j].x = fma_t(h.x, h.x, -(h.y * h.y));
+        wt[j].y = T(2) * h.x * h.y;
+      } else {
+        wt[j] = CisPi<T>::eval(x, A.wbits[s][j]);
+      }
+    }
+  }
+  if (!INV) {
+#pragma unroll
+    for (int j = RB - 1; j >= 0; --j) {
+      if ((sm >> j) & 1u) {
+        switch (j) {
+          case 0: qft_stage<T, RB, 0, false>(a, wt[0]); break;
+          case 1: if constexpr (RB > 1) qft_stage<T, RB, 1, false>(a, wt[j]); break;
+          case 2: if constexpr (RB > 2) qft_stage<T, RB, 2, false>(a, wt[j]); break;
+          case 3: if constexpr (RB > 3) qft_stage<T, RB, 3, false>(a, wt[j]); break;
+          case 4: if constexpr (RB > 4) qft_stage<T, RB, 4, false>(a, wt[j]); break;
+          default: break;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      if ((sm >> j) & 1u) {
+        switch (j) {
+          case 0: qft_stage<T, RB, 0, true>(a, wt[0]); break;
+          case 1: if constexpr (RB > 1) qft_stage<T, RB, 1, true>(a, wt[j]); break;
+          case 2: if constexpr (RB > 2) qft_stage<T, RB, 2, true>(a, wt[j]); break;
+          case 3: if constexpr (RB > 3) qft_stage<T, RB, 3, true>(a, wt[j]); break;
+          case 4: if constexpr (RB > 4) qft_stage<T, RB, 4, true>(a, wt[j]); break;
+          default: break;
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int RB, bool INV>
+__global__ void __launch_bounds__(512) k_qft(typename Vec2<T>::type *__restrict__ st,
+                                             const QftArgs<T> A) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const uint32_t tid = threadIdx.x;
+  uint64_t vrank = blockIdx.x;
+  if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
+    vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  const uint32_t last = A.nseg - 1;
+  for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
+    const uint64_t base = tile_base(A, tile);
+    V a[R];
+    for (uint32_t s = 0;; ++s) {
+      const TileSeg &sg = A.seg[s];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+      if (s == 0) {
+        uint64_t gstr[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
+      }
+      if (A.smask[s]) qft_segment<T, RB, INV>(a, A, s, p.gb);
+      if (s == last) break;
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)] = a[r];
+      __syncthreads();
+    }
+    {
+      const TileSeg &sg = A.seg[last];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        a[r].x *= A.scale;
+        a[r].y *= A.scale;
+      }
       uint64_t gstr[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
@@ -1009,25 +1181,65 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
   return hipGetLastError();
 }
 
+// Launch with more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU):
+// the kernel's dynamic LDS limit is raised once per kernel.
+template <typename... KArgs, typename... Args>
+hipError_t launch_big_lds(void (*kernel)(KArgs...), uint32_t grid, uint32_t threads, size_t lds,
+                          hipStream_t stream, Args &&...args) {
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), lds, stream, std::forward<Args>(args)...);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
                       hipStream_t stream) {
   using V = typename Vec2<T>::type;
   const uint32_t threads = 1u << (args.nlocal - rb);
-  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  // exchange buffer (64 KiB for a full tile) plus the tile-factor tables
+  const size_t lds = (args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0) + sizeof(V) * 32 * args.ntgrp;
   switch (rb) {
-    case 1: hipLaunchKernelGGL((k_rot<T, 1>), dim3(grid), dim3(threads), lds, stream, state, args); break;
-    case 2: hipLaunchKernelGGL((k_rot<T, 2>), dim3(grid), dim3(threads), lds, stream, state, args); break;
-    case 3: hipLaunchKernelGGL((k_rot<T, 3>), dim3(grid), dim3(threads), lds, stream, state, args); break;
-    case 4: hipLaunchKernelGGL((k_rot<T, 4>), dim3(grid), dim3(threads), lds, stream, state, args); break;
+    case 1: return launch_big_lds(k_rot<T, 1>, grid, threads, lds, stream, state, args);
+    case 2: return launch_big_lds(k_rot<T, 2>, grid, threads, lds, stream, state, args);
+    case 3: return launch_big_lds(k_rot<T, 3>, grid, threads, lds, stream, state, args);
+    case 4: return launch_big_lds(k_rot<T, 4>, grid, threads, lds, stream, state, args);
+    case 5:
+      if constexpr (TileCfg<T>::kRB >= 5) return launch_big_lds(k_rot<T, 5>, grid, threads, lds, stream, state, args);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
+                      uint32_t grid, hipStream_t stream) {
+  using V = typename Vec2<T>::type;
+  const uint32_t threads = 1u << (args.nlocal - rb);
+  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+#define MSV_QFT_LAUNCH(RBV)                                                                        \
+  if (inverse)                                                                                     \
+    hipLaunchKernelGGL((k_qft<T, RBV, true>), dim3(grid), dim3(threads), lds, stream, state, args); \
+  else                                                                                             \
+    hipLaunchKernelGGL((k_qft<T, RBV, false>), dim3(grid), dim3(threads), lds, stream, state, args);
+  switch (rb) {
+    case 1: MSV_QFT_LAUNCH(1) break;
+    case 2: MSV_QFT_LAUNCH(2) break;
+    case 3: MSV_QFT_LAUNCH(3) break;
+    case 4: MSV_QFT_LAUNCH(4) break;
     case 5:
       if constexpr (TileCfg<T>::kRB >= 5) {
-        hipLaunchKernelGGL((k_rot<T, 5>), dim3(grid), dim3(threads), lds, stream, state, args);
+        MSV_QFT_LAUNCH(5)
         break;
       }
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
+#undef MSV_QFT_LAUNCH
   return hipGetLastError();
 }
 
@@ -1134,6 +1346,8 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
   template hipError_t launch_tile<T>(Vec2<T>::type *, const TileArgs<T> &, int, uint32_t,        \
                                      hipStream_t);                                               \
   template hipError_t launch_rot<T>(Vec2<T>::type *, const RotArgs<T> &, int, uint32_t,          \
+                                    hipStream_t);                                                \
+  template hipError_t launch_qft<T>(Vec2<T>::type *, const QftArgs<T> &, int, bool, uint32_t,    \
                                     hipStream_t);                                                \
   template hipError_t launch_phase<T>(Vec2<T>::type *, const Vec2<T>::type *, const PhaseArgs &, \
                                       uint32_t, hipStream_t);                                    \

@@ -849,6 +849,46 @@ int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   return MSV_OK;
 }
 
+// A QFT sweep plan runs as k_qft when every segment with stages holds
+// consecutive index bits at or above lb in its registers (then the stage
+// twiddles split into a register part known at compile time and one thread
+// part per stage); otherwise through the generic tile program.
+template <typename T>
+int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
+  using V = typename Vec2<T>::type;
+  const int nlocal = static_cast<int>(plan.local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  QftArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  std::vector<int> wins;
+  bool ok = static_cast<int>(plan.segs.size()) <= kMaxSegs;
+  for (size_t si = 0; ok && si < plan.segs.size(); ++si) {
+    const SegPlan &sp = plan.segs[si];
+    wins.push_back(sp.reg0);
+    if (sp.ops.empty()) continue;
+    const int p0 = plan.local[sp.reg0];
+    for (int j = 0; j < rb; ++j) ok &= plan.local[sp.reg0 + j] == p0 + j;
+    ok &= p0 >= lb;
+    for (const PlanOp &po : sp.ops) {
+      const int j = po.tbit - p0;
+      ok &= (po.kind == OP_QFT_F || po.kind == OP_QFT_I) && j >= 0 && j < rb;
+      if (ok) A.smask[si] |= 1u << j;
+    }
+    if (!ok) break;
+    A.xbits[si] = static_cast<uint32_t>(p0 - lb);
+    for (int j = 0; j < rb; ++j) A.wbits[si][j] = static_cast<uint32_t>(p0 + j - lb);
+  }
+  if (!ok) return launch_sweep<T>(s, plan);
+  MSV_TRY(fill_tile_geometry(s->n, plan.local, wins, rb, &A));
+  A.lb = static_cast<uint32_t>(lb);
+  A.scale = static_cast<T>(plan.scale);
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+  MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
 // QFT (no terminal swaps) on index bits [lb, hb]: forward stages run from
 // hb down to lb; the inverse runs the adjoint stages from lb up to hb.
 template <typename T>
@@ -923,7 +963,7 @@ int apply_qft_T(msv_state *s, int hb, int lb, bool inverse, int only_stage_bit) 
       plan.segs.push_back(st);
     }
     plan.scale = std::pow(2.0, -0.5 * static_cast<double>(stage_bits.size()));
-    MSV_TRY(launch_sweep<T>(s, plan));
+    MSV_TRY(launch_qft_plan<T>(s, plan, lb, inverse));
   }
   return MSV_OK;
 }
