@@ -40,6 +40,7 @@ constexpr int kPhaseLutMax = 4096;   // entries of all windows together
 constexpr int kSampleBlock = 4096;   // amplitudes per sampling block
 constexpr int kSampleLanes = 64;     // lanes per block (one wave64)
 constexpr int kMaxWords = 16;        // Pauli words per reduction launch
+constexpr uint64_t kStreamChunkPacks = 4096;  // packs (16 B) per block iteration of the streaming kernels
 
 enum TileOpKind : uint8_t {
   OP_U2 = 1,     // 2x2 on a register bit, optional controls
@@ -180,7 +181,7 @@ struct PhaseArgs {
   uint32_t nwin;
   uint32_t nlow;           // windows [0, nlow) read index bits 0..1
   uint32_t lut_len;
-  uint32_t pad;
+  uint32_t xcd_groups;     // chunk-to-block mapping groups (as in the tile sweeps)
   PhaseWin win[kMaxPhaseWins];
 };
 

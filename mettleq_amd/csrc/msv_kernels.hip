@@ -749,10 +749,24 @@ __device__ __forceinline__ void store_pack(typename Vec2<T>::type *st, uint64_t 
   }
 }
 
-// Each thread keeps kUnroll packs in flight: iteration t covers packs
-// base + u * nthreads (u < kUnroll), so every load instruction of a wave
-// still reads consecutive addresses.
+// Streaming kernels walk the packs in chunks of kStreamThreads x kChunkPer
+// consecutive packs (64 KiB of complex64), one chunk per block and
+// iteration, in the XCD-grouped order of the tile sweeps; within a chunk a
+// thread keeps kUnroll packs in flight and every load instruction of a
+// wave reads consecutive addresses.
 constexpr int kUnroll = 4;
+constexpr int kStreamThreads = 256;
+constexpr int kChunkPer = 16;
+constexpr uint64_t kChunkPacks = static_cast<uint64_t>(kStreamThreads) * kChunkPer;
+static_assert(kChunkPacks == kStreamChunkPacks, "streaming chunk size");
+constexpr uint32_t kStreamXcdGroups = 8;  // gfx950: 8 XCDs
+
+__device__ __forceinline__ uint64_t stream_vrank(uint32_t groups) {
+  uint64_t vrank = blockIdx.x;
+  if (groups > 1 && gridDim.x % groups == 0)
+    vrank = (blockIdx.x % groups) * (gridDim.x / groups) + blockIdx.x / groups;
+  return vrank;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restrict__ st,
@@ -765,31 +779,34 @@ __global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restric
   __syncthreads();
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = A.n_amps / KA;
-  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
-       base += nthreads * kUnroll) {
-    V amp[kUnroll][KA];
+  const uint64_t nchunks = (npack + kChunkPacks - 1) / kChunkPacks;
+  for (uint64_t c = stream_vrank(A.xcd_groups); c < nchunks; c += gridDim.x) {
+#pragma unroll 1
+    for (int k0 = 0; k0 < kChunkPer; k0 += kUnroll) {
+      const uint64_t base = c * kChunkPacks + static_cast<uint64_t>(k0) * kStreamThreads + threadIdx.x;
+      V amp[kUnroll][KA];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t pk = base + u * nthreads;
-      if (pk < npack) load_pack<T>(st, pk, amp[u]);
-    }
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
+        if (pk < npack) load_pack<T>(st, pk, amp[u]);
+      }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t pk = base + u * nthreads;
-      if (pk < npack) {
-        const uint64_t i0 = pk * KA;
-        V ph;
-        ph.x = T(1);
-        ph.y = T(0);
-        for (uint32_t w = A.nlow; w < A.nwin; ++w) ph = cmul(ph, lut[win_index(A.win[w], i0)]);
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
+        if (pk < npack) {
+          const uint64_t i0 = pk * KA;
+          V ph;
+          ph.x = T(1);
+          ph.y = T(0);
+          for (uint32_t w = A.nlow; w < A.nwin; ++w) ph = cmul(ph, lut[win_index(A.win[w], i0)]);
 #pragma unroll
-        for (int d = 0; d < KA; ++d) {
-          V p = ph;
-          for (uint32_t w = 0; w < A.nlow; ++w) p = cmul(p, lut[win_index(A.win[w], i0 + d)]);
-          amp[u][d] = cmul(amp[u][d], p);
+          for (int d = 0; d < KA; ++d) {
+            V p = ph;
+            for (uint32_t w = 0; w < A.nlow; ++w) p = cmul(p, lut[win_index(A.win[w], i0 + d)]);
+            amp[u][d] = cmul(amp[u][d], p);
+          }
+          store_pack<T>(st, pk, amp[u]);
         }
-        store_pack<T>(st, pk, amp[u]);
       }
     }
   }
@@ -804,33 +821,36 @@ __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type
                                                     typename Vec2<T>::type *__restrict__ dst,
                                                     uint64_t n_amps,
                                                     const uint64_t *__restrict__ tables, int ntab,
-                                                    uint64_t c) {
+                                                    uint64_t c, uint32_t groups) {
   using V = typename Vec2<T>::type;
   __shared__ uint64_t tab[8 * 256];
   for (int i = threadIdx.x; i < ntab * 256; i += blockDim.x) tab[i] = tables[i];
   __syncthreads();
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = n_amps / KA;
-  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
-       base += nthreads * kUnroll) {
-    V amp[kUnroll][KA];
+  const uint64_t nchunks = (npack + kChunkPacks - 1) / kChunkPacks;
+  for (uint64_t ch = stream_vrank(groups); ch < nchunks; ch += gridDim.x) {
+#pragma unroll 1
+    for (int k0 = 0; k0 < kChunkPer; k0 += kUnroll) {
+      const uint64_t base = ch * kChunkPacks + static_cast<uint64_t>(k0) * kStreamThreads + threadIdx.x;
+      V amp[kUnroll][KA];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t pk = base + u * nthreads;
-      if (pk < npack) {
-        const uint64_t y0 = pk * KA;
-        uint64_t s0 = c;
-        for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
-        amp[u][0] = ld_amp(src + s0);
-        // y0 is even when KA = 2: src(y0 + 1) = src(y0) ^ column of index bit 0
-        if constexpr (KA == 2) amp[u][1] = ld_amp(src + (s0 ^ tab[1]));
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
+        if (pk < npack) {
+          const uint64_t y0 = pk * KA;
+          uint64_t s0 = c;
+          for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
+          amp[u][0] = ld_amp(src + s0);
+          // y0 is even when KA = 2: src(y0 + 1) = src(y0) ^ column of index bit 0
+          if constexpr (KA == 2) amp[u][1] = ld_amp(src + (s0 ^ tab[1]));
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t pk = base + u * nthreads;
-      if (pk < npack) store_pack<T>(dst, pk, amp[u]);
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
+        if (pk < npack) store_pack<T>(dst, pk, amp[u]);
+      }
     }
   }
 }
@@ -855,26 +875,29 @@ __global__ void __launch_bounds__(256) k_init_basis(typename Vec2<T>::type *__re
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_scale(typename Vec2<T>::type *__restrict__ st,
-                                               uint64_t n_amps, T re, T im) {
+                                               uint64_t n_amps, T re, T im, uint32_t groups) {
   using V = typename Vec2<T>::type;
   V f;
   f.x = re;
   f.y = im;
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = n_amps / KA;
-  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < npack;
-       base += nthreads * kUnroll) {
-    V amp[kUnroll][KA];
+  const uint64_t nchunks = (npack + kChunkPacks - 1) / kChunkPacks;
+  for (uint64_t ch = stream_vrank(groups); ch < nchunks; ch += gridDim.x) {
+#pragma unroll 1
+    for (int k0 = 0; k0 < kChunkPer; k0 += kUnroll) {
+      const uint64_t base = ch * kChunkPacks + static_cast<uint64_t>(k0) * kStreamThreads + threadIdx.x;
+      V amp[kUnroll][KA];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-      if (base + u * nthreads < npack) load_pack<T>(st, base + u * nthreads, amp[u]);
+      for (int u = 0; u < kUnroll; ++u)
+        if (base + u * kStreamThreads < npack) load_pack<T>(st, base + u * kStreamThreads, amp[u]);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      if (base + u * nthreads < npack) {
+      for (int u = 0; u < kUnroll; ++u) {
+        if (base + u * kStreamThreads < npack) {
 #pragma unroll
-        for (int d = 0; d < KA; ++d) amp[u][d] = cmul(amp[u][d], f);
-        store_pack<T>(st, base + u * nthreads, amp[u]);
+          for (int d = 0; d < KA; ++d) amp[u][d] = cmul(amp[u][d], f);
+          store_pack<T>(st, base + u * kStreamThreads, amp[u]);
+        }
       }
     }
   }
@@ -1257,7 +1280,7 @@ hipError_t launch_xor_gather(const typename Vec2<T>::type *src, typename Vec2<T>
                              uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
                              uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL((k_xor_gather<T>), dim3(grid), dim3(256), 0, stream, src, dst, n_amps, tables,
-                     ntab, c);
+                     ntab, c, kStreamXcdGroups);
   return hipGetLastError();
 }
 
@@ -1272,7 +1295,7 @@ template <typename T>
 hipError_t launch_scale(typename Vec2<T>::type *state, uint64_t n_amps, double re, double im,
                         uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL((k_scale<T>), dim3(grid), dim3(256), 0, stream, state, n_amps,
-                     static_cast<T>(re), static_cast<T>(im));
+                     static_cast<T>(re), static_cast<T>(im), kStreamXcdGroups);
   return hipGetLastError();
 }
 

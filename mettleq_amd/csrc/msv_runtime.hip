@@ -188,6 +188,16 @@ int ensure_alt(msv_state *s) {
   return MSV_OK;
 }
 
+// Grid of the chunked streaming kernels (phase, gather, scale): one block
+// per chunk of kStreamChunkPacks 16-byte packs, at most elem_blocks_per_cu
+// blocks per CU (each then takes several chunks).
+uint32_t stream_grid(const msv_state *s) {
+  const uint64_t packs = s->amps * (s->dtype == MSV_C128 ? 16 : 8) / 16;
+  const uint64_t want = (packs + kStreamChunkPacks - 1) / kStreamChunkPacks;
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.elem_blocks_per_cu;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+}
+
 uint32_t elem_grid(const msv_state *s, uint64_t work) {
   const uint64_t want = (work + 255) / 256;
   const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.elem_blocks_per_cu;
@@ -1254,11 +1264,12 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
     }
     A.nwin = nw;
     A.nlow = nlow;
+    A.xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
     A.lut_len = static_cast<uint32_t>(lut.size());
     void *d_lut = nullptr;
     MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &d_lut));
     const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
-                                                  : elem_grid(s, s->amps / 2);
+                                                  : stream_grid(s);
     MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A,
                             grid, s->stream));
     s->sweeps += 1;
@@ -1321,7 +1332,7 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
   MSV_TRY(stage(s, tab.data(), tab.size() * sizeof(uint64_t), &d_tab));
   MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
                                s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
-                               elem_grid(s, s->amps / 2), s->stream));
+                               stream_grid(s), s->stream));
   std::swap(s->buf, s->alt);
   s->sweeps += 1;
   return MSV_OK;
@@ -1854,10 +1865,10 @@ int msv_state_scale(msv_state *s, double re, double im) {
   MSV_TRY(set_dev(s));
   if (s->dtype == MSV_C128)
     MSV_HIP(launch_scale<double>(static_cast<double2 *>(s->buf), s->amps, re, im,
-                                 elem_grid(s, s->amps), s->stream));
+                                 stream_grid(s), s->stream));
   else
     MSV_HIP(launch_scale<float>(static_cast<float2 *>(s->buf), s->amps, re, im,
-                                elem_grid(s, s->amps), s->stream));
+                                stream_grid(s), s->stream));
   s->sweeps += 1;
   return MSV_OK;
 }
