@@ -18,7 +18,6 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
-#include <utility>
 
 #include "msv_device.h"
 #include "msv_internal.h"
@@ -464,7 +463,7 @@ __device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T 
 }
 
 // Per-tile factor of a diagonal: product over the 4-bit groups of the tile
-// number (wave-uniform reads of a table staged in LDS).
+// number (wave-uniform reads of a read-only table: scalar loads).
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec2<T>::type *tab,
                                                               uint32_t ngrp, uint64_t tile) {
@@ -478,6 +477,7 @@ __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec
 
 template <typename T, int RB>
 __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict__ st,
+                                             const typename Vec2<T>::type *__restrict__ ttab,
                                              const RotArgs<T> A) {
   using V = typename Vec2<T>::type;
   constexpr int R = 1 << RB;
@@ -487,10 +487,6 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
   uint64_t vrank = blockIdx.x;
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
-  // tile-factor tables (pre, then post) behind the exchange buffer
-  V *ttab = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
-  for (uint32_t i = tid; i < 32u * A.ntgrp; i += blockDim.x) ttab[i] = A.tile_tab[i];
-  __syncthreads();
   const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
@@ -1204,38 +1200,28 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
   return hipGetLastError();
 }
 
-// Launch with more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU):
-// the kernel's dynamic LDS limit is raised once per kernel.
-template <typename... KArgs, typename... Args>
-hipError_t launch_big_lds(void (*kernel)(KArgs...), uint32_t grid, uint32_t threads, size_t lds,
-                          hipStream_t stream, Args &&...args) {
-  if (lds > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             static_cast<int>(lds));
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), lds, stream, std::forward<Args>(args)...);
-  return hipGetLastError();
-}
-
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
                       hipStream_t stream) {
   using V = typename Vec2<T>::type;
   const uint32_t threads = 1u << (args.nlocal - rb);
   // exchange buffer (64 KiB for a full tile) plus the tile-factor tables
-  const size_t lds = (args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0) + sizeof(V) * 32 * args.ntgrp;
+  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  const V *ttab = args.tile_tab;
   switch (rb) {
-    case 1: return launch_big_lds(k_rot<T, 1>, grid, threads, lds, stream, state, args);
-    case 2: return launch_big_lds(k_rot<T, 2>, grid, threads, lds, stream, state, args);
-    case 3: return launch_big_lds(k_rot<T, 3>, grid, threads, lds, stream, state, args);
-    case 4: return launch_big_lds(k_rot<T, 4>, grid, threads, lds, stream, state, args);
+    case 1: hipLaunchKernelGGL((k_rot<T, 1>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
+    case 2: hipLaunchKernelGGL((k_rot<T, 2>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
+    case 3: hipLaunchKernelGGL((k_rot<T, 3>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
+    case 4: hipLaunchKernelGGL((k_rot<T, 4>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
     case 5:
-      if constexpr (TileCfg<T>::kRB >= 5) return launch_big_lds(k_rot<T, 5>, grid, threads, lds, stream, state, args);
+      if constexpr (TileCfg<T>::kRB >= 5) {
+        hipLaunchKernelGGL((k_rot<T, 5>), dim3(grid), dim3(threads), lds, stream, state, ttab, args);
+        break;
+      }
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
 }
 
 template <typename T>

@@ -53,8 +53,8 @@ struct Tuning {
   int gate_bits = 9;        // index bits above the columns per high sweep
   int phase_blocks = 0;     // 0 = CUs * 8
   int sweep_split = 0;      // high-sweep bit sets of 1q layers: 0 contiguous, 1 page-balanced
-  int tile_blocks_per_cu = 64;
-  int elem_blocks_per_cu = 128;
+  int tile_blocks_per_cu = 4096;  // large: one tile per block up to 2^20 tiles (measured best)
+  int elem_blocks_per_cu = 1024;  // one 64 KiB chunk per block up to 2^18 chunks
   int col_bits_c64 = 4;     // contiguous low index bits per tile row in high sweeps
   int rot_split_min = 2;    // min active gates of a 1q layer for the rotation-split form (0 = never)
   int xcd_groups = 8;       // tile-to-block mapping groups (gfx950: 8 XCDs); 0/1 = plain
