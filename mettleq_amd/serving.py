@@ -63,6 +63,21 @@ class Result:
         return self.done - self.arrival
 
 
+def qasm_request(source, id: int = 0, shots: int = 1024, seed: int = BASE_SEED,
+                 arrival: float = 0.0) -> Request:
+    """A request from OpenQASM 2.0 (source text, or a path to a .qasm file)
+    through the strict importer (mettleq_amd.qasm; reference qasm.py:290-563):
+    the QASM wire format of the serving loop."""
+    from pathlib import Path
+    from .qasm import parse_qasm, parse_qasm_file
+    if isinstance(source, Path) or (isinstance(source, str) and source.rstrip().endswith(".qasm")
+                                    and "\n" not in source):
+        n, ops = parse_qasm_file(source)
+    else:
+        n, ops = parse_qasm(str(source))
+    return Request(id=int(id), n=int(n), ops=ops, shots=int(shots), seed=int(seed), arrival=float(arrival))
+
+
 # ---------------------------------------------------------------------------
 # workload: Poisson arrivals of random circuits
 # ---------------------------------------------------------------------------
