@@ -120,7 +120,13 @@ int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bon
 
 /* Generic diagonal: n1 one-qubit diagonals (qubit q1[k], entries d1[4k..4k+3]
  * = d0.re d0.im d1.re d1.im) and n2 two-qubit diagonals (qubits q2[2k] (MSB),
- * q2[2k+1], entries d2[8k..8k+7]) applied as one pass. */
+ * q2[2k+1], entries d2[8k..8k+7]) applied as one pass.
+ *
+ * Both diagonal entry points may defer their pass (MSV_TUNE_DEFER_DIAG, on
+ * by default): the state then holds the diagonal until its next operation.
+ * msv_apply_xor_affine applies it inside the permutation's own sweep; every
+ * other call launches it first, so results never differ from immediate
+ * application (up to rounding of the phase products). */
 int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
                        const int32_t *q2, const double *d2, int n2);
 
@@ -265,6 +271,9 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
                                          2^bits (5: 256-thread tiles, 4: 512) */
 #define MSV_TUNE_TILE_PREFETCH 11     /* c64 512-thread tile sweeps: load the next
                                          tile while the current one runs (0/1) */
+#define MSV_TUNE_DEFER_DIAG 12        /* phase layers / diagonals wait for the next
+                                         operation on the state; an XOR permutation
+                                         then applies them in its own sweep (0/1) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

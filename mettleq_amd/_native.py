@@ -50,6 +50,7 @@ MSV_TUNE_ROT_SPLIT_MIN = 8
 MSV_TUNE_XCD_GROUPS = 9
 MSV_TUNE_TILE_REG_BITS = 10
 MSV_TUNE_TILE_PREFETCH = 11
+MSV_TUNE_DEFER_DIAG = 12
 
 
 class NativeUnavailableError(RuntimeError):

@@ -203,6 +203,11 @@ hipError_t launch_xor_gather(const typename Vec2<T>::type *src, typename Vec2<T>
                              uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
                              uint32_t grid, hipStream_t stream);
 template <typename T>
+hipError_t launch_xor_gather_phase(const typename Vec2<T>::type *src, typename Vec2<T>::type *dst,
+                                   uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
+                                   const typename Vec2<T>::type *lut, const PhaseArgs &phase,
+                                   uint32_t grid, hipStream_t stream);
+template <typename T>
 hipError_t launch_init_basis(typename Vec2<T>::type *state, uint64_t n_amps, uint64_t index,
                              uint32_t grid, hipStream_t stream);
 template <typename T>

@@ -17,6 +17,7 @@
 //               one/two-bit diagonals) executed segment by segment.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <type_traits>
 
 #include "msv_device.h"
@@ -812,15 +813,25 @@ __global__ void __launch_bounds__(256) k_phase(typename Vec2<T>::type *__restric
 // GF(2) affine gather: out[y] = in[c ^ T0[y&255] ^ T1[(y>>8)&255] ^ ...]
 // ---------------------------------------------------------------------------
 
-template <typename T>
+// PH: the gather also applies a deferred diagonal (a phase layer given as
+// window lookup tables, as k_phase evaluates it) at the source index:
+// out[y] = D(src(y)) in[src(y)], i.e. the phase layer followed by the
+// permutation in one sweep.
+template <typename T, bool PH>
 __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type *__restrict__ src,
                                                     typename Vec2<T>::type *__restrict__ dst,
                                                     uint64_t n_amps,
                                                     const uint64_t *__restrict__ tables, int ntab,
-                                                    uint64_t c, uint32_t groups) {
+                                                    uint64_t c, uint32_t groups,
+                                                    const typename Vec2<T>::type *__restrict__ lut_g,
+                                                    const PhaseArgs P) {
   using V = typename Vec2<T>::type;
   __shared__ uint64_t tab[8 * 256];
+  extern __shared__ unsigned char smem_raw[];
+  V *lut = reinterpret_cast<V *>(smem_raw);
   for (int i = threadIdx.x; i < ntab * 256; i += blockDim.x) tab[i] = tables[i];
+  if constexpr (PH)
+    for (uint32_t i = threadIdx.x; i < P.lut_len; i += blockDim.x) lut[i] = lut_g[i];
   __syncthreads();
   constexpr int KA = Pack<T>::kAmps;
   const uint64_t npack = n_amps / KA;
@@ -830,6 +841,7 @@ __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type
     for (int k0 = 0; k0 < kChunkPer; k0 += kUnroll) {
       const uint64_t base = ch * kChunkPacks + static_cast<uint64_t>(k0) * kStreamThreads + threadIdx.x;
       V amp[kUnroll][KA];
+      uint64_t sidx[kUnroll][KA];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
@@ -837,15 +849,31 @@ __global__ void __launch_bounds__(256) k_xor_gather(const typename Vec2<T>::type
           const uint64_t y0 = pk * KA;
           uint64_t s0 = c;
           for (int t = 0; t < ntab; ++t) s0 ^= tab[t * 256 + ((y0 >> (8 * t)) & 255u)];
+          sidx[u][0] = s0;
           amp[u][0] = ld_amp(src + s0);
           // y0 is even when KA = 2: src(y0 + 1) = src(y0) ^ column of index bit 0
-          if constexpr (KA == 2) amp[u][1] = ld_amp(src + (s0 ^ tab[1]));
+          if constexpr (KA == 2) {
+            sidx[u][1] = s0 ^ tab[1];
+            amp[u][1] = ld_amp(src + sidx[u][1]);
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const uint64_t pk = base + static_cast<uint64_t>(u) * kStreamThreads;
-        if (pk < npack) store_pack<T>(dst, pk, amp[u]);
+        if (pk < npack) {
+          if constexpr (PH) {
+#pragma unroll
+            for (int d = 0; d < KA; ++d) {
+              V ph;
+              ph.x = T(1);
+              ph.y = T(0);
+              for (uint32_t w = 0; w < P.nwin; ++w) ph = cmul(ph, lut[win_index(P.win[w], sidx[u][d])]);
+              amp[u][d] = cmul(amp[u][d], ph);
+            }
+          }
+          store_pack<T>(dst, pk, amp[u]);
+        }
       }
     }
   }
@@ -1265,8 +1293,22 @@ template <typename T>
 hipError_t launch_xor_gather(const typename Vec2<T>::type *src, typename Vec2<T>::type *dst,
                              uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
                              uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL((k_xor_gather<T>), dim3(grid), dim3(256), 0, stream, src, dst, n_amps, tables,
-                     ntab, c, kStreamXcdGroups);
+  PhaseArgs none;
+  std::memset(&none, 0, sizeof(none));
+  hipLaunchKernelGGL((k_xor_gather<T, false>), dim3(grid), dim3(256), 0, stream, src, dst, n_amps,
+                     tables, ntab, c, kStreamXcdGroups, static_cast<const typename Vec2<T>::type *>(nullptr),
+                     none);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_xor_gather_phase(const typename Vec2<T>::type *src, typename Vec2<T>::type *dst,
+                                   uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
+                                   const typename Vec2<T>::type *lut, const PhaseArgs &phase,
+                                   uint32_t grid, hipStream_t stream) {
+  using V = typename Vec2<T>::type;
+  hipLaunchKernelGGL((k_xor_gather<T, true>), dim3(grid), dim3(256), sizeof(V) * phase.lut_len, stream,
+                     src, dst, n_amps, tables, ntab, c, kStreamXcdGroups, lut, phase);
   return hipGetLastError();
 }
 
@@ -1363,6 +1405,10 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
   template hipError_t launch_xor_gather<T>(const Vec2<T>::type *, Vec2<T>::type *, uint64_t,     \
                                            const uint64_t *, int, uint64_t, uint32_t,            \
                                            hipStream_t);                                         \
+  template hipError_t launch_xor_gather_phase<T>(const Vec2<T>::type *, Vec2<T>::type *, uint64_t, \
+                                                 const uint64_t *, int, uint64_t,                \
+                                                 const Vec2<T>::type *, const PhaseArgs &,       \
+                                                 uint32_t, hipStream_t);                         \
   template hipError_t launch_init_basis<T>(Vec2<T>::type *, uint64_t, uint64_t, uint32_t,        \
                                            hipStream_t);                                         \
   template hipError_t launch_scale<T>(Vec2<T>::type *, uint64_t, double, double, uint32_t,       \

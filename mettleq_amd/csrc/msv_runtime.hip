@@ -60,6 +60,7 @@ struct Tuning {
   int xcd_groups = 8;       // tile-to-block mapping groups (gfx950: 8 XCDs); 0/1 = plain
   int reg_bits_c64 = 4;     // register bits per thread in c64 tile sweeps (4: 512-thread tiles)
   int tile_prefetch = 0;    // c64 512-thread tiles: next tile's loads overlap the current tile
+  int defer_diag = 1;       // phase layers wait for the next operation (an XOR permutation absorbs them)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -113,6 +114,14 @@ struct msv_state {
   size_t red_cap = 0;
   uint64_t sweeps = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timer on this state's stream
+  // Deferred diagonal: a phase layer that fits one k_phase launch is kept
+  // here (windows + host copy of its lookup tables) and applied by the next
+  // operation on the amplitudes. An XOR permutation absorbs it (the gather
+  // multiplies by the diagonal at the source index, one sweep for both);
+  // every other entry point launches it first (flush_pending).
+  bool pend_on = false;
+  PhaseArgs pend_args;
+  std::vector<unsigned char> pend_lut;
 };
 
 namespace {
@@ -1183,8 +1192,33 @@ bool fit_runs(const std::vector<int> &bits, int wmax, int *s1, int *w1, int *s2,
 }
 
 template <typename T>
-int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
+int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, size_t lut_bytes) {
   using V = typename Vec2<T>::type;
+  void *d_lut = nullptr;
+  MSV_TRY(stage(s, lut_host, lut_bytes, &d_lut));
+  const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
+                                                : stream_grid(s);
+  MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A, grid,
+                          s->stream));
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+template <typename T>
+int flush_pending_T(msv_state *s) {
+  if (!s->pend_on) return MSV_OK;
+  s->pend_on = false;
+  return launch_phase_chunk<T>(s, s->pend_args, s->pend_lut.data(), s->pend_lut.size());
+}
+
+// Largest deferred lookup table: the absorbing gather keeps it in LDS next
+// to its 16 KiB of permutation tables.
+constexpr size_t kPendLutMaxBytes = 48 * 1024;
+
+template <typename T>
+int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool defer) {
+  using V = typename Vec2<T>::type;
+  MSV_TRY(flush_pending_T<T>(s));
   if (terms.empty()) return MSV_OK;
   const int wmax = 8;
   std::vector<Win> wins;
@@ -1221,6 +1255,7 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
     const bool la = a.bits.front() == 0, lb2 = b.bits.front() == 0;
     return la && !lb2;
   });
+  std::vector<std::pair<PhaseArgs, std::vector<V>>> chunks;
   size_t wi = 0;
   while (wi < wins.size()) {
     PhaseArgs A;
@@ -1266,14 +1301,18 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms) {
     A.nlow = nlow;
     A.xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
     A.lut_len = static_cast<uint32_t>(lut.size());
-    void *d_lut = nullptr;
-    MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &d_lut));
-    const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
-                                                  : stream_grid(s);
-    MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A,
-                            grid, s->stream));
-    s->sweeps += 1;
+    chunks.emplace_back(A, std::move(lut));
   }
+  if (defer && g_tune.defer_diag && chunks.size() == 1 &&
+      chunks[0].second.size() * sizeof(V) <= kPendLutMaxBytes) {
+    s->pend_on = true;
+    s->pend_args = chunks[0].first;
+    const auto *b = reinterpret_cast<const unsigned char *>(chunks[0].second.data());
+    s->pend_lut.assign(b, b + chunks[0].second.size() * sizeof(V));
+    return MSV_OK;
+  }
+  for (auto &ch : chunks)
+    MSV_TRY(launch_phase_chunk<T>(s, ch.first, ch.second.data(), ch.second.size() * sizeof(V)));
   return MSV_OK;
 }
 
@@ -1330,9 +1369,20 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
   MSV_TRY(ensure_alt(s));
   void *d_tab = nullptr;
   MSV_TRY(stage(s, tab.data(), tab.size() * sizeof(uint64_t), &d_tab));
-  MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
-                               s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
-                               stream_grid(s), s->stream));
+  if (s->pend_on) {
+    // the deferred phase layer rides along: out[y] = D(src(y)) in[src(y)]
+    s->pend_on = false;
+    void *d_lut = nullptr;
+    MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d_lut));
+    MSV_HIP(launch_xor_gather_phase<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
+                                       s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
+                                       reinterpret_cast<const V *>(d_lut), s->pend_args,
+                                       stream_grid(s), s->stream));
+  } else {
+    MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
+                                 s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
+                                 stream_grid(s), s->stream));
+  }
   std::swap(s->buf, s->alt);
   s->sweeps += 1;
   return MSV_OK;
@@ -1554,6 +1604,15 @@ int init_basis_T(msv_state *s, uint64_t index) {
 
 #define MSV_DISPATCH(s, fn, ...) \
   ((s)->dtype == MSV_C128 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+// Launches a deferred phase layer (no-op when none is pending). Every entry
+// point that reads or writes amplitudes calls it first, except the XOR
+// permutation, which absorbs it, and the phase layers themselves.
+int flush_pending(msv_state *s) {
+  if (!s->pend_on) return MSV_OK;
+  MSV_TRY(set_dev(s));
+  return MSV_DISPATCH(s, flush_pending_T, s);
+}
 
 namespace {
 
@@ -1796,6 +1855,7 @@ int msv_state_set_basis(msv_state *s, uint64_t index) {
   MSV_TRY(check_state(s));
   if (index >= s->amps) return fail(MSV_E_INVALID, "basis index out of range");
   MSV_TRY(set_dev(s));
+  s->pend_on = false;  // the new basis state replaces what a pending diagonal would act on
   return MSV_DISPATCH(s, init_basis_T, s, index);
 }
 
@@ -1813,6 +1873,7 @@ int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t c
   if (offset > s->amps || count > s->amps - offset)
     return fail(MSV_E_INVALID, "upload range exceeds the state");
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   const size_t es = esize(s->dtype);
   MSV_HIP(hipMemcpyAsync(static_cast<char *>(s->buf) + offset * es, host, count * es,
                          hipMemcpyHostToDevice, s->stream));
@@ -1826,6 +1887,7 @@ int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count
   if (offset > s->amps || count > s->amps - offset)
     return fail(MSV_E_INVALID, "download range exceeds the state");
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   const size_t es = esize(s->dtype);
   MSV_HIP(hipMemcpyAsync(host, static_cast<const char *>(s->buf) + offset * es, count * es,
                          hipMemcpyDeviceToHost, s->stream));
@@ -1837,7 +1899,9 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   MSV_TRY(check_state(dst));
   MSV_TRY(check_state(src));
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
+  MSV_TRY(flush_pending(src));
   MSV_TRY(set_dev(dst));
+  if (dst != src) dst->pend_on = false;  // overwritten
   MSV_HIP(hipStreamSynchronize(src->stream));
   const uint32_t grid = elem_grid(dst, dst->amps);
   if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
@@ -1863,6 +1927,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
 int msv_state_scale(msv_state *s, double re, double im) {
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   if (s->dtype == MSV_C128)
     MSV_HIP(launch_scale<double>(static_cast<double2 *>(s->buf), s->amps, re, im,
                                  stream_grid(s), s->stream));
@@ -1876,6 +1941,7 @@ int msv_state_scale(msv_state *s, double re, double im) {
 int msv_state_sync(msv_state *s) {
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   MSV_HIP(hipStreamSynchronize(s->stream));
   return MSV_OK;
 }
@@ -1889,12 +1955,18 @@ int msv_state_release_scratch(msv_state *s) {
   return MSV_OK;
 }
 
-void *msv_state_device_ptr(msv_state *s) { return s ? s->buf : nullptr; }
+void *msv_state_device_ptr(msv_state *s) {
+  if (!s) return nullptr;
+  // the caller reads the amplitudes directly: a deferred diagonal goes first
+  if (flush_pending(s) != MSV_OK) return nullptr;
+  return s->buf;
+}
 
 int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) {
   MSV_TRY(check_state(s));
   if (!mats) return fail(MSV_E_INVALID, "null matrix array");
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, apply_1q_layer_T, s, mats, active);
 }
 
@@ -1938,7 +2010,7 @@ int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bon
     terms.push_back(d);
   }
   MSV_TRY(set_dev(s));
-  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms);
+  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms, true);
 }
 
 int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
@@ -1970,7 +2042,7 @@ int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1
     terms.push_back(d);
   }
   MSV_TRY(set_dev(s));
-  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms);
+  return MSV_DISPATCH(s, apply_diag_terms_T, s, terms, true);
 }
 
 int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c) {
@@ -1987,6 +2059,7 @@ int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
                                    std::to_string(m) + ") for " + std::to_string(s->n) +
                                    " qubits");
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   const int hb = s->n - 1 - first, lb = s->n - m;
   return MSV_DISPATCH(s, apply_qft_T, s, hb, lb, inverse != 0, -1);
 }
@@ -1997,6 +2070,7 @@ int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse) {
     return fail(MSV_E_INVALID, "qft_stage: invalid j=" + std::to_string(j) + ", m=" +
                                    std::to_string(m) + ", n=" + std::to_string(s->n));
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   const int p = s->n - 1 - j, lb = s->n - m;
   return MSV_DISPATCH(s, apply_qft_T, s, p, lb, inverse != 0, p);
 }
@@ -2069,12 +2143,14 @@ int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
     }
   }
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, apply_plan_ops, s, ops);
 }
 
 int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires, const double *params,
                       int n_ops) {
   MSV_TRY(check_state(s));
+  MSV_TRY(flush_pending(s));
   if (n_ops < 0 || (n_ops > 0 && (!codes || !wires || !params)))
     return fail(MSV_E_INVALID, "circuit: null arrays");
   std::vector<msv_gate> gates;
@@ -2111,6 +2187,7 @@ int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out) {
     }
   }
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, probabilities_T, s, wires, nw, out);
 }
 
@@ -2132,6 +2209,7 @@ int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint6
     seen |= 1ull << wires[w];
   }
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, sample_T, s, wires, nw, shots, seed, stream_id, out);
 }
 
@@ -2142,6 +2220,7 @@ int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask,
     return fail(MSV_E_INVALID, "expval_pauli: null arrays");
   if (n_words == 0) return MSV_OK;
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, pauli_T, s, xmask, zmask, n_words, out_re, out_im);
 }
 
@@ -2151,6 +2230,8 @@ int msv_inner(msv_state *a, msv_state *b, double *re, double *im) {
   if (!re || !im) return fail(MSV_E_INVALID, "null output");
   if (a->n != b->n) return fail(MSV_E_INVALID, "inner: qubit counts differ");
   if (a->device != b->device) return fail(MSV_E_INVALID, "inner: states on different devices");
+  MSV_TRY(flush_pending(a));
+  MSV_TRY(flush_pending(b));
   MSV_TRY(set_dev(a));
   if (a->dtype == MSV_C64 && b->dtype == MSV_C64) return inner_T<float, float>(a, b, re, im);
   if (a->dtype == MSV_C64) return inner_T<float, double>(a, b, re, im);
@@ -2162,6 +2243,7 @@ int msv_norm2(msv_state *s, double *out) {
   MSV_TRY(check_state(s));
   if (!out) return fail(MSV_E_INVALID, "null output");
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));
   std::vector<double> bs;
   int rc = s->dtype == MSV_C128 ? block_sums_T<double>(s, &bs) : block_sums_T<float>(s, &bs);
   if (rc != MSV_OK) return rc;
@@ -2219,6 +2301,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "tile prefetch must be 0 or 1");
       g_tune.tile_prefetch = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_DEFER_DIAG:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer diag must be 0 or 1");
+      g_tune.defer_diag = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2238,6 +2324,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_XCD_GROUPS: *value = g_tune.xcd_groups; return MSV_OK;
     case MSV_TUNE_TILE_REG_BITS: *value = g_tune.reg_bits_c64; return MSV_OK;
     case MSV_TUNE_TILE_PREFETCH: *value = g_tune.tile_prefetch; return MSV_OK;
+    case MSV_TUNE_DEFER_DIAG: *value = g_tune.defer_diag; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

@@ -526,3 +526,96 @@ def test_1q_layer_rotation_split_thresholds(gpu, split_min, n):
     finally:
         nat.set_tuning(nat.MSV_TUNE_ROT_SPLIT_MIN, old)
     assert np.max(np.abs(got - want)) < TOL64
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("after", ["xor", "probs", "download", "phase", "layer", "qft", "inner",
+                                   "sample", "reset", "scale", "gates"])
+def test_deferred_phase_layer_then_next_operation(gpu, dtype, after):
+    """A phase layer is held until the state's next operation: the XOR
+    permutation applies it in its own gather (D at the source index), any
+    other call launches it first. Each sequence must match the oracle."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import gate_u
+    n = 14
+    rng = np.random.default_rng(4242)
+    psi = _rand_state(n, 99)
+    bonds = [(i, (i + 1) % n) for i in range(n)] + [(0, 7), (3, 11)]
+    th = rng.uniform(0, math.pi, len(bonds))
+    want = O.zz_layer_diag(n, bonds, th) * psi
+    rows, c = O.compose_inverse_affine([{"name": "CNOT", "wires": [i, i + 1]} for i in range(n - 1)]
+                                       + [{"name": "X", "wires": [2]}], n)
+    mats = [_rand_u(rng) for _ in range(n)]
+    tol = TOL64 if dtype == "complex64" else TOL128
+    assert nat.get_tuning(nat.MSV_TUNE_DEFER_DIAG) == 1
+    with _state(n, dtype) as s:
+        s.from_numpy(psi)
+        s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, list(th))
+        if after == "xor":
+            s.apply_xor_affine(rows, c)
+            want = O.xor_affine_gather(want, n, rows, c)
+        elif after == "probs":
+            assert np.max(np.abs(s.probabilities() - np.abs(want) ** 2)) < tol
+        elif after == "download":
+            pass
+        elif after == "phase":
+            s.apply_phase_layer([nat.MSV_PHASE_CPHASE] * 3, [(0, 1), (4, 9), (12, 13)], [0.3, 1.1, math.pi])
+            want = O.cphase_layer_diag(n, [(0, 1), (4, 9), (12, 13)], [0.3, 1.1, math.pi]) * want
+            s.apply_xor_affine(rows, c)
+            want = O.xor_affine_gather(want, n, rows, c)
+        elif after == "layer":
+            s.apply_1q_layer(mats)
+            want = O.apply_1q_layer(want, n, mats)
+        elif after == "qft":
+            s.apply_qft(0, n)
+            want = O.qft_fft(want, n)
+        elif after == "inner":
+            with _state(n, dtype) as t:
+                t.from_numpy(psi)
+                ip = t.inner(s)
+            assert abs(ip - np.vdot(psi, want)) < 10 * tol
+        elif after == "sample":
+            got = s.sample(300, None, seed=11)
+            ref = O.sample_indices(O.amp_probs_c64(s.to_numpy()), 300, 11, 0)
+            assert np.array_equal(got, ref)
+        elif after == "reset":
+            s.reset()
+            want = O.zero_state(n)
+        elif after == "scale":
+            s.scale(1j)
+            want = 1j * want
+        elif after == "gates":
+            g = [gate_u(3, mats[0]), gate_u(9, mats[1], [2])]
+            s.apply_gates(g)
+            want = O.apply_controlled(O.apply_dense(want, n, mats[0], [3]), n, mats[1], [2], 9)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < tol
+
+
+def test_deferred_phase_matches_immediate_at_26q(gpu):
+    """RZZ ring then CNOT chain at 26 qubits: the gather that absorbs the
+    deferred phase layer equals the two separate sweeps (within complex64
+    rounding of the phase products), in one sweep instead of two."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.workloads import ring_bonds
+    n = 26
+    rng = np.random.default_rng(26)
+    mats = [_rand_u(rng) for _ in range(n)]
+    bonds = ring_bonds(n)
+    th = [float(x) for x in rng.uniform(0, math.pi, len(bonds))]
+    rows, c = O.compose_inverse_affine([{"name": "CNOT", "wires": [i, i + 1]} for i in range(n - 1)], n)
+    outs, sweeps = [], []
+    try:
+        for defer in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, defer)
+            with _state(n) as s:
+                s.apply_1q_layer(mats)
+                s0 = s.sweep_count()
+                s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
+                s.apply_xor_affine(rows, c)
+                sweeps.append(s.sweep_count() - s0)
+                outs.append(s.to_numpy())
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
+    assert sweeps == [2, 1]
+    assert np.max(np.abs(outs[0] - outs[1])) < 1e-6
