@@ -274,6 +274,10 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
 #define MSV_TUNE_DEFER_DIAG 12        /* phase layers / diagonals wait for the next
                                          operation on the state; an XOR permutation
                                          then applies them in its own sweep (0/1) */
+#define MSV_TUNE_DEFER_PERM 13        /* XOR permutations wait for the next operation;
+                                         a 1q layer then reads its first sweep through
+                                         the permutation (and a pending diagonal) (0/1) */
+#define MSV_TUNE_FUSED_BLOCKS_PER_CU 14 /* grid of such a fused first sweep (blocks per CU) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

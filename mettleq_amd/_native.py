@@ -51,6 +51,8 @@ MSV_TUNE_XCD_GROUPS = 9
 MSV_TUNE_TILE_REG_BITS = 10
 MSV_TUNE_TILE_PREFETCH = 11
 MSV_TUNE_DEFER_DIAG = 12
+MSV_TUNE_DEFER_PERM = 13
+MSV_TUNE_FUSED_BLOCKS_PER_CU = 14
 
 
 class NativeUnavailableError(RuntimeError):

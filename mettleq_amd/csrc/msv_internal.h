@@ -112,6 +112,24 @@ struct TileArgs {
 static_assert(sizeof(TileArgs<float>) <= 4096, "tile kernel arguments exceed 4 KiB");
 static_assert(sizeof(TileArgs<double>) <= 4096, "tile kernel arguments exceed 4 KiB");
 
+struct PhaseWin {
+  uint8_t s1, w1, s2, w2;  // lut index = ((i>>s1)&(2^w1-1)) | (((i>>s2)&(2^w2-1))<<w1)
+  uint32_t off;            // offset of this window's table
+};
+
+struct PhaseArgs {
+  uint64_t n_amps;
+  uint32_t nwin;
+  uint32_t nlow;           // windows [0, nlow) read index bits 0..1
+  uint32_t lut_len;
+  uint32_t xcd_groups;     // chunk-to-block mapping groups (as in the tile sweeps)
+  PhaseWin win[kMaxPhaseWins];
+};
+
+constexpr int kMaxTileBits = 32;   // bits of a tile number
+constexpr int kMaxTileGroups = 8;  // 4-bit groups of a tile number
+constexpr int kFusedRdep = 4;      // deferred-diagonal windows a fused sweep evaluates per register
+
 // Commuting rotation sweep (1q layers in Z-Y-Z split form, k_rot): per
 // segment, a rotation on each register bit set in gmask, given by its three
 // shear factors (a, b); a diagonal over all n index bits before the first
@@ -134,15 +152,27 @@ struct RotArgs {
   uint32_t has_post;
   uint32_t ntgrp;             // 4-bit groups of the tile number in tile_tab
   uint32_t pad;
-  const typename Vec2<T>::type *tile_tab;  // [2][ntgrp][16]: pre, post (null: ntgrp = 0)
+  const typename Vec2<T>::type *tile_tab;  // [2][kMaxTileGroups][16]: pre, post (unit entries past ntgrp)
   TileSeg seg[kMaxSegs];
   T shear[kMaxSegs][5][2];
   T pre[kDtabVals];
   T post[kDtabVals];
+  // Deferred permutation (k_rot<.., PERM>): the first segment loads
+  // amplitude y from src(y) = M'y ^ c of the source buffer, times a deferred
+  // diagonal D(src(y)), and the sweep stores into the other buffer. M' maps
+  // the tile's local bits into local bits, so each tile reads one source
+  // tile. src(y) = c ^ image of the tile number (byte tables at ptab) ^
+  // pthr(thread bits) ^ preg(register bits), each the XOR of the images of
+  // the set bits.
+  uint64_t pc;                          // c
+  uint64_t pthr[kTileThreadBits];       // image of segment 0's thread-id bit k
+  uint64_t preg[5];                     // image of segment 0's register bit j
+  uint32_t ph_rdep;                     // windows [0, ph_rdep) of ph see the register images
+  uint32_t ph_pad;
+  PhaseArgs ph;                         // the deferred diagonal (nwin = 0: none)
 };
 static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4 KiB");
 static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");
-constexpr int kMaxTileGroups = 8;  // tile numbers up to 32 bits
 
 // QFT sweep (k_qft): per segment, QFT stages on the register bits set in
 // smask, the window holding consecutive index bits p0 .. p0 + RB - 1 with
@@ -171,27 +201,18 @@ struct QftArgs {
 };
 static_assert(sizeof(QftArgs<double>) <= 4096, "QFT sweep arguments exceed 4 KiB");
 
-struct PhaseWin {
-  uint8_t s1, w1, s2, w2;  // lut index = ((i>>s1)&(2^w1-1)) | (((i>>s2)&(2^w2-1))<<w1)
-  uint32_t off;            // offset of this window's table
-};
-
-struct PhaseArgs {
-  uint64_t n_amps;
-  uint32_t nwin;
-  uint32_t nlow;           // windows [0, nlow) read index bits 0..1
-  uint32_t lut_len;
-  uint32_t xcd_groups;     // chunk-to-block mapping groups (as in the tile sweeps)
-  PhaseWin win[kMaxPhaseWins];
-};
-
 // Kernel launchers (msv_kernels.hip). All enqueue on `stream`.
 template <typename T>
 hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, int rb,
                        uint32_t grid, hipStream_t stream);
+// perm_src != nullptr: the first segment reads through the deferred
+// permutation from perm_src (with the diagonal's tables at ph_lut and the
+// tile-number images in four 256-entry byte tables at ptab) and the sweep
+// writes `state`.
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
-                      hipStream_t stream);
+                      hipStream_t stream, const typename Vec2<T>::type *perm_src = nullptr,
+                      const typename Vec2<T>::type *ph_lut = nullptr, const uint64_t *ptab = nullptr);
 template <typename T>
 hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
                       uint32_t grid, hipStream_t stream);

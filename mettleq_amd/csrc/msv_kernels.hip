@@ -291,6 +291,13 @@ __device__ __forceinline__ void load_first(const typename Vec2<T>::type *__restr
   for (int r = 0; r < (1 << RB); ++r) a[r] = ld_amp(st + gb + reg_goff<RB>(r, gstr));
 }
 
+// Lookup-table index of phase window w for amplitude index i.
+__device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) {
+  const uint32_t a = static_cast<uint32_t>(i >> w.s1) & ((1u << w.w1) - 1u);
+  const uint32_t b = static_cast<uint32_t>(i >> w.s2) & ((1u << w.w2) - 1u);
+  return w.off + (a | (b << w.w1));
+}
+
 // Segment `s` of a tile: its thread's local and index bits.
 struct SegPos {
   uint32_t tl, tm, tsw;
@@ -465,21 +472,36 @@ __device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T 
 
 // Per-tile factor of a diagonal: product over the 4-bit groups of the tile
 // number (wave-uniform reads of a read-only table: scalar loads).
+// The host pads the table to kMaxTileGroups groups (unit entries past
+// ngrp), so every group is read unconditionally: the loads are independent
+// and issue together.
 template <typename T>
 __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec2<T>::type *tab,
                                                               uint32_t ngrp, uint64_t tile) {
   using V = typename Vec2<T>::type;
-  V f;
-  f.x = T(1);
-  f.y = T(0);
-  for (uint32_t g = 0; g < ngrp; ++g) f = cmul(f, tab[g * 16 + ((tile >> (4 * g)) & 15u)]);
+  (void)ngrp;
+  V e[kMaxTileGroups];
+#pragma unroll
+  for (int g = 0; g < kMaxTileGroups; ++g) e[g] = tab[g * 16 + ((tile >> (4 * g)) & 15u)];
+  V f = e[0];
+#pragma unroll
+  for (int g = 1; g < kMaxTileGroups; ++g) f = cmul(f, e[g]);
   return f;
 }
 
-template <typename T, int RB>
+// PERM: the first segment gathers through a deferred permutation (see
+// RotArgs): thread-uniform and register parts of the source index are XORs
+// of precomputed images, the deferred diagonal's window tables sit in LDS
+// behind the exchange buffer, and windows the register images do not touch
+// are evaluated once per thread. Reads come from `src`, stores go to `st`
+// (the other buffer), so no tile reads what another tile writes.
+template <typename T, int RB, bool PERM>
 __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict__ st,
                                              const typename Vec2<T>::type *__restrict__ ttab,
-                                             const RotArgs<T> A) {
+                                             const RotArgs<T> A,
+                                             const typename Vec2<T>::type *__restrict__ src,
+                                             const typename Vec2<T>::type *__restrict__ ph_lut,
+                                             const uint64_t *__restrict__ ptab) {
   using V = typename Vec2<T>::type;
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
@@ -488,6 +510,11 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
   uint64_t vrank = blockIdx.x;
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  V *lut = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
+  if constexpr (PERM) {
+    for (uint32_t i = tid; i < A.ph.lut_len; i += blockDim.x) lut[i] = ph_lut[i];
+    __syncthreads();
+  }
   const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
@@ -496,11 +523,53 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
       const TileSeg &sg = A.seg[s];
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (s == 0) {
-        uint64_t gstr[RB];
+        if constexpr (PERM) {
+          // image of the tile number: four byte tables (scalar loads)
+          uint64_t sb = A.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
+                        ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
 #pragma unroll
-        for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+          for (int k = 0; k < kTileThreadBits; ++k)
+            if ((p.tm >> k) & 1u) sb ^= A.pthr[k];
+          uint64_t ri[R];
+          ri[0] = 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+          for (int r = 1; r < R; ++r) ri[r] = ri[r & (r - 1)] ^ A.preg[__builtin_ctz(r)];
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[r] = ld_amp(src + (sb ^ ri[r]));
+          if (A.ph.nwin) {
+            // windows [0, nrdep) see the register bits (the host orders
+            // them first); the others are one product per thread. A
+            // window's index is linear in the amplitude index, so the
+            // per-register part is an XOR of wave-uniform bits.
+            const uint32_t nrdep = A.ph_rdep;
+            V pt;
+            pt.x = T(1);
+            pt.y = T(0);
+            for (uint32_t w = nrdep; w < A.ph.nwin; ++w) pt = cmul(pt, lut[win_index(A.ph.win[w], sb)]);
+            uint32_t bw[kFusedRdep];
+#pragma unroll
+            for (int w = 0; w < kFusedRdep; ++w)
+              bw[w] = (w < static_cast<int>(nrdep)) ? win_index(A.ph.win[w], sb) - A.ph.win[w].off : 0u;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              V q = pt;
+#pragma unroll
+              for (int w = 0; w < kFusedRdep; ++w)
+                if (w < static_cast<int>(nrdep)) {
+                  const PhaseWin &pw = A.ph.win[w];
+                  const uint32_t rb = win_index(pw, ri[r]) - pw.off;
+                  q = cmul(q, lut[pw.off + (bw[w] ^ rb)]);
+                }
+              a[r] = cmul(a[r], q);
+            }
+          }
+        } else {
+          uint64_t gstr[RB];
+#pragma unroll
+          for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+        }
         if (A.has_pre)
           dtab_apply_g<T, RB>(a, A.pre, p.tm, tile_factor<T>(ttab, A.ntgrp, tile));
       } else {
@@ -512,13 +581,13 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         if ((gm >> j) & 1u) {
-          const T sa = A.shear[s][j][0], sb = A.shear[s][j][1];
+          const T sa = A.shear[s][j][0], sb2 = A.shear[s][j][1];
           switch (j) {
-            case 0: shear_pairs<T, RB, 0>(a, sa, sb); break;
-            case 1: if constexpr (RB > 1) shear_pairs<T, RB, 1>(a, sa, sb); break;
-            case 2: if constexpr (RB > 2) shear_pairs<T, RB, 2>(a, sa, sb); break;
-            case 3: if constexpr (RB > 3) shear_pairs<T, RB, 3>(a, sa, sb); break;
-            case 4: if constexpr (RB > 4) shear_pairs<T, RB, 4>(a, sa, sb); break;
+            case 0: shear_pairs<T, RB, 0>(a, sa, sb2); break;
+            case 1: if constexpr (RB > 1) shear_pairs<T, RB, 1>(a, sa, sb2); break;
+            case 2: if constexpr (RB > 2) shear_pairs<T, RB, 2>(a, sa, sb2); break;
+            case 3: if constexpr (RB > 3) shear_pairs<T, RB, 3>(a, sa, sb2); break;
+            case 4: if constexpr (RB > 4) shear_pairs<T, RB, 4>(a, sa, sb2); break;
             default: break;
           }
         }
@@ -533,7 +602,7 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
       const TileSeg &sg = A.seg[last];
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (A.has_post)
-        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(ttab + A.ntgrp * 16, A.ntgrp, tile));
+        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(ttab + kMaxTileGroups * 16, A.ntgrp, tile));
       uint64_t gstr[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
@@ -718,11 +787,6 @@ template <typename T> struct Pack;
 template <> struct Pack<float> { using P = float4; static constexpr int kAmps = 2; };
 template <> struct Pack<double> { using P = double2; static constexpr int kAmps = 1; };
 
-__device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) {
-  const uint32_t a = static_cast<uint32_t>(i >> w.s1) & ((1u << w.w1) - 1u);
-  const uint32_t b = static_cast<uint32_t>(i >> w.s2) & ((1u << w.w2) - 1u);
-  return w.off + (a | (b << w.w1));
-}
 
 template <typename T>
 __device__ __forceinline__ void load_pack(const typename Vec2<T>::type *st, uint64_t pk,
@@ -1228,28 +1292,50 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
   return hipGetLastError();
 }
 
+// Dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU) needs the kernel's
+// limit raised; done once per kernel.
+template <typename K>
+hipError_t allow_lds(K kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
+template <typename T, int RB>
+hipError_t launch_rot_rb(typename Vec2<T>::type *state, const RotArgs<T> &args, uint32_t grid,
+                         hipStream_t stream, const typename Vec2<T>::type *perm_src,
+                         const typename Vec2<T>::type *ph_lut, const uint64_t *ptab) {
+  using V = typename Vec2<T>::type;
+  const uint32_t threads = 1u << (args.nlocal - RB);
+  const size_t xlds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
+  if (perm_src) {
+    const size_t lds = xlds + sizeof(V) * args.ph.lut_len;
+    const hipError_t e = allow_lds(k_rot<T, RB, true>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_rot<T, RB, true>), dim3(grid), dim3(threads), lds, stream, state,
+                       args.tile_tab, args, perm_src, ph_lut, ptab);
+  } else {
+    hipLaunchKernelGGL((k_rot<T, RB, false>), dim3(grid), dim3(threads), xlds, stream, state,
+                       args.tile_tab, args, static_cast<const V *>(nullptr), static_cast<const V *>(nullptr),
+                       static_cast<const uint64_t *>(nullptr));
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
-                      hipStream_t stream) {
-  using V = typename Vec2<T>::type;
-  const uint32_t threads = 1u << (args.nlocal - rb);
-  // exchange buffer (64 KiB for a full tile) plus the tile-factor tables
-  const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
-  const V *ttab = args.tile_tab;
+                      hipStream_t stream, const typename Vec2<T>::type *perm_src,
+                      const typename Vec2<T>::type *ph_lut, const uint64_t *ptab) {
   switch (rb) {
-    case 1: hipLaunchKernelGGL((k_rot<T, 1>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
-    case 2: hipLaunchKernelGGL((k_rot<T, 2>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
-    case 3: hipLaunchKernelGGL((k_rot<T, 3>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
-    case 4: hipLaunchKernelGGL((k_rot<T, 4>), dim3(grid), dim3(threads), lds, stream, state, ttab, args); break;
+    case 1: return launch_rot_rb<T, 1>(state, args, grid, stream, perm_src, ph_lut, ptab);
+    case 2: return launch_rot_rb<T, 2>(state, args, grid, stream, perm_src, ph_lut, ptab);
+    case 3: return launch_rot_rb<T, 3>(state, args, grid, stream, perm_src, ph_lut, ptab);
+    case 4: return launch_rot_rb<T, 4>(state, args, grid, stream, perm_src, ph_lut, ptab);
     case 5:
-      if constexpr (TileCfg<T>::kRB >= 5) {
-        hipLaunchKernelGGL((k_rot<T, 5>), dim3(grid), dim3(threads), lds, stream, state, ttab, args);
-        break;
-      }
+      if constexpr (TileCfg<T>::kRB >= 5) return launch_rot_rb<T, 5>(state, args, grid, stream, perm_src, ph_lut, ptab);
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 template <typename T>
@@ -1397,7 +1483,8 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
   template hipError_t launch_tile<T>(Vec2<T>::type *, const TileArgs<T> &, int, uint32_t,        \
                                      hipStream_t);                                               \
   template hipError_t launch_rot<T>(Vec2<T>::type *, const RotArgs<T> &, int, uint32_t,          \
-                                    hipStream_t);                                                \
+                                    hipStream_t, const Vec2<T>::type *, const Vec2<T>::type *,   \
+                                    const uint64_t *);                                           \
   template hipError_t launch_qft<T>(Vec2<T>::type *, const QftArgs<T> &, int, bool, uint32_t,    \
                                     hipStream_t);                                                \
   template hipError_t launch_phase<T>(Vec2<T>::type *, const Vec2<T>::type *, const PhaseArgs &, \

@@ -61,6 +61,8 @@ struct Tuning {
   int reg_bits_c64 = 4;     // register bits per thread in c64 tile sweeps (4: 512-thread tiles)
   int tile_prefetch = 0;    // c64 512-thread tiles: next tile's loads overlap the current tile
   int defer_diag = 1;       // phase layers wait for the next operation (an XOR permutation absorbs them)
+  int defer_perm = 1;       // XOR permutations wait for the next operation (a 1q layer's first sweep reads through them)
+  int fused_blocks_per_cu = 32;  // grid of a first sweep that reads through deferred work (tables staged per block)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -122,6 +124,13 @@ struct msv_state {
   bool pend_on = false;
   PhaseArgs pend_args;
   std::vector<unsigned char> pend_lut;
+  // Deferred XOR permutation: the amplitudes are y -> D(src(y)) buf[src(y)]
+  // with src(y) = M'y ^ c (perm_col[j] = M' column j); D is the deferred
+  // diagonal above when pend_on. The next 1q layer reads its first sweep
+  // through it; every other entry point runs the gather first.
+  bool perm_on = false;
+  std::vector<uint64_t> perm_col;
+  uint64_t perm_c = 0;
 };
 
 namespace {
@@ -221,6 +230,12 @@ int check_qubit(const msv_state *s, int q, const char *what) {
                                    " out of range for " + std::to_string(s->n) + " qubits");
   return MSV_OK;
 }
+
+// Deferred work (defined with the phase and permutation planners below).
+template <typename T>
+int flush_pending_T(msv_state *s);
+template <typename T>
+int launch_gather(msv_state *s, const std::vector<uint64_t> &col, uint64_t c);
 
 // ---------------------------------------------------------------------------
 // tile-sweep planning
@@ -690,7 +705,7 @@ int fill_tile_geometry(int n, const std::vector<int> &local, const std::vector<i
 // segment `sg` of a tile: register table (constant included) and thread-bit
 // factors in `tab` (register entries at 0..63, thread-bit factors at
 // 64..81, as dtab_apply_g reads them), tile-number groups in `tgrp`
-// (ntgrp x 16 entries).
+// (kMaxTileGroups x 16 entries, unit past ntgrp).
 template <typename T, typename Args>
 void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &local, int rb,
                       const std::vector<std::complex<double>> &ratio, std::complex<double> cst,
@@ -718,10 +733,10 @@ void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &
   std::vector<int> tbit_index;
   for (uint32_t r = 0; r < A.nruns; ++r)
     for (uint32_t w = 0; w < A.run_width[r]; ++w) tbit_index.push_back(static_cast<int>(A.run_dst[r] + w));
-  for (int g = 0; g < ntgrp; ++g) {
+  for (int g = 0; g < kMaxTileGroups; ++g) {  // unit entries past ntgrp
     for (int v = 0; v < 16; ++v) {
       C f = 1.0;
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; g < ntgrp && i < 4; ++i) {
         const int t = 4 * g + i;
         if (((v >> i) & 1) && t < static_cast<int>(tbit_index.size())) f *= ratio[tbit_index[t]];
       }
@@ -740,6 +755,10 @@ void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &
 // first sweep's rotations, diag(a1, b1) after the last sweep's. The
 // diagonals commute with the rotations on other bits, so this equals the
 // gate-by-gate layer.
+// Largest deferred-diagonal table a fused first sweep keeps in LDS: two
+// blocks of 64 KiB exchange buffer + 16 KiB tables fill a CU's 160 KiB.
+constexpr size_t kFusedLutMaxBytes = 16 * 1024;
+
 template <typename T>
 int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vector<RotSplit> &zb) {
   using V = typename Vec2<T>::type;
@@ -767,7 +786,42 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
       if (w.local[k] >= L && act_bit[w.local[k]]) w.gl.push_back(k);
     if (!w.gl.empty()) sweeps.push_back(w);
   }
-  if (sweeps.empty()) return MSV_OK;
+  if (sweeps.empty()) return flush_pending_T<T>(s);
+  // Deferred permutation and/or diagonal: the first sweep reads through them
+  // (out of place) when every local bit of its tiles maps into local bits,
+  // so that each tile reads one source tile, and the diagonal's tables fit
+  // in LDS beside the exchange buffer; otherwise they run first.
+  bool fuse = false;
+  if (s->perm_on || s->pend_on) {
+    uint64_t lmask = 0;
+    for (int b : sweeps[0].local) lmask |= 1ull << b;
+    fuse = !s->pend_on || s->pend_lut.size() <= kFusedLutMaxBytes;
+    if (fuse && s->pend_on) {
+      // the first segment's register images may touch at most kFusedRdep windows
+      const int rb0 = std::min(rb_of<T>(), static_cast<int>(sweeps[0].local.size()));
+      uint32_t need = 0;
+      for (int k : sweeps[0].gl) need |= 1u << k;
+      const int nl0 = static_cast<int>(sweeps[0].local.size());
+      const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
+      uint64_t umask = 0;
+      for (int j = 0; !w0.empty() && j < rb0; ++j) {
+        const int b = sweeps[0].local[w0[0] + j];
+        umask |= s->perm_on ? s->perm_col[b] : (1ull << b);
+      }
+      int ndep = 0;
+      for (uint32_t w = 0; w < s->pend_args.nwin; ++w) {
+        const PhaseWin &pw = s->pend_args.win[w];
+        const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
+        ndep += (wm & umask) ? 1 : 0;
+      }
+      fuse = !w0.empty() && ndep <= kFusedRdep;
+    }
+    if (s->perm_on)
+      for (int b : sweeps[0].local) fuse = fuse && (s->perm_col[b] & ~lmask) == 0;
+    else
+      fuse = fuse && s->alt != nullptr;  // a lone diagonal: only when the second buffer exists
+  }
+  if (!fuse) MSV_TRY(flush_pending_T<T>(s));
   std::vector<C> pre(n, 1.0), post(n, 1.0);
   C cst = 1.0;
   bool pre_on = false;
@@ -807,23 +861,80 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
     if (ntgrp > kMaxTileGroups) return fail(MSV_E_INVALID, "internal: too many tile-number bits");
     A.ntgrp = static_cast<uint32_t>(ntgrp);
     std::vector<V> tgrp;
-    tgrp.reserve(2 * 16 * ntgrp);
+    tgrp.reserve(2 * 16 * kMaxTileGroups);
     std::vector<C> one(n, 1.0);
     A.has_pre = (i == 0 && pre_on) ? 1u : 0u;
     build_layer_diag<T>(A, A.seg[0], sw.local, rb, A.has_pre ? pre : one, 1.0, A.pre, ntgrp, &tgrp);
     A.has_post = (i + 1 == sweeps.size()) ? 1u : 0u;
     build_layer_diag<T>(A, A.seg[wins.size() - 1], sw.local, rb, A.has_post ? post : one,
                         A.has_post ? cst : C(1.0), A.post, ntgrp, &tgrp);
-    if (ntgrp > 0 && (A.has_pre || A.has_post)) {
+    if (A.has_pre || A.has_post) {
       void *d = nullptr;
       MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &d));
       A.tile_tab = reinterpret_cast<const V *>(d);
-    } else {
-      A.ntgrp = 0;
     }
     const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+    if (i == 0 && fuse) {
+      std::vector<uint64_t> col(n);
+      for (int b = 0; b < n; ++b) col[b] = s->perm_on ? s->perm_col[b] : (1ull << b);
+      A.pc = s->perm_on ? s->perm_c : 0;
+      // images of the tile-number bits, as four byte tables
+      std::vector<uint64_t> timg;
+      for (uint32_t r = 0; r < A.nruns; ++r)
+        for (uint32_t w = 0; w < A.run_width[r]; ++w) timg.push_back(col[A.run_dst[r] + w]);
+      if (timg.size() > static_cast<size_t>(kMaxTileBits))
+        return fail(MSV_E_INVALID, "internal: tile number too wide");
+      std::vector<uint64_t> ptab(4 * 256, 0);
+      for (int b = 0; b < 4; ++b)
+        for (int v = 0; v < 256; ++v) {
+          uint64_t x = 0;
+          for (int i = 0; i < 8; ++i) {
+            const size_t k = 8 * static_cast<size_t>(b) + i;
+            if (((v >> i) & 1) && k < timg.size()) x ^= timg[k];
+          }
+          ptab[256 * b + v] = x;
+        }
+      void *d_ptab = nullptr;
+      MSV_TRY(stage(s, ptab.data(), ptab.size() * sizeof(uint64_t), &d_ptab));
+      const TileSeg &s0 = A.seg[0];
+      for (int k = 0; k < kTileThreadBits; ++k)
+        A.pthr[k] = (k < nlocal - rb) ? col[s0.gi_bit[k]] : 0ull;
+      uint64_t umask = 0;
+      for (int j = 0; j < rb; ++j) {
+        A.preg[j] = col[sw.local[s0.reg0 + j]];
+        umask |= A.preg[j];
+      }
+      const V *d_lut = nullptr;
+      if (s->pend_on) {
+        // windows that see the register images first (their count in ph_rdep)
+        A.ph = s->pend_args;
+        std::vector<PhaseWin> dep, rest;
+        for (uint32_t w = 0; w < A.ph.nwin; ++w) {
+          const PhaseWin &pw = s->pend_args.win[w];
+          const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
+          (wm & umask ? dep : rest).push_back(pw);
+        }
+        A.ph_rdep = static_cast<uint32_t>(dep.size());
+        uint32_t w = 0;
+        for (const PhaseWin &pw : dep) A.ph.win[w++] = pw;
+        for (const PhaseWin &pw : rest) A.ph.win[w++] = pw;
+        void *d = nullptr;
+        MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d));
+        d_lut = reinterpret_cast<const V *>(d);
+      }
+      MSV_TRY(ensure_alt(s));
+      const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+      const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
+      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->alt), A, rb, fgrid, s->stream,
+                            reinterpret_cast<const V *>(s->buf), d_lut,
+                            reinterpret_cast<const uint64_t *>(d_ptab)));
+      std::swap(s->buf, s->alt);
+      s->perm_on = false;
+      s->pend_on = false;
+    } else {
+      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+    }
     s->sweeps += 1;
   }
   return MSV_OK;
@@ -850,6 +961,7 @@ int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
       if (act_bit[b]) ok = split_rotation(mats + 8 * (n - 1 - b), &zb[b]);
     if (ok) return run_rot_layer<T>(s, act_bit, zb);
   }
+  MSV_TRY(flush_pending_T<T>(s));
   // low sweep
   {
     std::vector<int> local = sweep_local_bits<T>(n, L, true, {0, 0});
@@ -1206,6 +1318,10 @@ int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, s
 
 template <typename T>
 int flush_pending_T(msv_state *s) {
+  if (s->perm_on) {
+    s->perm_on = false;
+    return launch_gather<T>(s, s->perm_col, s->perm_c);  // applies a pending diagonal too
+  }
   if (!s->pend_on) return MSV_OK;
   s->pend_on = false;
   return launch_phase_chunk<T>(s, s->pend_args, s->pend_lut.data(), s->pend_lut.size());
@@ -1355,6 +1471,29 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
     }
     if (rank != n) return fail(MSV_E_INVALID, "xor_affine: matrix is singular over GF(2)");
   }
+  if (g_tune.defer_perm) {
+    // A pending permutation is run first (two maps are not composed); a
+    // pending diagonal stays and now acts at the source index.
+    if (s->perm_on) {
+      s->perm_on = false;
+      MSV_TRY(launch_gather<T>(s, s->perm_col, s->perm_c));
+    }
+    MSV_TRY(ensure_alt(s));  // reserve the second buffer now: no allocation later
+    s->perm_on = true;
+    s->perm_col = col;
+    s->perm_c = c;
+    return MSV_OK;
+  }
+  MSV_TRY(flush_pending_T<T>(s));
+  return launch_gather<T>(s, col, c);
+}
+
+// out[y] = D(src(y)) in[src(y)], src(y) = c ^ XOR of col[j] over the set bits
+// j of y (D only when a diagonal is pending), into the second buffer.
+template <typename T>
+int launch_gather(msv_state *s, const std::vector<uint64_t> &col, uint64_t c) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
   const int ntab = (n + 7) / 8;
   std::vector<uint64_t> tab(static_cast<size_t>(ntab) * 256, 0);
   for (int t = 0; t < ntab; ++t)
@@ -1609,7 +1748,7 @@ int init_basis_T(msv_state *s, uint64_t index) {
 // point that reads or writes amplitudes calls it first, except the XOR
 // permutation, which absorbs it, and the phase layers themselves.
 int flush_pending(msv_state *s) {
-  if (!s->pend_on) return MSV_OK;
+  if (!s->pend_on && !s->perm_on) return MSV_OK;
   MSV_TRY(set_dev(s));
   return MSV_DISPATCH(s, flush_pending_T, s);
 }
@@ -1855,7 +1994,8 @@ int msv_state_set_basis(msv_state *s, uint64_t index) {
   MSV_TRY(check_state(s));
   if (index >= s->amps) return fail(MSV_E_INVALID, "basis index out of range");
   MSV_TRY(set_dev(s));
-  s->pend_on = false;  // the new basis state replaces what a pending diagonal would act on
+  s->pend_on = false;  // the new basis state replaces what deferred work would act on
+  s->perm_on = false;
   return MSV_DISPATCH(s, init_basis_T, s, index);
 }
 
@@ -1901,7 +2041,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
   MSV_TRY(flush_pending(src));
   MSV_TRY(set_dev(dst));
-  if (dst != src) dst->pend_on = false;  // overwritten
+  if (dst != src) dst->pend_on = dst->perm_on = false;  // overwritten
   MSV_HIP(hipStreamSynchronize(src->stream));
   const uint32_t grid = elem_grid(dst, dst->amps);
   if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
@@ -1949,6 +2089,7 @@ int msv_state_sync(msv_state *s) {
 int msv_state_release_scratch(msv_state *s) {
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
+  MSV_TRY(flush_pending(s));  // a deferred permutation still needs the buffer
   MSV_HIP(hipStreamSynchronize(s->stream));
   if (s->alt) MSV_HIP(hipFree(s->alt));
   s->alt = nullptr;
@@ -1966,7 +2107,6 @@ int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) 
   MSV_TRY(check_state(s));
   if (!mats) return fail(MSV_E_INVALID, "null matrix array");
   MSV_TRY(set_dev(s));
-  MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, apply_1q_layer_T, s, mats, active);
 }
 
@@ -2305,6 +2445,14 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer diag must be 0 or 1");
       g_tune.defer_diag = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_DEFER_PERM:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer perm must be 0 or 1");
+      g_tune.defer_perm = static_cast<int>(value);
+      return MSV_OK;
+    case MSV_TUNE_FUSED_BLOCKS_PER_CU:
+      if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "fused blocks per CU out of range");
+      g_tune.fused_blocks_per_cu = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2325,6 +2473,8 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_TILE_REG_BITS: *value = g_tune.reg_bits_c64; return MSV_OK;
     case MSV_TUNE_TILE_PREFETCH: *value = g_tune.tile_prefetch; return MSV_OK;
     case MSV_TUNE_DEFER_DIAG: *value = g_tune.defer_diag; return MSV_OK;
+    case MSV_TUNE_DEFER_PERM: *value = g_tune.defer_perm; return MSV_OK;
+    case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune.fused_blocks_per_cu; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

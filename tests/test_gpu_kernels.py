@@ -613,9 +613,141 @@ def test_deferred_phase_matches_immediate_at_26q(gpu):
                 s0 = s.sweep_count()
                 s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
                 s.apply_xor_affine(rows, c)
+                outs.append(s.to_numpy())  # runs the deferred work
                 sweeps.append(s.sweep_count() - s0)
-                outs.append(s.to_numpy())
     finally:
         nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
     assert sweeps == [2, 1]
     assert np.max(np.abs(outs[0] - outs[1])) < 1e-6
+
+
+def _chain(n):
+    return O.compose_inverse_affine([{"name": "CNOT", "wires": [i, i + 1]} for i in range(n - 1)], n)
+
+
+def _random_affine(n, rng):
+    ops = []
+    for _ in range(3 * n):
+        a, b = rng.choice(n, 2, replace=False)
+        ops.append({"name": "CNOT", "wires": [int(a), int(b)]})
+    ops.append({"name": "X", "wires": [int(rng.integers(n))]})
+    return O.compose_inverse_affine(ops, n)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("n", [9, 13, 14, 20])
+@pytest.mark.parametrize("with_phase", [False, True])
+@pytest.mark.parametrize("perm", ["chain", "random"])
+def test_deferred_permutation_read_by_next_1q_layer(gpu, dtype, n, with_phase, perm):
+    """An XOR permutation (with or without a deferred phase layer before it)
+    is held until the next operation; a 1q layer reads its first sweep
+    through it when the map keeps that sweep's tile bits in the tile (the
+    CNOT chain does), otherwise the gather runs first (random maps)."""
+    from mettleq_amd import _native as nat
+    rng = np.random.default_rng(7000 + n)
+    psi = _rand_state(n, 5 * n)
+    rows, c = _chain(n) if perm == "chain" else _random_affine(n, rng)
+    mats = [_rand_u(rng) for _ in range(n)]
+    bonds = [(i, (i + 1) % n) for i in range(n)]
+    th = rng.uniform(0, math.pi, n)
+    want = psi
+    with _state(n, dtype) as s:
+        s.from_numpy(psi)
+        if with_phase:
+            s.apply_phase_layer([nat.MSV_PHASE_ZZ] * n, bonds, list(th))
+            want = O.zz_layer_diag(n, bonds, th) * want
+        s.apply_xor_affine(rows, c)
+        want = O.xor_affine_gather(want, n, rows, c)
+        s.apply_1q_layer(mats)
+        want = O.apply_1q_layer(want, n, mats)
+        got = s.to_numpy()
+    tol = TOL64 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
+@pytest.mark.parametrize("after", ["probs", "qft", "gates", "xor", "phase", "inner", "sample",
+                                   "reset", "layer_subset", "scale"])
+def test_deferred_permutation_then_other_operations(gpu, after):
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import gate_u
+    n = 13
+    rng = np.random.default_rng(31337)
+    psi = _rand_state(n, 3)
+    rows, c = _chain(n)
+    rows2, c2 = _random_affine(n, rng)
+    mats = [_rand_u(rng) for _ in range(n)]
+    want = O.xor_affine_gather(psi, n, rows, c)
+    with _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_xor_affine(rows, c)
+        if after == "probs":
+            assert np.max(np.abs(s.probabilities() - np.abs(want) ** 2)) < TOL64
+        elif after == "qft":
+            s.apply_qft(0, n)
+            want = O.qft_fft(want, n)
+        elif after == "gates":
+            s.apply_gates([gate_u(4, mats[0]), gate_u(0, mats[1], [12])])
+            want = O.apply_controlled(O.apply_dense(want, n, mats[0], [4]), n, mats[1], [12], 0)
+        elif after == "xor":
+            s.apply_xor_affine(rows2, c2)
+            want = O.xor_affine_gather(want, n, rows2, c2)
+        elif after == "phase":
+            s.apply_phase_layer([nat.MSV_PHASE_CPHASE], [(2, 9)], [0.7])
+            want = O.cphase_layer_diag(n, [(2, 9)], [0.7]) * want
+        elif after == "inner":
+            with _state(n) as t:
+                t.from_numpy(psi)
+                assert abs(t.inner(s) - np.vdot(psi, want)) < 1e-5
+        elif after == "sample":
+            got = s.sample(200, None, seed=3)
+            assert np.array_equal(got, O.sample_indices(O.amp_probs_c64(s.to_numpy()), 200, 3, 0))
+        elif after == "reset":
+            s.reset()
+            want = O.zero_state(n)
+        elif after == "layer_subset":
+            act = [0, 5, 12]
+            s.apply_1q_layer(mats, active=act)
+            full = [mats[q] if q in act else np.eye(2) for q in range(n)]
+            want = O.apply_1q_layer(want, n, full)
+        elif after == "scale":
+            s.scale(-1j)
+            want = -1j * want
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+def test_deferred_permutation_fused_sweep_counts_at_26q(gpu):
+    """U3 layer, RZZ ring, CNOT chain, U3 layer at 26 qubits: with deferral
+    the ring and the chain ride in the second layer's first sweep (two
+    sweeps fewer); amplitudes agree with the undeferred run."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.workloads import ring_bonds
+    n = 26
+    rng = np.random.default_rng(2626)
+    m1 = [_rand_u(rng) for _ in range(n)]
+    m2 = [_rand_u(rng) for _ in range(n)]
+    bonds = ring_bonds(n)
+    th = [float(x) for x in rng.uniform(0, math.pi, len(bonds))]
+    rows, c = _chain(n)
+    outs, sweeps = [], []
+    with _state(n) as s:
+        s0 = s.sweep_count()
+        s.apply_1q_layer(m1)
+        u3 = s.sweep_count() - s0
+    try:
+        for d in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
+            nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, d)
+            with _state(n) as s:
+                s0 = s.sweep_count()
+                s.apply_1q_layer(m1)
+                s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
+                s.apply_xor_affine(rows, c)
+                s.apply_1q_layer(m2)
+                outs.append(s.to_numpy())
+                sweeps.append(s.sweep_count() - s0)
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
+        nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, 1)
+    assert sweeps == [u3 + 1 + 1 + u3, 2 * u3]
+    assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
