@@ -8,9 +8,11 @@ full QFT every 10th layer) on one GPU, inputs resident in HBM.
 value = sum over layers of (bytes moved by the layer's full-state sweeps)
         / (GPU time of the timed steps), in GB/s; one sweep moves
         2 * 2^n * 8 bytes (read + write of the complex64 state).
-roofline = the dominant kernel, k_rot (the rotation sweeps of the U3 layers):
-        algorithmic bytes per launch / mean launch duration from HIP events
-        on the state's stream; the other sweep kernels are listed beside it.
+roofline = the dominant kernel, k_rot<float, 4, false> (the rotation sweeps of
+        the U3 layers): algorithmic bytes per launch / mean launch duration
+        from HIP events on the state's stream (U3 layers that read no
+        deferred work: 3 launches each); the other sweep kernels, including
+        k_rot's gather variant that reads deferred layers, are listed beside it.
 cpu_baseline = the oracle's gate-by-gate CPU restatement (oracle/sv_cpu.c)
         timed on this host on a bounded sample (one layer of each type at
         --cpu-qubits), reported in the same unit.
@@ -439,15 +441,22 @@ def main():
     # per-layer breakdown (one instrumented pass)
     state.reset()
     state.sync()
+    # A U3 layer that follows deferred layers (S = 0) reads them in its first
+    # sweep (k_rot's gather variant): it is listed apart as "u3_fused".
     per_type = {}
+    labels = []
+    prev_deferred = False
     for layer in layers:
         s0 = state.sweep_count()
         state.timer_start()
         apply_layer(state, layer, n, lay)
         ms = state.timer_stop()
         S = state.sweep_count() - s0
-        d = per_type.setdefault(layer["type"], {"ms": [], "S": S})
+        label = "u3_fused" if layer["type"] == "u3" and prev_deferred else layer["type"]
+        labels.append(label)
+        d = per_type.setdefault(label, {"ms": [], "S": S})
         d["ms"].append(ms)
+        prev_deferred = S == 0
     log("per-layer: " + ", ".join(f"{k}: {np.mean(v['ms']):.3f} ms S={v['S']}" for k, v in per_type.items()))
 
     # timed region
@@ -463,21 +472,41 @@ def main():
     log(f"timed {args.steps} steps: {gpu_ms:.1f} ms GPU, {wall * 1e3:.1f} ms wall, {sweeps} sweeps")
     ms_per_step = gpu_ms / args.steps
     # value: bytes of all sweeps (the reset init writes the state once: 1/2 sweep)
-    layer_sweeps_per_step = sum(per_type[l["type"]]["S"] for l in layers)
-    layer_ms_per_step = sum(np.mean(per_type[l["type"]]["ms"]) for l in layers)
+    layer_sweeps_per_step = sum(per_type[l]["S"] for l in labels)
+    s_min = {"u3": 2, "qft": 2, "rzz": 1, "cnot": 1, "cz": 1}
+    s_min_per_step = sum(s_min[l["type"]] for l in layers)
     value = layer_sweeps_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9
 
-    # per sweep kernel: total time, launches, mean launch duration
-    kernel_of = {"u3": "k_rot", "qft": "k_qft", "rzz": "k_phase", "cnot": "k_xor_gather", "cz": "k_phase"}
+    # per sweep kernel: launches per step and mean launch duration (HIP events
+    # around each layer on the state's stream, divided by its sweeps)
+    kernel_of = {"u3": "k_rot<float, 4, false>", "qft": "k_qft<float, 4, false>", "rzz": "k_phase",
+                 "cnot": "k_xor_gather", "cz": "k_phase"}
     if per_type.get("rzz", {}).get("S", 1) == 0:
         kernel_of["cnot"] = "k_xor_gather (+ deferred RZZ ring)"
+    if per_type.get("cnot", {}).get("S", 1) == 0:
+        # the CNOT chain (and the RZZ ring before it) ride in the next layer's
+        # first sweep: k_rot's gather variant, or one k_xor_gather ahead of a QFT
+        kernel_of["qft"] = "k_qft<float, 4, false> (+ one k_xor_gather of the deferred RZZ ring + CNOT chain)"
     kernels = {}
     for t, v in per_type.items():
-        if v["S"] == 0:
+        if v["S"] == 0 or t == "u3_fused":
             continue  # deferred: its work runs inside the next layer's sweep
         k = kernels.setdefault(kernel_of[t], {"ms": 0.0, "launches": 0})
-        k["ms"] += float(np.sum(v["ms"]))
-        k["launches"] += v["S"] * len(v["ms"])
+        k["ms"] += float(np.mean(v["ms"])) * labels.count(t)
+        k["launches"] += v["S"] * labels.count(t)
+    if "u3_fused" in per_type:
+        # the fused layer = one gather-variant launch + (S - 1) plain ones,
+        # priced at the plain layers' mean launch when those exist
+        v = per_type["u3_fused"]
+        cnt = labels.count("u3_fused")
+        plain = kernels.get(kernel_of["u3"])
+        plain_avg = plain["ms"] / plain["launches"] if plain else float(np.mean(v["ms"])) / v["S"]
+        fused_ms = float(np.mean(v["ms"])) - (v["S"] - 1) * plain_avg
+        kernels["k_rot<float, 4, true> (first sweep reads the deferred RZZ ring + CNOT chain)"] = {
+            "ms": fused_ms * cnt, "launches": cnt}
+        k = kernels.setdefault(kernel_of["u3"], {"ms": 0.0, "launches": 0})
+        k["ms"] += (v["S"] - 1) * plain_avg * cnt
+        k["launches"] += (v["S"] - 1) * cnt
     for k in kernels.values():
         k["avg_launch_ms"] = round(k["ms"] / max(1, k["launches"]), 4)
         k["GBps"] = round(sweep_bytes / (k["avg_launch_ms"] * 1e-3) / 1e9, 1)
@@ -544,8 +573,13 @@ def main():
                                           / PEAK_GBPS, 1) if v["S"] else None)}
                    for k, v in per_type.items()},
         "sweeps_per_step": layer_sweeps_per_step,
-        "note": ("S = 0: the layer's work is deferred into the next layer's sweep (a phase layer "
-                 "rides in the following XOR gather); value counts only the sweeps performed"),
+        "layer_equivalent_GBps": round(s_min_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+        "note": ("S = 0: the layer's work is deferred into the next layer's first sweep (an RZZ ring "
+                 "and a CNOT chain are read through by the following U3 layer's first k_rot sweep, "
+                 "listed as u3_fused, or by one gather ahead of a QFT); value counts only the sweeps "
+                 "performed. "
+                 "layer_equivalent_GBps prices each layer at its lower bound of sweeps (diagonal 1, "
+                 "XOR 1, U3 2, QFT 2) over the same step time"),
         "roofline": {
             "bound": "hbm",
             "kernel": dom,

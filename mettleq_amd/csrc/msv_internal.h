@@ -129,6 +129,7 @@ struct PhaseArgs {
 constexpr int kMaxTileBits = 32;   // bits of a tile number
 constexpr int kMaxTileGroups = 8;  // 4-bit groups of a tile number
 constexpr int kFusedRdep = 4;      // deferred-diagonal windows a fused sweep evaluates per register
+constexpr int kFusedWins = 8;      // deferred-diagonal windows a fused sweep takes in all
 
 // Commuting rotation sweep (1q layers in Z-Y-Z split form, k_rot): per
 // segment, a rotation on each register bit set in gmask, given by its three
@@ -169,7 +170,8 @@ struct RotArgs {
   uint64_t preg[5];                     // image of segment 0's register bit j
   uint32_t ph_rdep;                     // windows [0, ph_rdep) of ph see the register images
   uint32_t ph_pad;
-  PhaseArgs ph;                         // the deferred diagonal (nwin = 0: none)
+  uint16_t ph_rimg[kFusedRdep][32];     // their table-index bits of register image r
+  PhaseArgs ph;                         // the deferred diagonal (nwin = 0: none; at most kFusedWins)
 };
 static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4 KiB");
 static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");

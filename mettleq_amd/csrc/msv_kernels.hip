@@ -292,11 +292,14 @@ __device__ __forceinline__ void load_first(const typename Vec2<T>::type *__restr
 }
 
 // Lookup-table index of phase window w for amplitude index i.
-__device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) {
+// A window's table index without its offset: linear (over XOR) in i.
+__device__ __forceinline__ uint32_t win_bits(const PhaseWin &w, uint64_t i) {
   const uint32_t a = static_cast<uint32_t>(i >> w.s1) & ((1u << w.w1) - 1u);
   const uint32_t b = static_cast<uint32_t>(i >> w.s2) & ((1u << w.w2) - 1u);
-  return w.off + (a | (b << w.w1));
+  return a | (b << w.w1);
 }
+
+__device__ __forceinline__ uint32_t win_index(const PhaseWin &w, uint64_t i) { return w.off + win_bits(w, i); }
 
 // Segment `s` of a tile: its thread's local and index bits.
 struct SegPos {
@@ -511,8 +514,20 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
   V *lut = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
+  // Thread part of the source index and of each window's table index:
+  // both are linear in the thread bits, so they are computed once here and
+  // each tile adds its wave-uniform part with one XOR.
+  uint64_t th = 0;
+  uint32_t thw[kFusedWins];
   if constexpr (PERM) {
     for (uint32_t i = tid; i < A.ph.lut_len; i += blockDim.x) lut[i] = ph_lut[i];
+    const uint32_t tm = tid & A.seg[0].tmask;
+#pragma unroll
+    for (int k = 0; k < kTileThreadBits; ++k)
+      if ((tm >> k) & 1u) th ^= A.pthr[k];
+#pragma unroll
+    for (int w = 0; w < kFusedWins; ++w)
+      thw[w] = (w < static_cast<int>(A.ph.nwin)) ? win_bits(A.ph.win[w], th) : 0u;
     __syncthreads();
   }
   const uint32_t last = A.nseg - 1;
@@ -525,11 +540,9 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
       if (s == 0) {
         if constexpr (PERM) {
           // image of the tile number: four byte tables (scalar loads)
-          uint64_t sb = A.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
-                        ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
-#pragma unroll
-          for (int k = 0; k < kTileThreadBits; ++k)
-            if ((p.tm >> k) & 1u) sb ^= A.pthr[k];
+          const uint64_t tsb = A.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
+                               ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
+          const uint64_t sb = tsb ^ th;
           uint64_t ri[R];
           ri[0] = 0;
 #pragma unroll
@@ -537,29 +550,32 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
 #pragma unroll
           for (int r = 0; r < R; ++r) a[r] = ld_amp(src + (sb ^ ri[r]));
           if (A.ph.nwin) {
-            // windows [0, nrdep) see the register bits (the host orders
-            // them first); the others are one product per thread. A
-            // window's index is linear in the amplitude index, so the
-            // per-register part is an XOR of wave-uniform bits.
-            const uint32_t nrdep = A.ph_rdep;
+            // windows [0, ph_rdep) see the register bits (the host orders
+            // them first; their register parts are in ph_rimg), the others
+            // give one product per thread
+            const uint32_t nw = A.ph.nwin, nrdep = A.ph_rdep;
             V pt;
             pt.x = T(1);
             pt.y = T(0);
-            for (uint32_t w = nrdep; w < A.ph.nwin; ++w) pt = cmul(pt, lut[win_index(A.ph.win[w], sb)]);
             uint32_t bw[kFusedRdep];
 #pragma unroll
-            for (int w = 0; w < kFusedRdep; ++w)
-              bw[w] = (w < static_cast<int>(nrdep)) ? win_index(A.ph.win[w], sb) - A.ph.win[w].off : 0u;
+            for (int w = 0; w < kFusedWins; ++w) {
+              if (w < static_cast<int>(nw)) {
+                const PhaseWin &pw = A.ph.win[w];
+                const uint32_t ix = win_bits(pw, tsb) ^ thw[w];
+                if (w < kFusedRdep && w < static_cast<int>(nrdep))
+                  bw[w < kFusedRdep ? w : 0] = ix;
+                else
+                  pt = cmul(pt, lut[pw.off + ix]);
+              }
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               V q = pt;
 #pragma unroll
               for (int w = 0; w < kFusedRdep; ++w)
-                if (w < static_cast<int>(nrdep)) {
-                  const PhaseWin &pw = A.ph.win[w];
-                  const uint32_t rb = win_index(pw, ri[r]) - pw.off;
-                  q = cmul(q, lut[pw.off + (bw[w] ^ rb)]);
-                }
+                if (w < static_cast<int>(nrdep))
+                  q = cmul(q, lut[A.ph.win[w].off + (bw[w] ^ A.ph_rimg[w][r])]);
               a[r] = cmul(a[r], q);
             }
           }

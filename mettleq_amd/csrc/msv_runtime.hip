@@ -814,7 +814,7 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
         const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
         ndep += (wm & umask) ? 1 : 0;
       }
-      fuse = !w0.empty() && ndep <= kFusedRdep;
+      fuse = !w0.empty() && ndep <= kFusedRdep && s->pend_args.nwin <= static_cast<uint32_t>(kFusedWins);
     }
     if (s->perm_on)
       for (int b : sweeps[0].local) fuse = fuse && (s->perm_col[b] & ~lmask) == 0;
@@ -915,10 +915,24 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
           const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
           (wm & umask ? dep : rest).push_back(pw);
         }
+        if (dep.size() > static_cast<size_t>(kFusedRdep) || A.ph.nwin > static_cast<uint32_t>(kFusedWins))
+          return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for the fused sweep");
         A.ph_rdep = static_cast<uint32_t>(dep.size());
         uint32_t w = 0;
         for (const PhaseWin &pw : dep) A.ph.win[w++] = pw;
         for (const PhaseWin &pw : rest) A.ph.win[w++] = pw;
+        // table-index bits of the register images (XOR of the set bits' images)
+        for (size_t k = 0; k < dep.size(); ++k) {
+          const PhaseWin &pw = dep[k];
+          for (int r = 0; r < (1 << rb); ++r) {
+            uint64_t img = 0;
+            for (int j = 0; j < rb; ++j)
+              if ((r >> j) & 1) img ^= A.preg[j];
+            const uint32_t lo = static_cast<uint32_t>(img >> pw.s1) & ((1u << pw.w1) - 1u);
+            const uint32_t hi = static_cast<uint32_t>(img >> pw.s2) & ((1u << pw.w2) - 1u);
+            A.ph_rimg[k][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
+          }
+        }
         void *d = nullptr;
         MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d));
         d_lut = reinterpret_cast<const V *>(d);
