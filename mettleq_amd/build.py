@@ -8,6 +8,13 @@ hipcc cross-compiles for gfx950 without a GPU. Flags:
       amplitude arrays of the tile kernel out of scratch memory (sinking the
       per-target butterfly cases into one block turns register indices into
       pointer phis, which forces the array into scratch).
+  -mllvm -instcombine-max-copied-from-constant-users=4096
+      keeps the sweep kernels' by-value argument blocks out of scratch memory:
+      the frontend copies a by-value kernel argument whose address is taken
+      into private memory, and InstCombine forwards that copy back to the
+      kernel argument segment only while it has at most this many users (the
+      default, 300, is below the use count of the fused first sweeps, which
+      then read their 2.6 KiB of arguments from scratch).
 """
 from __future__ import annotations
 
@@ -31,6 +38,7 @@ HIP_FLAGS = [
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-mllvm", "-simplifycfg-sink-common=false",
+    "-mllvm", "-instcombine-max-copied-from-constant-users=4096",
 ]
 
 

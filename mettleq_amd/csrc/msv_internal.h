@@ -131,6 +131,22 @@ constexpr int kMaxTileGroups = 8;  // 4-bit groups of a tile number
 constexpr int kFusedRdep = 4;      // deferred-diagonal windows a fused sweep evaluates per register
 constexpr int kFusedWins = 8;      // deferred-diagonal windows a fused sweep takes in all
 
+// First segment of a sweep that reads through deferred work (k_rot and k_qft
+// with PERM): it loads amplitude y from src(y) = M'y ^ c of the source
+// buffer (identity M' and c = 0 for a lone diagonal), times a deferred
+// diagonal D(src(y)), and the sweep stores into the other buffer. src(y) =
+// c ^ image of the tile number (four byte tables, ptab) ^ pthr(thread bits)
+// ^ preg(register bits), each the XOR of the images of the set bits.
+struct FusedRead {
+  uint64_t pc;                          // c
+  uint64_t pthr[kTileThreadBits];       // image of segment 0's thread-id bit k
+  uint64_t preg[5];                     // image of segment 0's register bit j
+  uint32_t ph_rdep;                     // windows [0, ph_rdep) of ph see the register images
+  uint32_t ph_pad;
+  uint16_t ph_rimg[kFusedRdep][32];     // their table-index bits of register image r
+  PhaseArgs ph;                         // the deferred diagonal (nwin = 0: none; at most kFusedWins)
+};
+
 // Commuting rotation sweep (1q layers in Z-Y-Z split form, k_rot): per
 // segment, a rotation on each register bit set in gmask, given by its three
 // shear factors (a, b); a diagonal over all n index bits before the first
@@ -158,20 +174,7 @@ struct RotArgs {
   T shear[kMaxSegs][5][2];
   T pre[kDtabVals];
   T post[kDtabVals];
-  // Deferred permutation (k_rot<.., PERM>): the first segment loads
-  // amplitude y from src(y) = M'y ^ c of the source buffer, times a deferred
-  // diagonal D(src(y)), and the sweep stores into the other buffer. M' maps
-  // the tile's local bits into local bits, so each tile reads one source
-  // tile. src(y) = c ^ image of the tile number (byte tables at ptab) ^
-  // pthr(thread bits) ^ preg(register bits), each the XOR of the images of
-  // the set bits.
-  uint64_t pc;                          // c
-  uint64_t pthr[kTileThreadBits];       // image of segment 0's thread-id bit k
-  uint64_t preg[5];                     // image of segment 0's register bit j
-  uint32_t ph_rdep;                     // windows [0, ph_rdep) of ph see the register images
-  uint32_t ph_pad;
-  uint16_t ph_rimg[kFusedRdep][32];     // their table-index bits of register image r
-  PhaseArgs ph;                         // the deferred diagonal (nwin = 0: none; at most kFusedWins)
+  FusedRead fr;                         // k_rot<.., PERM = true> only
 };
 static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4 KiB");
 static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");
@@ -200,16 +203,18 @@ struct QftArgs {
   uint32_t pad;
   T scale;
   TileSeg seg[kMaxSegs];
+  FusedRead fr;            // k_qft<.., PERM = true> only
 };
+static_assert(sizeof(QftArgs<float>) <= 4096, "QFT sweep arguments exceed 4 KiB");
 static_assert(sizeof(QftArgs<double>) <= 4096, "QFT sweep arguments exceed 4 KiB");
 
 // Kernel launchers (msv_kernels.hip). All enqueue on `stream`.
 template <typename T>
 hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, int rb,
                        uint32_t grid, hipStream_t stream);
-// perm_src != nullptr: the first segment reads through the deferred
-// permutation from perm_src (with the diagonal's tables at ph_lut and the
-// tile-number images in four 256-entry byte tables at ptab) and the sweep
+// perm_src != nullptr: the first segment reads through deferred work
+// (args.fr) from perm_src, with the diagonal's tables at ph_lut and the
+// tile-number images in four 256-entry byte tables at ptab, and the sweep
 // writes `state`.
 template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
@@ -217,7 +222,8 @@ hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int
                       const typename Vec2<T>::type *ph_lut = nullptr, const uint64_t *ptab = nullptr);
 template <typename T>
 hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
-                      uint32_t grid, hipStream_t stream);
+                      uint32_t grid, hipStream_t stream, const typename Vec2<T>::type *perm_src = nullptr,
+                      const typename Vec2<T>::type *ph_lut = nullptr, const uint64_t *ptab = nullptr);
 template <typename T>
 hipError_t launch_phase(typename Vec2<T>::type *state, const typename Vec2<T>::type *lut,
                         const PhaseArgs &args, uint32_t grid, hipStream_t stream);

@@ -492,14 +492,91 @@ __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec
   return f;
 }
 
-// PERM: the first segment gathers through a deferred permutation (see
-// RotArgs): thread-uniform and register parts of the source index are XORs
-// of precomputed images, the deferred diagonal's window tables sit in LDS
-// behind the exchange buffer, and windows the register images do not touch
-// are evaluated once per thread. Reads come from `src`, stores go to `st`
-// (the other buffer), so no tile reads what another tile writes.
+// Fused read of deferred work (FusedRead), per block: the diagonal's
+// tables into LDS, and the thread parts of the source index and of each
+// window's table index. Both are linear (over XOR) in the thread bits, so
+// each tile adds its wave-uniform part with one XOR.
+template <typename T>
+__device__ __forceinline__ void fused_setup(const FusedRead &F, const TileSeg &s0, uint32_t tid,
+                                            const typename Vec2<T>::type *__restrict__ ph_lut,
+                                            typename Vec2<T>::type *lut, uint64_t &th,
+                                            uint32_t (&thw)[kFusedWins]) {
+  for (uint32_t i = tid; i < F.ph.lut_len; i += blockDim.x) lut[i] = ph_lut[i];
+  const uint32_t tm = tid & s0.tmask;
+  th = 0;
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k)
+    if ((tm >> k) & 1u) th ^= F.pthr[k];
+#pragma unroll
+  for (int w = 0; w < kFusedWins; ++w)
+    thw[w] = (w < static_cast<int>(F.ph.nwin)) ? win_bits(F.ph.win[w], th) : 0u;
+  __syncthreads();
+}
+
+// Fused read, per tile: segment 0's amplitudes from src through the
+// deferred permutation, times the deferred diagonal. Windows [0, ph_rdep)
+// see the register bits (the host orders them first; their register parts
+// are in ph_rimg), the others give one product per thread.
+template <typename T, int RB>
+__device__ __forceinline__ void fused_load(typename Vec2<T>::type (&a)[1 << RB],
+                                           const typename Vec2<T>::type *__restrict__ src,
+                                           const FusedRead &F, const uint64_t *__restrict__ ptab,
+                                           uint64_t tile, uint64_t th, const uint32_t (&thw)[kFusedWins],
+                                           const typename Vec2<T>::type *lut) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  // image of the tile number: four byte tables (scalar loads)
+  const uint64_t tsb = F.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
+                       ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
+  const uint64_t sb = tsb ^ th;
+  uint64_t ri[R];
+  ri[0] = 0;
+#pragma unroll
+  for (int r = 1; r < R; ++r) ri[r] = ri[r & (r - 1)] ^ F.preg[__builtin_ctz(r)];
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] = ld_amp(src + (sb ^ ri[r]));
+  if (F.ph.nwin) {
+    const uint32_t nw = F.ph.nwin, nrdep = F.ph_rdep;
+    V pt;
+    pt.x = T(1);
+    pt.y = T(0);
+    uint32_t bw[kFusedRdep];
+#pragma unroll
+    for (int w = 0; w < kFusedWins; ++w) {
+      if (w < static_cast<int>(nw)) {
+        const PhaseWin &pw = F.ph.win[w];
+        const uint32_t ix = win_bits(pw, tsb) ^ thw[w];
+        if (w < kFusedRdep && w < static_cast<int>(nrdep))
+          bw[w < kFusedRdep ? w : 0] = ix;
+        else
+          pt = cmul(pt, lut[pw.off + ix]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V q = pt;
+#pragma unroll
+      for (int w = 0; w < kFusedRdep; ++w)
+        if (w < static_cast<int>(nrdep)) q = cmul(q, lut[F.ph.win[w].off + (bw[w] ^ F.ph_rimg[w][r])]);
+      a[r] = cmul(a[r], q);
+    }
+  }
+}
+
+// PERM: the first segment reads through deferred work (fused_load), with
+// the deferred diagonal's window tables in LDS behind the exchange buffer.
+// Reads come from `src`, stores go to `st` (the other buffer), so no tile
+// reads what another tile writes.
+// Waves per SIMD a rotation or QFT sweep must keep: complex64 tiles with
+// 4 register bits run 512-thread blocks, two per CU (64 KiB of LDS each),
+// which needs 4 waves per SIMD, i.e. at most 128 VGPRs.
+template <typename T, int RB>
+struct SweepWaves {
+  static constexpr int v = (RB == 4 && sizeof(T) == 4) ? 4 : 1;
+};
+
 template <typename T, int RB, bool PERM>
-__global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict__ st,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_rot(typename Vec2<T>::type *__restrict__ st,
                                              const typename Vec2<T>::type *__restrict__ ttab,
                                              const RotArgs<T> A,
                                              const typename Vec2<T>::type *__restrict__ src,
@@ -514,22 +591,9 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
   V *lut = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
-  // Thread part of the source index and of each window's table index:
-  // both are linear in the thread bits, so they are computed once here and
-  // each tile adds its wave-uniform part with one XOR.
   uint64_t th = 0;
   uint32_t thw[kFusedWins];
-  if constexpr (PERM) {
-    for (uint32_t i = tid; i < A.ph.lut_len; i += blockDim.x) lut[i] = ph_lut[i];
-    const uint32_t tm = tid & A.seg[0].tmask;
-#pragma unroll
-    for (int k = 0; k < kTileThreadBits; ++k)
-      if ((tm >> k) & 1u) th ^= A.pthr[k];
-#pragma unroll
-    for (int w = 0; w < kFusedWins; ++w)
-      thw[w] = (w < static_cast<int>(A.ph.nwin)) ? win_bits(A.ph.win[w], th) : 0u;
-    __syncthreads();
-  }
+  if constexpr (PERM) fused_setup<T>(A.fr, A.seg[0], tid, ph_lut, lut, th, thw);
   const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
@@ -539,46 +603,7 @@ __global__ void __launch_bounds__(512) k_rot(typename Vec2<T>::type *__restrict_
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (s == 0) {
         if constexpr (PERM) {
-          // image of the tile number: four byte tables (scalar loads)
-          const uint64_t tsb = A.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
-                               ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
-          const uint64_t sb = tsb ^ th;
-          uint64_t ri[R];
-          ri[0] = 0;
-#pragma unroll
-          for (int r = 1; r < R; ++r) ri[r] = ri[r & (r - 1)] ^ A.preg[__builtin_ctz(r)];
-#pragma unroll
-          for (int r = 0; r < R; ++r) a[r] = ld_amp(src + (sb ^ ri[r]));
-          if (A.ph.nwin) {
-            // windows [0, ph_rdep) see the register bits (the host orders
-            // them first; their register parts are in ph_rimg), the others
-            // give one product per thread
-            const uint32_t nw = A.ph.nwin, nrdep = A.ph_rdep;
-            V pt;
-            pt.x = T(1);
-            pt.y = T(0);
-            uint32_t bw[kFusedRdep];
-#pragma unroll
-            for (int w = 0; w < kFusedWins; ++w) {
-              if (w < static_cast<int>(nw)) {
-                const PhaseWin &pw = A.ph.win[w];
-                const uint32_t ix = win_bits(pw, tsb) ^ thw[w];
-                if (w < kFusedRdep && w < static_cast<int>(nrdep))
-                  bw[w < kFusedRdep ? w : 0] = ix;
-                else
-                  pt = cmul(pt, lut[pw.off + ix]);
-              }
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-              V q = pt;
-#pragma unroll
-              for (int w = 0; w < kFusedRdep; ++w)
-                if (w < static_cast<int>(nrdep))
-                  q = cmul(q, lut[A.ph.win[w].off + (bw[w] ^ A.ph_rimg[w][r])]);
-              a[r] = cmul(a[r], q);
-            }
-          }
+          fused_load<T, RB>(a, src, A.fr, ptab, tile, th, thw, lut);
         } else {
           uint64_t gstr[RB];
 #pragma unroll
@@ -742,9 +767,12 @@ __device__ __forceinline__ void qft_segment(typename Vec2<T>::type *a, const Qft
   }
 }
 
-template <typename T, int RB, bool INV>
-__global__ void __launch_bounds__(512) k_qft(typename Vec2<T>::type *__restrict__ st,
-                                             const QftArgs<T> A) {
+template <typename T, int RB, bool INV, bool PERM>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_qft(typename Vec2<T>::type *__restrict__ st,
+                                             const QftArgs<T> A,
+                                             const typename Vec2<T>::type *__restrict__ src,
+                                             const typename Vec2<T>::type *__restrict__ ph_lut,
+                                             const uint64_t *__restrict__ ptab) {
   using V = typename Vec2<T>::type;
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
@@ -753,6 +781,10 @@ __global__ void __launch_bounds__(512) k_qft(typename Vec2<T>::type *__restrict_
   uint64_t vrank = blockIdx.x;
   if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
     vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  V *lut = lds + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
+  uint64_t th = 0;
+  uint32_t thw[kFusedWins];
+  if constexpr (PERM) fused_setup<T>(A.fr, A.seg[0], tid, ph_lut, lut, th, thw);
   const uint32_t last = A.nseg - 1;
   for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
@@ -761,11 +793,15 @@ __global__ void __launch_bounds__(512) k_qft(typename Vec2<T>::type *__restrict_
       const TileSeg &sg = A.seg[s];
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (s == 0) {
-        uint64_t gstr[RB];
+        if constexpr (PERM) {
+          fused_load<T, RB>(a, src, A.fr, ptab, tile, th, thw, lut);
+        } else {
+          uint64_t gstr[RB];
 #pragma unroll
-        for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+          for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+          for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+        }
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -1325,7 +1361,7 @@ hipError_t launch_rot_rb(typename Vec2<T>::type *state, const RotArgs<T> &args, 
   const uint32_t threads = 1u << (args.nlocal - RB);
   const size_t xlds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
   if (perm_src) {
-    const size_t lds = xlds + sizeof(V) * args.ph.lut_len;
+    const size_t lds = xlds + sizeof(V) * args.fr.ph.lut_len;
     const hipError_t e = allow_lds(k_rot<T, RB, true>, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_rot<T, RB, true>), dim3(grid), dim3(threads), lds, stream, state,
@@ -1356,15 +1392,36 @@ hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int
 
 template <typename T>
 hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
-                      uint32_t grid, hipStream_t stream) {
+                      uint32_t grid, hipStream_t stream, const typename Vec2<T>::type *perm_src,
+                      const typename Vec2<T>::type *ph_lut, const uint64_t *ptab) {
   using V = typename Vec2<T>::type;
   const uint32_t threads = 1u << (args.nlocal - rb);
   const size_t lds = args.nseg > 1 ? (sizeof(V) << args.nlocal) : 0;
-#define MSV_QFT_LAUNCH(RBV)                                                                        \
-  if (inverse)                                                                                     \
-    hipLaunchKernelGGL((k_qft<T, RBV, true>), dim3(grid), dim3(threads), lds, stream, state, args); \
-  else                                                                                             \
-    hipLaunchKernelGGL((k_qft<T, RBV, false>), dim3(grid), dim3(threads), lds, stream, state, args);
+  if (perm_src) {
+    // the fused read exists for 4 register bits (both types' default window)
+    constexpr int RBF = 4;
+    if (rb != RBF) return hipErrorInvalidValue;
+    const size_t flds = lds + sizeof(V) * args.fr.ph.lut_len;
+    const hipError_t e = inverse ? allow_lds(k_qft<T, RBF, true, true>, flds)
+                                 : allow_lds(k_qft<T, RBF, false, true>, flds);
+    if (e != hipSuccess) return e;
+    if (inverse)
+      hipLaunchKernelGGL((k_qft<T, RBF, true, true>), dim3(grid), dim3(threads), flds, stream, state, args,
+                         perm_src, ph_lut, ptab);
+    else
+      hipLaunchKernelGGL((k_qft<T, RBF, false, true>), dim3(grid), dim3(threads), flds, stream, state, args,
+                         perm_src, ph_lut, ptab);
+    return hipGetLastError();
+  }
+  const V *nov = nullptr;
+  const uint64_t *nou = nullptr;
+#define MSV_QFT_LAUNCH(RBV)                                                                          \
+  if (inverse)                                                                                       \
+    hipLaunchKernelGGL((k_qft<T, RBV, true, false>), dim3(grid), dim3(threads), lds, stream, state,  \
+                       args, nov, nov, nou);                                                         \
+  else                                                                                               \
+    hipLaunchKernelGGL((k_qft<T, RBV, false, false>), dim3(grid), dim3(threads), lds, stream, state, \
+                       args, nov, nov, nou);
   switch (rb) {
     case 1: MSV_QFT_LAUNCH(1) break;
     case 2: MSV_QFT_LAUNCH(2) break;
@@ -1502,7 +1559,8 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
                                     hipStream_t, const Vec2<T>::type *, const Vec2<T>::type *,   \
                                     const uint64_t *);                                           \
   template hipError_t launch_qft<T>(Vec2<T>::type *, const QftArgs<T> &, int, bool, uint32_t,    \
-                                    hipStream_t);                                                \
+                                    hipStream_t, const Vec2<T>::type *, const Vec2<T>::type *,   \
+                                    const uint64_t *);                                           \
   template hipError_t launch_phase<T>(Vec2<T>::type *, const Vec2<T>::type *, const PhaseArgs &, \
                                       uint32_t, hipStream_t);                                    \
   template hipError_t launch_xor_gather<T>(const Vec2<T>::type *, Vec2<T>::type *, uint64_t,     \

@@ -759,6 +759,114 @@ void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &
 // blocks of 64 KiB exchange buffer + 16 KiB tables fill a CU's 160 KiB.
 constexpr size_t kFusedLutMaxBytes = 16 * 1024;
 
+// Whether a sweep whose segment 0 holds local bits [reg0, reg0 + rb) of
+// `local` in registers can read through the state's deferred work
+// (FusedRead): the diagonal's tables fit in LDS beside the exchange buffer
+// and the register images touch at most kFusedRdep of its windows; the
+// permutation keeps the lowest col_bits address bits among themselves, so
+// every 2^col_bits-amplitude run of the sweep reads one run of the source;
+// a lone diagonal only when the second buffer already exists.
+template <typename T>
+bool can_fuse_read(const msv_state *s, const std::vector<int> &local, int reg0, int rb) {
+  if (!s->perm_on && !s->pend_on) return false;
+  if (!s->perm_on && s->alt == nullptr) return false;
+  if (reg0 < 0 || reg0 + rb > static_cast<int>(local.size())) return false;
+  if (s->pend_on) {
+    if (s->pend_lut.size() > kFusedLutMaxBytes || s->pend_args.nwin > static_cast<uint32_t>(kFusedWins))
+      return false;
+    uint64_t umask = 0;
+    for (int j = 0; j < rb; ++j) {
+      const int b = local[reg0 + j];
+      umask |= s->perm_on ? s->perm_col[b] : (1ull << b);
+    }
+    int ndep = 0;
+    for (uint32_t w = 0; w < s->pend_args.nwin; ++w) {
+      const PhaseWin &pw = s->pend_args.win[w];
+      const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
+      ndep += (wm & umask) ? 1 : 0;
+    }
+    if (ndep > kFusedRdep) return false;
+  }
+  if (s->perm_on) {
+    const int cb = std::max(0, std::min(col_bits<T>(), s->n));
+    const uint64_t low = (1ull << cb) - 1ull;
+    for (int b = 0; b < cb; ++b)
+      if (s->perm_col[b] & ~low) return false;
+  }
+  return true;
+}
+
+// Fill A.fr for a fused read of the state's deferred work by a sweep with
+// tile geometry A (runs, segment 0) over `local`, and stage its tables.
+template <typename T, typename Args>
+int prepare_fused_read(msv_state *s, Args &A, const std::vector<int> &local, int rb, const uint64_t **d_ptab,
+                       const typename Vec2<T>::type **d_lut) {
+  using V = typename Vec2<T>::type;
+  const int n = s->n;
+  const int nlocal = static_cast<int>(local.size());
+  std::vector<uint64_t> col(n);
+  for (int b = 0; b < n; ++b) col[b] = s->perm_on ? s->perm_col[b] : (1ull << b);
+  FusedRead &F = A.fr;
+  F.pc = s->perm_on ? s->perm_c : 0;
+  // images of the tile-number bits, as four byte tables
+  std::vector<uint64_t> timg;
+  for (uint32_t r = 0; r < A.nruns; ++r)
+    for (uint32_t w = 0; w < A.run_width[r]; ++w) timg.push_back(col[A.run_dst[r] + w]);
+  if (timg.size() > static_cast<size_t>(kMaxTileBits)) return fail(MSV_E_INVALID, "internal: tile number too wide");
+  std::vector<uint64_t> ptab(4 * 256, 0);
+  for (int b = 0; b < 4; ++b)
+    for (int v = 0; v < 256; ++v) {
+      uint64_t x = 0;
+      for (int i = 0; i < 8; ++i) {
+        const size_t k = 8 * static_cast<size_t>(b) + i;
+        if (((v >> i) & 1) && k < timg.size()) x ^= timg[k];
+      }
+      ptab[256 * b + v] = x;
+    }
+  void *dp = nullptr;
+  MSV_TRY(stage(s, ptab.data(), ptab.size() * sizeof(uint64_t), &dp));
+  *d_ptab = reinterpret_cast<const uint64_t *>(dp);
+  const TileSeg &s0 = A.seg[0];
+  for (int k = 0; k < kTileThreadBits; ++k) F.pthr[k] = (k < nlocal - rb) ? col[s0.gi_bit[k]] : 0ull;
+  uint64_t umask = 0;
+  for (int j = 0; j < rb; ++j) {
+    F.preg[j] = col[local[s0.reg0 + j]];
+    umask |= F.preg[j];
+  }
+  *d_lut = nullptr;
+  if (!s->pend_on) return MSV_OK;
+  // windows that see the register images first (their count in ph_rdep)
+  F.ph = s->pend_args;
+  std::vector<PhaseWin> dep, rest;
+  for (uint32_t w = 0; w < F.ph.nwin; ++w) {
+    const PhaseWin &pw = s->pend_args.win[w];
+    const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
+    (wm & umask ? dep : rest).push_back(pw);
+  }
+  if (dep.size() > static_cast<size_t>(kFusedRdep) || F.ph.nwin > static_cast<uint32_t>(kFusedWins))
+    return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for the fused sweep");
+  F.ph_rdep = static_cast<uint32_t>(dep.size());
+  uint32_t w = 0;
+  for (const PhaseWin &pw : dep) F.ph.win[w++] = pw;
+  for (const PhaseWin &pw : rest) F.ph.win[w++] = pw;
+  // table-index bits of the register images (XOR of the set bits' images)
+  for (size_t k = 0; k < dep.size(); ++k) {
+    const PhaseWin &pw = dep[k];
+    for (int r = 0; r < (1 << rb); ++r) {
+      uint64_t img = 0;
+      for (int j = 0; j < rb; ++j)
+        if ((r >> j) & 1) img ^= F.preg[j];
+      const uint32_t lo = static_cast<uint32_t>(img >> pw.s1) & ((1u << pw.w1) - 1u);
+      const uint32_t hi = static_cast<uint32_t>(img >> pw.s2) & ((1u << pw.w2) - 1u);
+      F.ph_rimg[k][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
+    }
+  }
+  void *dl = nullptr;
+  MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &dl));
+  *d_lut = reinterpret_cast<const V *>(dl);
+  return MSV_OK;
+}
+
 template <typename T>
 int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vector<RotSplit> &zb) {
   using V = typename Vec2<T>::type;
@@ -787,39 +895,16 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
     if (!w.gl.empty()) sweeps.push_back(w);
   }
   if (sweeps.empty()) return flush_pending_T<T>(s);
-  // Deferred permutation and/or diagonal: the first sweep reads through them
-  // (out of place) when every local bit of its tiles maps into local bits,
-  // so that each tile reads one source tile, and the diagonal's tables fit
-  // in LDS beside the exchange buffer; otherwise they run first.
+  // Deferred permutation and/or diagonal: the first sweep reads through
+  // them (out of place) when can_fuse_read allows; otherwise they run first.
   bool fuse = false;
   if (s->perm_on || s->pend_on) {
-    uint64_t lmask = 0;
-    for (int b : sweeps[0].local) lmask |= 1ull << b;
-    fuse = !s->pend_on || s->pend_lut.size() <= kFusedLutMaxBytes;
-    if (fuse && s->pend_on) {
-      // the first segment's register images may touch at most kFusedRdep windows
-      const int rb0 = std::min(rb_of<T>(), static_cast<int>(sweeps[0].local.size()));
-      uint32_t need = 0;
-      for (int k : sweeps[0].gl) need |= 1u << k;
-      const int nl0 = static_cast<int>(sweeps[0].local.size());
-      const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
-      uint64_t umask = 0;
-      for (int j = 0; !w0.empty() && j < rb0; ++j) {
-        const int b = sweeps[0].local[w0[0] + j];
-        umask |= s->perm_on ? s->perm_col[b] : (1ull << b);
-      }
-      int ndep = 0;
-      for (uint32_t w = 0; w < s->pend_args.nwin; ++w) {
-        const PhaseWin &pw = s->pend_args.win[w];
-        const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
-        ndep += (wm & umask) ? 1 : 0;
-      }
-      fuse = !w0.empty() && ndep <= kFusedRdep && s->pend_args.nwin <= static_cast<uint32_t>(kFusedWins);
-    }
-    if (s->perm_on)
-      for (int b : sweeps[0].local) fuse = fuse && (s->perm_col[b] & ~lmask) == 0;
-    else
-      fuse = fuse && s->alt != nullptr;  // a lone diagonal: only when the second buffer exists
+    const int rb0 = std::min(rb_of<T>(), static_cast<int>(sweeps[0].local.size()));
+    uint32_t need = 0;
+    for (int k : sweeps[0].gl) need |= 1u << k;
+    const int nl0 = static_cast<int>(sweeps[0].local.size());
+    const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
+    fuse = !w0.empty() && can_fuse_read<T>(s, sweeps[0].local, w0[0], rb0);
   }
   if (!fuse) MSV_TRY(flush_pending_T<T>(s));
   std::vector<C> pre(n, 1.0), post(n, 1.0);
@@ -876,73 +961,14 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
     const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
     if (i == 0 && fuse) {
-      std::vector<uint64_t> col(n);
-      for (int b = 0; b < n; ++b) col[b] = s->perm_on ? s->perm_col[b] : (1ull << b);
-      A.pc = s->perm_on ? s->perm_c : 0;
-      // images of the tile-number bits, as four byte tables
-      std::vector<uint64_t> timg;
-      for (uint32_t r = 0; r < A.nruns; ++r)
-        for (uint32_t w = 0; w < A.run_width[r]; ++w) timg.push_back(col[A.run_dst[r] + w]);
-      if (timg.size() > static_cast<size_t>(kMaxTileBits))
-        return fail(MSV_E_INVALID, "internal: tile number too wide");
-      std::vector<uint64_t> ptab(4 * 256, 0);
-      for (int b = 0; b < 4; ++b)
-        for (int v = 0; v < 256; ++v) {
-          uint64_t x = 0;
-          for (int i = 0; i < 8; ++i) {
-            const size_t k = 8 * static_cast<size_t>(b) + i;
-            if (((v >> i) & 1) && k < timg.size()) x ^= timg[k];
-          }
-          ptab[256 * b + v] = x;
-        }
-      void *d_ptab = nullptr;
-      MSV_TRY(stage(s, ptab.data(), ptab.size() * sizeof(uint64_t), &d_ptab));
-      const TileSeg &s0 = A.seg[0];
-      for (int k = 0; k < kTileThreadBits; ++k)
-        A.pthr[k] = (k < nlocal - rb) ? col[s0.gi_bit[k]] : 0ull;
-      uint64_t umask = 0;
-      for (int j = 0; j < rb; ++j) {
-        A.preg[j] = col[sw.local[s0.reg0 + j]];
-        umask |= A.preg[j];
-      }
+      const uint64_t *d_ptab = nullptr;
       const V *d_lut = nullptr;
-      if (s->pend_on) {
-        // windows that see the register images first (their count in ph_rdep)
-        A.ph = s->pend_args;
-        std::vector<PhaseWin> dep, rest;
-        for (uint32_t w = 0; w < A.ph.nwin; ++w) {
-          const PhaseWin &pw = s->pend_args.win[w];
-          const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
-          (wm & umask ? dep : rest).push_back(pw);
-        }
-        if (dep.size() > static_cast<size_t>(kFusedRdep) || A.ph.nwin > static_cast<uint32_t>(kFusedWins))
-          return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for the fused sweep");
-        A.ph_rdep = static_cast<uint32_t>(dep.size());
-        uint32_t w = 0;
-        for (const PhaseWin &pw : dep) A.ph.win[w++] = pw;
-        for (const PhaseWin &pw : rest) A.ph.win[w++] = pw;
-        // table-index bits of the register images (XOR of the set bits' images)
-        for (size_t k = 0; k < dep.size(); ++k) {
-          const PhaseWin &pw = dep[k];
-          for (int r = 0; r < (1 << rb); ++r) {
-            uint64_t img = 0;
-            for (int j = 0; j < rb; ++j)
-              if ((r >> j) & 1) img ^= A.preg[j];
-            const uint32_t lo = static_cast<uint32_t>(img >> pw.s1) & ((1u << pw.w1) - 1u);
-            const uint32_t hi = static_cast<uint32_t>(img >> pw.s2) & ((1u << pw.w2) - 1u);
-            A.ph_rimg[k][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
-          }
-        }
-        void *d = nullptr;
-        MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d));
-        d_lut = reinterpret_cast<const V *>(d);
-      }
+      MSV_TRY(prepare_fused_read<T>(s, A, sw.local, rb, &d_ptab, &d_lut));
       MSV_TRY(ensure_alt(s));
       const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
       const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
       MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->alt), A, rb, fgrid, s->stream,
-                            reinterpret_cast<const V *>(s->buf), d_lut,
-                            reinterpret_cast<const uint64_t *>(d_ptab)));
+                            reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
       std::swap(s->buf, s->alt);
       s->perm_on = false;
       s->pend_on = false;
@@ -1023,13 +1049,35 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
     A.xbits[si] = static_cast<uint32_t>(p0 - lb);
     for (int j = 0; j < rb; ++j) A.wbits[si][j] = static_cast<uint32_t>(p0 + j - lb);
   }
-  if (!ok) return launch_sweep<T>(s, plan);
+  if (!ok) {
+    MSV_TRY(flush_pending_T<T>(s));
+    return launch_sweep<T>(s, plan);
+  }
   MSV_TRY(fill_tile_geometry(s->n, plan.local, wins, rb, &A));
   A.lb = static_cast<uint32_t>(lb);
   A.scale = static_cast<T>(plan.scale);
-  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
-  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-  MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
+  // deferred work: read through it in the first segment (k_qft's fused
+  // read exists for 4 register bits), or run it first
+  if ((s->perm_on || s->pend_on) &&
+      !(rb == 4 && can_fuse_read<T>(s, plan.local, static_cast<int>(A.seg[0].reg0), rb)))
+    MSV_TRY(flush_pending_T<T>(s));
+  if (s->perm_on || s->pend_on) {
+    const uint64_t *d_ptab = nullptr;
+    const V *d_lut = nullptr;
+    MSV_TRY(prepare_fused_read<T>(s, A, plan.local, rb, &d_ptab, &d_lut));
+    MSV_TRY(ensure_alt(s));
+    const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+    const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
+    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->alt), A, rb, inverse, fgrid, s->stream,
+                          reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
+    std::swap(s->buf, s->alt);
+    s->perm_on = false;
+    s->pend_on = false;
+  } else {
+    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
+  }
   s->sweeps += 1;
   return MSV_OK;
 }
@@ -2213,7 +2261,7 @@ int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
                                    std::to_string(m) + ") for " + std::to_string(s->n) +
                                    " qubits");
   MSV_TRY(set_dev(s));
-  MSV_TRY(flush_pending(s));
+  // deferred work is read by the first QFT sweep or flushed before it
   const int hb = s->n - 1 - first, lb = s->n - m;
   return MSV_DISPATCH(s, apply_qft_T, s, hb, lb, inverse != 0, -1);
 }

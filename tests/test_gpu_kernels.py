@@ -641,8 +641,8 @@ def _random_affine(n, rng):
 def test_deferred_permutation_read_by_next_1q_layer(gpu, dtype, n, with_phase, perm):
     """An XOR permutation (with or without a deferred phase layer before it)
     is held until the next operation; a 1q layer reads its first sweep
-    through it when the map keeps that sweep's tile bits in the tile (the
-    CNOT chain does), otherwise the gather runs first (random maps)."""
+    through it when the map keeps the lowest column bits among themselves
+    (the CNOT chain does), otherwise the gather runs first (random maps)."""
     from mettleq_amd import _native as nat
     rng = np.random.default_rng(7000 + n)
     psi = _rand_state(n, 5 * n)
@@ -750,4 +750,94 @@ def test_deferred_permutation_fused_sweep_counts_at_26q(gpu):
         nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
         nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, 1)
     assert sweeps == [u3 + 1 + 1 + u3, 2 * u3]
+    assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
+
+
+def _iqft_fft(psi, n):
+    # inverse of O.qft_fft: undo the bit reversal, then the forward FFT / sqrt(N)
+    size = 1 << n
+    idx = np.array([O.bit_reverse(i, n) for i in range(size)])
+    x = np.empty_like(np.asarray(psi, dtype=np.complex128))
+    x[idx] = psi
+    return np.fft.fft(x) / math.sqrt(size)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("n", [9, 14, 20])
+@pytest.mark.parametrize("work", ["phase+chain", "chain", "phase", "phase+random"])
+@pytest.mark.parametrize("qft", ["full", "inverse", "sub"])
+def test_deferred_work_read_by_next_qft(gpu, dtype, n, work, qft):
+    """Deferred phase layer and/or XOR permutation followed by a QFT: the
+    first QFT sweep reads through them when the map keeps the lowest column
+    bits among themselves (the CNOT chain), else they run first (random
+    maps). A lone phase layer is read when the second buffer exists (here:
+    after an earlier gather). Each sequence must match the oracle."""
+    from mettleq_amd import _native as nat
+    rng = np.random.default_rng(9100 + n)
+    psi = _rand_state(n, 17 * n)
+    bonds = [(i, (i + 1) % n) for i in range(n)] + [(0, n // 2)]
+    th = rng.uniform(0, math.pi, len(bonds))
+    rows, c = _random_affine(n, rng) if work == "phase+random" else _chain(n)
+    pre_rows, pre_c = _random_affine(n, rng)
+    want = psi
+    with _state(n, dtype) as s:
+        s.from_numpy(psi)
+        if work == "phase":
+            s.apply_xor_affine(pre_rows, pre_c)  # its gather runs at the phase layer
+            want = O.xor_affine_gather(want, n, pre_rows, pre_c)
+        if "phase" in work:
+            s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, list(th))
+            want = O.zz_layer_diag(n, bonds, th) * want
+        if work != "phase":
+            s.apply_xor_affine(rows, c)
+            want = O.xor_affine_gather(want, n, rows, c)
+        if qft == "full":
+            s.apply_qft(0, n)
+            want = O.qft_fft(want, n)
+        elif qft == "inverse":
+            s.apply_qft(0, n, inverse=True)
+            want = _iqft_fft(want, n)
+        else:
+            s.apply_qft(1, n - 1)
+            want = O.qft(want, n, list(range(1, n - 1)))
+        got = s.to_numpy()
+    tol = 2e-5 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
+def test_deferred_work_read_by_qft_sweep_counts_at_26q(gpu):
+    """U3 layer, RZZ ring, CNOT chain, full QFT at 26 qubits: with deferral
+    the ring and the chain ride in the QFT's first sweep (two sweeps fewer);
+    amplitudes agree with the undeferred run."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.workloads import ring_bonds
+    n = 26
+    rng = np.random.default_rng(2727)
+    m1 = [_rand_u(rng) for _ in range(n)]
+    bonds = ring_bonds(n)
+    th = [float(x) for x in rng.uniform(0, math.pi, len(bonds))]
+    rows, c = _chain(n)
+    with _state(n) as s:
+        s0 = s.sweep_count()
+        s.apply_1q_layer(m1)
+        u3 = s.sweep_count() - s0
+        s.apply_qft(0, n)
+        q = s.sweep_count() - s0 - u3
+    outs, sweeps = [], []
+    try:
+        for d in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
+            nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, d)
+            with _state(n) as s:
+                s0 = s.sweep_count()
+                s.apply_1q_layer(m1)
+                s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
+                s.apply_xor_affine(rows, c)
+                s.apply_qft(0, n)
+                outs.append(s.to_numpy())
+                sweeps.append(s.sweep_count() - s0)
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
+        nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, 1)
+    assert sweeps == [u3 + 1 + 1 + q, u3 + q]
     assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
