@@ -10,9 +10,11 @@ value = sum over layers of (bytes moved by the layer's full-state sweeps)
         2 * 2^n * 8 bytes (read + write of the complex64 state).
 roofline = the dominant kernel, k_rot<float, 4, false> (the rotation sweeps of
         the U3 layers): algorithmic bytes per launch / mean launch duration
-        from HIP events on the state's stream (U3 layers that read no
-        deferred work: 3 launches each); the other sweep kernels, including
-        k_rot's gather variant that reads deferred layers, are listed beside it.
+        from HIP events on the state's stream, over back-to-back U3 layers
+        that read no deferred work (3 launches each); the other sweep kernels,
+        including the gather variants of k_rot and k_qft that read deferred
+        layers (priced as an RZZ ring + CNOT chain + layer loop minus the
+        layer's plain launches), are listed beside it.
 cpu_baseline = the oracle's gate-by-gate CPU restatement (oracle/sv_cpu.c)
         timed on this host on a bounded sample (one layer of each type at
         --cpu-qubits), reported in the same unit.
@@ -441,8 +443,9 @@ def main():
     # per-layer breakdown (one instrumented pass)
     state.reset()
     state.sync()
-    # A U3 layer that follows deferred layers (S = 0) reads them in its first
-    # sweep (k_rot's gather variant): it is listed apart as "u3_fused".
+    # A U3 or QFT layer that follows deferred layers (S = 0) reads them in its
+    # first sweep (the kernels' gather variants): it is listed apart as
+    # "u3_fused" / "qft_fused".
     per_type = {}
     labels = []
     prev_deferred = False
@@ -452,12 +455,39 @@ def main():
         apply_layer(state, layer, n, lay)
         ms = state.timer_stop()
         S = state.sweep_count() - s0
-        label = "u3_fused" if layer["type"] == "u3" and prev_deferred else layer["type"]
+        label = layer["type"] + "_fused" if layer["type"] in ("u3", "qft") and prev_deferred else layer["type"]
         labels.append(label)
         d = per_type.setdefault(label, {"ms": [], "S": S})
         d["ms"].append(ms)
         prev_deferred = S == 0
     log("per-layer: " + ", ".join(f"{k}: {np.mean(v['ms']):.3f} ms S={v['S']}" for k, v in per_type.items()))
+    # mean launch of each sweep kernel, from short back-to-back loops (the
+    # queue stays full, so host-side set-up between launches is hidden):
+    # plain U3 layers and QFTs, and the same after an RZZ ring + CNOT chain,
+    # whose first sweep is the gather variant
+    first = {t: next((l for l in layers if l["type"] == t), None) for t in ("u3", "rzz", "cnot", "qft")}
+
+    def steady(seq, reps=4):
+        state.sync()
+        s0 = state.sweep_count()
+        state.timer_start()
+        for _ in range(reps):
+            for l in seq:
+                apply_layer(state, l, n, lay)
+        ms = state.timer_stop()
+        return ms / reps, (state.sweep_count() - s0) / reps
+
+    launch_ms = {}
+    for base in ("u3", "qft"):
+        if first[base] is None:
+            continue
+        ms, S = steady([first[base]])
+        launch_ms[base] = ms / S
+        if first["rzz"] is not None and first["cnot"] is not None:
+            ms_f, S_f = steady([first["rzz"], first["cnot"], first[base]])
+            if S_f == S:  # the ring and the chain ride in the first sweep
+                launch_ms[base + "_fused"] = ms_f - (S - 1) * launch_ms[base]
+    log("steady-state launch ms: " + ", ".join(f"{k}: {v:.4f}" for k, v in launch_ms.items()))
 
     # timed region
     state.sync()
@@ -479,36 +509,32 @@ def main():
 
     # per sweep kernel: launches per step and mean launch duration (HIP events
     # around each layer on the state's stream, divided by its sweeps)
-    kernel_of = {"u3": "k_rot<float, 4, false>", "qft": "k_qft<float, 4, false>", "rzz": "k_phase",
+    kernel_of = {"u3": "k_rot<float, 4, false>", "qft": "k_qft<float, 4, false, false>", "rzz": "k_phase",
                  "cnot": "k_xor_gather", "cz": "k_phase"}
+    fused_name = {"u3": "k_rot<float, 4, true>", "qft": "k_qft<float, 4, false, true>"}
     if per_type.get("rzz", {}).get("S", 1) == 0:
         kernel_of["cnot"] = "k_xor_gather (+ deferred RZZ ring)"
-    if per_type.get("cnot", {}).get("S", 1) == 0:
-        # the CNOT chain (and the RZZ ring before it) ride in the next layer's
-        # first sweep: k_rot's gather variant, or one k_xor_gather ahead of a QFT
-        kernel_of["qft"] = "k_qft<float, 4, false> (+ one k_xor_gather of the deferred RZZ ring + CNOT chain)"
     kernels = {}
     for t, v in per_type.items():
-        if v["S"] == 0 or t == "u3_fused":
-            continue  # deferred: its work runs inside the next layer's sweep
-        k = kernels.setdefault(kernel_of[t], {"ms": 0.0, "launches": 0})
-        k["ms"] += float(np.mean(v["ms"])) * labels.count(t)
-        k["launches"] += v["S"] * labels.count(t)
-    if "u3_fused" in per_type:
-        # the fused layer = one gather-variant launch + (S - 1) plain ones,
-        # priced at the plain layers' mean launch when those exist
-        v = per_type["u3_fused"]
-        cnt = labels.count("u3_fused")
-        plain = kernels.get(kernel_of["u3"])
-        plain_avg = plain["ms"] / plain["launches"] if plain else float(np.mean(v["ms"])) / v["S"]
-        fused_ms = float(np.mean(v["ms"])) - (v["S"] - 1) * plain_avg
-        kernels["k_rot<float, 4, true> (first sweep reads the deferred RZZ ring + CNOT chain)"] = {
-            "ms": fused_ms * cnt, "launches": cnt}
-        k = kernels.setdefault(kernel_of["u3"], {"ms": 0.0, "launches": 0})
-        k["ms"] += (v["S"] - 1) * plain_avg * cnt
-        k["launches"] += (v["S"] - 1) * cnt
+        if v["S"] == 0:
+            continue  # deferred: its work runs inside the next layer's first sweep
+        cnt = labels.count(t)
+        base = t[:-len("_fused")] if t.endswith("_fused") else t
+        if t.endswith("_fused") and t in launch_ms:
+            # one gather-variant launch + (S - 1) plain ones
+            name = fused_name[base] + " (first sweep reads the deferred RZZ ring + CNOT chain)"
+            k = kernels.setdefault(name, {"avg_ms": launch_ms[t], "launches": 0})
+            k["launches"] += cnt
+            k = kernels.setdefault(kernel_of[base], {"avg_ms": launch_ms[base], "launches": 0})
+            k["launches"] += (v["S"] - 1) * cnt
+        else:
+            avg = launch_ms.get(t, float(np.mean(v["ms"])) / v["S"])
+            k = kernels.setdefault(kernel_of[base], {"avg_ms": avg, "launches": 0})
+            k["launches"] += v["S"] * cnt
     for k in kernels.values():
-        k["avg_launch_ms"] = round(k["ms"] / max(1, k["launches"]), 4)
+        k["ms"] = k["avg_ms"] * k["launches"]
+    for k in kernels.values():
+        k["avg_launch_ms"] = round(k.pop("avg_ms"), 4)
         k["GBps"] = round(sweep_bytes / (k["avg_launch_ms"] * 1e-3) / 1e9, 1)
         k["frac"] = round(k["GBps"] / PEAK_GBPS, 4)
         k["ms"] = round(k["ms"], 3)
@@ -575,9 +601,8 @@ def main():
         "sweeps_per_step": layer_sweeps_per_step,
         "layer_equivalent_GBps": round(s_min_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "note": ("S = 0: the layer's work is deferred into the next layer's first sweep (an RZZ ring "
-                 "and a CNOT chain are read through by the following U3 layer's first k_rot sweep, "
-                 "listed as u3_fused, or by one gather ahead of a QFT); value counts only the sweeps "
-                 "performed. "
+                 "and a CNOT chain are read through by the first sweep of the following U3 layer or "
+                 "QFT, listed as u3_fused / qft_fused); value counts only the sweeps performed. "
                  "layer_equivalent_GBps prices each layer at its lower bound of sweeps (diagonal 1, "
                  "XOR 1, U3 2, QFT 2) over the same step time"),
         "roofline": {
