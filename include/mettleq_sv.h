@@ -124,15 +124,22 @@ int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bon
  *
  * Both diagonal entry points may defer their pass (MSV_TUNE_DEFER_DIAG, on
  * by default): the state then holds the diagonal until its next operation.
- * msv_apply_xor_affine applies it inside the permutation's own sweep; every
- * other call launches it first, so results never differ from immediate
- * application (up to rounding of the phase products). */
+ * msv_apply_xor_affine applies it inside the permutation's own sweep, and
+ * msv_apply_1q_layer and msv_apply_qft may read it in their first sweep;
+ * every other call launches it first, so results never differ from
+ * immediate application (up to rounding of the phase products). */
 int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
                        const int32_t *q2, const double *d2, int n2);
 
 /* GF(2) affine permutation out[y] = state[M'y ^ c]: qubit k of the source
  * index is parity(rows[k] & y) ^ bit(n-1-k) of c. Replaces
- * xor_affine_gather (shaders/xor_affine.py:79-116). */
+ * xor_affine_gather (shaders/xor_affine.py:79-116).
+ *
+ * May defer its pass (MSV_TUNE_DEFER_PERM, on by default), together with a
+ * pending diagonal: msv_apply_1q_layer and msv_apply_qft then read their
+ * first sweep through the permutation when it keeps the lowest 4 (complex64)
+ * or 3 (complex128) index bits among themselves, as a CNOT chain does;
+ * every other call runs the gather first. */
 int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c);
 
 /* QFT without terminal swaps on the contiguous qubits first..m-1 (first =
@@ -275,8 +282,9 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
                                          operation on the state; an XOR permutation
                                          then applies them in its own sweep (0/1) */
 #define MSV_TUNE_DEFER_PERM 13        /* XOR permutations wait for the next operation;
-                                         a 1q layer then reads its first sweep through
-                                         the permutation (and a pending diagonal) (0/1) */
+                                         a 1q layer or QFT then reads its first sweep
+                                         through the permutation (and a pending
+                                         diagonal) (0/1) */
 #define MSV_TUNE_FUSED_BLOCKS_PER_CU 14 /* grid of such a fused first sweep (blocks per CU) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
