@@ -149,6 +149,30 @@ def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
         total_passes += passes
     state_bytes = (1 << n_cpu) * 8
     gbps = total_passes * 2 * state_bytes / total_t / 1e9
+    # the same restatement on one thread, on a sample 8x smaller
+    n1 = max(4, n_cpu - 3)
+    lib().cpu_set_threads(1)
+    try:
+        eng1 = CpuEngine(n1)
+        eng1.state[:] = eng.state[: 1 << n1] / np.linalg.norm(eng.state[: 1 << n1])
+        t1 = 0.0
+        p1 = 0
+        for t, layer in seen.items():
+            src = dict(layer)
+            if t == "u3":
+                src = {"type": "u3", "angles": layer["angles"][:n1]}
+            if t == "rzz":
+                src = {"type": "rzz", "thetas": layer["thetas"][:n1]}
+            ops = layer_ops(src, n1)
+            q0 = eng1.passes
+            t0 = time.perf_counter()
+            eng1.run(ops)
+            t1 += time.perf_counter() - t0
+            p1 += eng1.passes - q0
+    finally:
+        lib().cpu_set_threads(threads)
+    single = {"value": round(p1 * 2 * (1 << n1) * 8 / t1 / 1e9, 3), "unit": "GB/s", "cores": 1,
+              "sample": f"one layer of each type at {n1} qubits complex64, gate by gate, once ({t1:.1f} s)"}
     return {
         "value": round(gbps, 3),
         "unit": "GB/s",
@@ -159,6 +183,7 @@ def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
                    f"{total_t * repeats:.1f} s in all)"),
         "layer_ms": {k: round(v["ms"], 2) for k, v in per.items()},
         "layer_ms_scaled_to_30q": {k: round(v["ms"] * (1 << (30 - n_cpu)), 1) for k, v in per.items()},
+        "single_thread": single,
     }
 
 

@@ -519,7 +519,7 @@ __device__ __forceinline__ void fused_setup(const FusedRead &F, const TileSeg &s
 // are in ph_rimg), the others give one product per thread.
 template <typename T, int RB>
 __device__ __forceinline__ void fused_load(typename Vec2<T>::type (&a)[1 << RB],
-                                           const typename Vec2<T>::type *__restrict__ src,
+                                           const typename Vec2<T>::type *src,
                                            const FusedRead &F, const uint64_t *__restrict__ ptab,
                                            uint64_t tile, uint64_t th, const uint32_t (&thw)[kFusedWins],
                                            const typename Vec2<T>::type *lut) {
@@ -565,8 +565,10 @@ __device__ __forceinline__ void fused_load(typename Vec2<T>::type (&a)[1 << RB],
 
 // PERM: the first segment reads through deferred work (fused_load), with
 // the deferred diagonal's window tables in LDS behind the exchange buffer.
-// Reads come from `src`, stores go to `st` (the other buffer), so no tile
-// reads what another tile writes.
+// Reads come from `src` and stores go to `st`: the other buffer when a
+// permutation is read, so no tile reads what another tile writes; the same
+// buffer for a lone diagonal, where each tile reads exactly the amplitudes
+// it writes, all of them before its first store.
 // Waves per SIMD a rotation or QFT sweep must keep: complex64 tiles with
 // 4 register bits run 512-thread blocks, two per CU (64 KiB of LDS each),
 // which needs 4 waves per SIMD, i.e. at most 128 VGPRs.
@@ -576,10 +578,10 @@ struct SweepWaves {
 };
 
 template <typename T, int RB, bool PERM>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_rot(typename Vec2<T>::type *__restrict__ st,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_rot(typename Vec2<T>::type *st,
                                              const typename Vec2<T>::type *__restrict__ ttab,
                                              const RotArgs<T> A,
-                                             const typename Vec2<T>::type *__restrict__ src,
+                                             const typename Vec2<T>::type *src,
                                              const typename Vec2<T>::type *__restrict__ ph_lut,
                                              const uint64_t *__restrict__ ptab) {
   using V = typename Vec2<T>::type;
@@ -768,9 +770,9 @@ __device__ __forceinline__ void qft_segment(typename Vec2<T>::type *a, const Qft
 }
 
 template <typename T, int RB, bool INV, bool PERM>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_qft(typename Vec2<T>::type *__restrict__ st,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v))) k_qft(typename Vec2<T>::type *st,
                                              const QftArgs<T> A,
-                                             const typename Vec2<T>::type *__restrict__ src,
+                                             const typename Vec2<T>::type *src,
                                              const typename Vec2<T>::type *__restrict__ ph_lut,
                                              const uint64_t *__restrict__ ptab) {
   using V = typename Vec2<T>::type;

@@ -764,12 +764,12 @@ constexpr size_t kFusedLutMaxBytes = 16 * 1024;
 // (FusedRead): the diagonal's tables fit in LDS beside the exchange buffer
 // and the register images touch at most kFusedRdep of its windows; the
 // permutation keeps the lowest col_bits address bits among themselves, so
-// every 2^col_bits-amplitude run of the sweep reads one run of the source;
-// a lone diagonal only when the second buffer already exists.
+// every 2^col_bits-amplitude run of the sweep reads one run of the source.
+// A permutation is read out of place (into the second buffer); a lone
+// diagonal in place, since each tile then reads exactly what it writes.
 template <typename T>
 bool can_fuse_read(const msv_state *s, const std::vector<int> &local, int reg0, int rb) {
   if (!s->perm_on && !s->pend_on) return false;
-  if (!s->perm_on && s->alt == nullptr) return false;
   if (reg0 < 0 || reg0 + rb > static_cast<int>(local.size())) return false;
   if (s->pend_on) {
     if (s->pend_lut.size() > kFusedLutMaxBytes || s->pend_args.nwin > static_cast<uint32_t>(kFusedWins))
@@ -964,12 +964,13 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
       const uint64_t *d_ptab = nullptr;
       const V *d_lut = nullptr;
       MSV_TRY(prepare_fused_read<T>(s, A, sw.local, rb, &d_ptab, &d_lut));
-      MSV_TRY(ensure_alt(s));
+      const bool oop = s->perm_on;  // a lone diagonal runs in place
+      if (oop) MSV_TRY(ensure_alt(s));
       const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
       const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->alt), A, rb, fgrid, s->stream,
+      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
                             reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
-      std::swap(s->buf, s->alt);
+      if (oop) std::swap(s->buf, s->alt);
       s->perm_on = false;
       s->pend_on = false;
     } else {
@@ -1065,12 +1066,13 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
     const uint64_t *d_ptab = nullptr;
     const V *d_lut = nullptr;
     MSV_TRY(prepare_fused_read<T>(s, A, plan.local, rb, &d_ptab, &d_lut));
-    MSV_TRY(ensure_alt(s));
+    const bool oop = s->perm_on;  // a lone diagonal runs in place
+    if (oop) MSV_TRY(ensure_alt(s));
     const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
     const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->alt), A, rb, inverse, fgrid, s->stream,
+    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, inverse, fgrid, s->stream,
                           reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
-    std::swap(s->buf, s->alt);
+    if (oop) std::swap(s->buf, s->alt);
     s->perm_on = false;
     s->pend_on = false;
   } else {

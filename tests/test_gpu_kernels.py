@@ -841,3 +841,45 @@ def test_deferred_work_read_by_qft_sweep_counts_at_26q(gpu):
         nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, 1)
     assert sweeps == [u3 + 1 + 1 + q, u3 + q]
     assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_lone_deferred_diagonal_read_in_place_by_next_layer(gpu, dtype):
+    """Brick circuit (U3 layer, CZ brick, U3 layer, CZ brick, QFT) at 20
+    qubits on a fresh state: each CZ brick rides in the next layer's first
+    sweep, in place (no second buffer), one sweep per brick fewer; the
+    amplitudes match the undeferred run and the oracle."""
+    from mettleq_amd import _native as nat
+    n = 20
+    rng = np.random.default_rng(2020)
+    m1 = [_rand_u(rng) for _ in range(n)]
+    m2 = [_rand_u(rng) for _ in range(n)]
+    b0 = [(q, q + 1) for q in range(0, n - 1, 2)]
+    b1 = [(q, q + 1) for q in range(1, n - 1, 2)]
+    psi = _rand_state(n, 77)
+    want = O.apply_1q_layer(psi, n, m1)
+    want = O.cphase_layer_diag(n, b0, [math.pi] * len(b0)) * want
+    want = O.apply_1q_layer(want, n, m2)
+    want = O.cphase_layer_diag(n, b1, [math.pi] * len(b1)) * want
+    want = O.qft_fft(want, n)
+    outs, sweeps = [], []
+    try:
+        for d in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
+            with _state(n, dtype) as s:
+                s.from_numpy(psi)
+                s0 = s.sweep_count()
+                s.apply_1q_layer(m1)
+                u3 = s.sweep_count() - s0
+                s.apply_phase_layer([nat.MSV_PHASE_CPHASE] * len(b0), b0, [math.pi] * len(b0))
+                s.apply_1q_layer(m2)
+                s.apply_phase_layer([nat.MSV_PHASE_CPHASE] * len(b1), b1, [math.pi] * len(b1))
+                s.apply_qft(0, n)
+                outs.append(s.to_numpy())
+                sweeps.append(s.sweep_count() - s0)
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, 1)
+    assert sweeps[0] == sweeps[1] + 2
+    tol = 2e-5 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(outs[1] - want)) < tol
+    assert np.max(np.abs(outs[0] - outs[1])) < tol
