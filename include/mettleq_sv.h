@@ -286,6 +286,9 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
                                          through the permutation (and a pending
                                          diagonal) (0/1) */
 #define MSV_TUNE_FUSED_BLOCKS_PER_CU 14 /* grid of such a fused first sweep (blocks per CU) */
+#define MSV_TUNE_REORDER_GATES 15     /* gate lists (msv_apply_gates, msv_apply_circuit):
+                                         commuting gates are regrouped into fewer tile
+                                         programs (0/1) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

@@ -53,6 +53,7 @@ MSV_TUNE_TILE_PREFETCH = 11
 MSV_TUNE_DEFER_DIAG = 12
 MSV_TUNE_DEFER_PERM = 13
 MSV_TUNE_FUSED_BLOCKS_PER_CU = 14
+MSV_TUNE_REORDER_GATES = 15
 
 
 class NativeUnavailableError(RuntimeError):

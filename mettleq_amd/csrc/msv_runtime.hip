@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <set>
 #include <vector>
 
 #include "../../include/mettleq_sv.h"
@@ -63,6 +64,7 @@ struct Tuning {
   int defer_diag = 1;       // phase layers wait for the next operation (an XOR permutation absorbs them)
   int defer_perm = 1;       // XOR permutations wait for the next operation (a 1q layer's first sweep reads through them)
   int fused_blocks_per_cu = 32;  // grid of a first sweep that reads through deferred work (tables staged per block)
+  int reorder_gates = 1;    // gate lists: commuting gates are regrouped into fewer tile programs
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -1297,8 +1299,131 @@ int count_segments(const msv_state *s, const std::vector<PlanOp> &ops) {
 
 int max_program_targets() { return 10; }
 
+// Gate list -> tile programs, with commuting gates regrouped. A gate's
+// non-diagonal action on its target bit commutes with every action of
+// another gate that does not touch that bit; diagonal actions (controls,
+// diagonal gates) commute with each other. The dependency graph keeps, per
+// bit, the last gate acting non-diagonally on it and the diagonal users
+// since. Programs are filled by list scheduling: of the gates whose
+// predecessors have run, diagonal gates first, then gates on a target the
+// program already has, then a new target while the program has fewer than
+// max_program_targets(), up to kMaxOps gates. Within a program the gates
+// are reordered (dependencies permitting) by target bit, so the register
+// windows advance in one direction, and the program is cut where the
+// segment count would exceed kMaxSegs.
+template <typename T>
+int apply_plan_ops_reordered(msv_state *s, const std::vector<PlanOp> &all) {
+  const int N = static_cast<int>(all.size());
+  std::vector<std::vector<int>> succ(N);
+  std::vector<int> npred(N, 0);
+  std::vector<int> last_nd(64, -1);
+  std::vector<std::vector<int>> diag_since(64);
+  auto edge = [&](int from, int to) {
+    succ[from].push_back(to);
+    ++npred[to];
+  };
+  for (int i = 0; i < N; ++i) {
+    const PlanOp &op = all[i];
+    if (op.kind != OP_U2 && op.kind != OP_DIAG) return fail(MSV_E_INVALID, "internal: unexpected gate-list op");
+    uint64_t dmask = op.kind == OP_U2 ? op.cmask : 0;
+    if (op.kind == OP_DIAG) {
+      dmask |= 1ull << op.abit;
+      if (op.bbit >= 0) dmask |= 1ull << op.bbit;
+    }
+    if (op.tbit >= 0) {
+      if (last_nd[op.tbit] >= 0) edge(last_nd[op.tbit], i);
+      for (int j : diag_since[op.tbit]) edge(j, i);
+      diag_since[op.tbit].clear();
+    }
+    for (uint64_t m = dmask; m; m &= m - 1) {
+      const int d = __builtin_ctzll(m);
+      if (last_nd[d] >= 0) edge(last_nd[d], i);
+    }
+    if (op.tbit >= 0) last_nd[op.tbit] = i;
+    for (uint64_t m = dmask; m; m &= m - 1) diag_since[__builtin_ctzll(m)].push_back(i);
+  }
+  // ready gates in list order; a pick looks at the first kScan of them
+  constexpr int kScan = 256;
+  std::set<int> ready;
+  for (int i = 0; i < N; ++i)
+    if (npred[i] == 0) ready.insert(i);
+  std::vector<int> in_prog(N, -1), lp(N, 0);
+  int prog_id = 0;
+  while (!ready.empty()) {
+    // fill one program
+    std::vector<int> prog;
+    uint64_t tmask = 0;
+    int nt = 0;
+    while (static_cast<int>(prog.size()) < TileCfg<T>::kMaxOps) {
+      int pick = -1, pick_rank = 3, seen = 0;
+      for (auto it = ready.begin(); it != ready.end() && seen < kScan && pick_rank > 0; ++it, ++seen) {
+        const PlanOp &op = all[*it];
+        int rank;
+        if (op.tbit < 0) rank = 0;
+        else if ((tmask >> op.tbit) & 1ull) rank = 1;
+        else if (nt < max_program_targets()) rank = 2;
+        else continue;
+        if (rank < pick_rank) {  // the set is in list order: the first of a rank wins
+          pick = *it;
+          pick_rank = rank;
+        }
+      }
+      if (pick < 0) break;
+      const int i = pick;
+      ready.erase(i);
+      prog.push_back(i);
+      in_prog[i] = prog_id;
+      if (all[i].tbit >= 0 && !((tmask >> all[i].tbit) & 1ull)) {
+        tmask |= 1ull << all[i].tbit;
+        ++nt;
+      }
+      for (int j : succ[i])
+        if (--npred[j] == 0) ready.insert(j);
+    }
+    // order inside the program: by target bit (diagonal gates first),
+    // dependencies within the program permitting
+    for (int i : prog)
+      for (int j : succ[i])
+        if (in_prog[j] == prog_id) ++lp[j];
+    std::vector<int> lready;
+    for (int i : prog)
+      if (lp[i] == 0) lready.push_back(i);
+    std::vector<PlanOp> ordered;
+    ordered.reserve(prog.size());
+    while (!lready.empty()) {
+      size_t best = 0;
+      for (size_t r = 1; r < lready.size(); ++r) {
+        const int a = all[lready[r]].tbit, b = all[lready[best]].tbit;
+        if (a < b || (a == b && lready[r] < lready[best])) best = r;
+      }
+      const int i = lready[best];
+      lready.erase(lready.begin() + best);
+      ordered.push_back(all[i]);
+      for (int j : succ[i])
+        if (in_prog[j] == prog_id && --lp[j] == 0) lready.push_back(j);
+    }
+    if (ordered.size() != prog.size()) return fail(MSV_E_INVALID, "internal: gate reorder lost an op");
+    // cut where the segment count would exceed kMaxSegs
+    std::vector<PlanOp> cur;
+    for (const PlanOp &po : ordered) {
+      cur.push_back(po);
+      if (cur.size() > 1 && count_segments<T>(s, cur) > kMaxSegs) {
+        cur.pop_back();
+        MSV_TRY(flush_program<T>(s, cur));
+        cur.assign(1, po);
+      }
+    }
+    MSV_TRY(flush_program<T>(s, cur));
+    ++prog_id;
+  }
+  for (int i = 0; i < N; ++i)
+    if (in_prog[i] < 0) return fail(MSV_E_INVALID, "internal: gate left unscheduled");
+  return MSV_OK;
+}
+
 template <typename T>
 int apply_plan_ops(msv_state *s, const std::vector<PlanOp> &all) {
+  if (g_tune.reorder_gates) return apply_plan_ops_reordered<T>(s, all);
   std::vector<PlanOp> cur;
   std::vector<int> targets;
   for (const PlanOp &po : all) {
@@ -2517,6 +2642,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "fused blocks per CU out of range");
       g_tune.fused_blocks_per_cu = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_REORDER_GATES:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "reorder gates must be 0 or 1");
+      g_tune.reorder_gates = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2539,6 +2668,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_DEFER_DIAG: *value = g_tune.defer_diag; return MSV_OK;
     case MSV_TUNE_DEFER_PERM: *value = g_tune.defer_perm; return MSV_OK;
     case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune.fused_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_REORDER_GATES: *value = g_tune.reorder_gates; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

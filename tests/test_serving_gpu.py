@@ -62,3 +62,38 @@ def test_realtime_poisson_stream_completes(gpu):
     s = summarize(res)
     assert s["completed"] == 60 and s["failed"] == 0
     assert all(int(r.counts.sum()) == q.shots for r, q in zip(res, reqs))
+
+
+@pytest.mark.parametrize("n,gates", [(14, 200), (24, 150), (28, 200)])
+def test_reordered_gate_lists_match_and_use_fewer_sweeps(gpu, n, gates):
+    """msv_apply_circuit regroups commuting gates into tile programs
+    (MSV_TUNE_REORDER_GATES): the state equals the in-order programs' state
+    (and the oracle at 14 qubits), with no more sweeps."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.circuit import encode_ops
+    from mettleq_amd.native_state import NativeState
+    from mettleq_amd.serving import random_circuit
+    from oracle import sv_oracle as O
+    rng = np.random.default_rng(900 + n)
+    ops = random_circuit(n, gates, rng)
+    enc = encode_ops(ops)
+    win = (0, min(1 << n, 1 << 16)) if n <= 20 else ((1 << (n - 1)) + 777, 1 << 16)
+    got, sweeps = [], []
+    old = nat.get_tuning(nat.MSV_TUNE_REORDER_GATES)
+    try:
+        for flag in (0, 1):
+            nat.set_tuning(nat.MSV_TUNE_REORDER_GATES, flag)
+            with NativeState(n) as st:
+                st.reset()
+                s0 = st.sweep_count()
+                st.apply_circuit(*enc)
+                sweeps.append(st.sweep_count() - s0)
+                got.append(st.to_numpy(*win))
+                if n == 14:
+                    full = st.to_numpy()
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_REORDER_GATES, old)
+    assert sweeps[1] <= sweeps[0]
+    assert np.max(np.abs(got[0] - got[1])) < 2e-6
+    if n == 14:
+        assert np.max(np.abs(full - O.run_ops(n, ops))) < 5e-6
