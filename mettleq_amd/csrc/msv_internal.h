@@ -47,6 +47,8 @@ enum TileOpKind : uint8_t {
   OP_QFT_F = 2,  // forward QFT stage: butterfly then ladder phase on the 1 branch
   OP_QFT_I = 3,  // inverse QFT stage: conjugate ladder phase then butterfly
   OP_DIAG = 4,   // diagonal over 1 or 2 index bits (anywhere in the index)
+  OP_X = 5,      // X on a register bit, optional controls (CNOT, Toffoli): a register swap
+  OP_SIGN = 6,   // diagonal whose entries are +1 or -1 (CZ, Z): sign flips, mask in jl
 };
 
 // One operation of a tile program. Matrix entries are stored in the
@@ -60,7 +62,7 @@ struct TileOp {
   int32_t rpos;     // target register bit (U2, QFT)
   int32_t ra;       // DIAG: register bit of index bit a, or -1
   int32_t rb;       // DIAG: register bit of index bit b, or -1
-  uint32_t jl;      // QFT: ladder register bits directly below rpos
+  uint32_t jl;      // QFT: ladder register bits directly below rpos; SIGN: bit k set = entry k is -1
   uint32_t nbits;   // DIAG: 1 or 2
   uint32_t has_ctrl;// U2: controls present
   uint32_t pbit;    // QFT: index bit p of the stage

@@ -91,6 +91,40 @@ __device__ __forceinline__ void u2_dispatch(typename Vec2<T>::type *a, const T *
   }
 }
 
+// X on register bit J: the pairs swap (under the controls).
+template <typename T, int RB, int J, bool CTRL>
+__device__ __forceinline__ void x_pairs(typename Vec2<T>::type *a, bool tok, uint32_t creg) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    const V v0 = a[r], v1 = a[r1];
+    if (CTRL) {
+      const bool ok = tok && ((static_cast<uint32_t>(r) & creg) == creg);
+      a[r].x = ok ? v1.x : v0.x;
+      a[r].y = ok ? v1.y : v0.y;
+      a[r1].x = ok ? v0.x : v1.x;
+      a[r1].y = ok ? v0.y : v1.y;
+    } else {
+      a[r] = v1;
+      a[r1] = v0;
+    }
+  }
+}
+
+template <typename T, int RB, bool CTRL>
+__device__ __forceinline__ void x_dispatch(typename Vec2<T>::type *a, int j, bool tok, uint32_t creg) {
+  switch (j) {
+    case 0: x_pairs<T, RB, 0, CTRL>(a, tok, creg); break;
+    case 1: if constexpr (RB > 1) x_pairs<T, RB, 1, CTRL>(a, tok, creg); break;
+    case 2: if constexpr (RB > 2) x_pairs<T, RB, 2, CTRL>(a, tok, creg); break;
+    case 3: if constexpr (RB > 3) x_pairs<T, RB, 3, CTRL>(a, tok, creg); break;
+    case 4: if constexpr (RB > 4) x_pairs<T, RB, 4, CTRL>(a, tok, creg); break;
+    default: break;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c) {
   if constexpr (std::is_same<T, float>::value) return __builtin_fmaf(a, b, c);
@@ -188,6 +222,25 @@ __device__ __forceinline__ void diag_apply(typename Vec2<T>::type *a, const Tile
   }
 }
 
+// Diagonal with entries +1 / -1 (bit k of jl: entry k, index (bit a, bit b)).
+template <typename T, int RB>
+__device__ __forceinline__ void sign_apply(typename Vec2<T>::type *a, const TileOp<T> &op, uint64_t gb) {
+  const int ra = op.ra, rb = op.rb;
+  const bool two = op.nbits == 2;
+  const uint32_t smask = op.jl;
+  const uint32_t ta = (gb & op.amask) ? 1u : 0u;
+  const uint32_t tb = (gb & op.bmask) ? 1u : 0u;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    const uint32_t ba = (ra >= 0) ? ((static_cast<uint32_t>(r) >> ra) & 1u) : ta;
+    const uint32_t bb = (rb >= 0) ? ((static_cast<uint32_t>(r) >> rb) & 1u) : tb;
+    const uint32_t idx = two ? ((ba << 1) | bb) : ba;
+    const bool neg = (smask >> idx) & 1u;
+    a[r].x = neg ? -a[r].x : a[r].x;
+    a[r].y = neg ? -a[r].y : a[r].y;
+  }
+}
+
 template <typename T, int RB>
 __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArgs<T> &A,
                                         uint32_t ob, uint32_t oe, uint64_t gb, uint32_t tm) {
@@ -212,6 +265,15 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
       qft_dispatch<T, RB>(a, wt, A.tab, op.rpos, op.jl, kind == OP_QFT_I);
     } else if (kind == OP_DIAG) {
       diag_apply<T, RB>(a, op, gb);
+    } else if (kind == OP_X) {
+      if (!op.has_ctrl) {
+        x_dispatch<T, RB, false>(a, op.rpos, true, 0u);
+      } else {
+        const bool tok = (gb & op.cmask_t) == op.cmask_t;
+        x_dispatch<T, RB, true>(a, op.rpos, tok, op.cmask_r);
+      }
+    } else if (kind == OP_SIGN) {
+      sign_apply<T, RB>(a, op, gb);
     }
   }
 }

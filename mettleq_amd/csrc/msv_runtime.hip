@@ -484,6 +484,11 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
           else op.cmask_t |= 1ull << bit;
         }
         op.has_ctrl = po.cmask != 0 ? 1u : 0u;
+        // an exact X (CNOT, Toffoli targets) runs as a register swap
+        const double xm[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+        bool is_x = true;
+        for (int k = 0; k < 8; ++k) is_x = is_x && po.m[k] == xm[k];
+        if (is_x) op.kind = OP_X;
       } else if (po.kind == OP_QFT_F || po.kind == OP_QFT_I) {
         const int j = op.rpos;
         const int p = po.tbit;
@@ -509,6 +514,18 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
           const int jb = reg_pos(po.bbit);
           op.rb = static_cast<int8_t>(jb);
           op.bmask = jb >= 0 ? 0ull : (1ull << po.bbit);
+        }
+        // entries of exactly +1 / -1 (CZ, Z) run as sign flips
+        uint32_t smask = 0;
+        bool is_sign = true;
+        for (uint32_t k = 0; k < (1u << op.nbits); ++k) {
+          const double re = po.m[2 * k], im = po.m[2 * k + 1];
+          is_sign = is_sign && im == 0.0 && (re == 1.0 || re == -1.0);
+          if (re == -1.0) smask |= 1u << k;
+        }
+        if (is_sign) {
+          op.kind = OP_SIGN;
+          op.jl = smask;
         }
       }
     }
