@@ -1326,7 +1326,8 @@ int max_program_targets() { return 10; }
 // program already has, then a new target while the program has fewer than
 // max_program_targets(), up to kMaxOps gates. Within a program the gates
 // are reordered (dependencies permitting) by target bit, so the register
-// windows advance in one direction, and the program is cut where the
+// windows advance in one direction, adjacent gates with the same target
+// and controls are multiplied into one, and the program is cut where the
 // segment count would exceed kMaxSegs.
 template <typename T>
 int apply_plan_ops_reordered(msv_state *s, const std::vector<PlanOp> &all) {
@@ -1420,9 +1421,50 @@ int apply_plan_ops_reordered(msv_state *s, const std::vector<PlanOp> &all) {
         if (in_prog[j] == prog_id && --lp[j] == 0) lready.push_back(j);
     }
     if (ordered.size() != prog.size()) return fail(MSV_E_INVALID, "internal: gate reorder lost an op");
+    // adjacent gates with the same target and controls (or diagonals on the
+    // same bits) become one: their product, formed in double precision
+    std::vector<PlanOp> merged;
+    merged.reserve(ordered.size());
+    for (const PlanOp &po : ordered) {
+      if (!merged.empty()) {
+        PlanOp &b = merged.back();
+        if (po.kind == OP_U2 && b.kind == OP_U2 && po.tbit == b.tbit && po.cmask == b.cmask) {
+          // po after b: M = M_po * M_b
+          std::complex<double> x[4], y[4], z[4];
+          for (int k = 0; k < 4; ++k) {
+            x[k] = {po.m[2 * k], po.m[2 * k + 1]};
+            y[k] = {b.m[2 * k], b.m[2 * k + 1]};
+          }
+          z[0] = x[0] * y[0] + x[1] * y[2];
+          z[1] = x[0] * y[1] + x[1] * y[3];
+          z[2] = x[2] * y[0] + x[3] * y[2];
+          z[3] = x[2] * y[1] + x[3] * y[3];
+          for (int k = 0; k < 4; ++k) {
+            b.m[2 * k] = z[k].real();
+            b.m[2 * k + 1] = z[k].imag();
+          }
+          const double id[8] = {1, 0, 0, 0, 0, 0, 1, 0};
+          bool is_id = true;
+          for (int k = 0; k < 8; ++k) is_id = is_id && b.m[k] == id[k];
+          if (is_id) merged.pop_back();  // e.g. X X
+          continue;
+        }
+        if (po.kind == OP_DIAG && b.kind == OP_DIAG && po.abit == b.abit && po.bbit == b.bbit) {
+          const int ne = po.bbit >= 0 ? 4 : 2;
+          for (int k = 0; k < ne; ++k) {
+            const std::complex<double> e = std::complex<double>(po.m[2 * k], po.m[2 * k + 1]) *
+                                           std::complex<double>(b.m[2 * k], b.m[2 * k + 1]);
+            b.m[2 * k] = e.real();
+            b.m[2 * k + 1] = e.imag();
+          }
+          continue;
+        }
+      }
+      merged.push_back(po);
+    }
     // cut where the segment count would exceed kMaxSegs
     std::vector<PlanOp> cur;
-    for (const PlanOp &po : ordered) {
+    for (const PlanOp &po : merged) {
       cur.push_back(po);
       if (cur.size() > 1 && count_segments<T>(s, cur) > kMaxSegs) {
         cur.pop_back();
