@@ -169,7 +169,9 @@ def _fuse(operations: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:
                     xmat = G.CONST_1Q["X"]
 
                     def _is(m, ref):
-                        return np.allclose(m, ref, rtol=0.0, atol=G.IDENTITY_ATOL)
+                        # entrywise |m - ref| <= atol (np.allclose with rtol = 0), on
+                        # the four entries directly: this runs once per wire and window
+                        return all(abs(x) <= G.IDENTITY_ATOL for x in (m - ref).ravel().tolist())
 
                     active = sorted(wq for wq, m in prods.items() if not _is(m, eye2))
                     if not active:
