@@ -156,7 +156,13 @@ int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse);
 /* Ordered list of generic gates, fused into tile sweeps by the library.
  * Replaces the per-gate StateVectorSimulator.apply_* calls dispatched by
  * Device._apply (device.py:1138-1290) and controlled_u_groups
- * (shaders/controlled_single.py:89-134). */
+ * (shaders/controlled_single.py:89-134).
+ *
+ * With MSV_TUNE_REORDER_GATES (on by default) the library may reorder gates
+ * that commute (a gate's non-diagonal action on its target commutes with
+ * any gate not touching that qubit; diagonal actions commute with each
+ * other) and multiply adjacent gates on the same target and controls, so
+ * results equal in-order application up to floating-point rounding. */
 int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates);
 
 /* Gate codes of msv_apply_circuit: names, wire order and angle conventions
