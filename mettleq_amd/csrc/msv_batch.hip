@@ -241,6 +241,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // address space the loads are scalar (s_load), not per-lane vector loads.
 typedef const __attribute__((address_space(4))) BOp *BOpCP;
 
+// The adjoint form holds phi and lambda (64 VGPRs of amplitudes, 160 in
+// all): both states exchange through one LDS buffer in turn, so its
+// registers, not the LDS, set the blocks per CU (three; holding it to 128
+// VGPRs for four spills and was 2.6x slower).
 template <int RB, bool ADJ>
 __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 *__restrict__ lam,
                                                const BOp *__restrict__ ops,
@@ -249,9 +253,8 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
   float2 *lds_a = reinterpret_cast<float2 *>(smem_raw);
-  float2 *lds_l = lds_a + (1u << A.nlocal);
   // the tile's rotations (cos, sin) staged once per tile
-  float2 *lds_m = lds_a + (ADJ ? 2u : 1u) * (A.nseg > 1 ? (1u << A.nlocal) : 0u);
+  float2 *lds_m = lds_a + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid >> 6;
   const BOpCP cops = (BOpCP)(ops);
@@ -284,14 +287,10 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
           a[r] = phi[i];
           if (ADJ) l[r] = lam[i];
         }
-      } else {
+      } else if (!ADJ) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const uint32_t k = tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0);
-          a[r] = lds_a[k];
-          if (ADJ) l[r] = lds_l[k];
-        }
-      }
+        for (int r = 0; r < R; ++r) a[r] = lds_a[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
+      }  // ADJ: phi and lambda were read at the end of the previous segment
       for (uint32_t o = A.op_begin[s]; o < A.op_end[s]; ++o) {
         const BOpCP opp = cops + o;
         const uint32_t kind = opp->kind;
@@ -371,12 +370,25 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
       } else {
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const uint32_t k = tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0);
-          lds_a[k] = a[r];
-          if (ADJ) lds_l[k] = l[r];
-        }
+        for (int r = 0; r < R; ++r) lds_a[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)] = a[r];
         __syncthreads();
+        if (ADJ) {
+          // phi into the next segment's registers, then lambda through the
+          // same buffer
+          const TileSeg &sn = A.seg[s + 1];
+          uint32_t tln, tmn;
+          uint64_t gbn;
+          seg_thread(sn, tid, base, tln, gbn, tmn);
+          const uint32_t tswn = lds_swz<RB>(tln);
+#pragma unroll
+          for (int r = 0; r < R; ++r) a[r] = lds_a[tswn ^ lds_swz<RB>(static_cast<uint32_t>(r) << sn.reg0)];
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < R; ++r) lds_a[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)] = l[r];
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < R; ++r) l[r] = lds_a[tswn ^ lds_swz<RB>(static_cast<uint32_t>(r) << sn.reg0)];
+        }
       }
     }
   }
@@ -820,7 +832,7 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       if (!p) return set_error(MSV_E_INVALID, "internal: rotation left unscheduled");
     if (put_phases) phases_done = true;
     sw.adj = backward;
-    sw.lds = (A.nseg > 1 ? (backward ? 2 : 1) * (sizeof(float2) << nlocal) : 0) +
+    sw.lds = (A.nseg > 1 ? (sizeof(float2) << nlocal) : 0) +
              sizeof(float2) * static_cast<size_t>(mat_cnt);
     out->push_back(sw);
   }
