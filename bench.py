@@ -466,8 +466,8 @@ def main():
         log(f"warmup {w + 1}/{args.warmup} done")
 
     # per-layer breakdown (one instrumented pass)
-    state.reset()
     state.sync()
+    state.reset()  # pending until the first layer (a U3 layer writes its product state)
     # A U3 or QFT layer that follows deferred layers (S = 0) reads them in its
     # first sweep (the kernels' gather variants): it is listed apart as
     # "u3_fused" / "qft_fused".
@@ -481,6 +481,8 @@ def main():
         ms = state.timer_stop()
         S = state.sweep_count() - s0
         label = layer["type"] + "_fused" if layer["type"] in ("u3", "qft") and prev_deferred else layer["type"]
+        if layer is layers[0] and layer["type"] == "u3" and S == 0:
+            label = "u3_on_basis"  # the reset state's product state, one write pass
         labels.append(label)
         d = per_type.setdefault(label, {"ms": [], "S": S})
         d["ms"].append(ms)
@@ -627,7 +629,9 @@ def main():
         "layer_equivalent_GBps": round(s_min_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
         "note": ("S = 0: the layer's work is deferred into the next layer's first sweep (an RZZ ring "
                  "and a CNOT chain are read through by the first sweep of the following U3 layer or "
-                 "QFT, listed as u3_fused / qft_fused); value counts only the sweeps performed. "
+                 "QFT, listed as u3_fused / qft_fused), or, for the first U3 layer (u3_on_basis), "
+                 "written as the product state of the reset |0...0> in one write pass; value counts "
+                 "only the sweeps performed. "
                  "layer_equivalent_GBps prices each layer at its lower bound of sweeps (diagonal 1, "
                  "XOR 1, U3 2, QFT 2) over the same step time"),
         "roofline": {

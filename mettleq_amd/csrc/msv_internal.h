@@ -41,6 +41,7 @@ constexpr int kSampleBlock = 4096;   // amplitudes per sampling block
 constexpr int kSampleLanes = 64;     // lanes per block (one wave64)
 constexpr int kMaxWords = 16;        // Pauli words per reduction launch
 constexpr uint64_t kStreamChunkPacks = 4096;  // packs (16 B) per block iteration of the streaming kernels
+constexpr int kProductGroupBits = 10;         // index bits per LDS factor table of k_product
 
 enum TileOpKind : uint8_t {
   OP_U2 = 1,     // 2x2 on a register bit, optional controls
@@ -238,6 +239,12 @@ hipError_t launch_xor_gather_phase(const typename Vec2<T>::type *src, typename V
                                    uint64_t n_amps, const uint64_t *tables, int ntab, uint64_t c,
                                    const typename Vec2<T>::type *lut, const PhaseArgs &phase,
                                    uint32_t grid, hipStream_t stream);
+// Product state from three index-bit-group factor tables (k_product):
+// groups of l1 and l2 <= kProductGroupBits low bits, the rest in `hi`.
+template <typename T>
+hipError_t launch_product(typename Vec2<T>::type *state, uint64_t n_amps, const typename Vec2<T>::type *lo,
+                          const typename Vec2<T>::type *mid, const typename Vec2<T>::type *hi, uint32_t l1,
+                          uint32_t l2, uint32_t grid, hipStream_t stream);
 template <typename T>
 hipError_t launch_init_basis(typename Vec2<T>::type *state, uint64_t n_amps, uint64_t index,
                              uint32_t grid, hipStream_t stream);

@@ -1077,6 +1077,47 @@ __global__ void __launch_bounds__(256) k_init_basis(typename Vec2<T>::type *__re
   }
 }
 
+// Product state: amplitude i = lo[i & (2^l1 - 1)] * mid[(i >> l1) & (2^l2 - 1)]
+// * hi[i >> (l1 + l2)], the per-bit factors of each index-bit group
+// multiplied out on the host. One write pass (lo and mid from LDS, hi a
+// broadcast read: it changes every 2^(l1+l2) amplitudes).
+template <typename T>
+__global__ void __launch_bounds__(256) k_product(typename Vec2<T>::type *__restrict__ st, uint64_t n_amps,
+                                                 const typename Vec2<T>::type *__restrict__ lo,
+                                                 const typename Vec2<T>::type *__restrict__ mid,
+                                                 const typename Vec2<T>::type *__restrict__ hi, uint32_t l1,
+                                                 uint32_t l2, uint32_t groups) {
+  using V = typename Vec2<T>::type;
+  __shared__ V tlo[1u << kProductGroupBits], tmid[1u << kProductGroupBits];
+  for (uint32_t i = threadIdx.x; i < (1u << l1); i += blockDim.x) tlo[i] = lo[i];
+  for (uint32_t i = threadIdx.x; i < (1u << l2); i += blockDim.x) tmid[i] = mid[i];
+  __syncthreads();
+  const uint64_t m1 = (1ull << l1) - 1ull, m2 = (1ull << l2) - 1ull;
+  const uint32_t sh = l1 + l2;
+  constexpr int KA = Pack<T>::kAmps;
+  const uint64_t npack = n_amps / KA;
+  const uint64_t nchunks = (npack + kChunkPacks - 1) / kChunkPacks;
+  for (uint64_t ch = stream_vrank(groups); ch < nchunks; ch += gridDim.x) {
+#pragma unroll 1
+    for (int k0 = 0; k0 < kChunkPer; k0 += kUnroll) {
+      const uint64_t base = ch * kChunkPacks + static_cast<uint64_t>(k0) * kStreamThreads + threadIdx.x;
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t pk = base + u * kStreamThreads;
+        if (pk < npack) {
+          V amp[KA];
+#pragma unroll
+          for (int d = 0; d < KA; ++d) {
+            const uint64_t i = pk * KA + d;
+            amp[d] = cmul(cmul(tlo[i & m1], tmid[(i >> l1) & m2]), hi[i >> sh]);
+          }
+          store_pack<T>(st, pk, amp);
+        }
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_scale(typename Vec2<T>::type *__restrict__ st,
                                                uint64_t n_amps, T re, T im, uint32_t groups) {
@@ -1543,6 +1584,17 @@ hipError_t launch_init_basis(typename Vec2<T>::type *state, uint64_t n_amps, uin
 }
 
 template <typename T>
+hipError_t launch_product(typename Vec2<T>::type *state, uint64_t n_amps, const typename Vec2<T>::type *lo,
+                          const typename Vec2<T>::type *mid, const typename Vec2<T>::type *hi, uint32_t l1,
+                          uint32_t l2, uint32_t grid, hipStream_t stream) {
+  if (l1 > static_cast<uint32_t>(kProductGroupBits) || l2 > static_cast<uint32_t>(kProductGroupBits))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_product<T>), dim3(grid), dim3(256), 0, stream, state, n_amps, lo, mid, hi, l1, l2,
+                     kStreamXcdGroups);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_scale(typename Vec2<T>::type *state, uint64_t n_amps, double re, double im,
                         uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL((k_scale<T>), dim3(grid), dim3(256), 0, stream, state, n_amps,
@@ -1636,6 +1688,9 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
                                                  uint32_t, hipStream_t);                         \
   template hipError_t launch_init_basis<T>(Vec2<T>::type *, uint64_t, uint64_t, uint32_t,        \
                                            hipStream_t);                                         \
+  template hipError_t launch_product<T>(Vec2<T>::type *, uint64_t, const Vec2<T>::type *,       \
+                                        const Vec2<T>::type *, const Vec2<T>::type *, uint32_t,  \
+                                        uint32_t, uint32_t, hipStream_t);                        \
   template hipError_t launch_scale<T>(Vec2<T>::type *, uint64_t, double, double, uint32_t,       \
                                       hipStream_t);                                              \
   template hipError_t launch_block_probsum<T>(const Vec2<T>::type *, uint64_t, double *,         \

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -133,6 +134,12 @@ struct msv_state {
   bool perm_on = false;
   std::vector<uint64_t> perm_col;
   uint64_t perm_c = 0;
+  // Deferred basis state: reset / set_basis keep the index here instead of
+  // writing the amplitudes; a 1q layer then writes its product state in one
+  // pass, and every other operation writes the basis state first. Never on
+  // together with the deferred work above.
+  bool basis_on = false;
+  uint64_t basis_k = 0;
 };
 
 namespace {
@@ -236,6 +243,8 @@ int check_qubit(const msv_state *s, int q, const char *what) {
 // Deferred work (defined with the phase and permutation planners below).
 template <typename T>
 int flush_pending_T(msv_state *s);
+template <typename T>
+int init_basis_T(msv_state *s, uint64_t index);
 template <typename T>
 int launch_gather(msv_state *s, const std::vector<uint64_t> &col, uint64_t c);
 
@@ -1000,8 +1009,62 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
   return MSV_OK;
 }
 
+// A 1q layer on a deferred basis state |k>: the result is the product
+// state with amplitude i = prod over index bits b of g_b(bit b of i), g_b(x)
+// = U_q[x][bit b of k] for the gate U_q on the qubit at bit b (the identity
+// for inactive qubits). The factors of each of three index-bit groups are
+// multiplied out in double precision on the host and k_product writes the
+// state in one pass.
+template <typename T>
+int product_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
+  using V = typename Vec2<T>::type;
+  using C = std::complex<double>;
+  const int n = s->n;
+  std::vector<std::array<C, 2>> g(n);
+  for (int q = 0; q < n; ++q) {
+    const int b = bit_of(s, q);
+    const int kb = static_cast<int>((s->basis_k >> b) & 1ull);
+    for (int x = 0; x < 2; ++x) {
+      if (!active || active[q]) {
+        const double *m = mats + 8 * q + 2 * (2 * x + kb);
+        g[b][x] = C(m[0], m[1]);
+      } else {
+        g[b][x] = (x == kb) ? C(1.0) : C(0.0);
+      }
+    }
+  }
+  const int l1 = std::min(kProductGroupBits, n);
+  const int l2 = std::min(kProductGroupBits, n - l1);
+  const int l3 = n - l1 - l2;
+  std::vector<V> tabs;
+  tabs.reserve((size_t(1) << l1) + (size_t(1) << l2) + (size_t(1) << l3));
+  auto group = [&](int b0, int w) {
+    for (uint64_t v = 0; v < (1ull << w); ++v) {
+      C f = 1.0;
+      for (int k = 0; k < w; ++k) f *= g[b0 + k][(v >> k) & 1ull];
+      V e;
+      e.x = static_cast<T>(f.real());
+      e.y = static_cast<T>(f.imag());
+      tabs.push_back(e);
+    }
+  };
+  group(0, l1);
+  group(l1, l2);
+  group(l1 + l2, l3);
+  void *d = nullptr;
+  MSV_TRY(stage(s, tabs.data(), tabs.size() * sizeof(V), &d));
+  const V *lo = reinterpret_cast<const V *>(d);
+  const V *mid = lo + (size_t(1) << l1);
+  const V *hi = mid + (size_t(1) << l2);
+  s->basis_on = false;
+  MSV_HIP(launch_product<T>(reinterpret_cast<V *>(s->buf), s->amps, lo, mid, hi, static_cast<uint32_t>(l1),
+                            static_cast<uint32_t>(l2), stream_grid(s), s->stream));
+  return MSV_OK;
+}
+
 template <typename T>
 int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
+  if (s->basis_on && !s->pend_on && !s->perm_on) return product_layer_T<T>(s, mats, active);
   const int n = s->n;
   int L = 0;
   std::vector<std::pair<int, int>> chunks;
@@ -1566,6 +1629,10 @@ int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, s
 
 template <typename T>
 int flush_pending_T(msv_state *s) {
+  if (s->basis_on) {
+    s->basis_on = false;
+    MSV_TRY(init_basis_T<T>(s, s->basis_k));
+  }
   if (s->perm_on) {
     s->perm_on = false;
     return launch_gather<T>(s, s->perm_col, s->perm_c);  // applies a pending diagonal too
@@ -1721,7 +1788,9 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
   }
   if (g_tune.defer_perm) {
     // A pending permutation is run first (two maps are not composed); a
-    // pending diagonal stays and now acts at the source index.
+    // pending diagonal stays and now acts at the source index; a pending
+    // basis state is written first.
+    if (s->basis_on) MSV_TRY(flush_pending_T<T>(s));
     if (s->perm_on) {
       s->perm_on = false;
       MSV_TRY(launch_gather<T>(s, s->perm_col, s->perm_c));
@@ -1996,7 +2065,7 @@ int init_basis_T(msv_state *s, uint64_t index) {
 // point that reads or writes amplitudes calls it first, except the XOR
 // permutation, which absorbs it, and the phase layers themselves.
 int flush_pending(msv_state *s) {
-  if (!s->pend_on && !s->perm_on) return MSV_OK;
+  if (!s->pend_on && !s->perm_on && !s->basis_on) return MSV_OK;
   MSV_TRY(set_dev(s));
   return MSV_DISPATCH(s, flush_pending_T, s);
 }
@@ -2244,7 +2313,9 @@ int msv_state_set_basis(msv_state *s, uint64_t index) {
   MSV_TRY(set_dev(s));
   s->pend_on = false;  // the new basis state replaces what deferred work would act on
   s->perm_on = false;
-  return MSV_DISPATCH(s, init_basis_T, s, index);
+  s->basis_on = true;  // written by the next operation (a 1q layer: as its product state)
+  s->basis_k = index;
+  return MSV_OK;
 }
 
 int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *bytes) {
@@ -2289,7 +2360,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
   MSV_TRY(flush_pending(src));
   MSV_TRY(set_dev(dst));
-  if (dst != src) dst->pend_on = dst->perm_on = false;  // overwritten
+  if (dst != src) dst->pend_on = dst->perm_on = dst->basis_on = false;  // overwritten
   MSV_HIP(hipStreamSynchronize(src->stream));
   const uint32_t grid = elem_grid(dst, dst->amps);
   if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
@@ -2447,7 +2518,9 @@ int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
                                    std::to_string(m) + ") for " + std::to_string(s->n) +
                                    " qubits");
   MSV_TRY(set_dev(s));
-  // deferred work is read by the first QFT sweep or flushed before it
+  // deferred work is read by the first QFT sweep or flushed before it; a
+  // deferred basis state is written first
+  if (s->basis_on) MSV_TRY(flush_pending(s));
   const int hb = s->n - 1 - first, lb = s->n - m;
   return MSV_DISPATCH(s, apply_qft_T, s, hb, lb, inverse != 0, -1);
 }

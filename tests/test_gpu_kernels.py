@@ -883,3 +883,103 @@ def test_lone_deferred_diagonal_read_in_place_by_next_layer(gpu, dtype):
     tol = 2e-5 if dtype == "complex64" else TOL128
     assert np.max(np.abs(outs[1] - want)) < tol
     assert np.max(np.abs(outs[0] - outs[1])) < tol
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("n", [1, 3, 10, 13, 20, 24])
+@pytest.mark.parametrize("subset", [False, True])
+def test_deferred_basis_state_then_1q_layer_writes_the_product_state(gpu, dtype, n, subset):
+    """set_basis(k) is kept pending; the next 1q layer writes the product
+    state prod_q U_q|k_q> in one pass (no sweep)."""
+    rng = np.random.default_rng(4100 + n + 7 * subset)
+    k = int(rng.integers(1 << n))
+    mats = [_rand_u(rng) for _ in range(n)]
+    act = sorted(int(q) for q in rng.choice(n, max(1, n // 2), replace=False)) if subset else None
+    with _state(n, dtype) as s:
+        s.set_basis(k)
+        s0 = s.sweep_count()
+        s.apply_1q_layer(mats, active=act)
+        sweeps = s.sweep_count() - s0
+        if n <= 20:
+            got = s.to_numpy()
+        else:
+            win = ((1 << n) - (1 << 16), 1 << 16)
+            got = s.to_numpy(*win)
+    assert sweeps == 0
+    # the product state, from per-qubit columns U_q[:, k_q]
+    cols = []
+    for q in range(n):
+        kq = (k >> (n - 1 - q)) & 1
+        u = mats[q] if (act is None or q in act) else np.eye(2)
+        cols.append(np.asarray(u)[:, kq])
+    if n <= 20:
+        want = cols[0]
+        for c in cols[1:]:
+            want = np.kron(want, c)
+        if n <= 13:
+            e = np.zeros(1 << n, dtype=np.complex128)
+            e[k] = 1
+            full = [mats[q] if (act is None or q in act) else np.eye(2) for q in range(n)]
+            assert np.max(np.abs(O.apply_1q_layer(e, n, full) - want)) < 1e-12
+    else:
+        idx = np.arange(*[win[0], win[0] + win[1]])
+        want = np.ones(len(idx), dtype=np.complex128)
+        for q in range(n):
+            want *= cols[q][(idx >> (n - 1 - q)) & 1]
+    tol = 2e-6 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
+@pytest.mark.parametrize("after", ["probs", "phase", "xor", "qft", "gates", "inner", "scale", "sample",
+                                   "download", "copy", "upload_part"])
+def test_deferred_basis_state_then_other_operations(gpu, after):
+    """Every operation other than a 1q layer writes the pending basis state
+    first: results equal those of |k>."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import gate_u
+    n = 12
+    rng = np.random.default_rng(4242)
+    k = int(rng.integers(1 << n))
+    e = np.zeros(1 << n, dtype=np.complex128)
+    e[k] = 1
+    want = e
+    with _state(n) as s:
+        s.reset()
+        s.set_basis(k)
+        if after == "probs":
+            assert np.array_equal(s.probabilities(), np.abs(e) ** 2)
+        elif after == "phase":
+            bonds = [(0, 5), (3, 4)]
+            s.apply_phase_layer([nat.MSV_PHASE_ZZ] * 2, bonds, [0.3, 0.9])
+            want = O.zz_layer_diag(n, bonds, [0.3, 0.9]) * e
+        elif after == "xor":
+            rows, c = _chain(n)
+            s.apply_xor_affine(rows, c)
+            want = O.xor_affine_gather(e, n, rows, c)
+        elif after == "qft":
+            s.apply_qft(0, n)
+            want = O.qft_fft(e, n)
+        elif after == "gates":
+            m = _rand_u(rng)
+            s.apply_gates([gate_u(4, m, [2])])
+            want = O.apply_controlled(e, n, m, [2], 4)
+        elif after == "inner":
+            with _state(n) as t:
+                t.from_numpy(e)
+                assert abs(t.inner(s) - 1.0) < 1e-6
+        elif after == "scale":
+            s.scale(1j)
+            want = 1j * e
+        elif after == "sample":
+            got = s.sample(50, None, seed=1)
+            assert np.all(got == k)
+        elif after == "copy":
+            with _state(n) as t:
+                t.copy_from(s)
+                assert np.array_equal(t.to_numpy(), e.astype(np.complex64))
+        elif after == "upload_part":
+            s.from_numpy(np.full(4, 0.5, dtype=np.complex64), offset=0)
+            want = e.copy()
+            want[:4] = 0.5
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < 2e-6
