@@ -86,6 +86,10 @@ int msv_get_device_info(int device, msv_device_info *out);
 /* Allocates 2^n amplitudes on `device` and sets |0...0>. */
 int msv_state_create(int n_qubits, int dtype, int device, msv_state **out);
 int msv_state_destroy(msv_state *s);
+/* |0...0> / |index>. Nothing is written until the state's next operation:
+ * a 1q layer then writes its product state in one pass, any other call
+ * writes the basis state first (msv_state_sync and msv_state_device_ptr
+ * included). */
 int msv_state_reset(msv_state *s);
 int msv_state_set_basis(msv_state *s, uint64_t index);
 int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *bytes);
