@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--cpu-qubits", type=int, default=27)
-    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--cpu-repeats", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-oracle", action="store_true", help="skip the complex128 fidelity check")
     ap.add_argument("--json-out", default=None)
@@ -113,6 +113,31 @@ def read_traffic(path):
     except OSError:
         pass
     return None, None
+
+
+def host_cpu_info() -> dict:
+    """Model name, logical CPUs and physical cores of this host (Linux)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": len(cores) or None}
 
 
 def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
@@ -184,6 +209,7 @@ def cpu_baseline(n_cpu: int, layers, repeats: int = 1):
         "layer_ms": {k: round(v["ms"], 2) for k, v in per.items()},
         "layer_ms_scaled_to_30q": {k: round(v["ms"] * (1 << (30 - n_cpu)), 1) for k, v in per.items()},
         "single_thread": single,
+        "host": host_cpu_info(),
     }
 
 
