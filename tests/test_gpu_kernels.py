@@ -983,3 +983,62 @@ def test_deferred_basis_state_then_other_operations(gpu, after):
             want[:4] = 0.5
         got = s.to_numpy()
     assert np.max(np.abs(got - want)) < 2e-6
+
+
+@pytest.mark.parametrize("n", [6, 11, 15])
+@pytest.mark.parametrize("reorder", [0, 1])
+def test_gate_lists_with_controls_swaps_and_diagonals(gpu, n, reorder):
+    """Random gate lists with multi-controlled 2x2 gates (X among them),
+    controlled SWAPs and one- and two-qubit diagonals (+-1 ones among
+    them), in order and regrouped (MSV_TUNE_REORDER_GATES): the state
+    matches the oracle's gate-by-gate application."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import gate_diag1, gate_diag2, gate_swap, gate_u
+    rng = np.random.default_rng(5150 + n)
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    gates = []
+    want = _rand_state(n, 3 * n)
+    psi = want.copy()
+    for _ in range(120):
+        kind = rng.integers(6)
+        qs = [int(q) for q in rng.choice(n, min(n, 4), replace=False)]
+        if kind == 0:  # U, 0-2 controls
+            nc = int(rng.integers(3))
+            u = _rand_u(rng)
+            gates.append(gate_u(qs[0], u, qs[1:1 + nc]))
+            want = O.apply_controlled(want, n, u, qs[1:1 + nc], qs[0]) if nc else O.apply_dense(want, n, u, [qs[0]])
+        elif kind == 1:  # X with 1-2 controls (register swaps)
+            nc = int(rng.integers(1, 3))
+            gates.append(gate_u(qs[0], X, qs[1:1 + nc]))
+            want = O.apply_controlled(want, n, X, qs[1:1 + nc], qs[0])
+        elif kind == 2:  # two-qubit diagonal, sometimes +-1 (CZ-like)
+            d = [1, 1, 1, -1] if rng.random() < 0.5 else list(np.exp(1j * rng.uniform(0, 6, 4)))
+            gates.append(gate_diag2(qs[0], qs[1], d))
+            want = O.apply_dense(want, n, np.diag(d), [qs[0], qs[1]])
+        elif kind == 3:  # one-qubit diagonal, sometimes Z
+            d = (1, -1) if rng.random() < 0.5 else tuple(np.exp(1j * rng.uniform(0, 6, 2)))
+            gates.append(gate_diag1(qs[0], *d))
+            want = O.apply_dense(want, n, np.diag(d), [qs[0]])
+        elif kind == 4:  # SWAP, 0-1 controls
+            nc = int(rng.integers(2))
+            gates.append(gate_swap(qs[0], qs[1], qs[2:2 + nc]))
+            if nc:
+                m = np.eye(8, dtype=complex)
+                m[-4:, -4:] = O.SWAP()
+                want = O.apply_dense(want, n, m, qs[2:3] + qs[:2])
+            else:
+                want = O.apply_dense(want, n, O.SWAP(), qs[:2])
+        else:  # repeated gate on one wire (merged when adjacent)
+            u = _rand_u(rng)
+            gates.append(gate_u(qs[0], u))
+            gates.append(gate_u(qs[0], u.conj().T))
+    old = nat.get_tuning(nat.MSV_TUNE_REORDER_GATES)
+    try:
+        nat.set_tuning(nat.MSV_TUNE_REORDER_GATES, reorder)
+        with _state(n) as s:
+            s.from_numpy(psi)
+            s.apply_gates(gates)
+            got = s.to_numpy()
+    finally:
+        nat.set_tuning(nat.MSV_TUNE_REORDER_GATES, old)
+    assert np.max(np.abs(got - want)) < 2e-5
