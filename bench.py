@@ -491,55 +491,106 @@ def main():
         state.sync()
         log(f"warmup {w + 1}/{args.warmup} done")
 
-    # per-layer breakdown (one instrumented pass)
+    # per-layer breakdown (one instrumented pass). A U3 or QFT layer that
+    # follows deferred layers (S = 0) reads them in its first sweep:
+    # "u3_fused" / "qft_fused" (gather variants of k_rot / k_qft). A U3 layer
+    # that finds the previous U3 layer's last rotations pending runs them in
+    # its first sweep, across the deferred ring and chain: "u3_cross"
+    # (k_rot_relabel). A QFT that finds pending rotations runs them first
+    # (one plain k_rot sweep).
     state.sync()
     state.reset()  # pending until the first layer (a U3 layer writes its product state)
-    # A U3 or QFT layer that follows deferred layers (S = 0) reads them in its
-    # first sweep (the kernels' gather variants): it is listed apart as
-    # "u3_fused" / "qft_fused".
     per_type = {}
     labels = []
-    prev_deferred = False
-    for layer in layers:
+    launches = {}  # sweep kernel -> launches per step
+    step_sweeps = 0
+    for i, layer in enumerate(layers):
+        pend = state.pending()
         s0 = state.sweep_count()
         state.timer_start()
         apply_layer(state, layer, n, lay)
         ms = state.timer_stop()
         S = state.sweep_count() - s0
-        label = layer["type"] + "_fused" if layer["type"] in ("u3", "qft") and prev_deferred else layer["type"]
-        if layer is layers[0] and layer["type"] == "u3" and S == 0:
+        step_sweeps += S
+        t = layer["type"]
+        tail = bool(pend & nat.MSV_PENDING_TAIL)
+        read = bool(pend & (nat.MSV_PENDING_DIAG | nat.MSV_PENDING_PERM))
+        label = t
+        if t == "u3" and S == 0 and i == 0:
             label = "u3_on_basis"  # the reset state's product state, one write pass
+        elif t == "u3" and tail and S > 0:
+            label = "u3_cross"
+        elif t in ("u3", "qft") and read and S > 0:
+            label = t + "_fused"
         labels.append(label)
-        d = per_type.setdefault(label, {"ms": [], "S": S})
+        d = per_type.setdefault(label, {"ms": [], "S": []})
         d["ms"].append(ms)
-        prev_deferred = S == 0
-    log("per-layer: " + ", ".join(f"{k}: {np.mean(v['ms']):.3f} ms S={v['S']}" for k, v in per_type.items()))
+        d["S"].append(S)
+
+        def add(k, c):
+            if c > 0:
+                launches[k] = launches.get(k, 0) + c
+
+        if label == "u3":
+            add("k_rot", S)
+        elif label == "u3_fused":
+            add("k_rot_gather", 1)
+            add("k_rot", S - 1)
+        elif label == "u3_cross":
+            add("k_rot_relabel", 1)
+            add("k_rot", S - 1)
+        elif t == "qft" and S > 0:
+            rest = S
+            if tail:
+                add("k_rot", 1)  # the pending rotations run first
+                rest -= 1
+            if read:
+                add("k_qft_gather", 1)
+                rest -= 1
+            add("k_qft", rest)
+        elif t in ("rzz", "cnot", "cz") and S > 0:
+            add("k_phase" if t != "cnot" else "k_xor_gather", S)
+    log("per-layer: " + ", ".join(f"{k}: {np.mean(v['ms']):.3f} ms S={max(v['S'])}" for k, v in per_type.items()))
+
     # mean launch of each sweep kernel, from short back-to-back loops (the
-    # queue stays full, so host-side set-up between launches is hidden):
-    # plain U3 layers and QFTs, and the same after an RZZ ring + CNOT chain,
-    # whose first sweep is the gather variant
+    # queue stays full, so host-side set-up between launches is hidden)
     first = {t: next((l for l in layers if l["type"] == t), None) for t in ("u3", "rzz", "cnot", "qft")}
 
-    def steady(seq, reps=4):
-        state.sync()
-        s0 = state.sweep_count()
-        state.timer_start()
-        for _ in range(reps):
-            for l in seq:
-                apply_layer(state, l, n, lay)
-        ms = state.timer_stop()
-        return ms / reps, (state.sweep_count() - s0) / reps
+    def steady(seq, reps=4, tail=1):
+        old = nat.get_tuning(nat.MSV_TUNE_DEFER_TAIL)
+        nat.set_tuning(nat.MSV_TUNE_DEFER_TAIL, tail)
+        try:
+            state.sync()
+            s0 = state.sweep_count()
+            state.timer_start()
+            for _ in range(reps):
+                for l in seq:
+                    apply_layer(state, l, n, lay)
+            ms = state.timer_stop()
+            return ms / reps, (state.sweep_count() - s0) / reps
+        finally:
+            nat.set_tuning(nat.MSV_TUNE_DEFER_TAIL, old)
 
     launch_ms = {}
-    for base in ("u3", "qft"):
-        if first[base] is None:
-            continue
-        ms, S = steady([first[base]])
-        launch_ms[base] = ms / S
-        if first["rzz"] is not None and first["cnot"] is not None:
-            ms_f, S_f = steady([first["rzz"], first["cnot"], first[base]])
-            if S_f == S:  # the ring and the chain ride in the first sweep
-                launch_ms[base + "_fused"] = ms_f - (S - 1) * launch_ms[base]
+    ring_chain = first["rzz"] is not None and first["cnot"] is not None
+    if first["u3"] is not None:
+        ms, S = steady([first["u3"]], tail=0)
+        launch_ms["k_rot"] = ms / S
+        if ring_chain:
+            ms, S = steady([first["rzz"], first["cnot"], first["u3"]], tail=0)
+            launch_ms["k_rot_gather"] = ms - (S - 1) * launch_ms["k_rot"]
+        if "u3_cross" in per_type:
+            # from the instrumented pass: the crossing sweep's cost depends on
+            # the tail it finds (8 bits after a layer's first plan, 9 after a
+            # crossing), so it is taken from the step's own U3 layers
+            v = per_type["u3_cross"]
+            launch_ms["k_rot_relabel"] = float(np.mean(v["ms"])) - (max(v["S"]) - 1) * launch_ms["k_rot"]
+    if first["qft"] is not None:
+        ms, S = steady([first["qft"]])
+        launch_ms["k_qft"] = ms / S
+        if ring_chain:
+            ms, S = steady([first["rzz"], first["cnot"], first["qft"]])
+            launch_ms["k_qft_gather"] = ms - (S - 1) * launch_ms["k_qft"]
     log("steady-state launch ms: " + ", ".join(f"{k}: {v:.4f}" for k, v in launch_ms.items()))
 
     # timed region
@@ -554,43 +605,29 @@ def main():
     sweeps = state.sweep_count() - sweeps0
     log(f"timed {args.steps} steps: {gpu_ms:.1f} ms GPU, {wall * 1e3:.1f} ms wall, {sweeps} sweeps")
     ms_per_step = gpu_ms / args.steps
-    # value: bytes of all sweeps (the reset init writes the state once: 1/2 sweep)
-    layer_sweeps_per_step = sum(per_type[l]["S"] for l in labels)
+    # value: bytes of the sweeps performed (the reset's product state is
+    # written in a half sweep, not counted)
+    layer_sweeps_per_step = step_sweeps
     s_min = {"u3": 2, "qft": 2, "rzz": 1, "cnot": 1, "cz": 1}
     s_min_per_step = sum(s_min[l["type"]] for l in layers)
     value = layer_sweeps_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9
 
     # per sweep kernel: launches per step and mean launch duration (HIP events
-    # around each layer on the state's stream, divided by its sweeps)
-    kernel_of = {"u3": "k_rot<float, 4, false>", "qft": "k_qft<float, 4, false, false>", "rzz": "k_phase",
-                 "cnot": "k_xor_gather", "cz": "k_phase"}
-    fused_name = {"u3": "k_rot<float, 4, true>", "qft": "k_qft<float, 4, false, true>"}
-    if per_type.get("rzz", {}).get("S", 1) == 0:
-        kernel_of["cnot"] = "k_xor_gather (+ deferred RZZ ring)"
+    # on the state's stream around back-to-back layers, as above)
+    names = {"k_rot": "k_rot<float, 4, false>",
+             "k_rot_gather": "k_rot<float, 4, true> (first sweep reads the deferred RZZ ring + CNOT chain)",
+             "k_rot_relabel": "k_rot_relabel<float, 4> (pending rotations, deferred ring + chain, first rotations)",
+             "k_qft": "k_qft<float, 4, false, false>",
+             "k_qft_gather": "k_qft<float, 4, false, true> (first sweep reads the deferred RZZ ring + CNOT chain)",
+             "k_phase": "k_phase", "k_xor_gather": "k_xor_gather"}
     kernels = {}
-    for t, v in per_type.items():
-        if v["S"] == 0:
-            continue  # deferred: its work runs inside the next layer's first sweep
-        cnt = labels.count(t)
-        base = t[:-len("_fused")] if t.endswith("_fused") else t
-        if t.endswith("_fused") and t in launch_ms:
-            # one gather-variant launch + (S - 1) plain ones
-            name = fused_name[base] + " (first sweep reads the deferred RZZ ring + CNOT chain)"
-            k = kernels.setdefault(name, {"avg_ms": launch_ms[t], "launches": 0})
-            k["launches"] += cnt
-            k = kernels.setdefault(kernel_of[base], {"avg_ms": launch_ms[base], "launches": 0})
-            k["launches"] += (v["S"] - 1) * cnt
-        else:
-            avg = launch_ms.get(t, float(np.mean(v["ms"])) / v["S"])
-            k = kernels.setdefault(kernel_of[base], {"avg_ms": avg, "launches": 0})
-            k["launches"] += v["S"] * cnt
-    for k in kernels.values():
-        k["ms"] = k["avg_ms"] * k["launches"]
-    for k in kernels.values():
-        k["avg_launch_ms"] = round(k.pop("avg_ms"), 4)
-        k["GBps"] = round(sweep_bytes / (k["avg_launch_ms"] * 1e-3) / 1e9, 1)
-        k["frac"] = round(k["GBps"] / PEAK_GBPS, 4)
-        k["ms"] = round(k["ms"], 3)
+    for k, cnt in launches.items():
+        avg = launch_ms.get(k)
+        if avg is None:
+            continue
+        kernels[names[k]] = {"launches": cnt, "ms": round(avg * cnt, 3), "avg_launch_ms": round(avg, 4),
+                             "GBps": round(sweep_bytes / (avg * 1e-3) / 1e9, 1),
+                             "frac": round(sweep_bytes / (avg * 1e-3) / 1e9 / PEAK_GBPS, 4)}
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     tile_avg_ms = kernels[dom]["avg_launch_ms"]
     achieved = sweep_bytes / (tile_avg_ms * 1e-3) / 1e9
@@ -645,21 +682,23 @@ def main():
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100.0 * value / PEAK_GBPS, 1),
-        "layers": {k: {"ms": round(float(np.mean(v["ms"])), 3), "S": v["S"],
-                       "GBps": (round(v["S"] * sweep_bytes / (np.mean(v["ms"]) * 1e-3) / 1e9, 1)
-                                if v["S"] else None),
-                       "pct_peak": (round(100 * v["S"] * sweep_bytes / (np.mean(v["ms"]) * 1e-3) / 1e9
-                                          / PEAK_GBPS, 1) if v["S"] else None)}
+        "layers": {k: {"ms": round(float(np.mean(v["ms"])), 3), "S": max(v["S"]),
+                       "GBps": (round(max(v["S"]) * sweep_bytes / (np.mean(v["ms"]) * 1e-3) / 1e9, 1)
+                                if max(v["S"]) else None),
+                       "pct_peak": (round(100 * max(v["S"]) * sweep_bytes / (np.mean(v["ms"]) * 1e-3) / 1e9
+                                          / PEAK_GBPS, 1) if max(v["S"]) else None)}
                    for k, v in per_type.items()},
         "sweeps_per_step": layer_sweeps_per_step,
         "layer_equivalent_GBps": round(s_min_per_step * sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-        "note": ("S = 0: the layer's work is deferred into the next layer's first sweep (an RZZ ring "
-                 "and a CNOT chain are read through by the first sweep of the following U3 layer or "
-                 "QFT, listed as u3_fused / qft_fused), or, for the first U3 layer (u3_on_basis), "
-                 "written as the product state of the reset |0...0> in one write pass; value counts "
-                 "only the sweeps performed. "
-                 "layer_equivalent_GBps prices each layer at its lower bound of sweeps (diagonal 1, "
-                 "XOR 1, U3 2, QFT 2) over the same step time"),
+        "note": ("S = sweeps a layer launches. S = 0: the layer's work is deferred into the next "
+                 "layer's first sweep (an RZZ ring and a CNOT chain are read through by the first sweep "
+                 "of the following U3 layer or QFT: u3_fused / qft_fused), or, for the first U3 layer "
+                 "(u3_on_basis), written as the product state of the reset |0...0> in one write pass. "
+                 "A U3 layer leaves its last sweep's rotations pending; the next U3 layer runs them in "
+                 "its first sweep across the deferred ring and chain (u3_cross), a QFT runs them first "
+                 "(one sweep, inside qft_fused). value counts only the sweeps performed; "
+                 "layer_equivalent_GBps prices each layer at a fixed count of sweeps (diagonal 1, "
+                 "XOR 1, U3 2, QFT 2) over the same step time, so it rises as the step gets faster"),
         "roofline": {
             "bound": "hbm",
             "kernel": dom,

@@ -299,6 +299,10 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
 #define MSV_TUNE_REORDER_GATES 15     /* gate lists (msv_apply_gates, msv_apply_circuit):
                                          commuting gates are regrouped into fewer tile
                                          programs (0/1) */
+#define MSV_TUNE_DEFER_TAIL 16        /* 1q layers of several sweeps leave their last
+                                         sweep's rotations pending; the next 1q layer
+                                         runs them in its first sweep, across a pending
+                                         XOR permutation (0/1) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 
@@ -306,6 +310,16 @@ int msv_get_tuning(int key, int64_t *value);
 /* Number of full-state sweep kernels launched on this state since creation
  * or the last reset of the counter. */
 int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset);
+/* Work the state holds back for the next operation (*out, a bit mask):
+ * MSV_PENDING_BASIS a basis state not yet written, MSV_PENDING_DIAG a phase
+ * layer, MSV_PENDING_PERM an XOR permutation, MSV_PENDING_TAIL a 1q layer's
+ * last rotations. No counterpart in the reference (its layers run eagerly);
+ * for execution reports and tests. */
+#define MSV_PENDING_BASIS 1
+#define MSV_PENDING_DIAG 2
+#define MSV_PENDING_PERM 4
+#define MSV_PENDING_TAIL 8
+int msv_state_pending(const msv_state *s, int *out);
 
 /* ---- timing ------------------------------------------------------------- */
 /* HIP events on the state's own stream: start records the first event,

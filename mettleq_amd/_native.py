@@ -54,6 +54,11 @@ MSV_TUNE_DEFER_DIAG = 12
 MSV_TUNE_DEFER_PERM = 13
 MSV_TUNE_FUSED_BLOCKS_PER_CU = 14
 MSV_TUNE_REORDER_GATES = 15
+MSV_TUNE_DEFER_TAIL = 16
+MSV_PENDING_BASIS = 1
+MSV_PENDING_DIAG = 2
+MSV_PENDING_PERM = 4
+MSV_PENDING_TAIL = 8
 
 
 class NativeUnavailableError(RuntimeError):
@@ -128,6 +133,7 @@ SIGNATURES = {
     "msv_set_tuning": (_I, [_I, ctypes.c_int64]),
     "msv_get_tuning": (_I, [_I, ctypes.POINTER(ctypes.c_int64)]),
     "msv_state_sweep_count": (_I, [_P, _U64P, _I]),
+    "msv_state_pending": (_I, [_P, ctypes.POINTER(ctypes.c_int)]),
     "msv_state_timer_start": (_I, [_P]),
     "msv_state_timer_stop": (_I, [_P, _DP]),
     "msv_probe_copy_bandwidth": (_I, [_I, _U64, _I, _DP]),

@@ -177,7 +177,14 @@ struct RotArgs {
   T shear[kMaxSegs][5][2];
   T pre[kDtabVals];
   T post[kDtabVals];
-  FusedRead fr;                         // k_rot<.., PERM = true> only
+  FusedRead fr;                         // k_rot<.., PERM = true> and k_rot_relabel only
+  // k_rot_relabel only: after segment rl_seg the tile changes frame through
+  // the deferred permutation. Local index u of that segment moves to local
+  // index Q u ^ delta(tile) of the next one; rl_thr[k] / rl_reg[j] are the
+  // images Q u of thread-id bit k / register bit j of segment rl_seg.
+  uint32_t rl_seg;
+  uint16_t rl_thr[kTileThreadBits];
+  uint16_t rl_reg[5];
 };
 static_assert(sizeof(RotArgs<float>) <= 4096, "rotation sweep arguments exceed 4 KiB");
 static_assert(sizeof(RotArgs<double>) <= 4096, "rotation sweep arguments exceed 4 KiB");
@@ -223,6 +230,17 @@ template <typename T>
 hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int rb, uint32_t grid,
                       hipStream_t stream, const typename Vec2<T>::type *perm_src = nullptr,
                       const typename Vec2<T>::type *ph_lut = nullptr, const uint64_t *ptab = nullptr);
+// Rotation sweep that changes frame mid-tile (k_rot_relabel): reads tiles of
+// src in the old frame, runs segments [0, rl_seg] there, multiplies by the
+// deferred diagonal (args.fr.ph, tables at ph_lut), moves every amplitude
+// to its new-frame slot, runs the remaining segments and stores into dst at
+// rep(tile) ^ images of the last segment's bits (args.fr.pthr / preg). The
+// tile-number byte tables at ptab hold rep | delta << 48 (entries 0..1023)
+// and the old-frame tile base (1024..2047).
+template <typename T>
+hipError_t launch_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> &args, int rb, uint32_t grid,
+                              hipStream_t stream, const typename Vec2<T>::type *src,
+                              const typename Vec2<T>::type *ph_lut, const uint64_t *ptab);
 template <typename T>
 hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
                       uint32_t grid, hipStream_t stream, const typename Vec2<T>::type *perm_src = nullptr,

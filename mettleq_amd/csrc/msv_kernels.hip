@@ -717,6 +717,154 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepW
   }
 }
 
+// Rotation sweep across a deferred permutation: the pending rotations of one
+// 1q layer (segments [0, so], old frame, tiles of src on coordinate bits),
+// that layer's post diagonal and the deferred diagonal at segment so, then
+// one exchange that writes each amplitude to its slot in the new frame
+// (local index Q u ^ delta, Q linear: the slot is an XOR of per-bit images,
+// swizzled the same way as every exchange), then the next layer's rotations
+// on the head bits (segments (so, last], their pre diagonal first). The new
+// frame's tile is the image of the old one under the permutation; its bits
+// are the head bits plus "extra" vectors, and its store address is
+// rep(tile) ^ the images of the last segment's bits. Tile numbers map to
+// old-frame bases and to rep | delta << 48 through byte tables (ptab[1024..]
+// and ptab[0..1023]): the host picks tile-number bits whose images are short
+// in both frames, so that the tiles running together read and write nearby
+// rows. Both diagonals are local to the tile (no per-tile factor). Reads and writes go to
+// different buffers when a permutation is pending (each output tile is the
+// image of exactly one input tile, so no tile reads what another writes).
+template <typename T, int RB>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepWaves<T, RB>::v)))
+k_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> A, const typename Vec2<T>::type *src,
+              const typename Vec2<T>::type *__restrict__ ph_lut, const uint64_t *__restrict__ ptab) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  extern __shared__ unsigned char smem_raw[];
+  V *lds = reinterpret_cast<V *>(smem_raw);
+  const uint32_t tid = threadIdx.x;
+  uint64_t vrank = blockIdx.x;
+  if (A.xcd_groups > 1 && gridDim.x % A.xcd_groups == 0)
+    vrank = (blockIdx.x % A.xcd_groups) * (gridDim.x / A.xcd_groups) + blockIdx.x / A.xcd_groups;
+  const FusedRead &F = A.fr;
+  V *lut = lds + (1u << A.nlocal);
+  for (uint32_t i = tid; i < F.ph.lut_len; i += blockDim.x) lut[i] = ph_lut[i];
+  const uint32_t so = A.rl_seg, last = A.nseg - 1;
+  __syncthreads();
+  for (uint64_t tile = vrank; tile < A.n_tiles; tile += gridDim.x) {
+    const uint64_t base = ptab[1024 + (tile & 255u)] ^ ptab[1280 + ((tile >> 8) & 255u)] ^
+                          ptab[1536 + ((tile >> 16) & 255u)] ^ ptab[1792 + ((tile >> 24) & 255u)];
+    V a[R];
+    for (uint32_t s = 0;; ++s) {
+      const TileSeg &sg = A.seg[s];
+      const SegPos p = seg_pos<RB>(sg, tid, base);
+      if (s == 0) {
+        uint64_t gstr[RB];
+#pragma unroll
+        for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = ld_amp(src + p.gb + reg_goff<RB>(r, gstr));
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
+        if (s == so + 1 && A.has_pre) {
+          V one;
+          one.x = T(1);
+          one.y = T(0);
+          dtab_apply_g<T, RB>(a, A.pre, p.tm, one);
+        }
+      }
+      const uint32_t gm = A.gmask[s];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        if ((gm >> j) & 1u) {
+          const T sa = A.shear[s][j][0], sb2 = A.shear[s][j][1];
+          switch (j) {
+            case 0: shear_pairs<T, RB, 0>(a, sa, sb2); break;
+            case 1: if constexpr (RB > 1) shear_pairs<T, RB, 1>(a, sa, sb2); break;
+            case 2: if constexpr (RB > 2) shear_pairs<T, RB, 2>(a, sa, sb2); break;
+            case 3: if constexpr (RB > 3) shear_pairs<T, RB, 3>(a, sa, sb2); break;
+            case 4: if constexpr (RB > 4) shear_pairs<T, RB, 4>(a, sa, sb2); break;
+            default: break;
+          }
+        }
+      }
+      if (s == last) break;
+      if (s == so) {
+        if (A.has_post) {
+          V one;
+          one.x = T(1);
+          one.y = T(0);
+          dtab_apply_g<T, RB>(a, A.post, p.tm, one);
+        }
+        if (F.ph.nwin) {
+          // this thread's index in segment so (old frame) without its register bits
+          const uint64_t gt = p.gb;
+          const uint32_t nw = F.ph.nwin, nrdep = F.ph_rdep;
+          V pt;
+          pt.x = T(1);
+          pt.y = T(0);
+          uint32_t bw[kFusedRdep];
+#pragma unroll
+          for (int w = 0; w < kFusedWins; ++w) {
+            if (w < static_cast<int>(nw)) {
+              const PhaseWin &pw = F.ph.win[w];
+              const uint32_t ix = win_bits(pw, gt);
+              if (w < kFusedRdep && w < static_cast<int>(nrdep))
+                bw[w < kFusedRdep ? w : 0] = ix;
+              else
+                pt = cmul(pt, lut[pw.off + ix]);
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            V q = pt;
+#pragma unroll
+            for (int w = 0; w < kFusedRdep; ++w)
+              if (w < static_cast<int>(nrdep)) q = cmul(q, lut[F.ph.win[w].off + (bw[w] ^ F.ph_rimg[w][r])]);
+            a[r] = cmul(a[r], q);
+          }
+        }
+        const uint64_t e = F.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
+                           ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
+        uint32_t qt = static_cast<uint32_t>(e >> 48);
+#pragma unroll
+        for (int k = 0; k < kTileThreadBits; ++k)
+          if ((p.tm >> k) & 1u) qt ^= A.rl_thr[k];
+        const uint32_t wbase = lds_swz<RB>(qt);
+        uint32_t ri[R];
+        ri[0] = 0;
+#pragma unroll
+        for (int r = 1; r < R; ++r) ri[r] = ri[r & (r - 1)] ^ A.rl_reg[__builtin_ctz(r)];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[wbase ^ lds_swz<RB>(ri[r])] = a[r];
+        __syncthreads();
+      } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)] = a[r];
+        __syncthreads();
+      }
+    }
+    {
+      const uint64_t e = F.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
+                         ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];
+      uint64_t sb = e & ((1ull << 48) - 1ull);
+      const uint32_t tm = tid & A.seg[last].tmask;
+#pragma unroll
+      for (int k = 0; k < kTileThreadBits; ++k)
+        if ((tm >> k) & 1u) sb ^= F.pthr[k];
+      uint64_t gi[R];
+      gi[0] = 0;
+#pragma unroll
+      for (int r = 1; r < R; ++r) gi[r] = gi[r & (r - 1)] ^ F.preg[__builtin_ctz(r)];
+#pragma unroll
+      for (int r = 0; r < R; ++r) st_amp(dst + (sb ^ gi[r]), a[r]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // QFT sweep
 // ---------------------------------------------------------------------------
@@ -1495,6 +1643,35 @@ hipError_t launch_rot(typename Vec2<T>::type *state, const RotArgs<T> &args, int
   }
 }
 
+template <typename T, int RB>
+hipError_t launch_rot_relabel_rb(typename Vec2<T>::type *dst, const RotArgs<T> &args, uint32_t grid,
+                                 hipStream_t stream, const typename Vec2<T>::type *src,
+                                 const typename Vec2<T>::type *ph_lut, const uint64_t *ptab) {
+  using V = typename Vec2<T>::type;
+  const uint32_t threads = 1u << (args.nlocal - RB);
+  const size_t lds = (sizeof(V) << args.nlocal) + sizeof(V) * args.fr.ph.lut_len;
+  const hipError_t e = allow_lds(k_rot_relabel<T, RB>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_rot_relabel<T, RB>), dim3(grid), dim3(threads), lds, stream, dst, args, src, ph_lut,
+                     ptab);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> &args, int rb, uint32_t grid,
+                              hipStream_t stream, const typename Vec2<T>::type *src,
+                              const typename Vec2<T>::type *ph_lut, const uint64_t *ptab) {
+  if (args.nseg < 2 || args.rl_seg + 1 >= args.nseg) return hipErrorInvalidValue;
+  switch (rb) {
+    case 3: return launch_rot_relabel_rb<T, 3>(dst, args, grid, stream, src, ph_lut, ptab);
+    case 4: return launch_rot_relabel_rb<T, 4>(dst, args, grid, stream, src, ph_lut, ptab);
+    case 5:
+      if constexpr (TileCfg<T>::kRB >= 5) return launch_rot_relabel_rb<T, 5>(dst, args, grid, stream, src, ph_lut, ptab);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <typename T>
 hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int rb, bool inverse,
                       uint32_t grid, hipStream_t stream, const typename Vec2<T>::type *perm_src,
@@ -1674,6 +1851,9 @@ hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uin
   template hipError_t launch_rot<T>(Vec2<T>::type *, const RotArgs<T> &, int, uint32_t,          \
                                     hipStream_t, const Vec2<T>::type *, const Vec2<T>::type *,   \
                                     const uint64_t *);                                           \
+  template hipError_t launch_rot_relabel<T>(Vec2<T>::type *, const RotArgs<T> &, int, uint32_t,  \
+                                            hipStream_t, const Vec2<T>::type *,                  \
+                                            const Vec2<T>::type *, const uint64_t *);            \
   template hipError_t launch_qft<T>(Vec2<T>::type *, const QftArgs<T> &, int, bool, uint32_t,    \
                                     hipStream_t, const Vec2<T>::type *, const Vec2<T>::type *,   \
                                     const uint64_t *);                                           \

@@ -66,6 +66,7 @@ struct Tuning {
   int defer_perm = 1;       // XOR permutations wait for the next operation (a 1q layer's first sweep reads through them)
   int fused_blocks_per_cu = 32;  // grid of a first sweep that reads through deferred work (tables staged per block)
   int reorder_gates = 1;    // gate lists: commuting gates are regrouped into fewer tile programs
+  int defer_tail = 1;       // 1q layers leave their last sweep's rotations to the next layer's first sweep
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -140,6 +141,18 @@ struct msv_state {
   // together with the deferred work above.
   bool basis_on = false;
   uint64_t basis_k = 0;
+  // Deferred rotation tail: the rotations of a 1q layer's last gate bits
+  // (shear factors per bit) and that layer's post diagonal (ratio per index
+  // bit, times tail_cst), not yet applied to buf. They act before the
+  // deferred diagonal and permutation above. The next 1q layer runs them in
+  // its first sweep ahead of its own rotations, crossing the permutation
+  // inside the tile (k_rot_relabel); every other entry point runs them
+  // first. Never on together with a deferred basis state.
+  bool tail_on = false;
+  std::vector<int> tail_bits;                   // ascending
+  std::vector<std::array<double, 2>> tail_sh;   // per index bit
+  std::vector<std::complex<double>> tail_post;  // per index bit
+  std::complex<double> tail_cst = 1.0;
 };
 
 namespace {
@@ -895,117 +908,657 @@ int prepare_fused_read(msv_state *s, Args &A, const std::vector<int> &local, int
   return MSV_OK;
 }
 
+using Shear = std::array<double, 2>;
+
+// Shear factors of a real rotation: [[c, -s], [s, c]] = X(a) Y(b) X(a) with
+// a = -tan(theta/2) = -s/(1+c), b = s.
+Shear shear_of(const RotSplit &z) { return {-z.s / (1.0 + z.c), z.s}; }
+
+// Segments of a rotation sweep over `local` with rotations on the local
+// positions `gl`: register windows, and each gate bit assigned to the first
+// window that holds it (A.gmask / A.shear from sh[index bit]).
 template <typename T>
-int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vector<RotSplit> &zb) {
+void assign_rotations(RotArgs<T> &A, const std::vector<int> &local, const std::vector<int> &gl,
+                      const std::vector<int> &wins, size_t seg0, int rb, const std::vector<Shear> &sh) {
+  std::vector<bool> done(local.size(), false);
+  for (size_t si = 0; si < wins.size(); ++si)
+    for (int k : gl) {
+      if (done[k] || k < wins[si] || k >= wins[si] + rb) continue;
+      done[k] = true;
+      const int j = k - wins[si];
+      A.gmask[seg0 + si] |= 1u << j;
+      A.shear[seg0 + si][j][0] = static_cast<T>(sh[local[k]][0]);
+      A.shear[seg0 + si][j][1] = static_cast<T>(sh[local[k]][1]);
+    }
+}
+
+// One k_rot sweep over the tile bits `local`: rotations on the index bits
+// `gates` (shear factors sh[bit]), the diagonal `pre` (ratio per index bit)
+// before them and `post` (ratio per index bit, times cst) after them, null
+// for none. fuse: the sweep reads through the state's deferred diagonal and
+// permutation (out of place when a permutation is pending).
+template <typename T>
+int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<int> &gates,
+                const std::vector<Shear> &sh, const std::vector<std::complex<double>> *pre,
+                const std::vector<std::complex<double>> *post, std::complex<double> cst, bool fuse) {
   using V = typename Vec2<T>::type;
   using C = std::complex<double>;
   const int n = s->n;
+  const int nlocal = static_cast<int>(local.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  std::vector<int> gl;
+  for (int k = 0; k < nlocal; ++k)
+    if (std::find(gates.begin(), gates.end(), local[k]) != gates.end()) gl.push_back(k);
+  uint32_t need = 0;
+  for (int k : gl) need |= 1u << k;
+  const std::vector<int> wins = commuting_windows(need, nlocal, rb, good_from(nlocal, rb));
+  if (wins.empty()) return fail(MSV_E_INVALID, "internal: no register window cover");
+  RotArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  MSV_TRY(fill_tile_geometry(n, local, wins, rb, &A));
+  assign_rotations<T>(A, local, gl, wins, 0, rb, sh);
+  const int ntgrp = (n - nlocal + 3) / 4;
+  if (ntgrp > kMaxTileGroups) return fail(MSV_E_INVALID, "internal: too many tile-number bits");
+  A.ntgrp = static_cast<uint32_t>(ntgrp);
+  std::vector<V> tgrp;
+  tgrp.reserve(2 * 16 * kMaxTileGroups);
+  const std::vector<C> one(n, 1.0);
+  A.has_pre = pre ? 1u : 0u;
+  build_layer_diag<T>(A, A.seg[0], local, rb, pre ? *pre : one, 1.0, A.pre, ntgrp, &tgrp);
+  A.has_post = post ? 1u : 0u;
+  build_layer_diag<T>(A, A.seg[wins.size() - 1], local, rb, post ? *post : one, post ? cst : C(1.0), A.post,
+                      ntgrp, &tgrp);
+  if (pre || post) {
+    void *d = nullptr;
+    MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &d));
+    A.tile_tab = reinterpret_cast<const V *>(d);
+  }
+  if (fuse) {
+    const uint64_t *d_ptab = nullptr;
+    const V *d_lut = nullptr;
+    MSV_TRY(prepare_fused_read<T>(s, A, local, rb, &d_ptab, &d_lut));
+    const bool oop = s->perm_on;  // a lone diagonal runs in place
+    if (oop) MSV_TRY(ensure_alt(s));
+    const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+    const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
+    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
+                          reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
+    if (oop) std::swap(s->buf, s->alt);
+    s->perm_on = false;
+    s->pend_on = false;
+  } else {
+    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+  }
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+// Runs a deferred rotation tail on its own (one sweep, in place): the
+// deferred diagonal and permutation after it stay pending.
+template <typename T>
+int flush_tail_T(msv_state *s) {
+  if (!s->tail_on) return MSV_OK;
+  s->tail_on = false;
+  const std::vector<int> local = high_sweep_local_bits<T>(s->tail_bits);
+  return rot_sweep_T<T>(s, local, s->tail_bits, s->tail_sh, nullptr, &s->tail_post, s->tail_cst, false);
+}
+
+// Inverse of the deferred permutation's linear part: img[g] = the new-frame
+// index vector whose source is old index bit g alone (col[q] = source of new
+// bit q), and img_c = the new-frame index of old index 0.
+bool perm_images(int n, const std::vector<uint64_t> &col, uint64_t c, std::vector<uint64_t> *img,
+                 uint64_t *img_c) {
+  // rows (old vector, new combination) reduced to the identity on the old side
+  std::vector<std::pair<uint64_t, uint64_t>> rows(n);
+  for (int q = 0; q < n; ++q) rows[q] = {col[q], 1ull << q};
+  for (int g = 0; g < n; ++g) {
+    int piv = -1;
+    for (int r = g; r < n; ++r)
+      if ((rows[r].first >> g) & 1ull) {
+        piv = r;
+        break;
+      }
+    if (piv < 0) return false;
+    std::swap(rows[g], rows[piv]);
+    for (int r = 0; r < n; ++r)
+      if (r != g && ((rows[r].first >> g) & 1ull)) {
+        rows[r].first ^= rows[g].first;
+        rows[r].second ^= rows[g].second;
+      }
+  }
+  img->assign(n, 0);
+  for (int g = 0; g < n; ++g) (*img)[g] = rows[g].second;
+  uint64_t y = 0;
+  for (int g = 0; g < n; ++g)
+    if ((c >> g) & 1ull) y ^= rows[g].second;
+  *img_c = y;
+  return true;
+}
+
+// Tile of a sweep that crosses the deferred permutation: old-frame tile bits
+// V (the columns, the pending tail's bits and spares chosen so that the new
+// frame gets the most coordinate bits), the new-frame coordinate ("head")
+// bits q, those whose source col[q] lies inside V, and the extra basis
+// vectors that complete the image of V.
+struct CrossTile {
+  std::vector<int> vloc;        // old-frame tile bits, ascending
+  uint64_t vmask = 0;
+  std::vector<int> head;        // new-frame coordinate bits, ascending
+  uint64_t hmask = 0;
+  std::vector<uint64_t> extra;  // further basis vectors (no head bits, reduced, pivots in piv)
+  std::vector<int> piv;
+};
+
+// Head bits of old-frame tile bits `vmask` under the permutation with source
+// columns col.
+uint64_t head_of(int n, const std::vector<uint64_t> &col, uint64_t vmask) {
+  uint64_t h = 0;
+  for (int q = 0; q < n; ++q)
+    if ((col[q] & ~vmask) == 0) h |= 1ull << q;
+  return h;
+}
+
+// V = columns + `fixed` + greedy spares (each adds the bit that makes the
+// most active bits head bits), up to nl bits.
+uint64_t choose_tile_bits(int n, int nl, int cb, const std::vector<int> &fixed, const std::vector<uint64_t> &col,
+                          const std::vector<bool> &act) {
+  uint64_t v = (1ull << cb) - 1ull;
+  for (int b : fixed) v |= 1ull << b;
+  auto score = [&](uint64_t m) {
+    const uint64_t h = head_of(n, col, m);
+    int c = 0;
+    for (int q = 0; q < n; ++q) c += ((h >> q) & 1ull) && act[q];
+    return c;
+  };
+  while (__builtin_popcountll(v) < nl) {
+    int best = -1, best_sc = -1;
+    for (int x = 0; x < n; ++x) {
+      if ((v >> x) & 1ull) continue;
+      const int sc = score(v | (1ull << x));
+      if (sc > best_sc) {
+        best_sc = sc;
+        best = x;
+      }
+    }
+    if (best < 0) break;
+    v |= 1ull << best;
+  }
+  return v;
+}
+
+bool cross_tile(int n, uint64_t vmask, const std::vector<uint64_t> &col, const std::vector<uint64_t> &img,
+                int cb, CrossTile *t) {
+  t->vmask = vmask;
+  t->vloc.clear();
+  for (int b = 0; b < n; ++b)
+    if ((vmask >> b) & 1ull) t->vloc.push_back(b);
+  t->hmask = head_of(n, col, vmask);
+  t->head.clear();
+  for (int q = 0; q < n; ++q)
+    if ((t->hmask >> q) & 1ull) t->head.push_back(q);
+  for (int q = 0; q < cb; ++q)
+    if (!((t->hmask >> q) & 1ull)) return false;  // the new tile's rows must stay contiguous
+  // extras: images of V without their head bits, reduced
+  std::vector<uint64_t> rows;
+  for (int g : t->vloc) rows.push_back(img[g] & ~t->hmask);
+  t->extra.clear();
+  t->piv.clear();
+  for (uint64_t r : rows) {
+    for (size_t k = 0; k < t->extra.size(); ++k)
+      if ((r >> t->piv[k]) & 1ull) r ^= t->extra[k];
+    if (!r) continue;
+    const int p = __builtin_ctzll(r);
+    for (size_t k = 0; k < t->extra.size(); ++k)
+      if ((t->extra[k] >> p) & 1ull) t->extra[k] ^= r;
+    t->extra.push_back(r);
+    t->piv.push_back(p);
+  }
+  return t->head.size() + t->extra.size() == t->vloc.size();
+}
+
+// Coordinates of new-frame index vector v in the cross tile's basis (head
+// bits in ascending order, then the extras), and what is left (zero on the
+// head bits and the pivots): v = rest ^ sum of the selected basis vectors.
+uint32_t cross_coords(const CrossTile &t, uint64_t v, uint64_t *rest) {
+  uint32_t u = 0;
+  for (size_t k = 0; k < t.head.size(); ++k)
+    if ((v >> t.head[k]) & 1ull) u |= 1u << k;
+  v &= ~t.hmask;
+  for (size_t k = 0; k < t.extra.size(); ++k)
+    if ((v >> t.piv[k]) & 1ull) {
+      u |= 1u << (t.head.size() + k);
+      v ^= t.extra[k];
+    }
+  *rest = v;
+  return u;
+}
+
+// The sweep that runs the pending tail (old frame: its rotations, its
+// layer's post diagonal, then the deferred diagonal), crosses the deferred
+// permutation inside the tile and runs this layer's rotations on the
+// active head bits (their pre diagonal first). Fails (MSV_E_INVALID,
+// nothing launched) when the tile does not fit the kernel's limits.
+template <typename T>
+int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act, const std::vector<Shear> &sh,
+                  const std::vector<std::complex<double>> &pre, bool pre_on, bool launch) {
+  using V = typename Vec2<T>::type;
+  using C = std::complex<double>;
+  const int n = s->n;
+  const int nlocal = static_cast<int>(t.vloc.size());
+  const int nh = static_cast<int>(t.head.size());
+  const int rb = std::min(rb_of<T>(), nlocal);
+  const int gf = good_from(nlocal, rb);
+  const int cb = std::max(0, std::min(col_bits<T>(), n));
+  std::vector<uint64_t> col(n), img;
+  uint64_t img_c = 0;
+  for (int q = 0; q < n; ++q) col[q] = s->perm_on ? s->perm_col[q] : (1ull << q);
+  if (!perm_images(n, col, s->perm_on ? s->perm_c : 0, &img, &img_c))
+    return fail(MSV_E_INVALID, "internal: deferred permutation not invertible");
+  // old frame: the tail's rotations
+  std::vector<int> gl_o;
+  for (int k = 0; k < nlocal; ++k)
+    if (std::find(s->tail_bits.begin(), s->tail_bits.end(), t.vloc[k]) != s->tail_bits.end()) gl_o.push_back(k);
+  if (gl_o.size() != s->tail_bits.size()) return fail(MSV_E_INVALID, "internal: cross tile");
+  uint32_t need_o = 0;
+  for (int k : gl_o) need_o |= 1u << k;
+  const std::vector<int> wo = commuting_windows(need_o, nlocal, rb, gf);
+  if (wo.empty()) return fail(MSV_E_INVALID, "internal: cross tile needs too many segments");
+  // Canonical coordinates of the new tile (head bits ascending, then the
+  // extras, as cross_coords gives them) go to local positions: the columns
+  // first; then, for each lane bit of the crossing segment in turn, one
+  // coordinate its image brings in (so that the lanes of a wave write slots
+  // that differ in the lowest bits, as in a plain exchange); then the rest.
+  std::vector<uint32_t> qc(nlocal);
+  for (int k = 0; k < nlocal; ++k) {
+    uint64_t rest = 0;
+    qc[k] = cross_coords(t, img[t.vloc[k]], &rest);
+    if (rest) return fail(MSV_E_INVALID, "internal: cross tile basis");
+  }
+  std::vector<int> cpos(nlocal, -1);
+  int next = 0;
+  for (int c = 0; c < cb; ++c) cpos[c] = next++;  // head bits 0 .. cb-1 are the columns
+  {
+    int lane = 0;
+    for (int k = 0; k < nlocal && lane < 6; ++k) {
+      if (k >= wo.back() && k < wo.back() + rb) continue;  // a register bit of the crossing segment
+      ++lane;
+      for (int c = 0; c < nlocal; ++c)
+        if (((qc[k] >> c) & 1u) && cpos[c] < 0) {
+          cpos[c] = next++;
+          break;
+        }
+    }
+  }
+  for (int c = 0; c < nlocal; ++c)
+    if (cpos[c] < 0) cpos[c] = next++;
+  auto to_local = [&](uint32_t uc) {
+    uint32_t u = 0;
+    for (int c = 0; c < nlocal; ++c)
+      if ((uc >> c) & 1u) u |= 1u << cpos[c];
+    return u;
+  };
+  // new-frame local positions as pseudo index bits (head bit q, extras n)
+  // and as index vectors
+  std::vector<int> nloc(nlocal, n);
+  std::vector<uint64_t> bvec(nlocal, 0);
+  for (int c = 0; c < nlocal; ++c) {
+    nloc[cpos[c]] = c < nh ? t.head[c] : n;
+    bvec[cpos[c]] = c < nh ? (1ull << t.head[c]) : t.extra[c - nh];
+  }
+  std::vector<int> gl_n;
+  for (int k = 0; k < nlocal; ++k)
+    if (nloc[k] < n && act[nloc[k]]) gl_n.push_back(k);
+  if (gl_n.empty()) return fail(MSV_E_INVALID, "internal: cross tile");
+  uint32_t need_n = 0;
+  for (int k : gl_n) need_n |= 1u << k;
+  const std::vector<int> wn = commuting_windows(need_n, nlocal, rb, gf);
+  if (wn.empty() || wo.size() + wn.size() > static_cast<size_t>(kMaxSegs))
+    return fail(MSV_E_INVALID, "internal: cross tile needs too many segments");
+  std::vector<int> wins = wo;
+  wins.insert(wins.end(), wn.begin(), wn.end());
+  RotArgs<T> A;
+  std::memset(&A, 0, sizeof(A));
+  MSV_TRY(fill_tile_geometry(n, t.vloc, wins, rb, &A));
+  const uint32_t so = static_cast<uint32_t>(wo.size() - 1);
+  A.rl_seg = so;
+  assign_rotations<T>(A, t.vloc, gl_o, wo, 0, rb, s->tail_sh);
+  std::vector<Shear> sh_n(n + 1, Shear{0.0, 0.0});
+  for (int q : t.head) sh_n[q] = sh[q];
+  assign_rotations<T>(A, nloc, gl_n, wn, wo.size(), rb, sh_n);
+  // diagonals, both local to the tile: this layer's pre on the active head
+  // bits after the crossing, the tail's post (on the tail bits) before it
+  A.ntgrp = 0;
+  std::vector<C> pre_n(n + 1, 1.0);
+  for (int q : t.head)
+    if (act[q]) pre_n[q] = pre[q];
+  std::vector<V> tgrp;
+  tgrp.reserve(2 * 16 * kMaxTileGroups);
+  A.has_pre = pre_on ? 1u : 0u;
+  build_layer_diag<T>(A, A.seg[so + 1], nloc, rb, pre_n, 1.0, A.pre, 0, &tgrp);
+  A.has_post = 1u;
+  build_layer_diag<T>(A, A.seg[so], t.vloc, rb, s->tail_post, s->tail_cst, A.post, 0, &tgrp);
+  // the crossing: images Q u of segment so's bits, per-tile rep | delta << 48
+  const TileSeg &sgo = A.seg[so];
+  for (int k = 0; k < kTileThreadBits; ++k)
+    A.rl_thr[k] = (k < nlocal - rb) ? static_cast<uint16_t>(to_local(qc[sgo.tl_bit[k]])) : 0;
+  for (int j = 0; j < rb; ++j) A.rl_reg[j] = static_cast<uint16_t>(to_local(qc[sgo.reg0 + j]));
+  auto packed = [&](uint64_t v) {
+    uint64_t rest = 0;
+    const uint32_t u = to_local(cross_coords(t, v, &rest));
+    return rest | (static_cast<uint64_t>(u) << 48);
+  };
+  FusedRead &F = A.fr;
+  F.pc = packed(img_c);
+  // Tile-number bits: one per new-frame index bit h outside the new tile
+  // (neither a head bit nor an extra's pivot), ascending; its old-frame base
+  // vector is the source of h with the tile's own bits cleared, so that
+  // consecutive tiles differ in nearby bits in both frames.
+  uint64_t pivm = 0;
+  for (int pv : t.piv) pivm |= 1ull << pv;
+  std::vector<uint64_t> tbase, timg;
+  for (int h = 0; h < n; ++h) {
+    if (((t.hmask | pivm) >> h) & 1ull) continue;
+    const uint64_t o = col[h] & ~t.vmask;
+    uint64_t y = 0;
+    for (int g = 0; g < n; ++g)
+      if ((o >> g) & 1ull) y ^= img[g];
+    tbase.push_back(o);
+    timg.push_back(packed(y));
+  }
+  if (static_cast<int>(tbase.size()) != n - nlocal || tbase.size() > static_cast<size_t>(kMaxTileBits))
+    return fail(MSV_E_INVALID, "internal: cross tile numbering");
+  {
+    // the bases must reach every coset of the tile: full rank on the bits outside it
+    std::vector<uint64_t> m(tbase);
+    int rank = 0;
+    for (int g = 0; g < n; ++g) {
+      if ((t.vmask >> g) & 1ull) continue;
+      int piv = -1;
+      for (size_t r = rank; r < m.size(); ++r)
+        if ((m[r] >> g) & 1ull) {
+          piv = static_cast<int>(r);
+          break;
+        }
+      if (piv < 0) continue;
+      std::swap(m[rank], m[piv]);
+      for (size_t r = 0; r < m.size(); ++r)
+        if (static_cast<int>(r) != rank && ((m[r] >> g) & 1ull)) m[r] ^= m[rank];
+      ++rank;
+    }
+    if (rank != n - nlocal) return fail(MSV_E_INVALID, "internal: cross tile numbering");
+  }
+  // store images: the new-frame index vector of each local position
+  const TileSeg &sgl = A.seg[wins.size() - 1];
+  for (int k = 0; k < kTileThreadBits; ++k) F.pthr[k] = (k < nlocal - rb) ? bvec[sgl.tl_bit[k]] : 0ull;
+  for (int j = 0; j < rb; ++j) F.preg[j] = bvec[sgl.reg0 + j];
+  // the deferred diagonal at segment so (old frame): windows that see its
+  // register bits first
+  uint64_t umask = 0;
+  for (int j = 0; j < rb; ++j) umask |= 1ull << t.vloc[sgo.reg0 + j];
+  if (s->pend_on) {
+    if (s->pend_lut.size() > kFusedLutMaxBytes || s->pend_args.nwin > static_cast<uint32_t>(kFusedWins))
+      return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for a cross sweep");
+    F.ph = s->pend_args;
+    std::vector<PhaseWin> dep, rest;
+    for (uint32_t w = 0; w < F.ph.nwin; ++w) {
+      const PhaseWin &pw = s->pend_args.win[w];
+      const uint64_t wm = (((1ull << pw.w1) - 1ull) << pw.s1) | (((1ull << pw.w2) - 1ull) << pw.s2);
+      (wm & umask ? dep : rest).push_back(pw);
+    }
+    if (dep.size() > static_cast<size_t>(kFusedRdep))
+      return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for a cross sweep");
+    F.ph_rdep = static_cast<uint32_t>(dep.size());
+    uint32_t w = 0;
+    for (const PhaseWin &pw : dep) F.ph.win[w++] = pw;
+    for (const PhaseWin &pw : rest) F.ph.win[w++] = pw;
+    for (size_t k = 0; k < dep.size(); ++k) {
+      const PhaseWin &pw = dep[k];
+      for (int r = 0; r < (1 << rb); ++r) {
+        uint64_t im = 0;
+        for (int j = 0; j < rb; ++j)
+          if ((r >> j) & 1) im |= 1ull << t.vloc[sgo.reg0 + j];
+        const uint32_t lo = static_cast<uint32_t>(im >> pw.s1) & ((1u << pw.w1) - 1u);
+        const uint32_t hi = static_cast<uint32_t>(im >> pw.s2) & ((1u << pw.w2) - 1u);
+        F.ph_rimg[k][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
+      }
+    }
+  } else {
+    F.ph.nwin = 0;
+    F.ph.lut_len = 0;
+  }
+  if (!launch) return MSV_OK;
+  A.tile_tab = nullptr;
+  std::vector<uint64_t> ptab(8 * 256, 0);
+  for (int b = 0; b < 4; ++b)
+    for (int v = 0; v < 256; ++v) {
+      uint64_t x = 0, o = 0;
+      for (int i = 0; i < 8; ++i) {
+        const size_t k = 8 * static_cast<size_t>(b) + i;
+        if (((v >> i) & 1) && k < timg.size()) {
+          x ^= timg[k];
+          o ^= tbase[k];
+        }
+      }
+      ptab[256 * b + v] = x;
+      ptab[1024 + 256 * b + v] = o;
+    }
+  void *dp = nullptr;
+  MSV_TRY(stage(s, ptab.data(), ptab.size() * sizeof(uint64_t), &dp));
+  const V *d_lut = nullptr;
+  if (s->pend_on) {
+    void *dl = nullptr;
+    MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &dl));
+    d_lut = reinterpret_cast<const V *>(dl);
+  }
+  const bool oop = s->perm_on;
+  if (oop) MSV_TRY(ensure_alt(s));
+  const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+  const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
+  MSV_HIP(launch_rot_relabel<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
+                                reinterpret_cast<const V *>(s->buf), d_lut,
+                                reinterpret_cast<const uint64_t *>(dp)));
+  if (oop) std::swap(s->buf, s->alt);
+  s->perm_on = false;
+  s->pend_on = false;
+  s->tail_on = false;
+  s->sweeps += 1;
+  return MSV_OK;
+}
+
+// Gate-bit sets of a 1q layer's sweeps without a crossing: the low sweep
+// (tile = bits [0, L)), then the high sweeps (columns + a chunk).
+template <typename T>
+std::vector<std::vector<int>> layer_gate_sets(int n, const std::vector<bool> &act) {
   int L = 0;
   std::vector<std::pair<int, int>> chunks;
   partition<T>(n, &L, &chunks);
-  struct Sw {
-    std::vector<int> local;
-    std::vector<int> gl;  // local positions of the gate bits
-  };
-  std::vector<Sw> sweeps;
-  {
-    Sw w;
-    w.local = sweep_local_bits<T>(n, L, true, {0, 0});
-    for (int k = 0; k < static_cast<int>(w.local.size()); ++k)
-      if (act_bit[w.local[k]]) w.gl.push_back(k);
-    if (!w.gl.empty()) sweeps.push_back(w);
+  std::vector<std::vector<int>> sets;
+  std::vector<int> low;
+  for (int b = 0; b < L; ++b)
+    if (act[b]) low.push_back(b);
+  if (!low.empty()) sets.push_back(low);
+  for (const std::vector<int> &g : high_gate_sets<T>(chunks)) {
+    std::vector<int> a;
+    for (int b : g)
+      if (act[b]) a.push_back(b);
+    if (!a.empty()) sets.push_back(a);
   }
-  for (const std::vector<int> &gates : high_gate_sets<T>(chunks)) {
-    Sw w;
-    w.local = high_sweep_local_bits<T>(gates);
-    for (int k = 0; k < static_cast<int>(w.local.size()); ++k)
-      if (w.local[k] >= L && act_bit[w.local[k]]) w.gl.push_back(k);
-    if (!w.gl.empty()) sweeps.push_back(w);
+  return sets;
+}
+
+// Splits the bits `rest` (ascending) left after a crossing sweep into a tail
+// (up to G bits, consecutive in `rest`, chosen so that the next crossing
+// through a permutation like `pred` gets the most head bits) and sweeps of
+// at most G bits.
+template <typename T>
+void split_rest(int n, const std::vector<int> &rest, int G, bool defer, const std::vector<uint64_t> &pred,
+                std::vector<std::vector<int>> *sets, std::vector<int> *tail) {
+  sets->clear();
+  tail->clear();
+  std::vector<int> plain = rest;
+  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int cb = std::max(0, std::min(col_bits<T>(), n));
+  if (defer && rest.size() > 1) {
+    const int k = std::min<int>(G, static_cast<int>(rest.size()) - 1);
+    const std::vector<bool> all(n, true);
+    int best = -1, best_sc = -1;
+    for (int i = 0; i + k <= static_cast<int>(rest.size()); ++i) {
+      const std::vector<int> cand(rest.begin() + i, rest.begin() + i + k);
+      const uint64_t v = choose_tile_bits(n, NL, cb, cand, pred, all);
+      const int sc = __builtin_popcountll(head_of(n, pred, v));
+      if (sc >= best_sc) {
+        best_sc = sc;
+        best = i;
+      }
+    }
+    tail->assign(rest.begin() + best, rest.begin() + best + k);
+    plain.clear();
+    for (int b : rest)
+      if (std::find(tail->begin(), tail->end(), b) == tail->end()) plain.push_back(b);
   }
-  if (sweeps.empty()) return flush_pending_T<T>(s);
-  // Deferred permutation and/or diagonal: the first sweep reads through
-  // them (out of place) when can_fuse_read allows; otherwise they run first.
-  bool fuse = false;
-  if (s->perm_on || s->pend_on) {
-    const int rb0 = std::min(rb_of<T>(), static_cast<int>(sweeps[0].local.size()));
-    uint32_t need = 0;
-    for (int k : sweeps[0].gl) need |= 1u << k;
-    const int nl0 = static_cast<int>(sweeps[0].local.size());
-    const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
-    fuse = !w0.empty() && can_fuse_read<T>(s, sweeps[0].local, w0[0], rb0);
+  const int m = static_cast<int>(plain.size());
+  const int ns = (m + G - 1) / G;
+  for (int c = 0, at = 0; c < ns; ++c) {
+    const int sz = m / ns + (c < m % ns ? 1 : 0);
+    sets->emplace_back(plain.begin() + at, plain.begin() + at + sz);
+    at += sz;
   }
-  if (!fuse) MSV_TRY(flush_pending_T<T>(s));
+}
+
+// A layer's post diagonal split into the part on the bits `tail` (post_t)
+// and the rest (post_k).
+void split_post(const std::vector<std::complex<double>> &post, const std::vector<int> &tail,
+                std::vector<std::complex<double>> *post_k, std::vector<std::complex<double>> *post_t) {
+  *post_k = post;
+  post_t->assign(post.size(), 1.0);
+  for (int b : tail) {
+    (*post_t)[b] = post[b];
+    (*post_k)[b] = 1.0;
+  }
+}
+
+// Leaves the rotations on `tail` pending, with their post diagonal post_t
+// and the layer's constant.
+void keep_tail(msv_state *s, const std::vector<int> &tail, const std::vector<Shear> &sh,
+               const std::vector<std::complex<double>> &post_t, std::complex<double> cst) {
+  s->tail_on = true;
+  s->tail_bits = tail;
+  s->tail_sh = sh;
+  s->tail_post = post_t;
+  s->tail_cst = cst;
+}
+
+template <typename T>
+int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vector<RotSplit> &zb) {
+  using C = std::complex<double>;
+  const int n = s->n;
+  std::vector<Shear> sh(n, Shear{0.0, 0.0});
   std::vector<C> pre(n, 1.0), post(n, 1.0);
   C cst = 1.0;
   bool pre_on = false;
   for (int b = 0; b < n; ++b) {
     if (!act_bit[b]) continue;
+    sh[b] = shear_of(zb[b]);
     pre[b] = zb[b].b2;
     post[b] = zb[b].b1 / zb[b].a1;
     cst *= zb[b].a1;
     pre_on |= std::abs(zb[b].b2 - 1.0) > 0.0;
   }
-  for (size_t i = 0; i < sweeps.size(); ++i) {
-    const Sw &sw = sweeps[i];
-    const int nlocal = static_cast<int>(sw.local.size());
-    const int rb = std::min(rb_of<T>(), nlocal);
-    uint32_t need = 0;
-    for (int k : sw.gl) need |= 1u << k;
-    const std::vector<int> wins = commuting_windows(need, nlocal, rb, good_from(nlocal, rb));
-    if (wins.empty()) return fail(MSV_E_INVALID, "internal: no register window cover");
-    RotArgs<T> A;
-    std::memset(&A, 0, sizeof(A));
-    MSV_TRY(fill_tile_geometry(n, sw.local, wins, rb, &A));
-    // each gate bit rotates in the first window that holds it
-    std::vector<bool> done(nlocal, false);
-    for (size_t si = 0; si < wins.size(); ++si) {
-      for (int k : sw.gl) {
-        if (done[k] || k < wins[si] || k >= wins[si] + rb) continue;
-        done[k] = true;
-        const int j = k - wins[si];
-        const RotSplit &z = zb[sw.local[k]];
-        // [[c, -s], [s, c]] = X(a) Y(b) X(a): a = -tan(theta/2) = -s/(1+c), b = s
-        A.gmask[si] |= 1u << j;
-        A.shear[si][j][0] = static_cast<T>(-z.s / (1.0 + z.c));
-        A.shear[si][j][1] = static_cast<T>(z.s);
+  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int cb = std::max(0, std::min(col_bits<T>(), n));
+  const int G = std::max(rb_of<T>(), std::min(g_tune.gate_bits, NL - col_bits<T>()));
+  const bool defer = g_tune.defer_tail != 0;
+  std::vector<uint64_t> pred(n);
+  for (int q = 0; q < n; ++q) pred[q] = s->perm_on ? s->perm_col[q] : (1ull << q);
+  const std::vector<std::vector<int>> std_sets = layer_gate_sets<T>(n, act_bit);
+  if (std_sets.empty()) return flush_pending_T<T>(s);
+  // A pending tail: cross the deferred permutation in one sweep when that
+  // launches fewer sweeps than running the tail on its own first.
+  if (s->tail_on) {
+    bool crossed = false;
+    const uint64_t vmask = choose_tile_bits(n, NL, cb, s->tail_bits, pred, act_bit);
+    CrossTile t;
+    std::vector<uint64_t> img;
+    uint64_t img0 = 0;
+    if (__builtin_popcountll(vmask) == NL && perm_images(n, pred, 0, &img, &img0) &&
+        cross_tile(n, vmask, pred, img, cb, &t)) {
+      std::vector<int> rest;
+      for (int b = 0; b < n; ++b)
+        if (act_bit[b] && !((t.hmask >> b) & 1ull)) rest.push_back(b);
+      std::vector<std::vector<int>> sets;
+      std::vector<int> tail;
+      split_rest<T>(n, rest, G, defer, pred, &sets, &tail);
+      const size_t cross_cost = 1 + sets.size();
+      const size_t plain_cost = 1 + std_sets.size() - ((defer && std_sets.size() > 1) ? 1 : 0);
+      if (!rest.empty() && !sets.empty() && cross_cost < plain_cost &&
+          cross_sweep_T<T>(s, t, act_bit, sh, pre, pre_on, false) == MSV_OK) {
+        MSV_TRY(cross_sweep_T<T>(s, t, act_bit, sh, pre, pre_on, true));
+        crossed = true;
+        // the rest: the first sweep also applies the pre diagonal of every
+        // remaining bit (tail included); the last one the post diagonal of
+        // every bit but the tail's, which the tail keeps
+        std::vector<C> pre_r(n, 1.0);
+        bool pre_r_on = false;
+        for (int b : rest) {
+          pre_r[b] = pre[b];
+          pre_r_on |= std::abs(pre[b] - 1.0) > 0.0;
+        }
+        std::vector<C> post_k, post_t;
+        split_post(post, tail, &post_k, &post_t);
+        for (size_t i = 0; i < sets.size(); ++i) {
+          const bool last = i + 1 == sets.size();
+          MSV_TRY(rot_sweep_T<T>(s, high_sweep_local_bits<T>(sets[i]), sets[i], sh,
+                                 (i == 0 && pre_r_on) ? &pre_r : nullptr, last ? &post_k : nullptr,
+                                 tail.empty() ? cst : C(1.0), false));
+        }
+        if (!tail.empty()) keep_tail(s, tail, sh, post_t, cst);
+        return MSV_OK;
       }
     }
-    const int ntgrp = (n - nlocal + 3) / 4;
-    if (ntgrp > kMaxTileGroups) return fail(MSV_E_INVALID, "internal: too many tile-number bits");
-    A.ntgrp = static_cast<uint32_t>(ntgrp);
-    std::vector<V> tgrp;
-    tgrp.reserve(2 * 16 * kMaxTileGroups);
-    std::vector<C> one(n, 1.0);
-    A.has_pre = (i == 0 && pre_on) ? 1u : 0u;
-    build_layer_diag<T>(A, A.seg[0], sw.local, rb, A.has_pre ? pre : one, 1.0, A.pre, ntgrp, &tgrp);
-    A.has_post = (i + 1 == sweeps.size()) ? 1u : 0u;
-    build_layer_diag<T>(A, A.seg[wins.size() - 1], sw.local, rb, A.has_post ? post : one,
-                        A.has_post ? cst : C(1.0), A.post, ntgrp, &tgrp);
-    if (A.has_pre || A.has_post) {
-      void *d = nullptr;
-      MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &d));
-      A.tile_tab = reinterpret_cast<const V *>(d);
-    }
-    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
-    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-    if (i == 0 && fuse) {
-      const uint64_t *d_ptab = nullptr;
-      const V *d_lut = nullptr;
-      MSV_TRY(prepare_fused_read<T>(s, A, sw.local, rb, &d_ptab, &d_lut));
-      const bool oop = s->perm_on;  // a lone diagonal runs in place
-      if (oop) MSV_TRY(ensure_alt(s));
-      const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
-      const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
-                            reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
-      if (oop) std::swap(s->buf, s->alt);
-      s->perm_on = false;
-      s->pend_on = false;
-    } else {
-      MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
-    }
-    s->sweeps += 1;
+    if (!crossed) MSV_TRY(flush_tail_T<T>(s));
   }
+  // sweep tiles: the low sweep's tile is bits [0, L), a high sweep's the
+  // columns + its chunk
+  int L = 0;
+  std::vector<std::pair<int, int>> chunks;
+  partition<T>(n, &L, &chunks);
+  auto tile_of = [&](const std::vector<int> &set) {
+    if (set.back() < L) {
+      std::vector<int> local;
+      for (int b = 0; b < L; ++b) local.push_back(b);
+      return local;
+    }
+    return high_sweep_local_bits<T>(set);
+  };
+  // the last sweep's rotations become the pending tail when the layer has
+  // several sweeps (and its tile is a high sweep's)
+  size_t nrun = std_sets.size();
+  const bool defer_last = defer && nrun > 1 && std_sets.back().front() >= L;
+  if (defer_last) --nrun;
+  // Deferred permutation and/or diagonal: the first sweep reads through
+  // them (out of place) when can_fuse_read allows; otherwise they run first.
+  bool fuse = false;
+  if (s->perm_on || s->pend_on) {
+    const std::vector<int> local0 = tile_of(std_sets[0]);
+    const int nl0 = static_cast<int>(local0.size());
+    const int rb0 = std::min(rb_of<T>(), nl0);
+    uint32_t need = 0;
+    for (int k = 0; k < nl0; ++k)
+      if (std::find(std_sets[0].begin(), std_sets[0].end(), local0[k]) != std_sets[0].end()) need |= 1u << k;
+    const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
+    fuse = !w0.empty() && can_fuse_read<T>(s, local0, w0[0], rb0);
+  }
+  if (!fuse) MSV_TRY(flush_pending_T<T>(s));
+  std::vector<C> post_k, post_t;
+  split_post(post, defer_last ? std_sets.back() : std::vector<int>(), &post_k, &post_t);
+  for (size_t i = 0; i < nrun; ++i) {
+    const bool last = i + 1 == nrun;
+    MSV_TRY(rot_sweep_T<T>(s, tile_of(std_sets[i]), std_sets[i], sh, (i == 0 && pre_on) ? &pre : nullptr,
+                           last ? &post_k : nullptr, defer_last ? C(1.0) : cst, i == 0 && fuse));
+  }
+  if (defer_last) keep_tail(s, std_sets.back(), sh, post_t, cst);
   return MSV_OK;
 }
 
@@ -1633,6 +2186,7 @@ int flush_pending_T(msv_state *s) {
     s->basis_on = false;
     MSV_TRY(init_basis_T<T>(s, s->basis_k));
   }
+  MSV_TRY(flush_tail_T<T>(s));  // the tail acts before the diagonal and the permutation
   if (s->perm_on) {
     s->perm_on = false;
     return launch_gather<T>(s, s->perm_col, s->perm_c);  // applies a pending diagonal too
@@ -1649,7 +2203,8 @@ constexpr size_t kPendLutMaxBytes = 48 * 1024;
 template <typename T>
 int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool defer) {
   using V = typename Vec2<T>::type;
-  MSV_TRY(flush_pending_T<T>(s));
+  // a pending tail stays when this diagonal is deferred behind it
+  if (s->pend_on || s->perm_on || s->basis_on) MSV_TRY(flush_pending_T<T>(s));
   if (terms.empty()) return MSV_OK;
   const int wmax = 8;
   std::vector<Win> wins;
@@ -1742,6 +2297,7 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool de
     s->pend_lut.assign(b, b + chunks[0].second.size() * sizeof(V));
     return MSV_OK;
   }
+  MSV_TRY(flush_tail_T<T>(s));
   for (auto &ch : chunks)
     MSV_TRY(launch_phase_chunk<T>(s, ch.first, ch.second.data(), ch.second.size() * sizeof(V)));
   return MSV_OK;
@@ -1792,6 +2348,7 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
     // basis state is written first.
     if (s->basis_on) MSV_TRY(flush_pending_T<T>(s));
     if (s->perm_on) {
+      MSV_TRY(flush_tail_T<T>(s));  // a pending tail acts before the pending permutation
       s->perm_on = false;
       MSV_TRY(launch_gather<T>(s, s->perm_col, s->perm_c));
     }
@@ -2065,7 +2622,7 @@ int init_basis_T(msv_state *s, uint64_t index) {
 // point that reads or writes amplitudes calls it first, except the XOR
 // permutation, which absorbs it, and the phase layers themselves.
 int flush_pending(msv_state *s) {
-  if (!s->pend_on && !s->perm_on && !s->basis_on) return MSV_OK;
+  if (!s->pend_on && !s->perm_on && !s->basis_on && !s->tail_on) return MSV_OK;
   MSV_TRY(set_dev(s));
   return MSV_DISPATCH(s, flush_pending_T, s);
 }
@@ -2313,6 +2870,7 @@ int msv_state_set_basis(msv_state *s, uint64_t index) {
   MSV_TRY(set_dev(s));
   s->pend_on = false;  // the new basis state replaces what deferred work would act on
   s->perm_on = false;
+  s->tail_on = false;
   s->basis_on = true;  // written by the next operation (a 1q layer: as its product state)
   s->basis_k = index;
   return MSV_OK;
@@ -2360,7 +2918,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
   MSV_TRY(flush_pending(src));
   MSV_TRY(set_dev(dst));
-  if (dst != src) dst->pend_on = dst->perm_on = dst->basis_on = false;  // overwritten
+  if (dst != src) dst->pend_on = dst->perm_on = dst->basis_on = dst->tail_on = false;  // overwritten
   MSV_HIP(hipStreamSynchronize(src->stream));
   const uint32_t grid = elem_grid(dst, dst->amps);
   if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
@@ -2519,8 +3077,9 @@ int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
                                    " qubits");
   MSV_TRY(set_dev(s));
   // deferred work is read by the first QFT sweep or flushed before it; a
-  // deferred basis state is written first
+  // deferred basis state is written first, a deferred rotation tail run first
   if (s->basis_on) MSV_TRY(flush_pending(s));
+  if (s->tail_on) MSV_TRY(MSV_DISPATCH(s, flush_tail_T, s));
   const int hb = s->n - 1 - first, lb = s->n - m;
   return MSV_DISPATCH(s, apply_qft_T, s, hb, lb, inverse != 0, -1);
 }
@@ -2778,6 +3337,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "reorder gates must be 0 or 1");
       g_tune.reorder_gates = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_DEFER_TAIL:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer tail must be 0 or 1");
+      g_tune.defer_tail = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2801,6 +3364,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_DEFER_PERM: *value = g_tune.defer_perm; return MSV_OK;
     case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune.fused_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_REORDER_GATES: *value = g_tune.reorder_gates; return MSV_OK;
+    case MSV_TUNE_DEFER_TAIL: *value = g_tune.defer_tail; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -2809,6 +3373,14 @@ int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset) {
   MSV_TRY(check_state(s));
   if (out) *out = s->sweeps;
   if (reset) s->sweeps = 0;
+  return MSV_OK;
+}
+
+int msv_state_pending(const msv_state *s, int *out) {
+  MSV_TRY(check_state(s));
+  if (!out) return fail(MSV_E_INVALID, "null output");
+  *out = (s->basis_on ? MSV_PENDING_BASIS : 0) | (s->pend_on ? MSV_PENDING_DIAG : 0) |
+         (s->perm_on ? MSV_PENDING_PERM : 0) | (s->tail_on ? MSV_PENDING_TAIL : 0);
   return MSV_OK;
 }
 

@@ -87,6 +87,12 @@ class NativeState:
         nat.check(nat.lib().msv_state_sweep_count(self.handle, ctypes.byref(out), int(reset)), "sweeps")
         return int(out.value)
 
+    def pending(self) -> int:
+        """Bit mask of deferred work (nat.MSV_PENDING_*)."""
+        out = ctypes.c_int(0)
+        nat.check(nat.lib().msv_state_pending(self.handle, ctypes.byref(out)), "pending")
+        return int(out.value)
+
     def timer_start(self) -> None:
         nat.check(nat.lib().msv_state_timer_start(self.handle), "timer_start")
 

@@ -90,8 +90,10 @@ def test_layer_cycle_then_inverse_returns_the_state_30q(gpu):
             apply(st, kind, arg, True)
         f = _fidelity(ref, st)
         assert abs(st.norm2() - 1.0) < 1e-5
-    # RZZ rings and CNOT chains ride in the next sweep: 3 U3 layers + QFT
-    assert forward_sweeps == 4 * 3
+    # RZZ rings and CNOT chains ride in the next sweep; each U3 layer leaves
+    # its last sweep's rotations to the next one (two sweeps per layer), the
+    # QFT runs the pending ones first (one sweep + three)
+    assert forward_sweeps == 2 + 2 + (1 + 3) + 2
     assert 1.0 - f < 1e-6
 
 

@@ -33,6 +33,27 @@ def _rand_u(rng):
     return O.U3(t, p, l)
 
 
+class _Tuned:
+    """Sets tuning keys for the duration of a with-block and restores them."""
+
+    def __init__(self, **kv):
+        from mettleq_amd import _native as nat
+        self.kv = {getattr(nat, "MSV_TUNE_" + k): v for k, v in kv.items()}
+        self.old = {}
+
+    def __enter__(self):
+        from mettleq_amd import _native as nat
+        for k, v in self.kv.items():
+            self.old[k] = nat.get_tuning(k)
+            nat.set_tuning(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        from mettleq_amd import _native as nat
+        for k, v in self.old.items():
+            nat.set_tuning(k, v)
+
+
 @pytest.mark.parametrize("n", [1, 2, 4, 5, 6, 9, 13, 14, 16, 17, 20, 22])
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_1q_layer_matches_oracle(gpu, n, dtype):
@@ -608,7 +629,7 @@ def test_deferred_phase_matches_immediate_at_26q(gpu):
     try:
         for defer in (0, 1):
             nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, defer)
-            with _state(n) as s:
+            with _Tuned(DEFER_TAIL=0), _state(n) as s:
                 s.apply_1q_layer(mats)
                 s0 = s.sweep_count()
                 s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
@@ -730,7 +751,7 @@ def test_deferred_permutation_fused_sweep_counts_at_26q(gpu):
     th = [float(x) for x in rng.uniform(0, math.pi, len(bonds))]
     rows, c = _chain(n)
     outs, sweeps = [], []
-    with _state(n) as s:
+    with _Tuned(DEFER_TAIL=0), _state(n) as s:
         s0 = s.sweep_count()
         s.apply_1q_layer(m1)
         u3 = s.sweep_count() - s0
@@ -738,7 +759,7 @@ def test_deferred_permutation_fused_sweep_counts_at_26q(gpu):
         for d in (0, 1):
             nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
             nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, d)
-            with _state(n) as s:
+            with _Tuned(DEFER_TAIL=0), _state(n) as s:
                 s0 = s.sweep_count()
                 s.apply_1q_layer(m1)
                 s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
@@ -817,7 +838,7 @@ def test_deferred_work_read_by_qft_sweep_counts_at_26q(gpu):
     bonds = ring_bonds(n)
     th = [float(x) for x in rng.uniform(0, math.pi, len(bonds))]
     rows, c = _chain(n)
-    with _state(n) as s:
+    with _Tuned(DEFER_TAIL=0), _state(n) as s:
         s0 = s.sweep_count()
         s.apply_1q_layer(m1)
         u3 = s.sweep_count() - s0
@@ -828,7 +849,7 @@ def test_deferred_work_read_by_qft_sweep_counts_at_26q(gpu):
         for d in (0, 1):
             nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
             nat.set_tuning(nat.MSV_TUNE_DEFER_PERM, d)
-            with _state(n) as s:
+            with _Tuned(DEFER_TAIL=0), _state(n) as s:
                 s0 = s.sweep_count()
                 s.apply_1q_layer(m1)
                 s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, th)
@@ -866,7 +887,7 @@ def test_lone_deferred_diagonal_read_in_place_by_next_layer(gpu, dtype):
     try:
         for d in (0, 1):
             nat.set_tuning(nat.MSV_TUNE_DEFER_DIAG, d)
-            with _state(n, dtype) as s:
+            with _Tuned(DEFER_TAIL=0), _state(n, dtype) as s:
                 s.from_numpy(psi)
                 s0 = s.sweep_count()
                 s.apply_1q_layer(m1)
@@ -1042,3 +1063,214 @@ def test_gate_lists_with_controls_swaps_and_diagonals(gpu, n, reorder):
     finally:
         nat.set_tuning(nat.MSV_TUNE_REORDER_GATES, old)
     assert np.max(np.abs(got - want)) < 2e-5
+
+
+def _layer_cycle(s, want, n, rng, layers, ring, perm):
+    """U3 layer, then (RZZ ring) (permutation) before each further U3 layer;
+    returns the oracle's state."""
+    from mettleq_amd import _native as nat
+    bonds = [(i, (i + 1) % n) for i in range(n)]
+    for layer in range(layers):
+        if layer:
+            if ring:
+                th = rng.uniform(0, math.pi, n)
+                s.apply_phase_layer([nat.MSV_PHASE_ZZ] * n, bonds, list(th))
+                want = O.zz_layer_diag(n, bonds, th) * want
+            if perm is not None:
+                rows, c = perm
+                s.apply_xor_affine(rows, c)
+                want = O.xor_affine_gather(want, n, rows, c)
+        mats = [_rand_u(rng) for _ in range(n)]
+        s.apply_1q_layer(mats)
+        want = O.apply_1q_layer(want, n, mats)
+    return want
+
+
+@pytest.mark.parametrize("n", [16, 17, 20])
+@pytest.mark.parametrize("perm", ["chain", "random", "none"])
+@pytest.mark.parametrize("ring", [False, True])
+def test_pending_tail_crosses_into_the_next_layer(gpu, n, perm, ring):
+    """With 9-bit tiles a complex64 layer takes three or four sweeps; its last
+    sweep's rotations stay pending and the next layer's first sweep runs
+    them, the deferred ring and permutation, and its own first rotations,
+    changing frame inside the tile (k_rot_relabel). The chain keeps the
+    column bits, so every layer after the first crosses; a random map does
+    not (its gather runs after the pending rotations). Four layers against
+    the oracle, with pending tails on and off; fewer sweeps when on."""
+    rng0 = np.random.default_rng(8100 + n)
+    psi = _rand_state(n, 81 * n)
+    p = {"chain": _chain(n), "random": _random_affine(n, rng0), "none": None}[perm]
+    outs, sweeps = [], []
+    for tail in (0, 1):
+        with _Tuned(TILE_LOCAL_BITS=9, DEFER_TAIL=tail):
+            rng = np.random.default_rng(8200 + n)
+            with _state(n) as s:
+                s.from_numpy(psi)
+                s0 = s.sweep_count()
+                want = _layer_cycle(s, psi, n, rng, 4, ring, p)
+                got = s.to_numpy()
+                sweeps.append(s.sweep_count() - s0)
+        assert np.max(np.abs(got - want)) < TOL64
+        outs.append(got)
+    assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
+    if perm != "random":
+        assert sweeps[1] < sweeps[0]
+
+
+@pytest.mark.parametrize("perm", ["chain", "none"])
+def test_pending_tail_crosses_complex128(gpu, perm):
+    """complex128 (12-bit tiles, three sweeps per layer at 22 qubits)."""
+    n = 22
+    rng = np.random.default_rng(8300)
+    psi = _rand_state(n, 83)
+    p = _chain(n) if perm == "chain" else None
+    with _state(n, "complex128") as s:
+        s.from_numpy(psi)
+        s0 = s.sweep_count()
+        want = _layer_cycle(s, psi, n, rng, 3, True, p)
+        got = s.to_numpy()
+        sw = s.sweep_count() - s0
+    assert np.max(np.abs(got - want)) < TOL128
+    assert sw == 2 + 2 + 2 + 1  # the last tail runs when the state is read
+
+
+@pytest.mark.parametrize("after", ["probs", "qft", "qft_after_ring_chain", "gates", "xor", "phase",
+                                   "diagonal", "inner", "sample", "reset", "layer_subset", "scale",
+                                   "copy", "upload_part", "second_layer"])
+def test_pending_tail_then_other_operations(gpu, after):
+    """A 1q layer leaves rotations pending; every other operation runs them
+    first (a QFT reads the deferred ring and chain after them), a reset drops
+    them, and a second layer runs them in its first sweep."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import gate_u
+    n = 16
+    rng = np.random.default_rng(8400)
+    psi = _rand_state(n, 84)
+    mats = [_rand_u(rng) for _ in range(n)]
+    rows, c = _chain(n)
+    rows2, c2 = _random_affine(n, rng)
+    want = O.apply_1q_layer(psi, n, mats)
+    # 9-bit tiles (three sweeps per layer) except before a QFT, whose sweep
+    # planner takes the default tiles
+    bits = 13 if after.startswith("qft") else 9
+    with _Tuned(TILE_LOCAL_BITS=bits, DEFER_TAIL=1), _state(n) as s:
+        s.from_numpy(psi)
+        s.apply_1q_layer(mats)
+        if after == "probs":
+            assert np.max(np.abs(s.probabilities() - np.abs(want) ** 2)) < TOL64
+        elif after == "qft":
+            s.apply_qft(0, n)
+            want = O.qft_fft(want, n)
+        elif after == "qft_after_ring_chain":
+            bonds = [(i, (i + 1) % n) for i in range(n)]
+            th = rng.uniform(0, math.pi, n)
+            s.apply_phase_layer([nat.MSV_PHASE_ZZ] * n, bonds, list(th))
+            s.apply_xor_affine(rows, c)
+            s.apply_qft(0, n)
+            want = O.qft_fft(O.xor_affine_gather(O.zz_layer_diag(n, bonds, th) * want, n, rows, c), n)
+        elif after == "gates":
+            s.apply_gates([gate_u(4, mats[0]), gate_u(0, mats[1], [12])])
+            want = O.apply_controlled(O.apply_dense(want, n, mats[0], [4]), n, mats[1], [12], 0)
+        elif after == "xor":
+            s.apply_xor_affine(rows2, c2)
+            want = O.xor_affine_gather(want, n, rows2, c2)
+        elif after == "phase":
+            s.apply_phase_layer([nat.MSV_PHASE_CPHASE], [(2, 9)], [0.7])
+            want = O.cphase_layer_diag(n, [(2, 9)], [0.7]) * want
+        elif after == "diagonal":
+            d0, d1 = np.exp(0.3j), np.exp(-1.1j)
+            s.apply_diagonal(one=[(3, d0, d1)])
+            want = O.apply_dense(want, n, np.diag([d0, d1]), [3])
+        elif after == "inner":
+            with _state(n) as t:
+                t.from_numpy(psi)
+                assert abs(t.inner(s) - np.vdot(psi, want)) < 1e-5
+        elif after == "sample":
+            got = s.sample(200, None, seed=3)
+            assert np.array_equal(got, O.sample_indices(O.amp_probs_c64(s.to_numpy()), 200, 3, 0))
+        elif after == "reset":
+            s.reset()
+            want = O.zero_state(n)
+        elif after == "layer_subset":
+            act = [0, 5, 12]
+            s.apply_1q_layer(mats, active=act)
+            full = [mats[q] if q in act else np.eye(2) for q in range(n)]
+            want = O.apply_1q_layer(want, n, full)
+        elif after == "scale":
+            s.scale(-1j)
+            want = -1j * want
+        elif after == "copy":
+            with _state(n) as t:
+                t.copy_from(s)
+                assert np.max(np.abs(t.to_numpy() - want)) < TOL64
+        elif after == "upload_part":
+            s.from_numpy(np.full(4, 0.5, dtype=np.complex64), offset=0)
+            want = want.copy()
+            want[:4] = 0.5
+        elif after == "second_layer":
+            m2 = [_rand_u(rng) for _ in range(n)]
+            s.apply_1q_layer(m2)
+            want = O.apply_1q_layer(want, n, m2)
+        got = s.to_numpy()
+    assert np.max(np.abs(got - want)) < TOL64
+
+
+def test_pending_tail_layered_sweep_counts_at_30q(gpu):
+    """U3 / RZZ ring / CNOT chain at 30 qubits: the first layer takes two
+    sweeps and leaves eight bits pending; every further layer takes two (the
+    crossing sweep and one plain sweep) and leaves nine; the last tail runs
+    when the state is read. Checked against the same layers without pending
+    tails."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.workloads import ring_bonds
+    n = 30
+    rng = np.random.default_rng(3000)
+    bonds = ring_bonds(n)
+    rows, c = _chain(n)
+    mats = [[_rand_u(rng) for _ in range(n)] for _ in range(3)]
+    ths = [[float(x) for x in rng.uniform(0, math.pi, len(bonds))] for _ in range(2)]
+    res = []
+    for tail in (1, 0):
+        with _Tuned(DEFER_TAIL=tail), _state(n) as s:
+            s.from_numpy(np.zeros(1, dtype=np.complex64) + 1, offset=0)
+            counts = []
+            for i, m in enumerate(mats):
+                if i:
+                    s.apply_phase_layer([nat.MSV_PHASE_ZZ] * len(bonds), bonds, ths[i - 1])
+                    s.apply_xor_affine(rows, c)
+                s0 = s.sweep_count()
+                s.apply_1q_layer(m)
+                counts.append(s.sweep_count() - s0)
+            s0 = s.sweep_count()
+            s.sync()
+            counts.append(s.sweep_count() - s0)
+            win = [s.to_numpy(o, 1 << 14) for o in (0, 1 << 29, (1 << 30) - (1 << 14))]
+            res.append((counts, win))
+    assert res[0][0] == [2, 2, 2, 1]
+    assert res[1][0] == [3, 3, 3, 0]
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.max(np.abs(a - b)) < 2e-6
+
+
+def test_pending_mask_follows_deferred_work(gpu):
+    """msv_state_pending reports what the state holds back: the reset's basis
+    state, a 1q layer's last rotations, a phase layer, a permutation; a sync
+    runs all of it."""
+    from mettleq_amd import _native as nat
+    n = 20
+    rng = np.random.default_rng(8500)
+    bonds = [(i, (i + 1) % n) for i in range(n)]
+    rows, c = _chain(n)
+    with _state(n) as s:
+        s.reset()
+        assert s.pending() == nat.MSV_PENDING_BASIS
+        s.apply_1q_layer([_rand_u(rng) for _ in range(n)])  # the product state: nothing left
+        assert s.pending() == 0
+        s.apply_1q_layer([_rand_u(rng) for _ in range(n)])  # two sweeps: the second stays pending
+        assert s.pending() == nat.MSV_PENDING_TAIL
+        s.apply_phase_layer([nat.MSV_PHASE_ZZ] * n, bonds, list(rng.uniform(0, 1, n)))
+        assert s.pending() == nat.MSV_PENDING_TAIL | nat.MSV_PENDING_DIAG
+        s.apply_xor_affine(rows, c)
+        assert s.pending() == nat.MSV_PENDING_TAIL | nat.MSV_PENDING_DIAG | nat.MSV_PENDING_PERM
+        s.sync()
+        assert s.pending() == 0
