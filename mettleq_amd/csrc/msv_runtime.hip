@@ -1328,6 +1328,49 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
     F.ph.nwin = 0;
     F.ph.lut_len = 0;
   }
+  // The tail's post diagonal (its bits' factors and the constant) folds into
+  // the deferred diagonal's tables when a window holds each of those bits:
+  // the diagonals multiply at the same point, and the sweep then skips one
+  // diagonal pass over its registers.
+  std::vector<V> lut;
+  if (s->pend_on) {
+    lut.resize(s->pend_lut.size() / sizeof(V));
+    std::memcpy(lut.data(), s->pend_lut.data(), lut.size() * sizeof(V));
+    std::vector<std::pair<uint32_t, int>> at;  // (window, bit) per folded factor
+    bool all = true;
+    for (int b : s->tail_bits) {
+      if (s->tail_post[b] == C(1.0)) continue;
+      int wf = -1;
+      for (uint32_t w = 0; w < F.ph.nwin && wf < 0; ++w) {
+        const PhaseWin &pw = F.ph.win[w];
+        if ((b >= pw.s1 && b < pw.s1 + pw.w1) || (b >= pw.s2 && b < pw.s2 + pw.w2)) wf = static_cast<int>(w);
+      }
+      if (wf < 0) {
+        all = false;
+        break;
+      }
+      at.push_back({static_cast<uint32_t>(wf), b});
+    }
+    if (all && F.ph.nwin > 0) {
+      auto scale = [&](const PhaseWin &pw, uint32_t idx, C f) {
+        V &e = lut[pw.off + idx];
+        const C v = C(e.x, e.y) * f;
+        e.x = static_cast<T>(v.real());
+        e.y = static_cast<T>(v.imag());
+      };
+      const uint32_t len0 = 1u << (F.ph.win[0].w1 + F.ph.win[0].w2);
+      for (uint32_t i = 0; i < len0; ++i) scale(F.ph.win[0], i, s->tail_cst);
+      for (auto &wb : at) {
+        const PhaseWin &pw = F.ph.win[wb.first];
+        const int b = wb.second;
+        const int pos = (b >= pw.s1 && b < pw.s1 + pw.w1) ? b - pw.s1 : pw.w1 + (b - pw.s2);
+        const uint32_t len = 1u << (pw.w1 + pw.w2);
+        for (uint32_t i = 0; i < len; ++i)
+          if ((i >> pos) & 1u) scale(pw, i, s->tail_post[b]);
+      }
+      A.has_post = 0u;
+    }
+  }
   if (!launch) return MSV_OK;
   A.tile_tab = nullptr;
   std::vector<uint64_t> ptab(8 * 256, 0);
@@ -1349,7 +1392,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   const V *d_lut = nullptr;
   if (s->pend_on) {
     void *dl = nullptr;
-    MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &dl));
+    MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &dl));
     d_lut = reinterpret_cast<const V *>(dl);
   }
   const bool oop = s->perm_on;
