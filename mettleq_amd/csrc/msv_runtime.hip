@@ -67,6 +67,7 @@ struct Tuning {
   int fused_blocks_per_cu = 32;  // grid of a first sweep that reads through deferred work (tables staged per block)
   int reorder_gates = 1;    // gate lists: commuting gates are regrouped into fewer tile programs
   int defer_tail = 1;       // 1q layers leave their last sweep's rotations to the next layer's first sweep
+                            // (1: it crosses a pending permutation only, 2: also without one)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -1142,7 +1143,8 @@ uint32_t cross_coords(const CrossTile &t, uint64_t v, uint64_t *rest) {
 // nothing launched) when the tile does not fit the kernel's limits.
 template <typename T>
 int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act, const std::vector<Shear> &sh,
-                  const std::vector<std::complex<double>> &pre, bool pre_on, bool launch) {
+                  const std::vector<std::complex<double>> &pre, bool pre_on, bool launch,
+                  uint64_t *rotated = nullptr) {
   using V = typename Vec2<T>::type;
   using C = std::complex<double>;
   const int n = s->n;
@@ -1207,15 +1209,25 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
     nloc[cpos[c]] = c < nh ? t.head[c] : n;
     bvec[cpos[c]] = c < nh ? (1ull << t.head[c]) : t.extra[c - nh];
   }
+  // the active head bits this sweep rotates: all of them, less the highest
+  // positions while the segments would not fit (the rest rotate in the
+  // plain sweeps that follow)
   std::vector<int> gl_n;
   for (int k = 0; k < nlocal; ++k)
     if (nloc[k] < n && act[nloc[k]]) gl_n.push_back(k);
-  if (gl_n.empty()) return fail(MSV_E_INVALID, "internal: cross tile");
-  uint32_t need_n = 0;
-  for (int k : gl_n) need_n |= 1u << k;
-  const std::vector<int> wn = commuting_windows(need_n, nlocal, rb, gf);
-  if (wn.empty() || wo.size() + wn.size() > static_cast<size_t>(kMaxSegs))
-    return fail(MSV_E_INVALID, "internal: cross tile needs too many segments");
+  std::vector<int> wn;
+  while (!gl_n.empty()) {
+    uint32_t need_n = 0;
+    for (int k : gl_n) need_n |= 1u << k;
+    wn = commuting_windows(need_n, nlocal, rb, gf);
+    if (!wn.empty() && wo.size() + wn.size() <= static_cast<size_t>(kMaxSegs)) break;
+    gl_n.pop_back();
+  }
+  if (gl_n.empty()) return fail(MSV_E_INVALID, "internal: cross tile needs too many segments");
+  if (rotated) {
+    *rotated = 0;
+    for (int k : gl_n) *rotated |= 1ull << nloc[k];
+  }
   std::vector<int> wins = wo;
   wins.insert(wins.end(), wn.begin(), wn.end());
   RotArgs<T> A;
@@ -1231,8 +1243,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   // bits after the crossing, the tail's post (on the tail bits) before it
   A.ntgrp = 0;
   std::vector<C> pre_n(n + 1, 1.0);
-  for (int q : t.head)
-    if (act[q]) pre_n[q] = pre[q];
+  for (int k : gl_n) pre_n[nloc[k]] = pre[nloc[k]];
   std::vector<V> tgrp;
   tgrp.reserve(2 * 16 * kMaxTileGroups);
   A.has_pre = pre_on ? 1u : 0u;
@@ -1518,25 +1529,30 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
   const std::vector<std::vector<int>> std_sets = layer_gate_sets<T>(n, act_bit);
   if (std_sets.empty()) return flush_pending_T<T>(s);
   // A pending tail: cross the deferred permutation in one sweep when that
-  // launches fewer sweeps than running the tail on its own first.
+  // launches fewer sweeps than running the tail on its own first. Without a
+  // pending permutation the first sweep would read a lone diagonal in place
+  // (or nothing), and the crossing sweep's extra rotations cost about what
+  // it saves (34-qubit brick layers measured slower), so by default it
+  // crosses only a permutation.
   if (s->tail_on) {
     bool crossed = false;
-    const uint64_t vmask = choose_tile_bits(n, NL, cb, s->tail_bits, pred, act_bit);
+    const bool try_cross = s->perm_on || g_tune.defer_tail == 2;
+    const uint64_t vmask = try_cross ? choose_tile_bits(n, NL, cb, s->tail_bits, pred, act_bit) : 0;
     CrossTile t;
     std::vector<uint64_t> img;
-    uint64_t img0 = 0;
-    if (__builtin_popcountll(vmask) == NL && perm_images(n, pred, 0, &img, &img0) &&
-        cross_tile(n, vmask, pred, img, cb, &t)) {
-      std::vector<int> rest;
+    uint64_t img0 = 0, rot = 0;
+    if (try_cross && __builtin_popcountll(vmask) == NL && perm_images(n, pred, 0, &img, &img0) &&
+        cross_tile(n, vmask, pred, img, cb, &t) &&
+        cross_sweep_T<T>(s, t, act_bit, sh, pre, pre_on, false, &rot) == MSV_OK) {
+      std::vector<int> rest;  // the active bits the crossing does not rotate
       for (int b = 0; b < n; ++b)
-        if (act_bit[b] && !((t.hmask >> b) & 1ull)) rest.push_back(b);
+        if (act_bit[b] && !((rot >> b) & 1ull)) rest.push_back(b);
       std::vector<std::vector<int>> sets;
       std::vector<int> tail;
       split_rest<T>(n, rest, G, defer, pred, &sets, &tail);
       const size_t cross_cost = 1 + sets.size();
       const size_t plain_cost = 1 + std_sets.size() - ((defer && std_sets.size() > 1) ? 1 : 0);
-      if (!rest.empty() && !sets.empty() && cross_cost < plain_cost &&
-          cross_sweep_T<T>(s, t, act_bit, sh, pre, pre_on, false) == MSV_OK) {
+      if (!rest.empty() && !sets.empty() && cross_cost < plain_cost) {
         MSV_TRY(cross_sweep_T<T>(s, t, act_bit, sh, pre, pre_on, true));
         crossed = true;
         // the rest: the first sweep also applies the pre diagonal of every
@@ -3381,7 +3397,7 @@ int msv_set_tuning(int key, int64_t value) {
       g_tune.reorder_gates = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_DEFER_TAIL:
-      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer tail must be 0 or 1");
+      if (value < 0 || value > 2) return fail(MSV_E_INVALID, "defer tail must be 0, 1 or 2");
       g_tune.defer_tail = static_cast<int>(value);
       return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");

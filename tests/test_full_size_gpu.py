@@ -91,9 +91,10 @@ def test_layer_cycle_then_inverse_returns_the_state_30q(gpu):
         f = _fidelity(ref, st)
         assert abs(st.norm2() - 1.0) < 1e-5
     # RZZ rings and CNOT chains ride in the next sweep; each U3 layer leaves
-    # its last sweep's rotations to the next one (two sweeps per layer), the
-    # QFT runs the pending ones first (one sweep + three)
-    assert forward_sweeps == 2 + 2 + (1 + 3) + 2
+    # its last sweep's rotations pending: the first U3 layer (no permutation
+    # before it) runs them first (one sweep + two), the second crosses the
+    # chain (two sweeps), the QFT runs them first (one sweep + three)
+    assert forward_sweeps == (1 + 2) + 2 + (1 + 3) + 2
     assert 1.0 - f < 1e-6
 
 
@@ -196,6 +197,30 @@ def test_complex64_against_complex128_after_a_cycle_30q(gpu):
             st.apply_xor_affine(rows, c)
             st.apply_qft(0, N)
             st.apply_1q_layer(mats)
+        f = _fidelity(states[0], states[1])
+    finally:
+        for st in states:
+            st.close()
+    assert 1.0 - f < 1e-6
+
+
+def test_brick_layers_complex64_against_complex128_30q(gpu):
+    """U3 layers and CZ bricks (a lone deferred diagonal between layers, each
+    layer's last sweep pending): complex64 against the complex128 mode after
+    six layers, 1 - F <= 1e-6."""
+    from mettleq_amd import _native as nat
+    from mettleq_amd.native_state import NativeState
+    rng = np.random.default_rng(7070)
+    layers = [(_mats(rng, N), [(q, q + 1) for q in range(l % 2, N - 1, 2)]) for l in range(6)]
+    states = []
+    try:
+        for dt in ("complex64", "complex128"):
+            st = NativeState(N, dt)
+            states.append(st)
+            st.reset()
+            for mats, bonds in layers:
+                st.apply_1q_layer(mats)
+                st.apply_phase_layer([nat.MSV_PHASE_CPHASE] * len(bonds), bonds, [math.pi] * len(bonds))
         f = _fidelity(states[0], states[1])
     finally:
         for st in states:

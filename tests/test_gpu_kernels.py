@@ -1101,7 +1101,9 @@ def test_pending_tail_crosses_into_the_next_layer(gpu, n, perm, ring):
     psi = _rand_state(n, 81 * n)
     p = {"chain": _chain(n), "random": _random_affine(n, rng0), "none": None}[perm]
     outs, sweeps = [], []
-    for tail in (0, 1):
+    # without a permutation the crossing is taken only on request (2); by
+    # default (1) the pending rotations run first, in their own sweep
+    for tail in ((0, 1) if perm != "none" else (0, 2, 1)):
         with _Tuned(TILE_LOCAL_BITS=9, DEFER_TAIL=tail):
             rng = np.random.default_rng(8200 + n)
             with _state(n) as s:
@@ -1112,9 +1114,12 @@ def test_pending_tail_crosses_into_the_next_layer(gpu, n, perm, ring):
                 sweeps.append(s.sweep_count() - s0)
         assert np.max(np.abs(got - want)) < TOL64
         outs.append(got)
-    assert np.max(np.abs(outs[0] - outs[1])) < 2e-6
+    for o in outs[1:]:
+        assert np.max(np.abs(outs[0] - o)) < 2e-6
     if perm != "random":
         assert sweeps[1] < sweeps[0]
+    if perm == "none":
+        assert sweeps[2] == sweeps[0]
 
 
 @pytest.mark.parametrize("perm", ["chain", "none"])
@@ -1124,7 +1129,7 @@ def test_pending_tail_crosses_complex128(gpu, perm):
     rng = np.random.default_rng(8300)
     psi = _rand_state(n, 83)
     p = _chain(n) if perm == "chain" else None
-    with _state(n, "complex128") as s:
+    with _Tuned(DEFER_TAIL=1 if perm == "chain" else 2), _state(n, "complex128") as s:
         s.from_numpy(psi)
         s0 = s.sweep_count()
         want = _layer_cycle(s, psi, n, rng, 3, True, p)
@@ -1209,7 +1214,8 @@ def test_pending_tail_then_other_operations(gpu, after):
             want[:4] = 0.5
         elif after == "second_layer":
             m2 = [_rand_u(rng) for _ in range(n)]
-            s.apply_1q_layer(m2)
+            with _Tuned(DEFER_TAIL=2):  # cross with no permutation pending
+                s.apply_1q_layer(m2)
             want = O.apply_1q_layer(want, n, m2)
         got = s.to_numpy()
     assert np.max(np.abs(got - want)) < TOL64
