@@ -125,6 +125,13 @@ class Device:
                 L.yy_layer(st, op["theta"], n, op["bonds"])
             elif name == "_CULISTLAYER":
                 L.controlled_u_groups(st, n, op["groups"])
+            elif name == "_CHAINPHASE_RXLAYER":
+                # the diagonal stays pending and the RX layer's first sweep reads it
+                if op["phase_kind"] == "zz":
+                    L.zz_chain_layer(st, op["theta"], n, op["bonds"])
+                else:
+                    L.diag_pair_layer(st, op["theta"], n, op["bonds"])
+                st.apply_1q_layer([G.single_qubit_matrix("RX", [op["rx_theta"]])] * n)
             else:
                 raise ValueError(f"unknown fused op {name}")
         flush()

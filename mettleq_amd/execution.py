@@ -299,8 +299,10 @@ def layer_sweeps(op: Dict[str, Any], n: int, dtype: str = STATE_DTYPE) -> int:
         return 1
     if name in ("_XORLAYER",):
         return 1
-    if name in ("_XXLAYER", "_YYLAYER", "_CHAINPHASE_RXLAYER"):
+    if name in ("_XXLAYER", "_YYLAYER"):
         return 2 * _sweeps_for_bits(list(range(n)), n, nl) + 1
+    if name == "_CHAINPHASE_RXLAYER":  # the diagonal rides in the RX layer's first sweep
+        return _sweeps_for_bits(list(range(n)), n, nl)
     if name == "_GATEBLOCK":
         return int(op.get("sweeps", 1))
     return 1

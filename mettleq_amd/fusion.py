@@ -12,6 +12,7 @@ src/mettleq) so the same gate streams become the same structured layers:
   CZ/CPHASE runs -> _DIAGLAYER / _DIAGWEIGHTED                    (898-938)
   ZZPHASE runs -> _ZZLAYER / _ZZWEIGHTED                          (939-965)
   uniform XX/YY runs -> _XXLAYER / _YYLAYER                       (970-989)
+  ZZ chain or CZ/CPHASE chain/ring + RX layer -> _CHAINPHASE_RXLAYER (1029-1087)
 
 Ops left unmatched become "_GATE" entries holding device gates
 (``gates_for_op``, the routing of device.py:1138-1290); the dispatcher
@@ -333,7 +334,49 @@ def _fuse(operations: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:
             collapsed.append(fused[index])
             index += 1
         fused = collapsed
+    if n >= 4 and _env_on("METTLEQ_CHAINPHASE_RX_FUSION"):
+        fused = _amortize_chain_phase_rx(fused, n)
     return fused
+
+
+def _amortize_chain_phase_rx(fused: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:
+    """A uniform ZZ layer on the open chain, or a uniform CZ/CPHASE layer on
+    distinct chain bonds (plus the (0, n-1) wrap), followed by an RX layer
+    becomes one _CHAINPHASE_RXLAYER (device.py:1029-1087). This engine runs
+    it as the diagonal deferred into the RX layer's first sweep; the entry
+    keeps the reference's fields (edge_mask: bit n-2-lo per bond (lo, lo+1),
+    wrap_mask) and the bonds."""
+    out: List[Dict[str, Any]] = []
+    i = 0
+    while i < len(fused):
+        cur = fused[i]
+        nxt = fused[i + 1] if i + 1 < len(fused) else None
+        if nxt is not None and nxt.get("name") == "_RXLAYER":
+            bonds = [tuple(sorted(int(w) for w in b)) for b in cur.get("bonds", [])]
+            entry = None
+            if cur.get("name") == "_ZZLAYER" and sorted(bonds) == [(q, q + 1) for q in range(n - 1)]:
+                entry = {"phase_kind": "zz", "n_bonds": n - 1, "edge_mask": (1 << (n - 1)) - 1, "wrap_mask": 0}
+            elif cur.get("name") == "_DIAGLAYER" and len(bonds) == len(set(bonds)):
+                edge, wrap, ok = 0, 0, True
+                for lo, hi in bonds:
+                    if hi - lo == 1:
+                        edge |= 1 << (n - 2 - lo)
+                    elif lo == 0 and hi == n - 1:
+                        wrap = (1 << (n - 1)) | 1
+                    else:
+                        ok = False
+                        break
+                if ok:
+                    entry = {"phase_kind": "cphase", "n_bonds": len(bonds), "edge_mask": edge, "wrap_mask": wrap}
+            if entry is not None:
+                entry.update({"name": "_CHAINPHASE_RXLAYER", "theta": cur["theta"], "bonds": list(cur["bonds"]),
+                              "rx_theta": nxt["theta"]})
+                out.append(entry)
+                i += 2
+                continue
+        out.append(cur)
+        i += 1
+    return out
 
 def _pair_ok(n: int, o) -> bool:
     w = o.get("wires", [])

@@ -87,6 +87,14 @@ def apply_fused(state, n, fused):
             pname = "XXPHASE" if name == "_XXLAYER" else "YYPHASE"
             for a, b in op["bonds"]:
                 state = O.apply_dense(state, n, O.pauli_pair_phase(pname, op["theta"]), [a, b])
+        elif name == "_CHAINPHASE_RXLAYER":
+            th = [op["theta"]] * len(op["bonds"])
+            diag = (O.zz_layer_diag(n, op["bonds"], th) if op["phase_kind"] == "zz"
+                    else O.cphase_layer_diag(n, op["bonds"], th))
+            state = state * diag
+            u = G.single_qubit_matrix("RX", [op["rx_theta"]])
+            for q in range(n):
+                state = O.apply_dense(state, n, u, [q])
         elif name == "_CULISTLAYER":
             for group in op["groups"]:
                 for g in group:

@@ -128,3 +128,31 @@ def test_common_helpers_bell_values(gpu):
     assert np.allclose(statevector_numpy(dev2), [0, 1, 0, 0], atol=1e-6)
     with pytest.raises(ValueError):
         core_operation("NOPE", [0])
+
+
+@pytest.mark.parametrize("phase_name", ["CPHASE", "ZZPHASE"])
+@pytest.mark.parametrize("n", [8, 20])
+def test_chain_phase_rx_amortization_matches_unfused(gpu, monkeypatch, phase_name, n):
+    """The reference's chain-phase + RX amortization case
+    (mlxQMetalKernelsTest.py:153-191): fused and unfused runs agree within
+    5e-6 and with the oracle; the plan reports one amortized layer and one
+    sweep fewer."""
+    from mettleq_amd.device import Device
+    bonds = [(q, q + 1) for q in range(n - 1)] + ([(n - 1, 0)] if phase_name == "CPHASE" else [])
+    ops = ([{"name": phase_name, "wires": list(b), "parameters": [0.23]} for b in bonds]
+           + [{"name": "RX", "wires": [w], "parameters": [-0.31]} for w in range(n)])
+    outs, plans = [], []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("METTLEQ_CHAINPHASE_RX_FUSION", flag)
+        dev = Device(n)
+        try:
+            dev.execute(ops, report=True)
+            dev.synchronize()
+            outs.append(dev.native.to_numpy())
+            plans.append(dev.last_execution_plan)
+        finally:
+            dev.close()
+    assert np.max(np.abs(outs[0] - outs[1])) < 5e-6
+    assert np.max(np.abs(outs[1] - O.run_ops(n, ops))) < TOL
+    assert plans[1]["pattern_counts"] == {"chainphase_rxlayer": 1}
+    assert plans[1]["expected_custom_kernel_launches"] == plans[0]["expected_custom_kernel_launches"] - 1

@@ -191,3 +191,46 @@ def test_layers_compose_inverse_affine_matches_oracle():
             a, b = (int(x) for x in rng.choice(n, 2, replace=False))
             ops.append({"name": ["CNOT", "SWAP"][k - 1], "wires": [a, b]})
     assert compose_inverse_affine(ops, n) == O.compose_inverse_affine(ops, n)
+
+
+def _chain_phase_rx_ops(n, phase_name):
+    # the reference's case (mlxQMetalKernelsTest.py:153-191): a uniform
+    # CPHASE ring or ZZ chain, then RX on every wire
+    bonds = [(q, q + 1) for q in range(n - 1)]
+    if phase_name == "CPHASE":
+        bonds.append((n - 1, 0))
+    return ([{"name": phase_name, "wires": list(b), "parameters": [0.23]} for b in bonds]
+            + [{"name": "RX", "wires": [w], "parameters": [-0.31]} for w in range(n)])
+
+
+@pytest.mark.parametrize("phase_name", ["CPHASE", "ZZPHASE"])
+def test_chain_phase_rx_amortized_layer(monkeypatch, phase_name):
+    """A diagonal chain/ring layer followed by an RX layer fuses into one
+    _CHAINPHASE_RXLAYER (device.py:1029-1087), with the reference's masks;
+    METTLEQ_CHAINPHASE_RX_FUSION=0 keeps the two layers. Both match the
+    gate-by-gate oracle."""
+    n = 8
+    ops = _chain_phase_rx_ops(n, phase_name)
+    monkeypatch.setenv("METTLEQ_CHAINPHASE_RX_FUSION", "1")
+    fused = _check(n, ops)
+    assert _names(fused) == ["_CHAINPHASE_RXLAYER"]
+    f = fused[0]
+    assert f["rx_theta"] == -0.31
+    if phase_name == "ZZPHASE":
+        assert (f["phase_kind"], f["n_bonds"], f["edge_mask"], f["wrap_mask"]) == ("zz", 7, 127, 0)
+    else:
+        assert (f["phase_kind"], f["n_bonds"], f["edge_mask"], f["wrap_mask"]) == ("cphase", 8, 127, 129)
+    monkeypatch.setenv("METTLEQ_CHAINPHASE_RX_FUSION", "0")
+    unfused = _check(n, ops)
+    assert _names(unfused) == ["_ZZLAYER" if phase_name == "ZZPHASE" else "_DIAGLAYER", "_RXLAYER"]
+    from mettleq_amd.execution import layer_sweeps
+    assert layer_sweeps(f, n) == 1
+    assert sum(layer_sweeps(u, n) for u in unfused) == 2
+
+
+def test_chain_phase_rx_not_amortized_for_other_bonds():
+    n = 6
+    ops = ([{"name": "CZ", "wires": [0, 2]}, {"name": "CZ", "wires": [3, 4]}]
+           + [{"name": "RX", "wires": [w], "parameters": [0.4]} for w in range(n)])
+    fused = _check(n, ops)
+    assert "_CHAINPHASE_RXLAYER" not in _names(fused)
