@@ -1086,20 +1086,35 @@ def _layer_cycle(s, want, n, rng, layers, ring, perm):
     return want
 
 
+def _sparse_affine(n, rng, count=4):
+    # a few CNOTs and an X among the wires of index bits >= 4: the map keeps
+    # the column bits and moves few bits, so a crossing tile exists that is
+    # not the CNOT chain's
+    ops = []
+    for _ in range(count):
+        a, b = rng.choice(n - 4, 2, replace=False)
+        ops.append({"name": "CNOT", "wires": [int(a), int(b)]})
+    ops.append({"name": "X", "wires": [int(rng.integers(n - 4))]})
+    return O.compose_inverse_affine(ops, n)
+
+
 @pytest.mark.parametrize("n", [16, 17, 20])
-@pytest.mark.parametrize("perm", ["chain", "random", "none"])
+@pytest.mark.parametrize("perm", ["chain", "random", "sparse", "none"])
 @pytest.mark.parametrize("ring", [False, True])
 def test_pending_tail_crosses_into_the_next_layer(gpu, n, perm, ring):
     """With 9-bit tiles a complex64 layer takes three or four sweeps; its last
     sweep's rotations stay pending and the next layer's first sweep runs
     them, the deferred ring and permutation, and its own first rotations,
-    changing frame inside the tile (k_rot_relabel). The chain keeps the
-    column bits, so every layer after the first crosses; a random map does
-    not (its gather runs after the pending rotations). Four layers against
-    the oracle, with pending tails on and off; fewer sweeps when on."""
+    changing frame inside the tile (k_rot_relabel). The chain and a sparse
+    map on the high bits keep the column bits, so every layer after the
+    first crosses (the sparse map's tile has extra basis vectors other than
+    the chain's); a random map does not (its gather runs after the pending
+    rotations). Four layers against the oracle, with pending tails on and
+    off; fewer sweeps when on."""
     rng0 = np.random.default_rng(8100 + n)
     psi = _rand_state(n, 81 * n)
-    p = {"chain": _chain(n), "random": _random_affine(n, rng0), "none": None}[perm]
+    p = {"chain": _chain(n), "random": _random_affine(n, rng0), "sparse": _sparse_affine(n, rng0),
+         "none": None}[perm]
     outs, sweeps = [], []
     # without a permutation the crossing is taken only on request (2); by
     # default (1) the pending rotations run first, in their own sweep
@@ -1116,7 +1131,8 @@ def test_pending_tail_crosses_into_the_next_layer(gpu, n, perm, ring):
         outs.append(got)
     for o in outs[1:]:
         assert np.max(np.abs(outs[0] - o)) < 2e-6
-    if perm != "random":
+    assert sweeps[1] <= sweeps[0]
+    if perm in ("chain", "sparse", "none"):
         assert sweeps[1] < sweeps[0]
     if perm == "none":
         assert sweeps[2] == sweeps[0]
