@@ -302,8 +302,9 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
 #define MSV_TUNE_DEFER_TAIL 16        /* 1q layers of several sweeps leave their last
                                          sweep's rotations pending; the next 1q layer
                                          runs them in its first sweep, across a pending
-                                         XOR permutation (0 off, 1 on, 2 also when no
-                                         permutation is pending) */
+                                         XOR permutation (0 off; 1 on, and without a
+                                         pending permutation for states up to 64 GiB;
+                                         2 on, also above) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

@@ -67,7 +67,7 @@ struct Tuning {
   int fused_blocks_per_cu = 32;  // grid of a first sweep that reads through deferred work (tables staged per block)
   int reorder_gates = 1;    // gate lists: commuting gates are regrouped into fewer tile programs
   int defer_tail = 1;       // 1q layers leave their last sweep's rotations to the next layer's first sweep
-                            // (1: it crosses a pending permutation only, 2: also without one)
+                            // (1: without a pending permutation only up to 64 GiB states, 2: always)
 };
 Tuning g_tune;
 std::mutex g_tune_mu;
@@ -1530,13 +1530,14 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
   if (std_sets.empty()) return flush_pending_T<T>(s);
   // A pending tail: cross the deferred permutation in one sweep when that
   // launches fewer sweeps than running the tail on its own first. Without a
-  // pending permutation the first sweep would read a lone diagonal in place
-  // (or nothing), and the crossing sweep's extra rotations cost about what
-  // it saves (34-qubit brick layers measured slower), so by default it
-  // crosses only a permutation.
+  // pending permutation (a lone diagonal or nothing), crossing measured
+  // faster for states up to 64 GiB (brick layers at 30 and 32 qubits: 5%)
+  // and slower for the 128 GiB complex64 state (34 qubits: 9%), so by
+  // default it is taken there only up to 64 GiB.
   if (s->tail_on) {
     bool crossed = false;
-    const bool try_cross = s->perm_on || g_tune.defer_tail == 2;
+    const uint64_t state_bytes = s->amps * static_cast<uint64_t>(sizeof(typename Vec2<T>::type));
+    const bool try_cross = s->perm_on || g_tune.defer_tail == 2 || state_bytes <= (1ull << 36);
     const uint64_t vmask = try_cross ? choose_tile_bits(n, NL, cb, s->tail_bits, pred, act_bit) : 0;
     CrossTile t;
     std::vector<uint64_t> img;

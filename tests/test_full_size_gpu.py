@@ -91,10 +91,10 @@ def test_layer_cycle_then_inverse_returns_the_state_30q(gpu):
         f = _fidelity(ref, st)
         assert abs(st.norm2() - 1.0) < 1e-5
     # RZZ rings and CNOT chains ride in the next sweep; each U3 layer leaves
-    # its last sweep's rotations pending: the first U3 layer (no permutation
-    # before it) runs them first (one sweep + two), the second crosses the
-    # chain (two sweeps), the QFT runs them first (one sweep + three)
-    assert forward_sweeps == (1 + 2) + 2 + (1 + 3) + 2
+    # its last sweep's rotations pending and the next U3 layer runs them in
+    # its first sweep (two sweeps per layer), the QFT runs them first (one
+    # sweep + three)
+    assert forward_sweeps == 2 + 2 + (1 + 3) + 2
     assert 1.0 - f < 1e-6
 
 
