@@ -460,7 +460,7 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
-              "engine's multi-GPU sharding is a design (see DESIGN.md), not a measured path. "
+              "engine's multi-GPU sharding is designed, not a measured path. "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from mettleq_amd import _native as nat
