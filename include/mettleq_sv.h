@@ -80,7 +80,56 @@ typedef struct msv_device_info {
 const char *msv_last_error(void);
 int msv_version(void);
 int msv_device_count(int *out);
+/* max_alloc_bytes is the runtime's largest single allocation (ALLOC_MAX_SIZE
+ * of the device's coarse-grained HSA memory pool), 0 when not reported. */
 int msv_get_device_info(int device, msv_device_info *out);
+
+/* ---- statevector preflight (reference execution.py:186-407) ------------- */
+/* Lower bound of the check: two states (the reference's out-of-place model,
+ * execution.py:195-223; circuits with XOR permutations) or one state plus
+ * the library's fixed 8 MiB of scratch (in-place layer kernels). */
+#define MSV_PF_OUT_OF_PLACE 0
+#define MSV_PF_IN_PLACE 1
+/* Decisions (the reference's decision strings, execution.py:316-331). */
+#define MSV_PF_ALLOWED 0            /* "allowed_within_reported_lower_bounds"  */
+#define MSV_PF_ALLOWED_UNVERIFIED 1 /* "allowed_with_unverified_device_limits" */
+#define MSV_PF_ALLOWED_OVERRIDE 2   /* "allowed_with_unsafe_override"          */
+#define MSV_PF_REFUSED 3            /* "refused"                               */
+/* Failure bits (failure_reasons, execution.py:306-311). */
+#define MSV_PF_FAIL_MAX_BUFFER 1    /* "single_state_exceeds_max_buffer_length" */
+#define MSV_PF_FAIL_WORKING_SET 2   /* "two_state_lower_bound_exceeds_recommended_working_set"
+                                       or "in_place_lower_bound_exceeds_recommended_working_set" */
+#define MSV_PF_FAIL_QUBITS 4        /* "qubit_count_exceeds_engine_limit" (above 40)  */
+typedef struct msv_preflight_limits {
+  uint64_t max_buffer_bytes;   /* largest single allocation; 0 = unknown            */
+  uint64_t working_set_bytes;  /* memory available for new states; 0 = unknown     */
+} msv_preflight_limits;
+typedef struct msv_preflight_report {
+  int32_t decision;            /* MSV_PF_ALLOWED ... MSV_PF_REFUSED                 */
+  int32_t failures;            /* MSV_PF_FAIL_* bits                                */
+  int32_t allowed;             /* 1 when the state may be allocated                 */
+  int32_t overridden;          /* 1 when allowed only by allow_unsafe               */
+  int32_t buffer_check;        /* state <= max buffer: 1 yes, 0 no, -1 unknown      */
+  int32_t working_check;       /* lower bound <= working set: 1, 0, -1              */
+  uint64_t state_bytes;        /* 2^n x 8 (complex64) or x 16 (complex128)          */
+  uint64_t lower_bound_bytes;  /* per `mode`                                        */
+  uint64_t max_buffer_bytes;   /* the limit checked (0 = unknown)                   */
+  uint64_t working_set_bytes;  /* working set minus reserved_bytes (0 = unknown)    */
+} msv_preflight_report;
+/* Checks an n-qubit state of `dtype` before allocation, like the reference's
+ * statevector_preflight (execution.py:270-374). `limits` == NULL reads the
+ * live limits of `device`: its largest single allocation and its free memory
+ * now, which already excludes every state allocated so far, so
+ * `reserved_bytes` is only memory admitted but not yet allocated (the
+ * serving loop's admissions in flight). allow_unsafe != 0 is the reference's
+ * override (METTLEQ_ALLOW_UNSAFE_STATEVECTOR): it admits a refusal, except a
+ * qubit count above the engine limit. Returns MSV_OK when allowed and
+ * MSV_E_REFUSED when refused; `out` is filled in both cases and a refusal's
+ * message (msv_last_error) says "before allocation" and names the override
+ * variable, as the reference's StatevectorMemoryError does
+ * (execution.py:377-407). */
+int msv_preflight(int n_qubits, int dtype, int mode, uint64_t reserved_bytes, int allow_unsafe,
+                  int device, const msv_preflight_limits *limits, msv_preflight_report *out);
 
 /* ---- state lifecycle (reference sim.py:17-45) -------------------------- */
 /* Allocates 2^n amplitudes on `device` and sets |0...0>. */

@@ -60,6 +60,16 @@ MSV_PENDING_DIAG = 2
 MSV_PENDING_PERM = 4
 MSV_PENDING_TAIL = 8
 
+MSV_PF_OUT_OF_PLACE = 0
+MSV_PF_IN_PLACE = 1
+MSV_PF_ALLOWED = 0
+MSV_PF_ALLOWED_UNVERIFIED = 1
+MSV_PF_ALLOWED_OVERRIDE = 2
+MSV_PF_REFUSED = 3
+MSV_PF_FAIL_MAX_BUFFER = 1
+MSV_PF_FAIL_WORKING_SET = 2
+MSV_PF_FAIL_QUBITS = 4
+
 
 class NativeUnavailableError(RuntimeError):
     """The HIP library is not built or cannot be loaded."""
@@ -90,6 +100,28 @@ class msv_device_info(ctypes.Structure):
     ]
 
 
+class msv_preflight_limits(ctypes.Structure):
+    _fields_ = [
+        ("max_buffer_bytes", ctypes.c_uint64),
+        ("working_set_bytes", ctypes.c_uint64),
+    ]
+
+
+class msv_preflight_report(ctypes.Structure):
+    _fields_ = [
+        ("decision", ctypes.c_int32),
+        ("failures", ctypes.c_int32),
+        ("allowed", ctypes.c_int32),
+        ("overridden", ctypes.c_int32),
+        ("buffer_check", ctypes.c_int32),
+        ("working_check", ctypes.c_int32),
+        ("state_bytes", ctypes.c_uint64),
+        ("lower_bound_bytes", ctypes.c_uint64),
+        ("max_buffer_bytes", ctypes.c_uint64),
+        ("working_set_bytes", ctypes.c_uint64),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _I32P = ctypes.POINTER(ctypes.c_int32)
@@ -105,6 +137,8 @@ SIGNATURES = {
     "msv_version": (_I, []),
     "msv_device_count": (_I, [ctypes.POINTER(_I)]),
     "msv_get_device_info": (_I, [_I, ctypes.POINTER(msv_device_info)]),
+    "msv_preflight": (_I, [_I, _I, _I, _U64, _I, _I, ctypes.POINTER(msv_preflight_limits),
+                           ctypes.POINTER(msv_preflight_report)]),
     "msv_state_create": (_I, [_I, _I, _I, ctypes.POINTER(_P)]),
     "msv_state_destroy": (_I, [_P]),
     "msv_state_reset": (_I, [_P]),
@@ -223,6 +257,25 @@ def device_info(device: int = 0) -> dict:
         "name": info.name.decode("utf-8", "replace"),
         "arch": info.arch.decode("utf-8", "replace"),
     }
+
+
+def preflight(n_qubits: int, dtype: int, mode: int, reserved_bytes: int, allow_unsafe: bool,
+              limits: tuple | None, device: int = 0) -> dict:
+    """msv_preflight: `limits` = (max_buffer_bytes, working_set_bytes) with 0
+    or None for unknown, or None for the device's live limits. Returns the
+    report fields and, when refused, the library's message."""
+    rep = msv_preflight_report()
+    lim_p = None
+    if limits is not None:
+        lim = msv_preflight_limits(int(limits[0] or 0), int(limits[1] or 0))
+        lim_p = ctypes.byref(lim)
+    code = lib().msv_preflight(int(n_qubits), int(dtype), int(mode), int(reserved_bytes),
+                               1 if allow_unsafe else 0, int(device), lim_p, ctypes.byref(rep))
+    if code not in (MSV_OK, MSV_E_REFUSED):
+        check(code, "preflight")
+    out = {name: int(getattr(rep, name)) for name, _ in msv_preflight_report._fields_}
+    out["message"] = last_error() if code == MSV_E_REFUSED else None
+    return out
 
 
 def as_ptr(array: np.ndarray, ctype):

@@ -301,5 +301,6 @@ constexpr uint32_t kReduceThreads = 256;
 // Host helpers of the runtime shared with the batched ansatz engine.
 int set_error(int code, const std::string &msg);  // records the thread's last error, returns code
 int device_cu_count(int device);
+uint64_t device_max_alloc(int device);  // largest single allocation (HSA pool), 0 if unknown
 
 }  // namespace msv

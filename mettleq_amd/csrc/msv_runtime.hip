@@ -2862,7 +2862,7 @@ int msv_get_device_info(int device, msv_device_info *out) {
   MSV_HIP(hipMemGetInfo(&fr, &tot));
   out->total_bytes = tot;
   out->free_bytes = fr;
-  out->max_alloc_bytes = tot;
+  out->max_alloc_bytes = msv::device_max_alloc(device);
   out->compute_units = prop.multiProcessorCount;
   out->lds_bytes_per_block = static_cast<int32_t>(prop.sharedMemPerBlock);
   std::snprintf(out->name, sizeof(out->name), "%s", prop.name);

@@ -52,10 +52,11 @@ def _device_info() -> Dict[str, Any]:
         info = _native.device_info(0)
     except Exception:
         return {}
+    # free memory now: it already excludes the states this process holds
     return {
         "device_name": info["name"],
         "architecture": info["arch"],
-        "max_buffer_length": info["max_alloc_bytes"],
+        "max_buffer_length": info["max_alloc_bytes"] or None,
         "max_recommended_working_set_size": info["free_bytes"],
         "total_bytes": info["total_bytes"],
         "compute_units": info["compute_units"],
@@ -117,6 +118,17 @@ def statevector_unsafe_override_policy(allow_unsafe: Optional[bool] = None) -> D
             "environment_variable": STATEVECTOR_UNSAFE_OVERRIDE_ENV, "environment_value": raw}
 
 
+_DECISIONS = {
+    0: ("allowed_within_reported_lower_bounds",
+        "statevector lower bounds fit within the reported device limits"),
+    1: ("allowed_with_unverified_device_limits",
+        "one or more device memory limits are unavailable"),
+    2: ("allowed_with_unsafe_override",
+        "reported device limits reject the allocation, but the explicit unsafe override permits it"),
+    3: ("refused", "reported device limits are below the statevector lower bound"),
+}
+
+
 def statevector_preflight(
     n_qubits: int,
     *,
@@ -127,11 +139,19 @@ def statevector_preflight(
 ) -> Dict[str, Any]:
     """One-sided check before allocating a statevector (execution.py:270-374).
 
+    The decision is the library's ``msv_preflight`` (one implementation for
+    every binder of the C ABI); this function supplies the limits and shapes
+    the reference's report dict around it.
+
     ``mode`` selects the lower bound: "out_of_place" (two states, the
-    reference model) or "in_place" (one state plus fixed scratch).
-    ``reserved_bytes`` is memory already promised to other resident states
-    (the serving loop's admission control).
+    reference model) or "in_place" (one state plus fixed scratch). The
+    working set is the device's free memory now, which already excludes
+    every state allocated so far; ``reserved_bytes`` is memory admitted but
+    not yet allocated (the serving loop's admissions in flight), so no byte
+    is counted twice.
     """
+    from . import _native
+
     n = int(n_qubits)
     if n < 0:
         raise ValueError("n_qubits must be non-negative")
@@ -141,49 +161,35 @@ def statevector_preflight(
     override = statevector_unsafe_override_policy(allow_unsafe)
     info = _device_info()
     max_buffer = _limit(info.get("max_buffer_length"))
-    working_set = _limit(info.get("max_recommended_working_set_size"))
-    if working_set is not None:
-        working_set = max(0, working_set - int(reserved_bytes))
-    state_bytes = memory["state_bytes"]
-    lower = (memory["minimum_input_plus_output_bytes"] if mode == "out_of_place"
-             else memory["minimum_in_place_bytes"])
-    buffer_check = state_bytes <= max_buffer if max_buffer is not None else None
-    working_check = lower <= working_set if working_set is not None else None
-    index_check = n <= QUBIT_LIMIT
+    working_total = _limit(info.get("max_recommended_working_set_size"))
+    rep = _native.preflight(
+        n, _native.MSV_C64 if dtype == "complex64" else _native.MSV_C128,
+        _native.MSV_PF_OUT_OF_PLACE if mode == "out_of_place" else _native.MSV_PF_IN_PLACE,
+        int(reserved_bytes), bool(override["enabled"]), (max_buffer, working_total))
+    working_set = rep["working_set_bytes"] if working_total is not None else None
+    lower = rep["lower_bound_bytes"]
     failures: List[str] = []
-    if buffer_check is False:
+    if rep["failures"] & _native.MSV_PF_FAIL_MAX_BUFFER:
         failures.append("single_state_exceeds_max_buffer_length")
-    if working_check is False:
+    if rep["failures"] & _native.MSV_PF_FAIL_WORKING_SET:
         failures.append("two_state_lower_bound_exceeds_recommended_working_set"
                         if mode == "out_of_place"
                         else "in_place_lower_bound_exceeds_recommended_working_set")
-    if not index_check:
+    if rep["failures"] & _native.MSV_PF_FAIL_QUBITS:
         failures.append("qubit_count_exceeds_engine_limit")
-    refused = bool(failures)
-    overridden = refused and bool(override["enabled"]) and index_check
-    allowed = (not refused) or overridden
-    available = buffer_check is not None and working_check is not None
-    if overridden:
-        decision = "allowed_with_unsafe_override"
-        reason = ("reported device limits reject the allocation, but the explicit "
-                  "unsafe override permits it")
-    elif refused:
-        decision = "refused"
-        reason = "reported device limits are below the statevector lower bound"
-    elif available:
-        decision = "allowed_within_reported_lower_bounds"
-        reason = "statevector lower bounds fit within the reported device limits"
-    else:
-        decision = "allowed_with_unverified_device_limits"
-        reason = "one or more device memory limits are unavailable"
+    decision, reason = _DECISIONS[rep["decision"]]
+
+    def _check(v):
+        return None if v < 0 else bool(v)
+
     second_key = ("two_state_lower_bound_within_recommended_working_set" if mode == "out_of_place"
                   else "in_place_lower_bound_within_recommended_working_set")
     return {
         "schema_version": 1,
-        "allowed": allowed,
+        "allowed": bool(rep["allowed"]),
         "decision": decision,
         "reason": reason,
-        "overridden": overridden,
+        "overridden": bool(rep["overridden"]),
         "override": override,
         "failure_reasons": failures,
         "qubits": n,
@@ -194,11 +200,12 @@ def statevector_preflight(
             "architecture": info.get("architecture"),
             "max_buffer_length": max_buffer,
             "max_recommended_working_set_size": working_set,
+            "free_bytes_reported": working_total,
             "reserved_bytes": int(reserved_bytes),
         },
         "checks": {
-            "single_state_within_max_buffer_length": buffer_check,
-            second_key: working_check,
+            "single_state_within_max_buffer_length": _check(rep["buffer_check"]),
+            second_key: _check(rep["working_check"]),
         },
         "cost_model": {
             **memory,
@@ -207,7 +214,7 @@ def statevector_preflight(
                                                        if working_set is not None else None),
             "scope": "allocation preflight lower bound",
             "not_included": ["full probability vectors requested by the caller",
-                             "memory used by other processes"],
+                             "memory other processes allocate after the check"],
         },
         "passing_guarantee": (
             "Passing does not guarantee that an arbitrary circuit fits; it only "

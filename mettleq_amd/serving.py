@@ -6,8 +6,8 @@ The reference executes one circuit per call on the caller's thread
 per `run`). Here a server keeps a pool of device states per width
 (allocated once and kept resident: freeing device memory synchronizes the
 whole device), admits each request by the complex64 statevector preflight
-with the pool's resident bytes as `reserved_bytes`
-(execution.statevector_preflight), and runs up to `workers` requests at a
+(execution.statevector_preflight: the device's free memory now, minus the
+bytes admitted by other workers whose states are not allocated yet), and runs up to `workers` requests at a
 time, each on its own state and HIP stream, so that their kernels overlap
 on the GPU. Requests that arrive while others run are taken as soon as a
 worker and a state of their width are free (continuous batching, not a
@@ -132,13 +132,16 @@ def poisson_stream(n_requests: int, rate: float, seed: int = BASE_SEED, widths=(
 
 class StatePool:
     """Device states kept resident per width. `acquire(n)` returns a free
-    state of width n, allocating one when the preflight admits it on top of
-    the pool's resident bytes, or blocks until one is released."""
+    state of width n, allocating one when the preflight admits it, or blocks
+    until one is released. The preflight's working set is the device's free
+    memory, which already excludes the resident states, so only admissions
+    whose allocation is still in flight are passed as reserved bytes."""
 
     def __init__(self, budget_bytes: Optional[int] = None, factory: Optional[Callable[[int], object]] = None,
                  preflight: Optional[Callable[[int, int], dict]] = None):
         self._free: Dict[int, List[object]] = {}
         self._resident = 0
+        self._inflight = 0  # admitted, allocation not finished
         self._cv = threading.Condition()
         self.budget = budget_bytes
         self._factory = factory or _native_state
@@ -160,9 +163,10 @@ class StatePool:
                 if free:
                     return free.pop()
                 within_budget = self.budget is None or self._resident + need <= self.budget
-                report = self._preflight(n, self._resident) if within_budget else {"allowed": False}
+                report = self._preflight(n, self._inflight) if within_budget else {"allowed": False}
                 if report.get("allowed"):
                     self._resident += need
+                    self._inflight += need
                     self.peak_resident = max(self.peak_resident, self._resident)
                     self.allocations += 1
                     break
@@ -185,12 +189,16 @@ class StatePool:
                     raise TimeoutError(f"no {n}-qubit state became free")
                 self._cv.wait(remaining)
         try:
-            return self._factory(n)
+            st = self._factory(n)
         except Exception:
             with self._cv:
                 self._resident -= need
+                self._inflight -= need
                 self._cv.notify_all()
             raise
+        with self._cv:
+            self._inflight -= need
+        return st
 
     def release(self, state) -> None:
         with self._cv:
