@@ -431,9 +431,11 @@ def bench_serve(args, rates=None, n_requests=None, quick=False):
 def extra_summaries(args):
     """Short runs of the other configurations for the default bench line."""
     res = {}
-    for name, fn in (("brick20", lambda: bench_brick20(args, quick=True)),
+    for name, fn in (("memlimit", lambda: bench_memlimit(args)),
+                     ("brick20", lambda: bench_brick20(args, quick=True)),
                      ("vqe16_adjoint", lambda: bench_vqe16(args, quick=True)),
-                     ("serve_1000_per_s", lambda: bench_serve(args, rates=[1000.0], n_requests=400, quick=True))):
+                     ("serve_sweep", lambda: bench_serve(args, rates=[50.0, 200.0, 1000.0, 2000.0],
+                                                         n_requests=400, quick=True))):
         try:
             t0 = time.perf_counter()
             res[name] = fn()

@@ -69,8 +69,33 @@ struct Tuning {
   int defer_tail = 1;       // 1q layers leave their last sweep's rotations to the next layer's first sweep
                             // (1: without a pending permutation only up to 64 GiB states, 2: always)
 };
-Tuning g_tune;
+Tuning g_tune_store;  // written by msv_set_tuning under g_tune_mu
 std::mutex g_tune_mu;
+thread_local Tuning t_tune;
+thread_local int t_tune_depth = 0;
+
+// One snapshot of the tuning knobs per entry point: every planner of a call
+// reads the same values even when another thread sets a knob meanwhile
+// (msv_set_tuning takes effect from the next call on).
+struct TuneScope {
+  TuneScope() {
+    if (t_tune_depth++ == 0) {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      t_tune = g_tune_store;
+    }
+  }
+  ~TuneScope() { --t_tune_depth; }
+  TuneScope(const TuneScope &) = delete;
+  TuneScope &operator=(const TuneScope &) = delete;
+};
+
+const Tuning &tune() {
+  if (t_tune_depth == 0) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    t_tune = g_tune_store;
+  }
+  return t_tune;
+}
 
 struct DevProps {
   int cu = 0;
@@ -235,13 +260,13 @@ int ensure_alt(msv_state *s) {
 uint32_t stream_grid(const msv_state *s) {
   const uint64_t packs = s->amps * (s->dtype == MSV_C128 ? 16 : 8) / 16;
   const uint64_t want = (packs + kStreamChunkPacks - 1) / kStreamChunkPacks;
-  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.elem_blocks_per_cu;
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().elem_blocks_per_cu;
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
 }
 
 uint32_t elem_grid(const msv_state *s, uint64_t work) {
   const uint64_t want = (work + 255) / 256;
-  const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.elem_blocks_per_cu;
+  const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().elem_blocks_per_cu;
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
 }
 
@@ -287,12 +312,12 @@ struct SweepPlan {
 };
 
 template <typename T> int rb_of() {
-  return std::is_same<T, float>::value ? g_tune.reg_bits_c64 : TileCfg<T>::kRB;
+  return std::is_same<T, float>::value ? tune().reg_bits_c64 : TileCfg<T>::kRB;
 }
 template <typename T> int local_max() {
-  return std::is_same<T, float>::value ? g_tune.local_bits_c64 : g_tune.local_bits_c128;
+  return std::is_same<T, float>::value ? tune().local_bits_c64 : tune().local_bits_c128;
 }
-template <typename T> int col_bits() { return std::is_same<T, float>::value ? g_tune.col_bits_c64 : 3; }
+template <typename T> int col_bits() { return std::is_same<T, float>::value ? tune().col_bits_c64 : 3; }
 
 // Partition of index bits [0, n) into a low sweep [0, L) and high chunks of
 // between rb and gate_bits bits each (ascending).
@@ -300,7 +325,7 @@ template <typename T>
 void partition(int n, int *low_len, std::vector<std::pair<int, int>> *chunks) {
   const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
   const int RB = rb_of<T>();
-  const int G = std::max(RB, std::min(g_tune.gate_bits, NL - col_bits<T>()));
+  const int G = std::max(RB, std::min(tune().gate_bits, NL - col_bits<T>()));
   chunks->clear();
   if (n <= NL) {
     *low_len = n;
@@ -356,7 +381,7 @@ std::vector<int> sweep_local_bits(int n, int low_len, bool is_low, std::pair<int
 template <typename T>
 std::vector<std::vector<int>> high_gate_sets(const std::vector<std::pair<int, int>> &chunks) {
   std::vector<std::vector<int>> sets;
-  if (g_tune.sweep_split != 1 || chunks.size() < 2) {
+  if (tune().sweep_split != 1 || chunks.size() < 2) {
     for (auto &ch : chunks) {
       std::vector<int> g;
       for (int b = ch.first; b < ch.first + ch.second; ++b) g.push_back(b);
@@ -447,7 +472,7 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     A.nruns = static_cast<uint32_t>(nr);
   }
   A.n_tiles = 1ull << (n - nlocal);
-  A.xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
+  A.xcd_groups = static_cast<uint32_t>(std::max(0, tune().xcd_groups));
   A.scale = static_cast<T>(plan.scale);
   // table pool: QFT twiddles at 0, or the segment diagonals
   bool has_qft = false;
@@ -554,12 +579,12 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     }
     sg.op_end = static_cast<uint32_t>(nops);
   }
-  uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+  uint64_t cap = static_cast<uint64_t>(s->cu) * tune().tile_blocks_per_cu;
   // The prefetching form runs a resident grid (two 512-thread blocks per CU,
   // the LDS limit) and pays one discarded tile load per block, so it is used
   // only when every block gets at least 32 tiles.
   A.prefetch = 0;
-  if (g_tune.tile_prefetch && std::is_same<T, float>::value && rb == 4) {
+  if (tune().tile_prefetch && std::is_same<T, float>::value && rb == 4) {
     const uint64_t resident = static_cast<uint64_t>(s->cu) * 2;
     if (A.n_tiles >= 32 * resident) {
       A.prefetch = 1;
@@ -725,7 +750,7 @@ int fill_tile_geometry(int n, const std::vector<int> &local, const std::vector<i
   A->nlocal = static_cast<uint32_t>(nlocal);
   A->nseg = static_cast<uint32_t>(windows.size());
   A->n_tiles = 1ull << (n - nlocal);
-  A->xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
+  A->xcd_groups = static_cast<uint32_t>(std::max(0, tune().xcd_groups));
   for (size_t si = 0; si < windows.size(); ++si) {
     const int w = windows[si];
     if (w < 0 || w + rb > nlocal) return fail(MSV_E_INVALID, "internal: register window outside the tile");
@@ -980,7 +1005,7 @@ int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<i
     MSV_TRY(prepare_fused_read<T>(s, A, local, rb, &d_ptab, &d_lut));
     const bool oop = s->perm_on;  // a lone diagonal runs in place
     if (oop) MSV_TRY(ensure_alt(s));
-    const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+    const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
     const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
     MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
                           reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
@@ -988,7 +1013,7 @@ int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<i
     s->perm_on = false;
     s->pend_on = false;
   } else {
-    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+    const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
     MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
   }
@@ -1003,7 +1028,10 @@ int flush_tail_T(msv_state *s) {
   if (!s->tail_on) return MSV_OK;
   s->tail_on = false;
   const std::vector<int> local = high_sweep_local_bits<T>(s->tail_bits);
-  return rot_sweep_T<T>(s, local, s->tail_bits, s->tail_sh, nullptr, &s->tail_post, s->tail_cst, false);
+  const int rc =
+      rot_sweep_T<T>(s, local, s->tail_bits, s->tail_sh, nullptr, &s->tail_post, s->tail_cst, false);
+  if (rc != MSV_OK) s->tail_on = true;  // not enqueued: the tail stays pending
+  return rc;
 }
 
 // Inverse of the deferred permutation's linear part: img[g] = the new-frame
@@ -1408,7 +1436,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   }
   const bool oop = s->perm_on;
   if (oop) MSV_TRY(ensure_alt(s));
-  const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+  const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
   const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
   MSV_HIP(launch_rot_relabel<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
                                 reinterpret_cast<const V *>(s->buf), d_lut,
@@ -1522,8 +1550,8 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
   }
   const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
   const int cb = std::max(0, std::min(col_bits<T>(), n));
-  const int G = std::max(rb_of<T>(), std::min(g_tune.gate_bits, NL - col_bits<T>()));
-  const bool defer = g_tune.defer_tail != 0;
+  const int G = std::max(rb_of<T>(), std::min(tune().gate_bits, NL - col_bits<T>()));
+  const bool defer = tune().defer_tail != 0;
   std::vector<uint64_t> pred(n);
   for (int q = 0; q < n; ++q) pred[q] = s->perm_on ? s->perm_col[q] : (1ull << q);
   const std::vector<std::vector<int>> std_sets = layer_gate_sets<T>(n, act_bit);
@@ -1537,7 +1565,7 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
   if (s->tail_on) {
     bool crossed = false;
     const uint64_t state_bytes = s->amps * static_cast<uint64_t>(sizeof(typename Vec2<T>::type));
-    const bool try_cross = s->perm_on || g_tune.defer_tail == 2 || state_bytes <= (1ull << 36);
+    const bool try_cross = s->perm_on || tune().defer_tail == 2 || state_bytes <= (1ull << 36);
     const uint64_t vmask = try_cross ? choose_tile_bits(n, NL, cb, s->tail_bits, pred, act_bit) : 0;
     CrossTile t;
     std::vector<uint64_t> img;
@@ -1669,9 +1697,9 @@ int product_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   const V *lo = reinterpret_cast<const V *>(d);
   const V *mid = lo + (size_t(1) << l1);
   const V *hi = mid + (size_t(1) << l2);
-  s->basis_on = false;
   MSV_HIP(launch_product<T>(reinterpret_cast<V *>(s->buf), s->amps, lo, mid, hi, static_cast<uint32_t>(l1),
                             static_cast<uint32_t>(l2), stream_grid(s), s->stream));
+  s->basis_on = false;  // cleared once enqueued
   return MSV_OK;
 }
 
@@ -1690,7 +1718,7 @@ int apply_1q_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
       ++n_act;
     }
   // rotation-split form when every active gate is a (scaled) unitary
-  if (g_tune.rot_split_min > 0 && n_act >= g_tune.rot_split_min) {
+  if (tune().rot_split_min > 0 && n_act >= tune().rot_split_min) {
     std::vector<RotSplit> zb(n);
     bool ok = true;
     for (int b = 0; ok && b < n; ++b)
@@ -1763,7 +1791,7 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
     MSV_TRY(prepare_fused_read<T>(s, A, plan.local, rb, &d_ptab, &d_lut));
     const bool oop = s->perm_on;  // a lone diagonal runs in place
     if (oop) MSV_TRY(ensure_alt(s));
-    const uint64_t fcap = static_cast<uint64_t>(s->cu) * g_tune.fused_blocks_per_cu;
+    const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
     const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
     MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, inverse, fgrid, s->stream,
                           reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
@@ -1771,7 +1799,7 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
     s->perm_on = false;
     s->pend_on = false;
   } else {
-    const uint64_t cap = static_cast<uint64_t>(s->cu) * g_tune.tile_blocks_per_cu;
+    const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
     MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
   }
@@ -2158,7 +2186,7 @@ int apply_plan_ops_reordered(msv_state *s, const std::vector<PlanOp> &all) {
 
 template <typename T>
 int apply_plan_ops(msv_state *s, const std::vector<PlanOp> &all) {
-  if (g_tune.reorder_gates) return apply_plan_ops_reordered<T>(s, all);
+  if (tune().reorder_gates) return apply_plan_ops_reordered<T>(s, all);
   std::vector<PlanOp> cur;
   std::vector<int> targets;
   for (const PlanOp &po : all) {
@@ -2232,7 +2260,7 @@ int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, s
   using V = typename Vec2<T>::type;
   void *d_lut = nullptr;
   MSV_TRY(stage(s, lut_host, lut_bytes, &d_lut));
-  const uint32_t grid = g_tune.phase_blocks > 0 ? static_cast<uint32_t>(g_tune.phase_blocks)
+  const uint32_t grid = tune().phase_blocks > 0 ? static_cast<uint32_t>(tune().phase_blocks)
                                                 : stream_grid(s);
   MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A, grid,
                           s->stream));
@@ -2242,18 +2270,22 @@ int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, s
 
 template <typename T>
 int flush_pending_T(msv_state *s) {
+  // Each flag is cleared only once its work is enqueued: a failed launch or
+  // staging allocation leaves the work pending, so the error is not lost.
   if (s->basis_on) {
-    s->basis_on = false;
     MSV_TRY(init_basis_T<T>(s, s->basis_k));
+    s->basis_on = false;
   }
   MSV_TRY(flush_tail_T<T>(s));  // the tail acts before the diagonal and the permutation
   if (s->perm_on) {
+    MSV_TRY(launch_gather<T>(s, s->perm_col, s->perm_c));  // applies a pending diagonal too
     s->perm_on = false;
-    return launch_gather<T>(s, s->perm_col, s->perm_c);  // applies a pending diagonal too
+    return MSV_OK;
   }
   if (!s->pend_on) return MSV_OK;
+  MSV_TRY(launch_phase_chunk<T>(s, s->pend_args, s->pend_lut.data(), s->pend_lut.size()));
   s->pend_on = false;
-  return launch_phase_chunk<T>(s, s->pend_args, s->pend_lut.data(), s->pend_lut.size());
+  return MSV_OK;
 }
 
 // Largest deferred lookup table: the absorbing gather keeps it in LDS next
@@ -2345,11 +2377,11 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool de
     }
     A.nwin = nw;
     A.nlow = nlow;
-    A.xcd_groups = static_cast<uint32_t>(std::max(0, g_tune.xcd_groups));
+    A.xcd_groups = static_cast<uint32_t>(std::max(0, tune().xcd_groups));
     A.lut_len = static_cast<uint32_t>(lut.size());
     chunks.emplace_back(A, std::move(lut));
   }
-  if (defer && g_tune.defer_diag && chunks.size() == 1 &&
+  if (defer && tune().defer_diag && chunks.size() == 1 &&
       chunks[0].second.size() * sizeof(V) <= kPendLutMaxBytes) {
     s->pend_on = true;
     s->pend_args = chunks[0].first;
@@ -2402,15 +2434,15 @@ int apply_xor_T(msv_state *s, const uint64_t *rows, uint64_t c) {
     }
     if (rank != n) return fail(MSV_E_INVALID, "xor_affine: matrix is singular over GF(2)");
   }
-  if (g_tune.defer_perm) {
+  if (tune().defer_perm) {
     // A pending permutation is run first (two maps are not composed); a
     // pending diagonal stays and now acts at the source index; a pending
     // basis state is written first.
     if (s->basis_on) MSV_TRY(flush_pending_T<T>(s));
     if (s->perm_on) {
       MSV_TRY(flush_tail_T<T>(s));  // a pending tail acts before the pending permutation
-      s->perm_on = false;
       MSV_TRY(launch_gather<T>(s, s->perm_col, s->perm_c));
+      s->perm_on = false;
     }
     MSV_TRY(ensure_alt(s));  // reserve the second buffer now: no allocation later
     s->perm_on = true;
@@ -2444,13 +2476,13 @@ int launch_gather(msv_state *s, const std::vector<uint64_t> &col, uint64_t c) {
   MSV_TRY(stage(s, tab.data(), tab.size() * sizeof(uint64_t), &d_tab));
   if (s->pend_on) {
     // the deferred phase layer rides along: out[y] = D(src(y)) in[src(y)]
-    s->pend_on = false;
     void *d_lut = nullptr;
     MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d_lut));
     MSV_HIP(launch_xor_gather_phase<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
                                        s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
                                        reinterpret_cast<const V *>(d_lut), s->pend_args,
                                        stream_grid(s), s->stream));
+    s->pend_on = false;  // cleared once enqueued
   } else {
     MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
                                  s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
@@ -2871,6 +2903,7 @@ int msv_get_device_info(int device, msv_device_info *out) {
 }
 
 int msv_state_create(int n_qubits, int dtype, int device, msv_state **out) {
+  TuneScope ts_;
   if (!out) return fail(MSV_E_INVALID, "null output handle");
   *out = nullptr;
   if (n_qubits < 1 || n_qubits > 40)
@@ -2907,6 +2940,7 @@ int msv_state_create(int n_qubits, int dtype, int device, msv_state **out) {
 }
 
 int msv_state_destroy(msv_state *s) {
+  TuneScope ts_;
   if (!s) return MSV_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -2925,6 +2959,7 @@ int msv_state_destroy(msv_state *s) {
 int msv_state_reset(msv_state *s) { return msv_state_set_basis(s, 0); }
 
 int msv_state_set_basis(msv_state *s, uint64_t index) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (index >= s->amps) return fail(MSV_E_INVALID, "basis index out of range");
   MSV_TRY(set_dev(s));
@@ -2937,6 +2972,7 @@ int msv_state_set_basis(msv_state *s, uint64_t index) {
 }
 
 int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *bytes) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (n_qubits) *n_qubits = s->n;
   if (dtype) *dtype = s->dtype;
@@ -2945,6 +2981,7 @@ int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *byte
 }
 
 int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t count) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
   if (offset > s->amps || count > s->amps - offset)
@@ -2959,6 +2996,7 @@ int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t c
 }
 
 int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
   if (offset > s->amps || count > s->amps - offset)
@@ -2973,6 +3011,7 @@ int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count
 }
 
 int msv_state_copy(msv_state *dst, msv_state *src) {
+  TuneScope ts_;
   MSV_TRY(check_state(dst));
   MSV_TRY(check_state(src));
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
@@ -3002,6 +3041,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
 }
 
 int msv_state_scale(msv_state *s, double re, double im) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));
@@ -3016,6 +3056,7 @@ int msv_state_scale(msv_state *s, double re, double im) {
 }
 
 int msv_state_sync(msv_state *s) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));
@@ -3024,6 +3065,7 @@ int msv_state_sync(msv_state *s) {
 }
 
 int msv_state_release_scratch(msv_state *s) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));  // a deferred permutation still needs the buffer
@@ -3034,6 +3076,7 @@ int msv_state_release_scratch(msv_state *s) {
 }
 
 void *msv_state_device_ptr(msv_state *s) {
+  TuneScope ts_;
   if (!s) return nullptr;
   // the caller reads the amplitudes directly: a deferred diagonal goes first
   if (flush_pending(s) != MSV_OK) return nullptr;
@@ -3041,6 +3084,7 @@ void *msv_state_device_ptr(msv_state *s) {
 }
 
 int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!mats) return fail(MSV_E_INVALID, "null matrix array");
   MSV_TRY(set_dev(s));
@@ -3049,6 +3093,7 @@ int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) 
 
 int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bonds,
                           const double *thetas, int n_terms) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (n_terms < 0 || (n_terms > 0 && (!kinds || !bonds || !thetas)))
     return fail(MSV_E_INVALID, "phase_layer: null term arrays");
@@ -3092,6 +3137,7 @@ int msv_apply_phase_layer(msv_state *s, const int32_t *kinds, const int32_t *bon
 
 int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1,
                        const int32_t *q2, const double *d2, int n2) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   std::vector<DiagTerm> terms;
   for (int t = 0; t < n1; ++t) {
@@ -3123,6 +3169,7 @@ int msv_apply_diagonal(msv_state *s, const int32_t *q1, const double *d1, int n1
 }
 
 int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!rows) return fail(MSV_E_INVALID, "null rows");
   MSV_TRY(set_dev(s));
@@ -3130,6 +3177,7 @@ int msv_apply_xor_affine(msv_state *s, const uint64_t *rows, uint64_t c) {
 }
 
 int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!(0 <= first && first < m && m <= s->n))
     return fail(MSV_E_INVALID, "qft: invalid qubit range [" + std::to_string(first) + ", " +
@@ -3145,6 +3193,7 @@ int msv_apply_qft(msv_state *s, int first, int m, int inverse) {
 }
 
 int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!(0 <= j && j < m && m <= s->n))
     return fail(MSV_E_INVALID, "qft_stage: invalid j=" + std::to_string(j) + ", m=" +
@@ -3156,6 +3205,7 @@ int msv_apply_qft_stage(msv_state *s, int j, int m, int inverse) {
 }
 
 int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (n_gates < 0 || (n_gates > 0 && !gates)) return fail(MSV_E_INVALID, "null gate list");
   std::vector<PlanOp> ops;
@@ -3229,6 +3279,7 @@ int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
 
 int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires, const double *params,
                       int n_ops) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   MSV_TRY(flush_pending(s));
   if (n_ops < 0 || (n_ops > 0 && (!codes || !wires || !params)))
@@ -3255,6 +3306,7 @@ int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires, 
 }
 
 int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!out) return fail(MSV_E_INVALID, "null output");
   if (wires) {
@@ -3273,6 +3325,7 @@ int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out) {
 
 int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_t seed,
                uint64_t stream_id, uint64_t *out) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (shots && !out) return fail(MSV_E_INVALID, "null output");
   std::vector<int32_t> all;
@@ -3295,6 +3348,7 @@ int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint6
 
 int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask, int n_words,
                      double *out_re, double *out_im) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (n_words < 0 || (n_words > 0 && (!xmask || !zmask || !out_re || !out_im)))
     return fail(MSV_E_INVALID, "expval_pauli: null arrays");
@@ -3305,6 +3359,7 @@ int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask,
 }
 
 int msv_inner(msv_state *a, msv_state *b, double *re, double *im) {
+  TuneScope ts_;
   MSV_TRY(check_state(a));
   MSV_TRY(check_state(b));
   if (!re || !im) return fail(MSV_E_INVALID, "null output");
@@ -3320,6 +3375,7 @@ int msv_inner(msv_state *a, msv_state *b, double *re, double *im) {
 }
 
 int msv_norm2(msv_state *s, double *out) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!out) return fail(MSV_E_INVALID, "null output");
   MSV_TRY(set_dev(s));
@@ -3339,67 +3395,67 @@ int msv_set_tuning(int key, int64_t value) {
     case MSV_TUNE_TILE_LOCAL_BITS:
       if (value < 6 || value > TileCfg<float>::kMaxLocal)
         return fail(MSV_E_INVALID, "tile local bits out of range");
-      g_tune.local_bits_c64 = static_cast<int>(value);
+      g_tune_store.local_bits_c64 = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_TILE_GATE_BITS:
       if (value < 5 || value > 10) return fail(MSV_E_INVALID, "gate bits out of range");
-      g_tune.gate_bits = static_cast<int>(value);
+      g_tune_store.gate_bits = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_PHASE_BLOCKS:
       if (value < 0 || value > (1 << 20)) return fail(MSV_E_INVALID, "phase blocks out of range");
-      g_tune.phase_blocks = static_cast<int>(value);
+      g_tune_store.phase_blocks = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_SWEEP_SPLIT:
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "sweep split policy out of range");
-      g_tune.sweep_split = static_cast<int>(value);
+      g_tune_store.sweep_split = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_TILE_BLOCKS_PER_CU:
       if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "tile blocks per CU out of range");
-      g_tune.tile_blocks_per_cu = static_cast<int>(value);
+      g_tune_store.tile_blocks_per_cu = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_ELEM_BLOCKS_PER_CU:
       if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "elementwise blocks per CU out of range");
-      g_tune.elem_blocks_per_cu = static_cast<int>(value);
+      g_tune_store.elem_blocks_per_cu = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_COL_BITS:
       if (value < 0 || value > 8) return fail(MSV_E_INVALID, "column bits out of range");
-      g_tune.col_bits_c64 = static_cast<int>(value);
+      g_tune_store.col_bits_c64 = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_ROT_SPLIT_MIN:
       if (value < 0 || value > 64) return fail(MSV_E_INVALID, "rotation split threshold out of range");
-      g_tune.rot_split_min = static_cast<int>(value);
+      g_tune_store.rot_split_min = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_XCD_GROUPS:
       if (value < 0 || value > 64) return fail(MSV_E_INVALID, "XCD groups out of range");
-      g_tune.xcd_groups = static_cast<int>(value);
+      g_tune_store.xcd_groups = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_TILE_REG_BITS:
       if (value != 4 && value != 5) return fail(MSV_E_INVALID, "tile register bits must be 4 or 5");
-      g_tune.reg_bits_c64 = static_cast<int>(value);
+      g_tune_store.reg_bits_c64 = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_TILE_PREFETCH:
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "tile prefetch must be 0 or 1");
-      g_tune.tile_prefetch = static_cast<int>(value);
+      g_tune_store.tile_prefetch = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_DEFER_DIAG:
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer diag must be 0 or 1");
-      g_tune.defer_diag = static_cast<int>(value);
+      g_tune_store.defer_diag = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_DEFER_PERM:
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "defer perm must be 0 or 1");
-      g_tune.defer_perm = static_cast<int>(value);
+      g_tune_store.defer_perm = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_FUSED_BLOCKS_PER_CU:
       if (value < 1 || value > 4096) return fail(MSV_E_INVALID, "fused blocks per CU out of range");
-      g_tune.fused_blocks_per_cu = static_cast<int>(value);
+      g_tune_store.fused_blocks_per_cu = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_REORDER_GATES:
       if (value < 0 || value > 1) return fail(MSV_E_INVALID, "reorder gates must be 0 or 1");
-      g_tune.reorder_gates = static_cast<int>(value);
+      g_tune_store.reorder_gates = static_cast<int>(value);
       return MSV_OK;
     case MSV_TUNE_DEFER_TAIL:
       if (value < 0 || value > 2) return fail(MSV_E_INVALID, "defer tail must be 0, 1 or 2");
-      g_tune.defer_tail = static_cast<int>(value);
+      g_tune_store.defer_tail = static_cast<int>(value);
       return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
@@ -3409,27 +3465,28 @@ int msv_get_tuning(int key, int64_t *value) {
   if (!value) return fail(MSV_E_INVALID, "null output");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   switch (key) {
-    case MSV_TUNE_TILE_LOCAL_BITS: *value = g_tune.local_bits_c64; return MSV_OK;
-    case MSV_TUNE_TILE_GATE_BITS: *value = g_tune.gate_bits; return MSV_OK;
-    case MSV_TUNE_PHASE_BLOCKS: *value = g_tune.phase_blocks; return MSV_OK;
-    case MSV_TUNE_SWEEP_SPLIT: *value = g_tune.sweep_split; return MSV_OK;
-    case MSV_TUNE_TILE_BLOCKS_PER_CU: *value = g_tune.tile_blocks_per_cu; return MSV_OK;
-    case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune.elem_blocks_per_cu; return MSV_OK;
-    case MSV_TUNE_COL_BITS: *value = g_tune.col_bits_c64; return MSV_OK;
-    case MSV_TUNE_ROT_SPLIT_MIN: *value = g_tune.rot_split_min; return MSV_OK;
-    case MSV_TUNE_XCD_GROUPS: *value = g_tune.xcd_groups; return MSV_OK;
-    case MSV_TUNE_TILE_REG_BITS: *value = g_tune.reg_bits_c64; return MSV_OK;
-    case MSV_TUNE_TILE_PREFETCH: *value = g_tune.tile_prefetch; return MSV_OK;
-    case MSV_TUNE_DEFER_DIAG: *value = g_tune.defer_diag; return MSV_OK;
-    case MSV_TUNE_DEFER_PERM: *value = g_tune.defer_perm; return MSV_OK;
-    case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune.fused_blocks_per_cu; return MSV_OK;
-    case MSV_TUNE_REORDER_GATES: *value = g_tune.reorder_gates; return MSV_OK;
-    case MSV_TUNE_DEFER_TAIL: *value = g_tune.defer_tail; return MSV_OK;
+    case MSV_TUNE_TILE_LOCAL_BITS: *value = g_tune_store.local_bits_c64; return MSV_OK;
+    case MSV_TUNE_TILE_GATE_BITS: *value = g_tune_store.gate_bits; return MSV_OK;
+    case MSV_TUNE_PHASE_BLOCKS: *value = g_tune_store.phase_blocks; return MSV_OK;
+    case MSV_TUNE_SWEEP_SPLIT: *value = g_tune_store.sweep_split; return MSV_OK;
+    case MSV_TUNE_TILE_BLOCKS_PER_CU: *value = g_tune_store.tile_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_ELEM_BLOCKS_PER_CU: *value = g_tune_store.elem_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_COL_BITS: *value = g_tune_store.col_bits_c64; return MSV_OK;
+    case MSV_TUNE_ROT_SPLIT_MIN: *value = g_tune_store.rot_split_min; return MSV_OK;
+    case MSV_TUNE_XCD_GROUPS: *value = g_tune_store.xcd_groups; return MSV_OK;
+    case MSV_TUNE_TILE_REG_BITS: *value = g_tune_store.reg_bits_c64; return MSV_OK;
+    case MSV_TUNE_TILE_PREFETCH: *value = g_tune_store.tile_prefetch; return MSV_OK;
+    case MSV_TUNE_DEFER_DIAG: *value = g_tune_store.defer_diag; return MSV_OK;
+    case MSV_TUNE_DEFER_PERM: *value = g_tune_store.defer_perm; return MSV_OK;
+    case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune_store.fused_blocks_per_cu; return MSV_OK;
+    case MSV_TUNE_REORDER_GATES: *value = g_tune_store.reorder_gates; return MSV_OK;
+    case MSV_TUNE_DEFER_TAIL: *value = g_tune_store.defer_tail; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
 
 int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (out) *out = s->sweeps;
   if (reset) s->sweeps = 0;
@@ -3437,6 +3494,7 @@ int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset) {
 }
 
 int msv_state_pending(const msv_state *s, int *out) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!out) return fail(MSV_E_INVALID, "null output");
   *out = (s->basis_on ? MSV_PENDING_BASIS : 0) | (s->pend_on ? MSV_PENDING_DIAG : 0) |
@@ -3445,6 +3503,7 @@ int msv_state_pending(const msv_state *s, int *out) {
 }
 
 int msv_state_timer_start(msv_state *s) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   if (!s->ev0) MSV_HIP(hipEventCreate(&s->ev0));
@@ -3454,6 +3513,7 @@ int msv_state_timer_start(msv_state *s) {
 }
 
 int msv_state_timer_stop(msv_state *s, double *ms) {
+  TuneScope ts_;
   MSV_TRY(check_state(s));
   if (!ms) return fail(MSV_E_INVALID, "null output");
   if (!s->ev0 || !s->ev1) return fail(MSV_E_INVALID, "timer was not started");
@@ -3467,6 +3527,7 @@ int msv_state_timer_stop(msv_state *s, double *ms) {
 }
 
 int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_per_copy) {
+  TuneScope ts_;
   if (!ms_per_copy || iters < 1 || bytes < 16) return fail(MSV_E_INVALID, "probe: bad arguments");
   MSV_HIP(hipSetDevice(device));
   void *a = nullptr, *b = nullptr;
