@@ -275,6 +275,20 @@ int msv_probabilities(msv_state *s, const int32_t *wires, int nw, double *out);
 int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots,
                uint64_t seed, uint64_t stream_id, uint64_t *out);
 
+/* msv_sample for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Same arguments and the same output as
+ * msv_sample, bit for bit, but it calls no hipMalloc/hipFree and uses no
+ * host buffer besides `out`: the block-sum prefix and the per-shot staging
+ * live in a static 8 MiB device array per GPU (held until the process ends),
+ * shots run in chunks that fit behind the prefix. States of at most 31
+ * qubits (MSV_E_REFUSED above). Deferred work of the state is flushed with
+ * the state's existing buffers; if it would need a new one, the call
+ * returns MSV_E_REFUSED (run msv_state_sync once beforehand). Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+int msv_sample_noalloc(msv_state *s, const int32_t *wires, int nw, uint64_t shots,
+                       uint64_t seed, uint64_t stream_id, uint64_t *out);
+
 /* <psi|P_w|psi> for Pauli words P_w = i^{ny_w} X^{xmask_w} Z^{zmask_w}
  * (masks over qubits: bit q = qubit q; ny_w = number of Y factors). Returns
  * real and imaginary parts. Replaces pauli_product_expectations

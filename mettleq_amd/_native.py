@@ -161,6 +161,7 @@ SIGNATURES = {
     "msv_apply_circuit": (_I, [_P, _I32P, _I32P, _DP, _I]),
     "msv_probabilities": (_I, [_P, _I32P, _I, _DP]),
     "msv_sample": (_I, [_P, _I32P, _I, _U64, _U64, _U64, _U64P]),
+    "msv_sample_noalloc": (_I, [_P, _I32P, _I, _U64, _U64, _U64, _U64P]),
     "msv_expval_pauli": (_I, [_P, _U64P, _U64P, _I, _DP, _DP]),
     "msv_inner": (_I, [_P, _P, _DP, _DP]),
     "msv_norm2": (_I, [_P, _DP]),

@@ -295,6 +295,14 @@ hipError_t launch_inner(const typename Vec2<Ta>::type *a, const typename Vec2<Tb
                         uint64_t n_amps, double *partials, uint32_t grid, hipStream_t stream);
 hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uint32_t grid,
                              hipStream_t stream);
+// Sampler steps on the device (msv_sample_noalloc): the in-place sequential
+// prefix of the block sums at p[1..nb] (p[0] = 0), and per shot k of
+// [shot0, shot0 + count) its block and residual.
+hipError_t launch_prefix_seq(double *p, uint64_t nb, hipStream_t stream);
+hipError_t launch_sample_search(const double *prefix, uint64_t nb, double total, uint64_t seed,
+                                uint64_t stream_id, uint64_t shot0, uint64_t count, uint64_t *shot_block,
+                                double *shot_resid, hipStream_t stream);
+
 
 constexpr uint32_t kReduceThreads = 256;
 

@@ -22,6 +22,7 @@
 
 #include "msv_device.h"
 #include "msv_internal.h"
+#include "msv_philox.h"
 
 namespace msv {
 
@@ -1417,6 +1418,69 @@ __global__ void __launch_bounds__(256) k_sample_resolve(const typename Vec2<T>::
   }
 }
 
+// Device side of the sampler's host steps, for callers that hand over no
+// host buffers but the output (msv_sample_noalloc). Both reproduce the host
+// code of msv_sample bit for bit.
+//
+// Prefix of the block sums, in place and strictly left to right (the order
+// of the host's sequential prefix): p[b + 1] holds block sum b on entry and
+// p[0] + ... + (block sum b) on exit, p[0] = 0. One wave: each round loads 64
+// block sums (the next round's load is issued before this round's adds), and
+// every lane replays the 64 adds in lane order on wave-uniform values (read
+// with readlane), keeping the partial sum at its own position.
+__global__ void __launch_bounds__(64) k_prefix_seq(double *p, uint64_t nb) {
+#pragma clang fp contract(off)
+  const uint32_t lane = threadIdx.x;
+  double c = 0.0;
+  const uint64_t rounds = (nb + 63) / 64;
+  double cur = (lane < nb) ? p[1 + lane] : 0.0;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t i = r * 64 + lane;
+    const uint64_t inext = i + 64;
+    const double nxt = (inext < nb) ? p[1 + inext] : 0.0;
+    const uint64_t bits = static_cast<uint64_t>(__double_as_longlong(cur));
+    const int lo32 = static_cast<int>(static_cast<uint32_t>(bits));
+    const int hi32 = static_cast<int>(static_cast<uint32_t>(bits >> 32));
+    double mine = 0.0;
+#pragma unroll
+    for (int l = 0; l < 64; ++l) {
+      const double v = __hiloint2double(__builtin_amdgcn_readlane(hi32, l),
+                                        __builtin_amdgcn_readlane(lo32, l));
+      c = c + v;
+      if (static_cast<int>(lane) == l) mine = c;
+    }
+    if (i < nb) p[1 + i] = mine;
+    cur = nxt;
+  }
+  if (lane == 0) p[0] = 0.0;
+}
+
+// Per shot: its uniform from the Philox stream, the block whose prefix
+// interval holds it (first b with p[b + 1] > t, the host's upper_bound) and
+// the residual t - p[b].
+__global__ void __launch_bounds__(256) k_sample_search(const double *__restrict__ p, uint64_t nb,
+                                                       double total, uint64_t seed, uint64_t stream_id,
+                                                       uint64_t shot0, uint64_t count,
+                                                       uint64_t *__restrict__ shot_block,
+                                                       double *__restrict__ shot_resid) {
+#pragma clang fp contract(off)
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  double t = msv::philox_u01(seed, stream_id, shot0 + k) * total;
+  if (t >= total) t = nextafter(total, 0.0);
+  uint64_t lo = 0, hi = nb;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (p[mid + 1] > t)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  const uint64_t b = lo < nb ? lo : nb - 1;
+  shot_block[k] = b;
+  shot_resid[k] = t - p[b];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_probs_full(const typename Vec2<T>::type *__restrict__ st,
                                                     uint64_t n_amps, double *__restrict__ out) {
@@ -1836,6 +1900,20 @@ template <typename Ta, typename Tb>
 hipError_t launch_inner(const typename Vec2<Ta>::type *a, const typename Vec2<Tb>::type *b,
                         uint64_t n_amps, double *partials, uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL((k_inner<Ta, Tb>), dim3(grid), dim3(256), 0, stream, a, b, n_amps, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_prefix_seq(double *p, uint64_t nb, hipStream_t stream) {
+  hipLaunchKernelGGL(k_prefix_seq, dim3(1), dim3(64), 0, stream, p, nb);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_search(const double *prefix, uint64_t nb, double total, uint64_t seed,
+                                uint64_t stream_id, uint64_t shot0, uint64_t count, uint64_t *shot_block,
+                                double *shot_resid, hipStream_t stream) {
+  const uint64_t grid = (count + 255) / 256;
+  hipLaunchKernelGGL(k_sample_search, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, stream, prefix, nb,
+                     total, seed, stream_id, shot0, count, shot_block, shot_resid);
   return hipGetLastError();
 }
 

@@ -5,6 +5,8 @@
 // seed, so every shot is independent of batch order and thread count.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace msv {
@@ -13,7 +15,7 @@ struct Philox4 {
   uint32_t v[4];
 };
 
-inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+__host__ __device__ inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                              uint32_t k1) {
   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
@@ -35,7 +37,7 @@ inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
 }
 
 // Uniform double in [0, 1) with 53 random bits from the first two words.
-inline double philox_u01(uint64_t seed, uint64_t stream_id, uint64_t counter) {
+__host__ __device__ inline double philox_u01(uint64_t seed, uint64_t stream_id, uint64_t counter) {
   const Philox4 r = philox4x32_10(static_cast<uint32_t>(counter), static_cast<uint32_t>(counter >> 32),
                                   static_cast<uint32_t>(stream_id),
                                   static_cast<uint32_t>(stream_id >> 32),

@@ -19,6 +19,14 @@
 
 using namespace msv;
 
+// Device scratch of the allocation-free sampler (msv_sample_noalloc): the
+// block-sum prefix and per-shot staging. Each GPU the library runs on holds
+// its own copy until the process ends. Host code never uses this name as an
+// address: it takes the device address from hipGetSymbolAddress once per
+// device (noalloc_scratch below).
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------
@@ -196,12 +204,23 @@ int set_dev(const msv_state *s) {
   return MSV_OK;
 }
 
+// Set while an allocation-free entry point runs on this thread: the buffer
+// helpers below then refuse to allocate instead of growing a buffer.
+thread_local bool t_noalloc = false;
+
+int refuse_alloc(const char *what, size_t bytes) {
+  return fail(MSV_E_REFUSED, std::string("allocation-free call: the state's deferred work needs a new ") +
+                                 what + " of " + std::to_string(bytes) +
+                                 " bytes; call msv_state_sync first");
+}
+
 // Copies `bytes` of host data into the staging ring and enqueues the upload
 // on the state's stream; returns the device address. Wrapping the ring first
 // drains the stream, so no in-flight copy or kernel still reads a slot.
 int stage(msv_state *s, const void *src, size_t bytes, void **dev) {
   const size_t need = (bytes + 255) & ~size_t(255);
   if (need > s->stage_cap) {
+    if (t_noalloc) return refuse_alloc("staging ring", std::max<size_t>(need, size_t(1) << 20));
     MSV_HIP(hipStreamSynchronize(s->stream));
     if (s->h_stage) MSV_HIP(hipHostFree(s->h_stage));
     if (s->d_stage) MSV_HIP(hipFree(s->d_stage));
@@ -231,6 +250,7 @@ int stage(msv_state *s, const void *src, size_t bytes, void **dev) {
 
 int ensure_red(msv_state *s, size_t bytes) {
   if (bytes <= s->red_cap) return MSV_OK;
+  if (t_noalloc) return refuse_alloc("reduction buffer", bytes);
   MSV_HIP(hipStreamSynchronize(s->stream));
   if (s->d_red) MSV_HIP(hipFree(s->d_red));
   s->d_red = nullptr;
@@ -244,6 +264,7 @@ int ensure_red(msv_state *s, size_t bytes) {
 
 int ensure_alt(msv_state *s) {
   if (s->alt) return MSV_OK;
+  if (t_noalloc) return refuse_alloc("second state buffer", s->amps * esize(s->dtype));
   const size_t bytes = s->amps * esize(s->dtype);
   if (hipMalloc(&s->alt, bytes) != hipSuccess) {
     s->alt = nullptr;
@@ -2701,6 +2722,73 @@ int init_basis_T(msv_state *s, uint64_t index) {
   return MSV_OK;
 }
 
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device with hipGetSymbolAddress and kept: each GPU holds its own
+// copy of the array at its own address.
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];
+
+int noalloc_scratch(int device, char **out) {
+  if (device < 0 || device >= 64) return fail(MSV_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      (void)hipGetLastError();
+      return fail(MSV_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = static_cast<char *>(g_scratch_ptr[device]);
+  return MSV_OK;
+}
+
+// msv_sample without host buffers or device allocations: the block sums and
+// their prefix (device, same order as the host's), then shots in chunks that
+// fit the scratch behind the prefix; each chunk's indices are copied straight
+// into `out` and packed there. Same results as sample_T.
+template <typename T>
+int sample_noalloc_T(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_t seed,
+                     uint64_t stream_id, uint64_t *out, char *scr) {
+  using V = typename Vec2<T>::type;
+  const uint64_t nb = (s->amps + kSampleBlock - 1) / kSampleBlock;
+  const size_t prefix_bytes = ((nb + 1) * sizeof(double) + 255) & ~size_t(255);
+  double *d_prefix = reinterpret_cast<double *>(scr);
+  MSV_HIP(launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_prefix + 1, nb,
+                                  s->stream));
+  MSV_HIP(launch_prefix_seq(d_prefix, nb, s->stream));
+  double total = 0.0;
+  MSV_HIP(hipMemcpyAsync(&total, d_prefix + nb, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  MSV_HIP(hipStreamSynchronize(s->stream));
+  if (!(total > 0.0)) return fail(MSV_E_INVALID, "sample: state has zero norm");
+  const uint64_t per_shot = 2 * sizeof(uint64_t) + sizeof(double);
+  const uint64_t chunk = (kNoAllocScratchBytes - prefix_bytes) / per_shot;
+  uint64_t *d_block = reinterpret_cast<uint64_t *>(scr + prefix_bytes);
+  double *d_resid = reinterpret_cast<double *>(d_block + chunk);
+  uint64_t *d_index = reinterpret_cast<uint64_t *>(d_resid + chunk);
+  const int n = s->n;
+  for (uint64_t k0 = 0; k0 < shots; k0 += chunk) {
+    const uint64_t cnt = std::min(chunk, shots - k0);
+    MSV_HIP(launch_sample_search(d_prefix, nb, total, seed, stream_id, k0, cnt, d_block, d_resid,
+                                 s->stream));
+    MSV_HIP(launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
+                                     d_index, cnt, s->stream));
+    MSV_HIP(hipMemcpyAsync(out + k0, d_index, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
+    MSV_HIP(hipStreamSynchronize(s->stream));
+    for (uint64_t k = k0; k < k0 + cnt; ++k) {
+      const uint64_t idx = out[k];
+      uint64_t v = 0;
+      for (int w = 0; w < nw; ++w) {
+        const int bit = n - 1 - (wires ? wires[w] : w);
+        v = (v << 1) | ((idx >> bit) & 1ull);
+      }
+      out[k] = v;
+    }
+  }
+  return MSV_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3344,6 +3432,37 @@ int msv_sample(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint6
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, sample_T, s, wires, nw, shots, seed, stream_id, out);
+}
+
+int msv_sample_noalloc(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_t seed,
+                       uint64_t stream_id, uint64_t *out) {
+  TuneScope ts_;
+  MSV_TRY(check_state(s));
+  if (shots && !out) return fail(MSV_E_INVALID, "null output");
+  if (!wires) nw = s->n;
+  if (nw < 0 || nw > s->n || nw > 64) return fail(MSV_E_INVALID, "sample: bad wire count");
+  uint64_t seen = 0;
+  for (int w = 0; wires && w < nw; ++w) {
+    MSV_TRY(check_qubit(s, wires[w], "sample"));
+    if ((seen >> wires[w]) & 1ull) return fail(MSV_E_INVALID, "Duplicate qubit indices");
+    seen |= 1ull << wires[w];
+  }
+  const uint64_t nb = (s->amps + kSampleBlock - 1) / kSampleBlock;
+  const size_t prefix_bytes = ((nb + 1) * sizeof(double) + 255) & ~size_t(255);
+  if (prefix_bytes + 64 * (2 * sizeof(uint64_t) + sizeof(double)) > kNoAllocScratchBytes)
+    return fail(MSV_E_REFUSED, "sample_noalloc: a " + std::to_string(s->n) +
+                                   "-qubit state needs " + std::to_string(prefix_bytes) +
+                                   " bytes of prefix scratch; the static scratch holds " +
+                                   std::to_string(kNoAllocScratchBytes) + " (at most 31 qubits)");
+  MSV_TRY(set_dev(s));
+  char *scr = nullptr;
+  MSV_TRY(noalloc_scratch(s->device, &scr));
+  t_noalloc = true;
+  int rc = flush_pending(s);
+  t_noalloc = false;
+  MSV_TRY(rc);
+  if (shots == 0) return MSV_OK;
+  return MSV_DISPATCH(s, sample_noalloc_T, s, wires, nw, shots, seed, stream_id, out, scr);
 }
 
 int msv_expval_pauli(msv_state *s, const uint64_t *xmask, const uint64_t *zmask, int n_words,
