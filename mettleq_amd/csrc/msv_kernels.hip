@@ -510,8 +510,43 @@ __device__ __forceinline__ void shear_pairs(typename Vec2<T>::type *a, T sa, T s
 // Diagonal over all index bits, applied to a segment's amplitudes: tab[0..63]
 // holds the register part d(r) for the segment's register bits (constant
 // included), tab[64..81] the factor of each thread-id bit when it is set; the
-// thread part is the product of the factors of its set bits, times `g` (the
-// tile's factor).
+// thread part (thread_factor) is the product of the factors of its set bits,
+// multiplied by the tile's factor before dtab_apply_f.
+template <typename T>
+__device__ __forceinline__ typename Vec2<T>::type thread_factor(const T *tab, uint32_t tm) {
+  using V = typename Vec2<T>::type;
+  V f;
+  f.x = T(1);
+  f.y = T(0);
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) {
+    V q;
+    q.x = tab[64 + 2 * k];
+    q.y = tab[65 + 2 * k];
+    const V h = cmul(f, q);
+    const bool on = (tm >> k) & 1u;
+    f.x = on ? h.x : f.x;
+    f.y = on ? h.y : f.y;
+  }
+  return f;
+}
+
+// Applies the diagonal with the thread and tile parts already multiplied
+// into `f` (thread_factor, once per block, times the tile's factor).
+template <typename T, int RB>
+__device__ __forceinline__ void dtab_apply_f(typename Vec2<T>::type *a, const T *tab, typename Vec2<T>::type f) {
+  using V = typename Vec2<T>::type;
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    V d;
+    d.x = tab[2 * r];
+    d.y = tab[2 * r + 1];
+    a[r] = cmul(a[r], cmul(d, f));
+  }
+}
+
+// Applies the diagonal whose tile part is `g`: the thread part's factors
+// multiply into g, then each register's table entry.
 template <typename T, int RB>
 __device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T *tab, uint32_t tm,
                                              typename Vec2<T>::type g) {
@@ -527,14 +562,9 @@ __device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T 
     f.x = on ? h.x : f.x;
     f.y = on ? h.y : f.y;
   }
-#pragma unroll
-  for (int r = 0; r < (1 << RB); ++r) {
-    V d;
-    d.x = tab[2 * r];
-    d.y = tab[2 * r + 1];
-    a[r] = cmul(a[r], cmul(d, f));
-  }
+  dtab_apply_f<T, RB>(a, tab, f);
 }
+
 
 // Per-tile factor of a diagonal: product over the 4-bit groups of the tile
 // number (wave-uniform reads of a read-only table: scalar loads).
@@ -553,6 +583,52 @@ __device__ __forceinline__ typename Vec2<T>::type tile_factor(const typename Vec
 #pragma unroll
   for (int g = 1; g < kMaxTileGroups; ++g) f = cmul(f, e[g]);
   return f;
+}
+
+// The deferred diagonal of a fused sweep on one thread's amplitudes: ix[w]
+// is window w's table index without the register bits; windows [0, ph_rdep)
+// also see the register bits (index ix[w] ^ ph_rimg[w][r]). All kFusedWins
+// slots are read (the host points slots past nwin at a unit entry), so the
+// thread's window reads issue together and multiply into one factor; each
+// register-dependent window's reads then issue in groups of eight before
+// their products.
+template <typename T, int RB>
+__device__ __forceinline__ void fused_diag(typename Vec2<T>::type (&a)[1 << RB], const FusedRead &F,
+                                           const typename Vec2<T>::type *lut,
+                                           const uint32_t (&ix)[kFusedWins]) {
+  using V = typename Vec2<T>::type;
+  constexpr int R = 1 << RB;
+  constexpr int G = R < 8 ? R : 8;
+  const uint32_t nrdep = F.ph_rdep;
+  V e[kFusedWins];
+#pragma unroll
+  for (int w = 0; w < kFusedWins; ++w) e[w] = lut[F.ph.win[w].off + ix[w]];
+  V pt;
+  pt.x = T(1);
+  pt.y = T(0);
+#pragma unroll
+  for (int w = 0; w < kFusedWins; ++w) {
+    const V f = cmul(pt, e[w]);
+    const bool use = static_cast<uint32_t>(w) >= nrdep;
+    pt.x = use ? f.x : pt.x;
+    pt.y = use ? f.y : pt.y;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[r] = cmul(a[r], pt);
+#pragma unroll
+  for (int w = 0; w < kFusedRdep; ++w) {
+    if (static_cast<uint32_t>(w) < nrdep) {
+      const uint32_t off = F.ph.win[w].off, b = ix[w];
+#pragma unroll
+      for (int h = 0; h < R; h += G) {
+        V g[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) g[k] = lut[off + (b ^ F.ph_rimg[w][h + k])];
+#pragma unroll
+        for (int k = 0; k < G; ++k) a[h + k] = cmul(a[h + k], g[k]);
+      }
+    }
+  }
 }
 
 // Fused read of deferred work (FusedRead), per block: the diagonal's
@@ -599,30 +675,10 @@ __device__ __forceinline__ void fused_load(typename Vec2<T>::type (&a)[1 << RB],
 #pragma unroll
   for (int r = 0; r < R; ++r) a[r] = ld_amp(src + (sb ^ ri[r]));
   if (F.ph.nwin) {
-    const uint32_t nw = F.ph.nwin, nrdep = F.ph_rdep;
-    V pt;
-    pt.x = T(1);
-    pt.y = T(0);
-    uint32_t bw[kFusedRdep];
+    uint32_t ix[kFusedWins];
 #pragma unroll
-    for (int w = 0; w < kFusedWins; ++w) {
-      if (w < static_cast<int>(nw)) {
-        const PhaseWin &pw = F.ph.win[w];
-        const uint32_t ix = win_bits(pw, tsb) ^ thw[w];
-        if (w < kFusedRdep && w < static_cast<int>(nrdep))
-          bw[w < kFusedRdep ? w : 0] = ix;
-        else
-          pt = cmul(pt, lut[pw.off + ix]);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      V q = pt;
-#pragma unroll
-      for (int w = 0; w < kFusedRdep; ++w)
-        if (w < static_cast<int>(nrdep)) q = cmul(q, lut[F.ph.win[w].off + (bw[w] ^ F.ph_rimg[w][r])]);
-      a[r] = cmul(a[r], q);
-    }
+    for (int w = 0; w < kFusedWins; ++w) ix[w] = win_bits(F.ph.win[w], tsb) ^ thw[w];
+    fused_diag<T, RB>(a, F, lut, ix);
   }
 }
 
@@ -769,10 +825,8 @@ k_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> A, const typename Ve
         for (int r = 0; r < R; ++r)
           a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
         if (s == so + 1 && A.has_pre) {
-          V one;
-          one.x = T(1);
-          one.y = T(0);
-          dtab_apply_g<T, RB>(a, A.pre, p.tm, one);
+          // the thread part is formed per tile here: kept across tiles it spills registers
+          dtab_apply_f<T, RB>(a, A.pre, thread_factor<T>(A.pre, p.tm));
         }
       }
       const uint32_t gm = A.gmask[s];
@@ -793,38 +847,14 @@ k_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> A, const typename Ve
       if (s == last) break;
       if (s == so) {
         if (A.has_post) {
-          V one;
-          one.x = T(1);
-          one.y = T(0);
-          dtab_apply_g<T, RB>(a, A.post, p.tm, one);
+          dtab_apply_f<T, RB>(a, A.post, thread_factor<T>(A.post, p.tm));
         }
         if (F.ph.nwin) {
           // this thread's index in segment so (old frame) without its register bits
-          const uint64_t gt = p.gb;
-          const uint32_t nw = F.ph.nwin, nrdep = F.ph_rdep;
-          V pt;
-          pt.x = T(1);
-          pt.y = T(0);
-          uint32_t bw[kFusedRdep];
+          uint32_t ix[kFusedWins];
 #pragma unroll
-          for (int w = 0; w < kFusedWins; ++w) {
-            if (w < static_cast<int>(nw)) {
-              const PhaseWin &pw = F.ph.win[w];
-              const uint32_t ix = win_bits(pw, gt);
-              if (w < kFusedRdep && w < static_cast<int>(nrdep))
-                bw[w < kFusedRdep ? w : 0] = ix;
-              else
-                pt = cmul(pt, lut[pw.off + ix]);
-            }
-          }
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            V q = pt;
-#pragma unroll
-            for (int w = 0; w < kFusedRdep; ++w)
-              if (w < static_cast<int>(nrdep)) q = cmul(q, lut[F.ph.win[w].off + (bw[w] ^ F.ph_rimg[w][r])]);
-            a[r] = cmul(a[r], q);
-          }
+          for (int w = 0; w < kFusedWins; ++w) ix[w] = win_bits(F.ph.win[w], p.gb);
+          fused_diag<T, RB>(a, F, lut, ix);
         }
         const uint64_t e = F.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
                            ptab[512 + ((tile >> 16) & 255u)] ^ ptab[768 + ((tile >> 24) & 255u)];

@@ -105,6 +105,12 @@ const Tuning &tune() {
   return t_tune;
 }
 
+// One factor of a diagonal: a table over one or two index bits.
+struct DiagTerm {
+  int bits[2];  // index bits; bits[1] < 0 for one-bit terms; bits[0] is the MSB of the table
+  double re[4], im[4];
+};
+
 struct DevProps {
   int cu = 0;
   bool ok = false;
@@ -162,6 +168,7 @@ struct msv_state {
   bool pend_on = false;
   PhaseArgs pend_args;
   std::vector<unsigned char> pend_lut;
+  std::vector<DiagTerm> pend_terms;  // its factors (a crossing sweep re-windows them)
   // Deferred XOR permutation: the amplitudes are y -> D(src(y)) buf[src(y)]
   // with src(y) = M'y ^ c (perm_col[j] = M' column j); D is the deferred
   // diagonal above when pend_on. The next 1q layer reads its first sweep
@@ -845,7 +852,24 @@ void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &
 // gate-by-gate layer.
 // Largest deferred-diagonal table a fused first sweep keeps in LDS: two
 // blocks of 64 KiB exchange buffer + 16 KiB tables fill a CU's 160 KiB.
-constexpr size_t kFusedLutMaxBytes = 16 * 1024;
+constexpr size_t kFusedLutMaxBytes = 16 * 1024 - 16;  // plus the unit entry (pad_fused_windows)
+
+// The fused sweeps read all kFusedWins window slots unconditionally, so that
+// the table reads issue together: slots past nwin get a window of no bits
+// whose table is a unit entry appended to `lut`.
+template <typename V>
+void pad_fused_windows(PhaseArgs &ph, std::vector<V> &lut) {
+  V one;
+  one.x = 1;
+  one.y = 0;
+  const uint32_t unit = static_cast<uint32_t>(lut.size());
+  lut.push_back(one);
+  for (uint32_t w = ph.nwin; w < static_cast<uint32_t>(kFusedWins); ++w) {
+    ph.win[w].s1 = ph.win[w].w1 = ph.win[w].s2 = ph.win[w].w2 = 0;
+    ph.win[w].off = unit;
+  }
+  ph.lut_len = static_cast<uint32_t>(lut.size());
+}
 
 // Whether a sweep whose segment 0 holds local bits [reg0, reg0 + rb) of
 // `local` in registers can read through the state's deferred work
@@ -949,8 +973,11 @@ int prepare_fused_read(msv_state *s, Args &A, const std::vector<int> &local, int
       F.ph_rimg[k][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
     }
   }
+  std::vector<V> lut(s->pend_lut.size() / sizeof(V));
+  std::memcpy(lut.data(), s->pend_lut.data(), lut.size() * sizeof(V));
+  pad_fused_windows(F.ph, lut);
   void *dl = nullptr;
-  MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &dl));
+  MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &dl));
   *d_lut = reinterpret_cast<const V *>(dl);
   return MSV_OK;
 }
@@ -1190,6 +1217,213 @@ uint32_t cross_coords(const CrossTile &t, uint64_t v, uint64_t *rest) {
 // permutation inside the tile and runs this layer's rotations on the
 // active head bits (their pre diagonal first). Fails (MSV_E_INVALID,
 // nothing launched) when the tile does not fit the kernel's limits.
+struct Win {
+  std::vector<int> bits;  // distinct index bits, ascending
+  int s1 = 0, w1 = 0, s2 = 0, w2 = 0;
+  std::vector<int> terms;
+};
+
+// Cover a sorted bit set with at most two runs of total width <= wmax.
+bool fit_runs(const std::vector<int> &bits, int wmax, int *s1, int *w1, int *s2, int *w2) {
+  if (bits.empty()) return false;
+  const int lo = bits.front(), hi = bits.back();
+  if (hi - lo + 1 <= wmax) {
+    *s1 = lo;
+    *w1 = hi - lo + 1;
+    *s2 = 0;
+    *w2 = 0;
+    return true;
+  }
+  int gap_at = -1, gap = -1;
+  for (size_t i = 0; i + 1 < bits.size(); ++i) {
+    const int g = bits[i + 1] - bits[i];
+    if (g > gap) {
+      gap = g;
+      gap_at = static_cast<int>(i);
+    }
+  }
+  const int a_lo = lo, a_hi = bits[gap_at], b_lo = bits[gap_at + 1], b_hi = hi;
+  const int wa = a_hi - a_lo + 1, wb = b_hi - b_lo + 1;
+  if (wa + wb > wmax) return false;
+  *s1 = a_lo;
+  *w1 = wa;
+  *s2 = b_lo;
+  *w2 = wb;
+  return true;
+}
+
+constexpr int kWinMaxBits = 8;  // index bits of one window's table
+
+// Greedy windows: term ti joins the first window whose bits, with the
+// term's, still fit two runs of kWinMaxBits in all; else it opens a new one.
+// Windows never cover a bit of `avoid` (their tables then do not depend on
+// those bits), except window `keep` (< 0: none) placed first, which takes
+// the terms listed in `first`.
+bool place_terms(const std::vector<DiagTerm> &terms, const std::vector<int> &first, uint64_t avoid,
+                 std::vector<Win> *out) {
+  std::vector<Win> &wins = *out;
+  auto covers_avoid = [&](const Win &w) {
+    for (int b = w.s1; b < w.s1 + w.w1; ++b)
+      if ((avoid >> b) & 1ull) return true;
+    for (int b = w.s2; b < w.s2 + w.w2; ++b)
+      if ((avoid >> b) & 1ull) return true;
+    return false;
+  };
+  std::vector<bool> done(terms.size(), false);
+  if (!first.empty()) {
+    Win w;
+    for (int ti : first) {
+      const DiagTerm &t = terms[ti];
+      for (int k = 0; k < 2; ++k)
+        if (t.bits[k] >= 0 && std::find(w.bits.begin(), w.bits.end(), t.bits[k]) == w.bits.end())
+          w.bits.push_back(t.bits[k]);
+      w.terms.push_back(ti);
+      done[ti] = true;
+    }
+    std::sort(w.bits.begin(), w.bits.end());
+    if (!fit_runs(w.bits, kWinMaxBits, &w.s1, &w.w1, &w.s2, &w.w2)) return false;
+    wins.push_back(w);
+  }
+  const size_t fixed = wins.size();
+  for (size_t ti = 0; ti < terms.size(); ++ti) {
+    if (done[ti]) continue;
+    const DiagTerm &t = terms[ti];
+    std::vector<int> tb = {t.bits[0]};
+    if (t.bits[1] >= 0) tb.push_back(t.bits[1]);
+    bool placed = false;
+    for (size_t wi = fixed; wi < wins.size() && !placed; ++wi) {
+      Win &w = wins[wi];
+      Win c = w;
+      for (int b : tb)
+        if (std::find(c.bits.begin(), c.bits.end(), b) == c.bits.end()) c.bits.push_back(b);
+      std::sort(c.bits.begin(), c.bits.end());
+      if (fit_runs(c.bits, kWinMaxBits, &c.s1, &c.w1, &c.s2, &c.w2) && !covers_avoid(c)) {
+        c.terms.push_back(static_cast<int>(ti));
+        w = c;
+        placed = true;
+      }
+    }
+    if (!placed) {
+      Win w;
+      w.bits = tb;
+      std::sort(w.bits.begin(), w.bits.end());
+      fit_runs(w.bits, kWinMaxBits, &w.s1, &w.w1, &w.s2, &w.w2);
+      if (covers_avoid(w)) return false;
+      w.terms.push_back(static_cast<int>(ti));
+      wins.push_back(w);
+    }
+  }
+  return true;
+}
+
+// Tables of windows [wi, ...) into one PhaseArgs (up to kMaxPhaseWins
+// windows and kPhaseLutMax entries), products formed in double precision
+// and rounded once; `cst` multiplies the first window's table. Returns the
+// first window not taken.
+template <typename T>
+size_t window_tables(const msv_state *s, const std::vector<DiagTerm> &terms, const std::vector<Win> &wins,
+                     size_t wi, std::complex<double> cst, PhaseArgs *out,
+                     std::vector<typename Vec2<T>::type> *lut) {
+  using V = typename Vec2<T>::type;
+  PhaseArgs &A = *out;
+  std::memset(&A, 0, sizeof(A));
+  A.n_amps = s->amps;
+  uint32_t nw = 0, nlow = 0;
+  while (wi < wins.size() && nw < static_cast<uint32_t>(kMaxPhaseWins)) {
+    const Win &w = wins[wi];
+    const uint32_t len = 1u << (w.w1 + w.w2);
+    if (lut->size() + len > static_cast<size_t>(kPhaseLutMax) && nw > 0) break;
+    PhaseWin pw;
+    pw.s1 = static_cast<uint8_t>(w.s1);
+    pw.w1 = static_cast<uint8_t>(w.w1);
+    pw.s2 = static_cast<uint8_t>(w.s2);
+    pw.w2 = static_cast<uint8_t>(w.w2);
+    pw.off = static_cast<uint32_t>(lut->size());
+    for (uint32_t v = 0; v < len; ++v) {
+      auto bit_val = [&](int bit) -> int {
+        if (bit >= w.s1 && bit < w.s1 + w.w1) return (v >> (bit - w.s1)) & 1;
+        return (v >> (w.w1 + bit - w.s2)) & 1;
+      };
+      double pr = nw == 0 ? cst.real() : 1.0, pi = nw == 0 ? cst.imag() : 0.0;
+      for (int ti : w.terms) {
+        const DiagTerm &t = terms[ti];
+        int sel = bit_val(t.bits[0]);
+        if (t.bits[1] >= 0) sel = sel * 2 + bit_val(t.bits[1]);
+        const double nr = pr * t.re[sel] - pi * t.im[sel];
+        const double ni = pr * t.im[sel] + pi * t.re[sel];
+        pr = nr;
+        pi = ni;
+      }
+      V e;
+      e.x = static_cast<T>(pr);
+      e.y = static_cast<T>(pi);
+      lut->push_back(e);
+    }
+    A.win[nw++] = pw;
+    if (w.bits.front() == 0) nlow = nw;
+    ++wi;
+  }
+  A.nwin = nw;
+  A.nlow = nlow;
+  A.xcd_groups = static_cast<uint32_t>(std::max(0, tune().xcd_groups));
+  A.lut_len = static_cast<uint32_t>(lut->size());
+  return wi;
+}
+
+// The deferred diagonal of a crossing sweep, windowed for the crossing
+// segment's register bits (umask): every factor that reads one of them goes
+// into window 0, evaluated per register; the other windows avoid those bits
+// and give one product per thread and tile. The pending tail's post diagonal
+// joins as one-bit factors (its constant scales window 0's table), so the
+// sweep has no post pass of its own. False when the factors do not fit the
+// fused sweep's limits; the deferred diagonal's own windows are used then.
+template <typename T>
+bool crossing_windows(msv_state *s, const std::vector<int> &vloc, const TileSeg &sgo, int rb,
+                      uint64_t umask, FusedRead *F, std::vector<typename Vec2<T>::type> *lut) {
+  using V = typename Vec2<T>::type;
+  using C = std::complex<double>;
+  if (s->pend_terms.empty()) return false;
+  std::vector<DiagTerm> terms = s->pend_terms;
+  for (int b : s->tail_bits) {
+    if (s->tail_post[b] == C(1.0)) continue;
+    DiagTerm t;
+    std::memset(&t, 0, sizeof(t));
+    t.bits[0] = b;
+    t.bits[1] = -1;
+    t.re[0] = 1.0;
+    t.re[1] = s->tail_post[b].real();
+    t.im[1] = s->tail_post[b].imag();
+    terms.push_back(t);
+  }
+  std::vector<int> first;
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const DiagTerm &t = terms[i];
+    if (((umask >> t.bits[0]) & 1ull) || (t.bits[1] >= 0 && ((umask >> t.bits[1]) & 1ull)))
+      first.push_back(static_cast<int>(i));
+  }
+  std::vector<Win> wins;
+  if (!place_terms(terms, first, umask, &wins)) return false;
+  if (wins.size() > static_cast<size_t>(kFusedWins)) return false;
+  PhaseArgs A;
+  lut->clear();
+  if (window_tables<T>(s, terms, wins, 0, s->tail_cst, &A, lut) != wins.size()) return false;
+  if (lut->size() * sizeof(V) > kFusedLutMaxBytes) return false;
+  F->ph = A;
+  F->ph_rdep = first.empty() ? 0u : 1u;
+  if (!first.empty()) {
+    const PhaseWin &pw = A.win[0];
+    for (int r = 0; r < (1 << rb); ++r) {
+      uint64_t im = 0;
+      for (int j = 0; j < rb; ++j)
+        if ((r >> j) & 1) im |= 1ull << vloc[sgo.reg0 + j];
+      const uint32_t lo = static_cast<uint32_t>(im >> pw.s1) & ((1u << pw.w1) - 1u);
+      const uint32_t hi = static_cast<uint32_t>(im >> pw.s2) & ((1u << pw.w2) - 1u);
+      F->ph_rimg[0][r] = static_cast<uint16_t>(lo | (hi << pw.w1));
+    }
+  }
+  return true;
+}
+
 template <typename T>
 int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act, const std::vector<Shear> &sh,
                   const std::vector<std::complex<double>> &pre, bool pre_on, bool launch,
@@ -1357,7 +1591,11 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   // register bits first
   uint64_t umask = 0;
   for (int j = 0; j < rb; ++j) umask |= 1ull << t.vloc[sgo.reg0 + j];
-  if (s->pend_on) {
+  std::vector<V> lut;
+  const bool rewin = s->pend_on && crossing_windows<T>(s, t.vloc, sgo, rb, umask, &F, &lut);
+  if (rewin) {
+    A.has_post = 0u;  // folded into the windows
+  } else if (s->pend_on) {
     if (s->pend_lut.size() > kFusedLutMaxBytes || s->pend_args.nwin > static_cast<uint32_t>(kFusedWins))
       return fail(MSV_E_INVALID, "internal: deferred diagonal too wide for a cross sweep");
     F.ph = s->pend_args;
@@ -1392,8 +1630,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   // the deferred diagonal's tables when a window holds each of those bits:
   // the diagonals multiply at the same point, and the sweep then skips one
   // diagonal pass over its registers.
-  std::vector<V> lut;
-  if (s->pend_on) {
+  if (s->pend_on && !rewin) {
     lut.resize(s->pend_lut.size() / sizeof(V));
     std::memcpy(lut.data(), s->pend_lut.data(), lut.size() * sizeof(V));
     std::vector<std::pair<uint32_t, int>> at;  // (window, bit) per folded factor
@@ -1451,6 +1688,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   MSV_TRY(stage(s, ptab.data(), ptab.size() * sizeof(uint64_t), &dp));
   const V *d_lut = nullptr;
   if (s->pend_on) {
+    pad_fused_windows(F.ph, lut);
     void *dl = nullptr;
     MSV_TRY(stage(s, lut.data(), lut.size() * sizeof(V), &dl));
     d_lut = reinterpret_cast<const V *>(dl);
@@ -2236,46 +2474,6 @@ int apply_plan_ops(msv_state *s, const std::vector<PlanOp> &all) {
 // diagonal phase layers
 // ---------------------------------------------------------------------------
 
-struct DiagTerm {
-  int bits[2];  // index bits; bits[1] < 0 for one-bit terms; bits[0] is the MSB of the table
-  double re[4], im[4];
-};
-
-struct Win {
-  std::vector<int> bits;  // distinct index bits, ascending
-  int s1 = 0, w1 = 0, s2 = 0, w2 = 0;
-  std::vector<int> terms;
-};
-
-// Cover a sorted bit set with at most two runs of total width <= wmax.
-bool fit_runs(const std::vector<int> &bits, int wmax, int *s1, int *w1, int *s2, int *w2) {
-  if (bits.empty()) return false;
-  const int lo = bits.front(), hi = bits.back();
-  if (hi - lo + 1 <= wmax) {
-    *s1 = lo;
-    *w1 = hi - lo + 1;
-    *s2 = 0;
-    *w2 = 0;
-    return true;
-  }
-  int gap_at = -1, gap = -1;
-  for (size_t i = 0; i + 1 < bits.size(); ++i) {
-    const int g = bits[i + 1] - bits[i];
-    if (g > gap) {
-      gap = g;
-      gap_at = static_cast<int>(i);
-    }
-  }
-  const int a_lo = lo, a_hi = bits[gap_at], b_lo = bits[gap_at + 1], b_hi = hi;
-  const int wa = a_hi - a_lo + 1, wb = b_hi - b_lo + 1;
-  if (wa + wb > wmax) return false;
-  *s1 = a_lo;
-  *w1 = wa;
-  *s2 = b_lo;
-  *w2 = wb;
-  return true;
-}
-
 template <typename T>
 int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, size_t lut_bytes) {
   using V = typename Vec2<T>::type;
@@ -2316,39 +2514,12 @@ constexpr size_t kPendLutMaxBytes = 48 * 1024;
 template <typename T>
 int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool defer) {
   using V = typename Vec2<T>::type;
+  using C = std::complex<double>;
   // a pending tail stays when this diagonal is deferred behind it
   if (s->pend_on || s->perm_on || s->basis_on) MSV_TRY(flush_pending_T<T>(s));
   if (terms.empty()) return MSV_OK;
-  const int wmax = 8;
   std::vector<Win> wins;
-  for (size_t ti = 0; ti < terms.size(); ++ti) {
-    const DiagTerm &t = terms[ti];
-    std::vector<int> tb = {t.bits[0]};
-    if (t.bits[1] >= 0) tb.push_back(t.bits[1]);
-    bool placed = false;
-    for (Win &w : wins) {
-      std::vector<int> u = w.bits;
-      for (int b : tb)
-        if (std::find(u.begin(), u.end(), b) == u.end()) u.push_back(b);
-      std::sort(u.begin(), u.end());
-      int s1, w1, s2, w2;
-      if (fit_runs(u, wmax, &s1, &w1, &s2, &w2)) {
-        w.bits = u;
-        w.s1 = s1; w.w1 = w1; w.s2 = s2; w.w2 = w2;
-        w.terms.push_back(static_cast<int>(ti));
-        placed = true;
-        break;
-      }
-    }
-    if (!placed) {
-      Win w;
-      w.bits = tb;
-      std::sort(w.bits.begin(), w.bits.end());
-      fit_runs(w.bits, wmax, &w.s1, &w.w1, &w.s2, &w.w2);
-      w.terms.push_back(static_cast<int>(ti));
-      wins.push_back(w);
-    }
-  }
+  place_terms(terms, std::vector<int>(), 0, &wins);
   // order: windows reading index bit 0 first (evaluated per amplitude)
   std::stable_sort(wins.begin(), wins.end(), [](const Win &a, const Win &b) {
     const bool la = a.bits.front() == 0, lb2 = b.bits.front() == 0;
@@ -2358,53 +2529,14 @@ int apply_diag_terms_T(msv_state *s, const std::vector<DiagTerm> &terms, bool de
   size_t wi = 0;
   while (wi < wins.size()) {
     PhaseArgs A;
-    std::memset(&A, 0, sizeof(A));
-    A.n_amps = s->amps;
     std::vector<V> lut;
-    uint32_t nw = 0, nlow = 0;
-    while (wi < wins.size() && nw < static_cast<uint32_t>(kMaxPhaseWins)) {
-      const Win &w = wins[wi];
-      const uint32_t len = 1u << (w.w1 + w.w2);
-      if (lut.size() + len > static_cast<size_t>(kPhaseLutMax) && nw > 0) break;
-      PhaseWin pw;
-      pw.s1 = static_cast<uint8_t>(w.s1);
-      pw.w1 = static_cast<uint8_t>(w.w1);
-      pw.s2 = static_cast<uint8_t>(w.s2);
-      pw.w2 = static_cast<uint8_t>(w.w2);
-      pw.off = static_cast<uint32_t>(lut.size());
-      for (uint32_t v = 0; v < len; ++v) {
-        auto bit_val = [&](int bit) -> int {
-          if (bit >= w.s1 && bit < w.s1 + w.w1) return (v >> (bit - w.s1)) & 1;
-          return (v >> (w.w1 + bit - w.s2)) & 1;
-        };
-        double pr = 1.0, pi = 0.0;
-        for (int ti : w.terms) {
-          const DiagTerm &t = terms[ti];
-          int sel = bit_val(t.bits[0]);
-          if (t.bits[1] >= 0) sel = sel * 2 + bit_val(t.bits[1]);
-          const double nr = pr * t.re[sel] - pi * t.im[sel];
-          const double ni = pr * t.im[sel] + pi * t.re[sel];
-          pr = nr;
-          pi = ni;
-        }
-        V e;
-        e.x = static_cast<T>(pr);
-        e.y = static_cast<T>(pi);
-        lut.push_back(e);
-      }
-      A.win[nw++] = pw;
-      if (w.bits.front() == 0) nlow = nw;
-      ++wi;
-    }
-    A.nwin = nw;
-    A.nlow = nlow;
-    A.xcd_groups = static_cast<uint32_t>(std::max(0, tune().xcd_groups));
-    A.lut_len = static_cast<uint32_t>(lut.size());
+    wi = window_tables<T>(s, terms, wins, wi, C(1.0), &A, &lut);
     chunks.emplace_back(A, std::move(lut));
   }
   if (defer && tune().defer_diag && chunks.size() == 1 &&
       chunks[0].second.size() * sizeof(V) <= kPendLutMaxBytes) {
     s->pend_on = true;
+    s->pend_terms = terms;
     s->pend_args = chunks[0].first;
     const auto *b = reinterpret_cast<const unsigned char *>(chunks[0].second.data());
     s->pend_lut.assign(b, b + chunks[0].second.size() * sizeof(V));
