@@ -372,6 +372,43 @@ int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 
 /* ---- launch accounting (execution report) ------------------------------- */
+/* Kernel kinds of the dispatch log. Sweeps over the full state: */
+#define MSV_K_TILE 1              /* generic tile program (gate lists, controlled groups) */
+#define MSV_K_ROT 2               /* 1q-layer rotation sweep                              */
+#define MSV_K_ROT_FUSED 3         /* rotation sweep reading deferred work (gather form)   */
+#define MSV_K_ROT_RELABEL 4       /* pending rotations, deferred work, next layer's first */
+#define MSV_K_PRODUCT 5           /* product state of a 1q layer on a basis state         */
+#define MSV_K_QFT 6               /* QFT sweep                                            */
+#define MSV_K_QFT_FUSED 7         /* QFT sweep reading deferred work                      */
+#define MSV_K_PHASE 8             /* diagonal phase layer                                 */
+#define MSV_K_XOR_GATHER 9        /* XOR permutation                                      */
+#define MSV_K_XOR_GATHER_PHASE 10 /* XOR permutation with a deferred diagonal             */
+#define MSV_K_INIT_BASIS 11       /* basis state                                          */
+#define MSV_K_SCALE 12            /* global factor                                        */
+#define MSV_K_CONVERT 13          /* state copy / precision conversion                    */
+/* Reductions: */
+#define MSV_K_BLOCK_PROBSUM 14
+#define MSV_K_SAMPLE_RESOLVE 15
+#define MSV_K_PROBS_FULL 16
+#define MSV_K_MARGINAL 17
+#define MSV_K_PAULI 18
+#define MSV_K_INNER 19
+#define MSV_K_PREFIX 20
+#define MSV_K_SAMPLE_SEARCH 21
+/* A virtual state: the planner of a real state of that width and dtype,
+ * without device memory. Layer calls (and msv_state_sync) plan their sweeps,
+ * defer work and log their kernels exactly as on a real state but launch
+ * nothing; calls that read or write amplitudes return MSV_E_UNSUPPORTED.
+ * The execution plan's predicted launches come from it: the counterpart of
+ * the reference's launch prediction (execution.py:783-1029), measured on the
+ * planner itself instead of a separate cost model. */
+int msv_state_create_virtual(int n_qubits, int dtype, int device, msv_state **out);
+/* Kinds (MSV_K_*) of the kernels launched on `s` since creation or the last
+ * reset, in launch order: up to `cap` into `kinds` (the log keeps the first
+ * 65536), their total in *count; reset != 0 clears the log. The
+ * counterpart of the reference's observed_custom_dispatches
+ * (execution.py:1032-1049). */
+int msv_state_dispatch_log(msv_state *s, int32_t *kinds, int cap, uint64_t *count, int reset);
 /* Number of full-state sweep kernels launched on this state since creation
  * or the last reset of the counter. */
 int msv_state_sweep_count(msv_state *s, uint64_t *out, int reset);

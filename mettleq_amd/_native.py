@@ -60,6 +60,15 @@ MSV_PENDING_DIAG = 2
 MSV_PENDING_PERM = 4
 MSV_PENDING_TAIL = 8
 
+# kernel kinds of the dispatch log (msv_state_dispatch_log)
+KERNEL_KINDS = {
+    1: "k_tile", 2: "k_rot", 3: "k_rot_fused", 4: "k_rot_relabel", 5: "k_product", 6: "k_qft",
+    7: "k_qft_fused", 8: "k_phase", 9: "k_xor_gather", 10: "k_xor_gather_phase", 11: "k_init_basis",
+    12: "k_scale", 13: "k_convert", 14: "k_block_probsum", 15: "k_sample_resolve", 16: "k_probs_full",
+    17: "k_marginal", 18: "k_pauli", 19: "k_inner", 20: "k_prefix_seq", 21: "k_sample_search",
+}
+SWEEP_KINDS = frozenset(range(1, 14))
+
 MSV_PF_OUT_OF_PLACE = 0
 MSV_PF_IN_PLACE = 1
 MSV_PF_ALLOWED = 0
@@ -168,6 +177,8 @@ SIGNATURES = {
     "msv_set_tuning": (_I, [_I, ctypes.c_int64]),
     "msv_get_tuning": (_I, [_I, ctypes.POINTER(ctypes.c_int64)]),
     "msv_state_sweep_count": (_I, [_P, _U64P, _I]),
+    "msv_state_create_virtual": (_I, [_I, _I, _I, ctypes.POINTER(_P)]),
+    "msv_state_dispatch_log": (_I, [_P, _I32P, _I, _U64P, _I]),
     "msv_state_pending": (_I, [_P, ctypes.POINTER(ctypes.c_int)]),
     "msv_state_timer_start": (_I, [_P]),
     "msv_state_timer_stop": (_I, [_P, _DP]),

@@ -53,6 +53,14 @@ int fail(int code, const std::string &msg) {
     if (rc_ != MSV_OK) return rc_; \
   } while (0)
 
+// A kernel launch on state `st`: logged (kind MSV_K_*), and enqueued unless
+// the state is virtual.
+#define MSV_LAUNCH(st, kind, expr)        \
+  do {                                    \
+    record_dispatch((st), (kind));        \
+    if (!(st)->dry) MSV_HIP(expr);        \
+  } while (0)
+
 // ---------------------------------------------------------------------------
 // tuning
 // ---------------------------------------------------------------------------
@@ -144,6 +152,11 @@ int device_cu_count(int device) { return dev_cu(device); }
 // ---------------------------------------------------------------------------
 
 struct msv_state {
+  // virtual (planning) state: no device memory; every layer call plans its
+  // sweeps and logs its kernels without launching them
+  bool dry = false;
+  std::vector<int8_t> dlog;  // kinds of the kernels launched (MSV_K_*), in order
+  uint64_t dlog_total = 0;   // launches since the last reset (the log keeps the first kDispatchLogMax)
   int n = 0;
   int dtype = MSV_C64;
   int device = 0;
@@ -200,6 +213,13 @@ namespace {
 
 size_t esize(int dtype) { return dtype == MSV_C128 ? 16 : 8; }
 
+constexpr size_t kDispatchLogMax = size_t(1) << 16;
+
+void record_dispatch(msv_state *s, int kind) {
+  if (s->dlog.size() < kDispatchLogMax) s->dlog.push_back(static_cast<int8_t>(kind));
+  ++s->dlog_total;
+}
+
 int check_state(const msv_state *s) {
   if (!s) return fail(MSV_E_INVALID, "null state handle");
   if (!s->buf) return fail(MSV_E_INVALID, "state has no amplitude buffer");
@@ -207,6 +227,7 @@ int check_state(const msv_state *s) {
 }
 
 int set_dev(const msv_state *s) {
+  if (s->dry) return MSV_OK;
   MSV_HIP(hipSetDevice(s->device));
   return MSV_OK;
 }
@@ -226,6 +247,10 @@ int refuse_alloc(const char *what, size_t bytes) {
 // drains the stream, so no in-flight copy or kernel still reads a slot.
 int stage(msv_state *s, const void *src, size_t bytes, void **dev) {
   const size_t need = (bytes + 255) & ~size_t(255);
+  if (s->dry) {  // nothing is copied: a virtual state launches nothing
+    *dev = reinterpret_cast<void *>(uintptr_t(256));
+    return MSV_OK;
+  }
   if (need > s->stage_cap) {
     if (t_noalloc) return refuse_alloc("staging ring", std::max<size_t>(need, size_t(1) << 20));
     MSV_HIP(hipStreamSynchronize(s->stream));
@@ -257,6 +282,7 @@ int stage(msv_state *s, const void *src, size_t bytes, void **dev) {
 
 int ensure_red(msv_state *s, size_t bytes) {
   if (bytes <= s->red_cap) return MSV_OK;
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes to reduce");
   if (t_noalloc) return refuse_alloc("reduction buffer", bytes);
   MSV_HIP(hipStreamSynchronize(s->stream));
   if (s->d_red) MSV_HIP(hipFree(s->d_red));
@@ -271,6 +297,10 @@ int ensure_red(msv_state *s, size_t bytes) {
 
 int ensure_alt(msv_state *s) {
   if (s->alt) return MSV_OK;
+  if (s->dry) {
+    s->alt = reinterpret_cast<void *>(uintptr_t(512));  // never dereferenced: nothing launches
+    return MSV_OK;
+  }
   if (t_noalloc) return refuse_alloc("second state buffer", s->amps * esize(s->dtype));
   const size_t bytes = s->amps * esize(s->dtype);
   if (hipMalloc(&s->alt, bytes) != hipSuccess) {
@@ -620,7 +650,7 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     }
   }
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-  MSV_HIP(launch_tile<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+  MSV_LAUNCH(s, MSV_K_TILE, launch_tile<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
   s->sweeps += 1;
   return MSV_OK;
 }
@@ -1055,7 +1085,7 @@ int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<i
     if (oop) MSV_TRY(ensure_alt(s));
     const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
     const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
+    MSV_LAUNCH(s, MSV_K_ROT_FUSED, launch_rot<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
                           reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
     if (oop) std::swap(s->buf, s->alt);
     s->perm_on = false;
@@ -1063,7 +1093,7 @@ int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<i
   } else {
     const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-    MSV_HIP(launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+    MSV_LAUNCH(s, MSV_K_ROT, launch_rot<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
   }
   s->sweeps += 1;
   return MSV_OK;
@@ -1697,7 +1727,7 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   if (oop) MSV_TRY(ensure_alt(s));
   const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
   const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-  MSV_HIP(launch_rot_relabel<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
+  MSV_LAUNCH(s, MSV_K_ROT_RELABEL, launch_rot_relabel<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, fgrid, s->stream,
                                 reinterpret_cast<const V *>(s->buf), d_lut,
                                 reinterpret_cast<const uint64_t *>(dp)));
   if (oop) std::swap(s->buf, s->alt);
@@ -1956,7 +1986,7 @@ int product_layer_T(msv_state *s, const double *mats, const uint8_t *active) {
   const V *lo = reinterpret_cast<const V *>(d);
   const V *mid = lo + (size_t(1) << l1);
   const V *hi = mid + (size_t(1) << l2);
-  MSV_HIP(launch_product<T>(reinterpret_cast<V *>(s->buf), s->amps, lo, mid, hi, static_cast<uint32_t>(l1),
+  MSV_LAUNCH(s, MSV_K_PRODUCT, launch_product<T>(reinterpret_cast<V *>(s->buf), s->amps, lo, mid, hi, static_cast<uint32_t>(l1),
                             static_cast<uint32_t>(l2), stream_grid(s), s->stream));
   s->basis_on = false;  // cleared once enqueued
   return MSV_OK;
@@ -2052,7 +2082,7 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
     if (oop) MSV_TRY(ensure_alt(s));
     const uint64_t fcap = static_cast<uint64_t>(s->cu) * tune().fused_blocks_per_cu;
     const uint32_t fgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, fcap)));
-    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, inverse, fgrid, s->stream,
+    MSV_LAUNCH(s, MSV_K_QFT_FUSED, launch_qft<T>(reinterpret_cast<V *>(oop ? s->alt : s->buf), A, rb, inverse, fgrid, s->stream,
                           reinterpret_cast<const V *>(s->buf), d_lut, d_ptab));
     if (oop) std::swap(s->buf, s->alt);
     s->perm_on = false;
@@ -2060,7 +2090,7 @@ int launch_qft_plan(msv_state *s, const SweepPlan &plan, int lb, bool inverse) {
   } else {
     const uint64_t cap = static_cast<uint64_t>(s->cu) * tune().tile_blocks_per_cu;
     const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
-    MSV_HIP(launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
+    MSV_LAUNCH(s, MSV_K_QFT, launch_qft<T>(reinterpret_cast<V *>(s->buf), A, rb, inverse, grid, s->stream));
   }
   s->sweeps += 1;
   return MSV_OK;
@@ -2481,7 +2511,7 @@ int launch_phase_chunk(msv_state *s, const PhaseArgs &A, const void *lut_host, s
   MSV_TRY(stage(s, lut_host, lut_bytes, &d_lut));
   const uint32_t grid = tune().phase_blocks > 0 ? static_cast<uint32_t>(tune().phase_blocks)
                                                 : stream_grid(s);
-  MSV_HIP(launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A, grid,
+  MSV_LAUNCH(s, MSV_K_PHASE, launch_phase<T>(reinterpret_cast<V *>(s->buf), reinterpret_cast<const V *>(d_lut), A, grid,
                           s->stream));
   s->sweeps += 1;
   return MSV_OK;
@@ -2631,13 +2661,13 @@ int launch_gather(msv_state *s, const std::vector<uint64_t> &col, uint64_t c) {
     // the deferred phase layer rides along: out[y] = D(src(y)) in[src(y)]
     void *d_lut = nullptr;
     MSV_TRY(stage(s, s->pend_lut.data(), s->pend_lut.size(), &d_lut));
-    MSV_HIP(launch_xor_gather_phase<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
+    MSV_LAUNCH(s, MSV_K_XOR_GATHER_PHASE, launch_xor_gather_phase<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
                                        s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
                                        reinterpret_cast<const V *>(d_lut), s->pend_args,
                                        stream_grid(s), s->stream));
     s->pend_on = false;  // cleared once enqueued
   } else {
-    MSV_HIP(launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
+    MSV_LAUNCH(s, MSV_K_XOR_GATHER, launch_xor_gather<T>(reinterpret_cast<const V *>(s->buf), reinterpret_cast<V *>(s->alt),
                                  s->amps, reinterpret_cast<const uint64_t *>(d_tab), ntab, c,
                                  stream_grid(s), s->stream));
   }
@@ -2655,7 +2685,7 @@ int block_sums_T(msv_state *s, std::vector<double> *out) {
   using V = typename Vec2<T>::type;
   const uint64_t nb = (s->amps + kSampleBlock - 1) / kSampleBlock;
   MSV_TRY(ensure_red(s, nb * sizeof(double)));
-  MSV_HIP(launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps,
+  MSV_LAUNCH(s, MSV_K_BLOCK_PROBSUM, launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps,
                                   reinterpret_cast<double *>(s->d_red), nb, s->stream));
   out->resize(nb);
   MSV_HIP(hipMemcpyAsync(out->data(), s->d_red, nb * sizeof(double), hipMemcpyDeviceToHost,
@@ -2697,7 +2727,7 @@ int sample_T(msv_state *s, const int32_t *wires, int nw, uint64_t shots, uint64_
                          hipMemcpyHostToDevice, s->stream));
   MSV_HIP(hipMemcpyAsync(d_resid, shot_resid.data(), shots * sizeof(double),
                          hipMemcpyHostToDevice, s->stream));
-  MSV_HIP(launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
+  MSV_LAUNCH(s, MSV_K_SAMPLE_RESOLVE, launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
                                    d_index, shots, s->stream));
   std::vector<uint64_t> idx(shots);
   MSV_HIP(hipMemcpyAsync(idx.data(), d_index, shots * sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -2726,7 +2756,7 @@ int probabilities_T(msv_state *s, const int32_t *wires, int nw, double *out) {
   }
   if (identity) {
     MSV_TRY(ensure_red(s, s->amps * sizeof(double)));
-    MSV_HIP(launch_probs_full<T>(reinterpret_cast<const V *>(s->buf), s->amps,
+    MSV_LAUNCH(s, MSV_K_PROBS_FULL, launch_probs_full<T>(reinterpret_cast<const V *>(s->buf), s->amps,
                                  reinterpret_cast<double *>(s->d_red), elem_grid(s, s->amps),
                                  s->stream));
     MSV_HIP(hipMemcpyAsync(out, s->d_red, s->amps * sizeof(double), hipMemcpyDeviceToHost,
@@ -2755,7 +2785,7 @@ int probabilities_T(msv_state *s, const int32_t *wires, int nw, double *out) {
   uint64_t *d_bits = reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(s->d_red) + part_bytes);
   MSV_HIP(hipMemcpyAsync(d_bits, obits.data(), nw * sizeof(uint64_t), hipMemcpyHostToDevice,
                          s->stream));
-  MSV_HIP(launch_marginal<T>(reinterpret_cast<const V *>(s->buf), n, d_bits, nw, d_part, chunks,
+  MSV_LAUNCH(s, MSV_K_MARGINAL, launch_marginal<T>(reinterpret_cast<const V *>(s->buf), n, d_bits, nw, d_part, chunks,
                              chunk_len, s->stream));
   std::vector<double> part(K * chunks);
   MSV_HIP(hipMemcpyAsync(part.data(), d_part, part_bytes, hipMemcpyDeviceToHost, s->stream));
@@ -2796,7 +2826,7 @@ int pauli_T(msv_state *s, const uint64_t *xq, const uint64_t *zq, int nwords, do
     uint64_t *d_z = d_x + kMaxWords;
     MSV_HIP(hipMemcpyAsync(d_x, xm.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
     MSV_HIP(hipMemcpyAsync(d_z, zm.data(), cnt * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
-    MSV_HIP(launch_pauli<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_x, d_z, cnt, d_part,
+    MSV_LAUNCH(s, MSV_K_PAULI, launch_pauli<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_x, d_z, cnt, d_part,
                             grid, s->stream));
     std::vector<double> part(static_cast<size_t>(grid) * cnt * 2);
     MSV_HIP(hipMemcpyAsync(part.data(), d_part, part_bytes, hipMemcpyDeviceToHost, s->stream));
@@ -2829,7 +2859,7 @@ int inner_T(msv_state *a, msv_state *b, double *re, double *im) {
   MSV_HIP(hipStreamSynchronize(b->stream));
   const uint32_t grid = std::min<uint32_t>(elem_grid(a, a->amps), static_cast<uint32_t>(a->cu * 4));
   MSV_TRY(ensure_red(a, static_cast<size_t>(grid) * 2 * sizeof(double)));
-  MSV_HIP((launch_inner<Ta, Tb>(reinterpret_cast<const Va *>(a->buf),
+  MSV_LAUNCH(a, MSV_K_INNER, (launch_inner<Ta, Tb>(reinterpret_cast<const Va *>(a->buf),
                                 reinterpret_cast<const Vb *>(b->buf), a->amps,
                                 reinterpret_cast<double *>(a->d_red), grid, a->stream)));
   std::vector<double> part(static_cast<size_t>(grid) * 2);
@@ -2849,7 +2879,7 @@ int inner_T(msv_state *a, msv_state *b, double *re, double *im) {
 template <typename T>
 int init_basis_T(msv_state *s, uint64_t index) {
   using V = typename Vec2<T>::type;
-  MSV_HIP(launch_init_basis<T>(reinterpret_cast<V *>(s->buf), s->amps, index,
+  MSV_LAUNCH(s, MSV_K_INIT_BASIS, launch_init_basis<T>(reinterpret_cast<V *>(s->buf), s->amps, index,
                                elem_grid(s, s->amps), s->stream));
   return MSV_OK;
 }
@@ -2887,9 +2917,9 @@ int sample_noalloc_T(msv_state *s, const int32_t *wires, int nw, uint64_t shots,
   const uint64_t nb = (s->amps + kSampleBlock - 1) / kSampleBlock;
   const size_t prefix_bytes = ((nb + 1) * sizeof(double) + 255) & ~size_t(255);
   double *d_prefix = reinterpret_cast<double *>(scr);
-  MSV_HIP(launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_prefix + 1, nb,
+  MSV_LAUNCH(s, MSV_K_BLOCK_PROBSUM, launch_block_probsum<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_prefix + 1, nb,
                                   s->stream));
-  MSV_HIP(launch_prefix_seq(d_prefix, nb, s->stream));
+  MSV_LAUNCH(s, MSV_K_PREFIX, launch_prefix_seq(d_prefix, nb, s->stream));
   double total = 0.0;
   MSV_HIP(hipMemcpyAsync(&total, d_prefix + nb, sizeof(double), hipMemcpyDeviceToHost, s->stream));
   MSV_HIP(hipStreamSynchronize(s->stream));
@@ -2902,9 +2932,9 @@ int sample_noalloc_T(msv_state *s, const int32_t *wires, int nw, uint64_t shots,
   const int n = s->n;
   for (uint64_t k0 = 0; k0 < shots; k0 += chunk) {
     const uint64_t cnt = std::min(chunk, shots - k0);
-    MSV_HIP(launch_sample_search(d_prefix, nb, total, seed, stream_id, k0, cnt, d_block, d_resid,
+    MSV_LAUNCH(s, MSV_K_SAMPLE_SEARCH, launch_sample_search(d_prefix, nb, total, seed, stream_id, k0, cnt, d_block, d_resid,
                                  s->stream));
-    MSV_HIP(launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
+    MSV_LAUNCH(s, MSV_K_SAMPLE_RESOLVE, launch_sample_resolve<T>(reinterpret_cast<const V *>(s->buf), s->amps, d_block, d_resid,
                                      d_index, cnt, s->stream));
     MSV_HIP(hipMemcpyAsync(out + k0, d_index, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream));
     MSV_HIP(hipStreamSynchronize(s->stream));
@@ -3159,9 +3189,45 @@ int msv_state_create(int n_qubits, int dtype, int device, msv_state **out) {
   return MSV_OK;
 }
 
+int msv_state_create_virtual(int n_qubits, int dtype, int device, msv_state **out) {
+  TuneScope ts_;
+  if (!out) return fail(MSV_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (n_qubits < 1 || n_qubits > 40)
+    return fail(MSV_E_INVALID, "n_qubits must be in [1, 40], got " + std::to_string(n_qubits));
+  if (dtype != MSV_C64 && dtype != MSV_C128) return fail(MSV_E_INVALID, "unknown dtype");
+  msv_state *s = new msv_state();
+  s->dry = true;
+  s->n = n_qubits;
+  s->dtype = dtype;
+  s->device = device;
+  s->cu = 256;  // grid sizes only: the plan's sweeps do not depend on them
+  s->amps = 1ull << n_qubits;
+  s->buf = reinterpret_cast<void *>(uintptr_t(256));  // never dereferenced
+  *out = s;
+  return MSV_OK;
+}
+
+int msv_state_dispatch_log(msv_state *s, int32_t *kinds, int cap, uint64_t *count, int reset) {
+  if (!s) return fail(MSV_E_INVALID, "null state handle");
+  if (cap < 0 || (cap > 0 && !kinds)) return fail(MSV_E_INVALID, "bad output buffer");
+  if (count) *count = s->dlog_total;
+  const size_t m = std::min(s->dlog.size(), static_cast<size_t>(cap));
+  for (size_t i = 0; i < m; ++i) kinds[i] = s->dlog[i];
+  if (reset) {
+    s->dlog.clear();
+    s->dlog_total = 0;
+  }
+  return MSV_OK;
+}
+
 int msv_state_destroy(msv_state *s) {
   TuneScope ts_;
   if (!s) return MSV_OK;
+  if (s->dry) {  // a virtual state owns no device resources
+    delete s;
+    return MSV_OK;
+  }
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->buf) (void)hipFree(s->buf);
@@ -3203,6 +3269,7 @@ int msv_state_info(const msv_state *s, int *n_qubits, int *dtype, uint64_t *byte
 int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t count) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
   if (offset > s->amps || count > s->amps - offset)
     return fail(MSV_E_INVALID, "upload range exceeds the state");
@@ -3218,6 +3285,7 @@ int msv_state_upload(msv_state *s, const void *host, uint64_t offset, uint64_t c
 int msv_state_download(msv_state *s, void *host, uint64_t offset, uint64_t count) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (!host && count) return fail(MSV_E_INVALID, "null host buffer");
   if (offset > s->amps || count > s->amps - offset)
     return fail(MSV_E_INVALID, "download range exceeds the state");
@@ -3234,6 +3302,7 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   TuneScope ts_;
   MSV_TRY(check_state(dst));
   MSV_TRY(check_state(src));
+  if (dst->dry || src->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (dst->n != src->n) return fail(MSV_E_INVALID, "state_copy: qubit counts differ");
   MSV_TRY(flush_pending(src));
   MSV_TRY(set_dev(dst));
@@ -3241,19 +3310,19 @@ int msv_state_copy(msv_state *dst, msv_state *src) {
   MSV_HIP(hipStreamSynchronize(src->stream));
   const uint32_t grid = elem_grid(dst, dst->amps);
   if (src->dtype == MSV_C64 && dst->dtype == MSV_C64)
-    MSV_HIP((launch_convert<float, float>(static_cast<const float2 *>(src->buf),
+    MSV_LAUNCH(dst, MSV_K_CONVERT, (launch_convert<float, float>(static_cast<const float2 *>(src->buf),
                                           static_cast<float2 *>(dst->buf), dst->amps, grid,
                                           dst->stream)));
   else if (src->dtype == MSV_C64)
-    MSV_HIP((launch_convert<float, double>(static_cast<const float2 *>(src->buf),
+    MSV_LAUNCH(dst, MSV_K_CONVERT, (launch_convert<float, double>(static_cast<const float2 *>(src->buf),
                                            static_cast<double2 *>(dst->buf), dst->amps, grid,
                                            dst->stream)));
   else if (dst->dtype == MSV_C64)
-    MSV_HIP((launch_convert<double, float>(static_cast<const double2 *>(src->buf),
+    MSV_LAUNCH(dst, MSV_K_CONVERT, (launch_convert<double, float>(static_cast<const double2 *>(src->buf),
                                            static_cast<float2 *>(dst->buf), dst->amps, grid,
                                            dst->stream)));
   else
-    MSV_HIP((launch_convert<double, double>(static_cast<const double2 *>(src->buf),
+    MSV_LAUNCH(dst, MSV_K_CONVERT, (launch_convert<double, double>(static_cast<const double2 *>(src->buf),
                                             static_cast<double2 *>(dst->buf), dst->amps, grid,
                                             dst->stream)));
   MSV_HIP(hipStreamSynchronize(dst->stream));
@@ -3266,10 +3335,10 @@ int msv_state_scale(msv_state *s, double re, double im) {
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));
   if (s->dtype == MSV_C128)
-    MSV_HIP(launch_scale<double>(static_cast<double2 *>(s->buf), s->amps, re, im,
+    MSV_LAUNCH(s, MSV_K_SCALE, launch_scale<double>(static_cast<double2 *>(s->buf), s->amps, re, im,
                                  stream_grid(s), s->stream));
   else
-    MSV_HIP(launch_scale<float>(static_cast<float2 *>(s->buf), s->amps, re, im,
+    MSV_LAUNCH(s, MSV_K_SCALE, launch_scale<float>(static_cast<float2 *>(s->buf), s->amps, re, im,
                                 stream_grid(s), s->stream));
   s->sweeps += 1;
   return MSV_OK;
@@ -3280,6 +3349,7 @@ int msv_state_sync(msv_state *s) {
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));
+  if (s->dry) return MSV_OK;
   MSV_HIP(hipStreamSynchronize(s->stream));
   return MSV_OK;
 }
@@ -3289,6 +3359,10 @@ int msv_state_release_scratch(msv_state *s) {
   MSV_TRY(check_state(s));
   MSV_TRY(set_dev(s));
   MSV_TRY(flush_pending(s));  // a deferred permutation still needs the buffer
+  if (s->dry) {
+    s->alt = nullptr;
+    return MSV_OK;
+  }
   MSV_HIP(hipStreamSynchronize(s->stream));
   if (s->alt) MSV_HIP(hipFree(s->alt));
   s->alt = nullptr;
@@ -3300,7 +3374,7 @@ void *msv_state_device_ptr(msv_state *s) {
   if (!s) return nullptr;
   // the caller reads the amplitudes directly: a deferred diagonal goes first
   if (flush_pending(s) != MSV_OK) return nullptr;
-  return s->buf;
+  return s->dry ? nullptr : s->buf;
 }
 
 int msv_apply_1q_layer(msv_state *s, const double *mats, const uint8_t *active) {
@@ -3570,6 +3644,7 @@ int msv_sample_noalloc(msv_state *s, const int32_t *wires, int nw, uint64_t shot
                        uint64_t stream_id, uint64_t *out) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (shots && !out) return fail(MSV_E_INVALID, "null output");
   if (!wires) nw = s->n;
   if (nw < 0 || nw > s->n || nw > 64) return fail(MSV_E_INVALID, "sample: bad wire count");
@@ -3614,6 +3689,7 @@ int msv_inner(msv_state *a, msv_state *b, double *re, double *im) {
   MSV_TRY(check_state(a));
   MSV_TRY(check_state(b));
   if (!re || !im) return fail(MSV_E_INVALID, "null output");
+  if (a->dry || b->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (a->n != b->n) return fail(MSV_E_INVALID, "inner: qubit counts differ");
   if (a->device != b->device) return fail(MSV_E_INVALID, "inner: states on different devices");
   MSV_TRY(flush_pending(a));
@@ -3756,6 +3832,7 @@ int msv_state_pending(const msv_state *s, int *out) {
 int msv_state_timer_start(msv_state *s) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   MSV_TRY(set_dev(s));
   if (!s->ev0) MSV_HIP(hipEventCreate(&s->ev0));
   if (!s->ev1) MSV_HIP(hipEventCreate(&s->ev1));
@@ -3766,6 +3843,7 @@ int msv_state_timer_start(msv_state *s) {
 int msv_state_timer_stop(msv_state *s, double *ms) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
+  if (s->dry) return fail(MSV_E_UNSUPPORTED, "a virtual state holds no amplitudes");
   if (!ms) return fail(MSV_E_INVALID, "null output");
   if (!s->ev0 || !s->ev1) return fail(MSV_E_INVALID, "timer was not started");
   MSV_TRY(set_dev(s));

@@ -15,9 +15,8 @@ from typing import Any, Dict, List, Optional
 
 import numpy as np
 
-from . import gates as G
-from . import layers as L
-from .execution import build_execution_plan, mark_synchronized
+from .dispatch import dispatch as _dispatch
+from .execution import build_execution_plan, mark_synchronized, record_observed_dispatches
 from .fusion import fuse_operations
 from .sim import StateVectorSimulator
 
@@ -56,8 +55,13 @@ class Device:
         return self.sim.native
 
     def synchronize(self):
+        if self.last_execution_plan is None:
+            self.native.sync()
+            return self.native
+        self.native.dispatch_log(reset=True)
         self.native.sync()
-        mark_synchronized(self.last_execution_plan, self.native.sweep_count())
+        mark_synchronized(self.last_execution_plan, self.native.sweep_count(),
+                          flush_kernels=self.native.dispatch_log())
         return self.native
 
     # -- execution ---------------------------------------------------------
@@ -66,10 +70,21 @@ class Device:
         fused = self.fuse(ops)
         if report is None:
             report = os.environ.get("METTLEQ_EXECUTION_REPORT", "0") == "1"
+        basis = True
+        if report:
+            from . import _native as nat
+            pend = self.native.pending()
+            if pend & ~nat.MSV_PENDING_BASIS:
+                # work deferred by an earlier call runs first, so that this
+                # call's observed launches are its own
+                self.native.sync()
+                pend = self.native.pending()
+            basis = bool(pend & nat.MSV_PENDING_BASIS)
         self.native.sweep_count(reset=True)
         self.last_execution_plan = (
             build_execution_plan(n_qubits=self.wires, operations=ops, optimized_operations=fused,
-                                 statevector_preflight_data=self.statevector_preflight, dtype=self.dtype)
+                                 statevector_preflight_data=self.statevector_preflight, dtype=self.dtype,
+                                 basis=basis)
             if report else None)
         self.dispatch(fused)
         return self.native
@@ -82,59 +97,12 @@ class Device:
                                     dtype=self.dtype)
 
     def dispatch(self, fused: List[Dict[str, Any]]) -> None:
-        n = self.wires
-        st = self.native
-        pending: List = []
-
-        def flush():
-            if pending:
-                st.apply_gates(pending)
-                pending.clear()
-
-        for op in fused:
-            name = op.get("name")
-            if name == "_GATE":
-                pending.extend(op["gates"])
-                continue
-            flush()
-            if name == "_U2LISTLAYER":
-                st.apply_1q_layer(list(np.asarray(op["mats"]).reshape(n, 2, 2)), active=op.get("active"))
-            elif name in ("_U2LAYER", "_RXLAYER"):
-                if name == "_RXLAYER":
-                    u = G.single_qubit_matrix("RX", [op["theta"]])
-                else:
-                    u = G.single_qubit_matrix(op["gate"], op.get("params", []))
-                st.apply_1q_layer([u] * n)
-            elif name == "_XORLAYER":
-                st.apply_xor_affine(op["rows"], op["c"])
-            elif name == "_QFTSTAGE":
-                st.apply_qft_stage(op["j"], op["m"], inverse=op["inverse"])
-            elif name == "_QFTLAYER":
-                st.apply_qft(0, n, inverse=False)
-            elif name == "_DIAGLAYER":
-                L.diag_pair_layer(st, op["theta"], n, op["bonds"])
-            elif name == "_DIAGWEIGHTED":
-                L.diag_weighted_layer(st, n, op["bonds"], op["thetas"])
-            elif name == "_ZZLAYER":
-                L.zz_chain_layer(st, op["theta"], n, op["bonds"])
-            elif name == "_ZZWEIGHTED":
-                L.zz_weighted_layer(st, n, op["bonds"], op["thetas"])
-            elif name == "_XXLAYER":
-                L.xx_layer(st, op["theta"], n, op["bonds"])
-            elif name == "_YYLAYER":
-                L.yy_layer(st, op["theta"], n, op["bonds"])
-            elif name == "_CULISTLAYER":
-                L.controlled_u_groups(st, n, op["groups"])
-            elif name == "_CHAINPHASE_RXLAYER":
-                # the diagonal stays pending and the RX layer's first sweep reads it
-                if op["phase_kind"] == "zz":
-                    L.zz_chain_layer(st, op["theta"], n, op["bonds"])
-                else:
-                    L.diag_pair_layer(st, op["theta"], n, op["bonds"])
-                st.apply_1q_layer([G.single_qubit_matrix("RX", [op["rx_theta"]])] * n)
-            else:
-                raise ValueError(f"unknown fused op {name}")
-        flush()
+        if self.last_execution_plan is not None:
+            per_op: list = []
+            _dispatch(self.native, fused, self.wires, per_op)
+            record_observed_dispatches(self.last_execution_plan, per_op)
+        else:
+            _dispatch(self.native, fused, self.wires)
 
     # -- fusion ------------------------------------------------------------
     def fuse(self, operations: List[Dict[str, Any]]) -> List[Dict[str, Any]]:

@@ -288,78 +288,53 @@ def hip_runtime_status(n_qubits: Optional[int] = None, *, dtype: str = STATE_DTY
             "kernel_constraints": {"index_bits": INDEX_BITS, "index_qubit_limit": QUBIT_LIMIT}}
 
 
-def layer_sweeps(op: Dict[str, Any], n: int, dtype: str = STATE_DTYPE) -> int:
-    """Full-state sweeps an optimized op costs on this engine (planning model
-    of the native tile planner: low sweep of 13 (c64) / 12 (c128) local bits,
-    high chunks of at most 9 bits)."""
-    name = str(op.get("name", "")).upper()
-    nl = 13 if dtype == "complex64" else 12
-    if name in ("_U2LISTLAYER", "_U2LAYER", "_RXLAYER"):
-        active = op.get("active")
-        bits = sorted(n - 1 - q for q in (active if active is not None else range(n)))
-        return _sweeps_for_bits(bits, n, nl)
-    if name in ("_QFTLAYER",):
-        return _sweeps_for_bits(list(range(n)), n, nl)
-    if name in ("_QFTSTAGE",):
-        return 1
-    if name in ("_DIAGLAYER", "_DIAGWEIGHTED", "_ZZLAYER", "_ZZWEIGHTED", "_DIAGONAL"):
-        return 1
-    if name in ("_XORLAYER",):
-        return 1
-    if name in ("_XXLAYER", "_YYLAYER"):
-        return 2 * _sweeps_for_bits(list(range(n)), n, nl) + 1
-    if name == "_CHAINPHASE_RXLAYER":  # the diagonal rides in the RX layer's first sweep
-        return _sweeps_for_bits(list(range(n)), n, nl)
-    if name == "_GATEBLOCK":
-        return int(op.get("sweeps", 1))
-    return 1
-
-
-def _sweeps_for_bits(bits: List[int], n: int, nl: int) -> int:
-    if not bits:
-        return 0
-    if n <= nl:
-        return 1
-    rb = 5 if nl == 13 else 4
-    h = n - nl
-    low = nl
-    if h < rb:
-        low = n - rb
-        h = rb
-    k = (h + 8) // 9
-    chunks = []
-    start = low
-    for c in range(k):
-        sz = h // k + (1 if c < h % k else 0)
-        chunks.append((start, start + sz))
-        start += sz
-    count = 1 if any(b < low for b in bits) else 0
-    for lo, hi in chunks:
-        if any(lo <= b < hi for b in bits):
-            count += 1
-    return count
+def _family(op: Dict[str, Any]) -> str:
+    return str(op.get("name", "")).lower().lstrip("_") or "gate"
 
 
 def build_execution_plan(*, n_qubits: int, operations: List[Dict[str, Any]],
                          optimized_operations: List[Dict[str, Any]],
                          statevector_preflight_data: Optional[Dict[str, Any]] = None,
-                         dtype: str = STATE_DTYPE) -> Dict[str, Any]:
-    """Launch-accounting report (execution.py:783-1029): matched structured
-    patterns, predicted sweeps and their byte traffic."""
+                         dtype: str = STATE_DTYPE, basis: bool = True) -> Dict[str, Any]:
+    """Launch-accounting report (execution.py:783-1029).
+
+    The prediction is the native planner's own: the optimized operations run
+    on a virtual state of the same width and dtype (msv_state_create_virtual),
+    which plans, defers and logs every kernel exactly as the real state will
+    but launches nothing. `basis`: the real state starts from a deferred basis
+    state (after a reset). selected_custom_kernels lists, per dispatched
+    entry, the kernels it launches (deferred layers launch none: their work
+    rides in a later entry's sweep); the last entry is the flush of deferred
+    work that a read of the state triggers. Observed dispatches are filled
+    in as the real state runs (record_observed_dispatches, mark_synchronized).
+    """
+    from . import _native as nat
+    from .dispatch import planned_run
+
     n = int(n_qubits)
     status = hip_runtime_status(n, dtype=dtype)
     memory = state_memory_estimate(n, dtype)
-    matched = []
-    for idx, op in enumerate(optimized_operations):
-        name = str(op.get("name", ""))
-        if name.startswith("_"):
-            s = layer_sweeps(op, n, dtype)
-            matched.append({"optimized_operation_index": idx, "kernel_family": name.lower().lstrip("_"),
-                            "expected_sweeps": s})
+    per_op: list = []
+    run = planned_run(list(optimized_operations), n, dtype, basis=basis, per_op=per_op)
+    sweep_names = {nat.KERNEL_KINDS[k] for k in nat.SWEEP_KINDS}
+    selected = []
+    for idx, kinds in per_op:
+        fam = ("deferred_work_flush" if idx is None
+               else "+".join(sorted({_family(optimized_operations[i]) for i in idx})))
+        selected.append({
+            "optimized_operation_indices": idx,
+            "kernel_family": fam,
+            "expected_launches": sum(1 for k in kinds if k in sweep_names),
+            "concrete_kernels": list(kinds),
+        })
+    matched = [
+        {"optimized_operation_index": i, "kernel_family": _family(op)}
+        for i, op in enumerate(optimized_operations) if str(op.get("name", "")).startswith("_")
+    ]
+    launches = sum(item["expected_launches"] for item in selected)
     sweep_bytes = 2 * memory["state_bytes"]
-    predicted = sum(m["expected_sweeps"] for m in matched)
     return {
-        "schema_version": 1,
+        "schema_version": 2,
         "engine": "mettleq_amd HIP statevector (gfx950)",
         "qubits": n,
         "dtype": dtype,
@@ -367,26 +342,50 @@ def build_execution_plan(*, n_qubits: int, operations: List[Dict[str, Any]],
         "optimized_operation_count": len(optimized_operations),
         "matched_structured_patterns": matched,
         "pattern_counts": dict(Counter(m["kernel_family"] for m in matched)),
-        "expected_custom_kernel_launches": predicted,
+        "selected_custom_kernels": selected,
+        "selected_concrete_kernels": sorted({k for item in selected for k in item["concrete_kernels"]}),
+        "expected_custom_kernel_launches": launches,
         "capabilities": status,
         "memory": memory,
         "statevector_preflight": statevector_preflight_data,
         "execution_cost_model": {
-            "scope": "full-state sweeps of structured layers",
+            "scope": "full-state sweeps planned by the native planner (virtual state)",
             "state_bytes": memory["state_bytes"],
             "bytes_per_sweep": sweep_bytes,
-            "predicted_sweeps": predicted,
-            "predicted_input_output_bytes": predicted * sweep_bytes,
+            "predicted_sweeps": run["sweeps"],
+            "predicted_input_output_bytes": run["sweeps"] * sweep_bytes,
         },
         "observed_custom_dispatches": [],
+        "observed_custom_kernel_launches": None,
+        "observed_sweeps": None,
         "execution_status": "planned",
         "synchronized": False,
     }
 
 
-def mark_synchronized(plan: Optional[Dict[str, Any]], observed_sweeps: Optional[int] = None) -> None:
+def record_observed_dispatches(plan: Optional[Dict[str, Any]], per_op: list) -> None:
+    """Kernels the real state launched per dispatched entry (its dispatch
+    log): the counterpart of record_custom_dispatch (execution.py:1032-1049)."""
     if plan is None:
         return
+    for idx, kinds in per_op:
+        plan["observed_custom_dispatches"].append(
+            {"optimized_operation_indices": idx, "concrete_kernels": list(kinds)})
+
+
+def mark_synchronized(plan: Optional[Dict[str, Any]], observed_sweeps: Optional[int] = None,
+                      flush_kernels: Optional[list] = None) -> None:
+    """execution.py:1094-1111: the state has been synchronized; `flush_kernels`
+    are the kernels the synchronize launched for deferred work."""
+    if plan is None:
+        return
+    from . import _native as nat
+    if flush_kernels is not None:
+        plan["observed_custom_dispatches"].append(
+            {"optimized_operation_indices": None, "concrete_kernels": list(flush_kernels)})
+    sweep_names = {nat.KERNEL_KINDS[k] for k in nat.SWEEP_KINDS}
+    plan["observed_custom_kernel_launches"] = sum(
+        1 for d in plan["observed_custom_dispatches"] for k in d["concrete_kernels"] if k in sweep_names)
     plan["execution_status"] = "synchronized"
     plan["synchronized"] = True
     if observed_sweeps is not None:

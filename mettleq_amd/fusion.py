@@ -74,19 +74,24 @@ def dependency_schedule(operations: List[Dict[str, Any]]) -> List[Dict[str, Any]
         raise RuntimeError("operation dependency graph contains a cycle")
     return scheduled
 
+_PLAN_CACHE: Dict[bytes, float] = {}
+
+
 def predicted_sweeps(fused: List[Dict[str, Any]], n: int) -> float:
-    """Cost model used to choose between fusion candidates: full-state sweeps
-    of the structured layers plus about one sweep per eight queued gates."""
-    from .execution import layer_sweeps
-    cost = 0.0
-    queued = 0
-    for op in fused:
-        if op["name"] == "_GATE":
-            queued += max(1, len(op["gates"]))
-            continue
-        cost += layer_sweeps(op, n) + (queued + 7) // 8
-        queued = 0
-    return cost + (queued + 7) // 8
+    """Cost model used to choose between fusion candidates: the full-state
+    sweeps the native planner launches for them (a virtual state, which plans
+    and defers exactly as a real one but launches nothing), flush included.
+    Results are cached per candidate (a repeated circuit is planned once)."""
+    import hashlib
+    import pickle
+    from .dispatch import planned_run
+    key = hashlib.sha1(pickle.dumps((n, fused), protocol=4)).digest()
+    hit = _PLAN_CACHE.get(key)
+    if hit is None:
+        if len(_PLAN_CACHE) > 4096:
+            _PLAN_CACHE.clear()
+        hit = _PLAN_CACHE[key] = float(planned_run(fused, n)["sweeps"])
+    return hit
 
 
 def fuse_operations(operations: List[Dict[str, Any]], n: int) -> List[Dict[str, Any]]:

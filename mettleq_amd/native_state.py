@@ -28,16 +28,34 @@ def _mat8(m) -> np.ndarray:
 class NativeState:
     """An n-qubit amplitude buffer on one GPU, initialised to |0...0>."""
 
-    def __init__(self, n_qubits: int, dtype: str = "complex64", device: int = 0):
+    def __init__(self, n_qubits: int, dtype: str = "complex64", device: int = 0, virtual: bool = False):
         if dtype not in DTYPES:
             raise ValueError(f"dtype must be one of {sorted(DTYPES)}")
         self.n = int(n_qubits)
         self.dtype = dtype
         self.code = DTYPES[dtype]
         self.device = int(device)
+        self.virtual = bool(virtual)
         self._h = ctypes.c_void_p()
-        nat.check(nat.lib().msv_state_create(self.n, self.code, self.device, ctypes.byref(self._h)),
-                  "state_create")
+        create = nat.lib().msv_state_create_virtual if self.virtual else nat.lib().msv_state_create
+        nat.check(create(self.n, self.code, self.device, ctypes.byref(self._h)), "state_create")
+
+    @classmethod
+    def planner(cls, n_qubits: int, dtype: str = "complex64", device: int = 0) -> "NativeState":
+        """A virtual state (msv_state_create_virtual): the native planner of
+        an n-qubit state with no device memory; layer calls plan, defer and
+        log their kernels without launching them."""
+        return cls(n_qubits, dtype, device, virtual=True)
+
+    def dispatch_log(self, reset: bool = False) -> list:
+        """Kernel kinds launched since the last reset, in order (nat.KERNEL_KINDS names)."""
+        cap = 1 << 16
+        buf = np.zeros(cap, dtype=np.int32)
+        total = ctypes.c_uint64(0)
+        nat.check(nat.lib().msv_state_dispatch_log(self.handle, nat.as_ptr(buf, ctypes.c_int32), cap,
+                                                   ctypes.byref(total), int(bool(reset))), "dispatch_log")
+        m = min(int(total.value), cap)
+        return [nat.KERNEL_KINDS.get(int(k), f"kind{int(k)}") for k in buf[:m]]
 
     # -- lifecycle ---------------------------------------------------------
     @property
