@@ -5,9 +5,15 @@ wire k of the device is core wire k (MSB first, pennylane.py:397), Ising
 angles are halved on entry (pennylane.py:394-396), GlobalPhase(phi) is
 exp(-i phi) I, and State / Probability / Expectation / Variance / Sample /
 Counts measurements are served from the device-resident statevector.
-Gradients stay with PennyLane's parameter shift (use_device_gradient=False,
-pennylane.py:216-223); the engine's own adjoint kernel is exposed through
-mettleq_amd.adjoint instead.
+By default gradients stay with PennyLane's parameter shift
+(use_device_gradient=False, pennylane.py:216-223). With
+diff_method="adjoint" or "device", a tape that is a rotation ansatz (RX /
+RY / RZ on trainable or fixed angles, fixed CZ / ControlledPhaseShift /
+IsingZZ) measured by one expectation of a Z-string sum is differentiated by
+the batched engine's adjoint pass (mettleq_amd.batched: energies and every
+trainable parameter's derivative in one launch); other tapes are refused
+for device derivatives, as PennyLane's own adjoint device refuses what it
+cannot differentiate.
 """
 from __future__ import annotations
 
@@ -85,10 +91,80 @@ class MettleQDevice(PennyLaneDevice):
     def name(self) -> str:
         return "mettleq"
 
+    _DEVICE_GRADIENTS = ("adjoint", "device")
+
     def setup_execution_config(self, config: Optional[ExecutionConfig] = None, circuit=None):
         config = ExecutionConfig() if config is None else config
-        return replace(config, convert_to_numpy=True, use_device_gradient=False,
+        device_grad = getattr(config, "gradient_method", None) in self._DEVICE_GRADIENTS
+        return replace(config, convert_to_numpy=True, use_device_gradient=device_grad,
                        use_device_jacobian_product=False, grad_on_execution=False)
+
+    def supports_derivatives(self, execution_config=None, circuit=None) -> bool:
+        if execution_config is not None and getattr(execution_config, "gradient_method", None) \
+                not in self._DEVICE_GRADIENTS:
+            return False
+        return circuit is None or self._batched_tape(circuit) is not None
+
+    def _batched_tape(self, tape):
+        """(ansatz structure, Z-string terms, constant, trainable values) of a
+        tape the batched engine differentiates, or None."""
+        from .. import batched
+        if len(tape.measurements) != 1 or not isinstance(tape.measurements[0], ExpectationMP):
+            return None
+        sentence = self._sentence(tape.measurements[0].obs)
+        if sentence is None:
+            return None
+        words, coeffs = [], []
+        for word, c in sentence.items():
+            words.append({self.wires.index(w): p for w, p in word.items()})
+            coeffs.append(complex(c))
+        tz = batched.z_terms(words, coeffs)
+        if tz is None:
+            return None
+        trainable = list(tape.trainable_params)
+        slot_of = {t: i for i, t in enumerate(trainable)}
+        values = np.zeros(len(trainable))
+        ops, counter = [], 0
+        for op in tape.operations:
+            params = []
+            for v in op.parameters:
+                val = self._parameter(v, op.name)
+                if counter in slot_of:
+                    values[slot_of[counter]] = val
+                    params.append(batched.ParamRef(slot_of[counter]))
+                else:
+                    params.append(val)
+                counter += 1
+            name = {"RX": "RX", "RY": "RY", "RZ": "RZ", "CZ": "CZ", "ControlledPhaseShift": "CPHASE",
+                    "IsingZZ": "ZZPHASE", "Identity": "I"}.get(op.name)
+            if name is None:
+                return None
+            if name in ("CZ", "CPHASE", "ZZPHASE") and any(isinstance(p, batched.ParamRef) for p in params):
+                return None
+            if name == "ZZPHASE":
+                params = [params[0] / 2.0]
+            ops.append({"name": name, "wires": [self.wires.index(w) for w in op.wires],
+                        "parameters": params})
+        st = batched.ansatz_structure(len(self.wires), ops, n_params=len(trainable))
+        if st is None:
+            return None
+        return st, tz[0], tz[1], values
+
+    def compute_derivatives(self, circuits, execution_config=None):
+        from .. import batched
+        jacs = []
+        for tape in circuits:
+            match = self._batched_tape(tape)
+            if match is None:
+                raise DeviceError("device derivatives need a rotation ansatz with one Z-string expectation")
+            st, terms, const, values = match
+            _, g = batched.shared_estimator().evaluate(st, terms, const, values[None, :], grads=True)
+            row = g[0]
+            jacs.append(float(row[0]) if len(row) == 1 else tuple(float(x) for x in row))
+        return tuple(jacs)
+
+    def execute_and_compute_derivatives(self, circuits, execution_config=None):
+        return self.execute(circuits, execution_config), self.compute_derivatives(circuits, execution_config)
 
     def _supports_operation(self, op) -> bool:
         return op.name in _PENNYLANE_TO_CORE or op.name in {"Identity", "GlobalPhase"}

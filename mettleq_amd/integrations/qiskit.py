@@ -262,10 +262,68 @@ class MettleQSamplerV2(BackendSamplerV2):
         super().__init__(backend=backend or MettleQBackend(**backend_options), options=options)
 
 
+def _affine_angle(expr, position):
+    """A gate angle as a float, or ParamRef(a * p + b) when it is an affine
+    function of one circuit parameter p; None otherwise."""
+    from ..batched import ParamRef
+    try:
+        return float(expr)
+    except (TypeError, ValueError):
+        pass
+    free = list(getattr(expr, "parameters", ()))
+    if len(free) != 1 or free[0] not in position:
+        return None
+    p = free[0]
+    try:
+        coeff = float(expr.gradient(p))  # a constant gradient: the angle is affine in p
+        offset = float(expr.bind({p: 0.0}))
+    except (TypeError, ValueError, AttributeError):
+        return None
+    return ParamRef(position[p], coeff, offset)
+
+
+def _parametric_operations(circuit, parameters):
+    """Canonical ops of a parameterized rotation circuit (rx / ry / rz with
+    affine angles, cz, cp and rzz with fixed angles), wire 0 = the most
+    significant qubit as in _translate_circuit; None for anything else."""
+    n = circuit.num_qubits
+    position = {p: i for i, p in enumerate(parameters)}
+    ops = []
+    for instruction in circuit.data:
+        op = instruction.operation
+        name = op.name
+        if getattr(op, "condition", None) is not None:
+            return None
+        if name in ("barrier", "id"):
+            continue
+        wires = [n - 1 - circuit.find_bit(q).index for q in instruction.qubits]
+        if name in ("rx", "ry", "rz"):
+            angle = _affine_angle(op.params[0], position)
+            if angle is None:
+                return None
+            ops.append({"name": name.upper(), "wires": wires, "parameters": [angle]})
+        elif name == "cz":
+            ops.append({"name": "CZ", "wires": wires, "parameters": []})
+        elif name in ("cp", "rzz"):
+            try:
+                theta = float(op.params[0])
+            except (TypeError, ValueError):
+                return None
+            ops.append({"name": "CPHASE" if name == "cp" else "ZZPHASE", "wires": wires,
+                        "parameters": [theta if name == "cp" else theta / 2.0]})
+        else:
+            return None
+    return ops
+
+
 class MettleQEstimatorV2(BaseEstimatorV2):
     """Exact Pauli-sum expectation values from the device statevector
-    (qiskit.py:906-1090): each Pauli term is one on-device parity
-    reduction, evaluated 16 terms per state pass."""
+    (qiskit.py:906-1090). A pub whose circuit is a rotation ansatz (rx / ry /
+    rz with affine angles, fixed cz / cp / rzz) and whose observables are
+    Z-string sums runs every binding in one batched-engine launch
+    (mettleq_amd.batched); any other pub runs one bound circuit per binding,
+    each Pauli term one on-device parity reduction (16 terms per state
+    pass)."""
 
     def __init__(self, *, backend: Optional[MettleQBackend] = None, default_precision: float = 0.0,
                  **backend_options):
@@ -278,9 +336,55 @@ class MettleQEstimatorV2(BaseEstimatorV2):
         job._submit()
         return job
 
+    def _batched_evs(self, pub):
+        """Expectation values of every binding through the batched engine, or
+        None when the pub is outside its form."""
+        from .. import batched
+        circuit, observables, values = pub.circuit, pub.observables, pub.parameter_values
+        if getattr(circuit, "num_clbits", 0):
+            circuit = circuit.remove_final_measurements(inplace=False)
+        parameters = list(circuit.parameters)
+        ops = _parametric_operations(circuit, parameters)
+        if ops is None:
+            return None
+        n = circuit.num_qubits
+        st = batched.ansatz_structure(n, ops, n_params=len(parameters))
+        if st is None:
+            return None
+        bshape = tuple(values.shape)
+        arr = (np.asarray(values.as_array(parameters), dtype=np.float64) if parameters
+               else np.zeros(bshape + (0,)))
+        shape = np.broadcast_shapes(tuple(observables.shape), bshape)
+        flat_vals = np.broadcast_to(arr, shape + (len(parameters),)).reshape(-1, len(parameters))
+        obs_ids = np.broadcast_to(np.arange(max(1, observables.size)).reshape(observables.shape),
+                                  shape).reshape(-1)
+        evs = np.zeros(flat_vals.shape[0], dtype=np.float64)
+        est = batched.shared_estimator()
+        for k in np.unique(obs_ids):
+            obs = observables[np.unravel_index(int(k), observables.shape)] if observables.shape else observables[()]
+            words, coeffs = [], []
+            for label, coeff in obs.items():
+                words.append({n - 1 - q: ch for q, ch in enumerate(reversed(label)) if ch != "I"})
+                coeffs.append(complex(coeff))
+            tz = batched.z_terms(words, coeffs)
+            if tz is None:
+                return None
+            sel = obs_ids == k
+            evs[sel] = est.evaluate(st, tz[0], tz[1], flat_vals[sel])
+        return evs.reshape(shape)
+
     def _run(self, pubs):
         results = []
         for pub in pubs:
+            try:
+                batched_evs = self._batched_evs(pub)
+            except (ValueError, NotImplementedError, MemoryError):
+                batched_evs = None
+            if batched_evs is not None:
+                results.append(PubResult(
+                    DataBin(evs=batched_evs, stds=np.zeros_like(batched_evs), shape=batched_evs.shape),
+                    metadata={"target_precision": pub.precision, "mettleq_path": "batched_adjoint_engine"}))
+                continue
             circuit, observables, params = pub.circuit, pub.observables, pub.parameter_values
             bound = params.bind_all(circuit)
             shape = np.broadcast_shapes(observables.shape, bound.shape)
@@ -309,7 +413,8 @@ class MettleQEstimatorV2(BaseEstimatorV2):
                 for dev in cache.values():
                     dev.close()
             results.append(PubResult(DataBin(evs=evs, stds=stds, shape=shape),
-                                     metadata={"target_precision": pub.precision}))
+                                     metadata={"target_precision": pub.precision,
+                                               "mettleq_path": "per_binding_statevector"}))
         return PrimitiveResult(results, metadata={"version": 2})
 
 

@@ -141,6 +141,11 @@ def test_torch_autograd_function(gpu):
         for b in range(3):
             want = O.ansatz_param_shift(5, gates, params[b].detach().numpy(), terms)
             assert np.max(np.abs(params.grad[b].numpy() - weights[b].item() * want)) < 3 * TOL_G
+        # inference: no gradient requested, the forward pass alone, same energies
+        with torch.no_grad():
+            e_inf = ansatz_energy(params.detach(), a)
+        assert not e_inf.requires_grad
+        assert np.max(np.abs(e_inf.numpy() - energy.detach().numpy())) < 1e-12
 
 
 def test_invalid_arguments(gpu):
