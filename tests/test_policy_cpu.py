@@ -1,0 +1,95 @@
+"""Execution selection and SDK engine recommendation for one MI355X
+(mettleq_amd.planning, mettleq_amd.integrations.policy), mirroring the
+reference's planning and policy tests (tests/mlxQExecutionPlanningTest.py:
+19-113, tests/mlxQSdkPolicyTest.py) with this engine's calibration."""
+import pytest
+
+import mettleq_amd.execution as execution
+import mettleq_amd.planning as planning
+from mettleq_amd.integrations.policy import profile_operations, recommend_sdk_engine
+
+GIB = 1 << 30
+
+
+@pytest.fixture(autouse=True)
+def _limits(monkeypatch):
+    monkeypatch.setattr(execution, "_device_info", lambda: {
+        "device_name": "MI355X (test limits)", "architecture": "gfx950",
+        "max_buffer_length": 288 * 10**9, "max_recommended_working_set_size": 280 * GIB})
+
+
+def _layer(n):
+    return ([{"name": "U3", "wires": [q], "parameters": [0.1, 0.2, 0.3]} for q in range(n)]
+            + [{"name": "CNOT", "wires": [q, q + 1], "parameters": []} for q in range(0, n - 1, 2)])
+
+
+def test_cpu_below_and_gpu_at_the_crossover(monkeypatch):
+    monkeypatch.setattr(planning, "_gpu_available", lambda: True)
+    small = planning.select_execution(5, [], statevector_gpu_min_qubits=8)
+    large = planning.select_execution(8, [], statevector_gpu_min_qubits=8)
+    assert small.selected_method == "statevector" and small.selected_device == "cpu"
+    assert large.selected_device == "gpu"
+    assert "not combined into a speed claim" in large.cpu_gpu_policy
+
+
+def test_default_crossover_is_the_mi355x_calibration(monkeypatch):
+    monkeypatch.setattr(planning, "_gpu_available", lambda: True)
+    assert planning.DEFAULT_STATEVECTOR_GPU_MIN_QUBITS == 14
+    assert planning.select_execution(13, []).selected_device == "cpu"
+    assert planning.select_execution(14, []).selected_device == "gpu"
+
+
+def test_no_gpu_and_explicit_requests(monkeypatch):
+    monkeypatch.setattr(planning, "_gpu_available", lambda: False)
+    assert planning.select_execution(20, []).selected_device == "cpu"
+    assert "automatic_gpu_unavailable" in planning.select_execution(20, []).reason
+    with pytest.raises(RuntimeError):
+        planning.select_execution(20, [], device="gpu")
+    with pytest.raises(ValueError, match="method must be"):
+        planning.select_execution(4, [], method="hybrid")
+    with pytest.raises(ValueError, match="device must be"):
+        planning.select_execution(4, [], device="neural-engine")
+    with pytest.raises(ValueError, match="statevector method only"):
+        planning.select_execution(4, [], method="mps")
+
+
+def test_refused_statevector_keeps_the_exact_request(monkeypatch):
+    monkeypatch.setattr(planning, "_gpu_available", lambda: True)
+    sel = planning.select_execution(38, [])
+    assert sel.selected_method == "statevector"
+    assert sel.reason.startswith("automatic_exact_request_has_no_allowed_approximate_fallback")
+    assert sel.statevector_preflight["decision"] == "refused"
+
+
+def test_profile_reports_gate_mix_topology_and_planned_launches():
+    profile = profile_operations(8, _layer(8))
+    assert profile.operations == 12
+    assert profile.single_qubit_operations == 8
+    assert profile.topology_class == "nearest_neighbor"
+    assert profile.estimated_fusion_coverage == pytest.approx(1.0)
+    # the planner: one 1q-layer product write from |0...0>, the CNOTs as one
+    # deferred permutation flushed at the end
+    assert profile.planned_kernel_launches >= 1
+
+
+def test_small_circuits_stay_on_the_sdk_cpu_engine():
+    assert recommend_sdk_engine("qiskit", 12, _layer(12)).engine == "qiskit_aer_cpu_statevector"
+    assert recommend_sdk_engine("pennylane", 12, _layer(12)).engine == "pennylane_lightning_cpu"
+
+
+def test_gpu_above_the_crossover_and_double_precision_stays_on_gpu():
+    d = recommend_sdk_engine("qiskit", 22, _layer(22), output_contract="expectation")
+    assert d.engine == "mettleq_gpu_statevector"
+    assert d.adjusted_gpu_crossover_qubits <= 16
+    dd = recommend_sdk_engine("qiskit", 24, _layer(24), precision="double")
+    assert dd.engine == "mettleq_gpu_statevector_complex128"
+
+
+def test_mps_requests():
+    assert recommend_sdk_engine("qiskit", 40, _layer(40), method="mps").engine == "qiskit_aer_cpu_mps"
+    assert recommend_sdk_engine("pennylane", 40, _layer(40), method="mps").engine == "refused"
+
+
+def test_refusal_and_bounded_local_fallback():
+    assert recommend_sdk_engine("qiskit", 38, _layer(38)).engine == "refused"
+    assert recommend_sdk_engine("qiskit", 38, _layer(38), allow_approximation=True).engine == "qiskit_aer_cpu_mps"
