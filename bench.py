@@ -380,7 +380,11 @@ def bench_vqe16(args, quick=False):
             a.evaluate(params, grads=False)
             fwd.append(time.perf_counter() - t0)
         info = a.info(B)
+        sw = a.sweep_times(B, grads=True, reps=3)
+        a.evaluate(params)
     t_adj = float(np.median(ts))
+    fwd_frac = sw["forward_bytes"] / (sw["forward_ms"] * 1e-3) / (PEAK_GBPS * 1e9)
+    bwd_frac = sw["backward_bytes"] / (sw["backward_ms"] * 1e-3) / (PEAK_GBPS * 1e9)
     n_plain = 1 if quick else 2
     t0 = time.perf_counter()
     plain = [_vqe_plain_gradients(params[b].copy(), n, L) for b in range(n_plain)]
@@ -394,6 +398,11 @@ def bench_vqe16(args, quick=False):
         "forward_only_ms": round(1e3 * float(np.median(fwd)), 3),
         "first_call_s_autotune_and_graph_capture": round(first, 3),
         "plan": info,
+        "sweeps": {"forward_ms": round(sw["forward_ms"], 3), "backward_ms": round(sw["backward_ms"], 3),
+                   "forward_GBps": round(sw["forward_bytes"] / (sw["forward_ms"] * 1e-3) / 1e9, 1),
+                   "backward_GBps": round(sw["backward_bytes"] / (sw["backward_ms"] * 1e-3) / 1e9, 1),
+                   "forward_hbm_frac": round(fwd_frac, 3), "backward_hbm_frac": round(bwd_frac, 3),
+                   "bytes": "forward sweep 2 x B x 2^16 x 8 B (phi read + write); backward 4 x (phi and lambda)"},
         "plain_per_circuit_gradients_per_s": round(params.shape[1] / t_plain, 1),
         "plain_bindings_timed": n_plain,
         "speedup_vs_plain": round((B * params.shape[1] / t_adj) / (params.shape[1] / t_plain), 1),

@@ -331,6 +331,13 @@ int msv_ansatz_evaluate(msv_ansatz *a, const double *params, int batch, int n_pa
  * out[2] forward sweeps, out[3] backward sweeps, out[4] tile ops,
  * out[5] autotuned time in microseconds. */
 int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n_out);
+/* Times the cached plan's sweeps alone (no graph), averaged over `reps`:
+ * out_ms[0] the forward sweeps, out_ms[1] the backward sweeps (0 without
+ * gradients), measured with HIP events on the ansatz's stream. The states
+ * are left holding no meaningful values until the next evaluation. No
+ * counterpart in the reference: it is the measurement behind the sweeps'
+ * HBM fractions. */
+int msv_ansatz_time_sweeps(msv_ansatz *a, int batch, int with_grad, int reps, double *out_ms);
 
 /* ---- tuning knobs ------------------------------------------------------- */
 /* Sets an integer tuning parameter (MSV_TUNE_*); returns MSV_E_INVALID for

@@ -190,6 +190,7 @@ SIGNATURES = {
     "msv_ansatz_observable": (_I, [_P, _U64P, _DP, _I]),
     "msv_ansatz_evaluate": (_I, [_P, _DP, _I, _I, _DP, _DP]),
     "msv_ansatz_info": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_int64), _I]),
+    "msv_ansatz_time_sweeps": (_I, [_P, _I, _I, _I, ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None

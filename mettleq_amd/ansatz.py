@@ -96,6 +96,22 @@ class LayeredAnsatz:
         return {"tile_local_bits": out[0], "blocks_per_cu": out[1], "forward_sweeps": out[2],
                 "backward_sweeps": out[3], "tile_ops": out[4], "autotuned_ms": out[5] / 1000.0}
 
+    def sweep_times(self, batch: int, grads: bool = True, reps: int = 3) -> dict:
+        """Forward and backward sweeps of the cached plan for `batch`, timed
+        alone with HIP events (ms per evaluation), with their algorithmic
+        HBM bytes: a forward sweep reads and writes every binding's state
+        once (2 x batch x 2^n x 8 B), a backward sweep both phi and lambda.
+        Leaves the states without meaningful values: evaluate again before
+        reading results."""
+        out = (ctypes.c_double * 2)()
+        nat.check(nat.lib().msv_ansatz_time_sweeps(self._h, int(batch), int(bool(grads)), int(reps), out),
+                  "ansatz sweep timing")
+        info = self.info(batch, grads)
+        state = float(batch) * 8.0 * 2.0 ** max(self.n_qubits, 10)
+        return {"forward_ms": out[0], "backward_ms": out[1],
+                "forward_bytes": 2.0 * state * info["forward_sweeps"],
+                "backward_bytes": 4.0 * state * info["backward_sweeps"]}
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             nat.lib().msv_ansatz_destroy(self._h)

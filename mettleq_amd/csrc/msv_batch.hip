@@ -60,32 +60,28 @@ constexpr int kMaxTerms = 64;  // observable terms
 constexpr int kMinBits = 10;   // index bits per binding (>= 6 thread bits + kBRB)
 
 enum BOpKind : uint32_t {
-  BOP_ROT = 1,      // phi <- R phi (R = the binding's rotation `gate`)
-  BOP_ROT_ADJ = 2,  // phi <- R^dagger phi, lambda <- R^dagger lambda
-  BOP_GRAD = 3,     // acc[slot] += per-thread Im <lambda|P|phi>, P on register bit rpos
-  BOP_DIAG = 4,     // diagonal on index bits a (MSB) and b of phi (and lambda)
-  BOP_CZ = 5,       // product of CZ signs over a bond list (phi and lambda)
-  BOP_FLUSH = 6,    // wave sums of acc[0..count) -> partials of gates[0..count)
+  BOP_STAGE = 1,  // rotations of generator `gen`, at most one per register bit (see bstage)
+  BOP_ROT = 2,    // rotation record of a stage (never a top-level op)
+  BOP_DIAG = 4,   // diagonal on index bits a (MSB) and b of phi (and lambda)
+  BOP_CZ = 5,     // product of CZ signs over a bond list (phi and lambda)
 };
 
-constexpr int kMaxAcc = 8;     // gradient accumulators per flush
 constexpr int kMaxCzList = 16; // non register-register CZ bonds per op
-constexpr int kMaxSweepGates = 256;
 
 // Device-resident op record (a reverse layer of a 16-qubit ansatz has
-// ~60 ops: more than the kernel-argument space).
+// ~40 ops: more than the kernel-argument space).
 struct BOp {
   uint32_t kind;
-  int32_t rpos;    // ROT/GRAD: register bit
-  uint32_t gen;    // ROT/GRAD: 0 = X, 1 = Y, 2 = Z
-  uint32_t gate;   // ROT/GRAD: gate number minus the sweep's first gate
-  uint32_t count;  // GRAD: accumulator slot; FLUSH: slots; CZ: list entries
+  int32_t rpos;    // ROT: register bit
+  uint32_t gen;    // STAGE: 0 = X, 1 = Y, 2 = Z
+  uint32_t gate;   // ROT: rotation number minus the sweep's first rotation
+  uint32_t count;  // ROT: gradient row (the rotation number); CZ: list entries
   int32_t ra, rb;  // DIAG: register bits of index bits a / b, or -1
   uint32_t qmask;  // CZ: parity of the register-register bonds, bit r = register index r
   uint64_t amask, bmask;  // DIAG: index masks of a / b when not register bits
   float d[8];             // DIAG: d(a,b) for ab = 00, 01, 10, 11 as (re, im)
-  // FLUSH: gate of slot k; CZ: bond k = a | b << 8 | kind << 16 (kind 0:
-  // index bits a, b; kind 1: register bit a, index bit b)
+  // STAGE: list[j] = the rotation record on register bit j, or ~0; CZ: bond k = a | b << 8 | kind << 16 (kind 0: index bits a, b; kind 1:
+  // register bit a, index bit b)
   uint32_t list[kMaxCzList];
 };
 
@@ -96,14 +92,14 @@ struct BArgs {
   uint32_t nseg;
   uint32_t nruns;
   uint32_t n_gates;  // rotations per binding
-  uint32_t mat_lo;   // this sweep's rotations: [mat_lo, mat_lo + mat_cnt)
+  uint32_t mat_lo;   // this sweep's rotations: [mat_lo, mat_lo + mat_cnt) of the binding's table
   uint32_t mat_cnt;
   uint32_t pad;
   uint32_t run_src[kMaxRuns];
   uint32_t run_width[kMaxRuns];
   uint32_t run_dst[kMaxRuns];
   TileSeg seg[kMaxSegs];
-  uint32_t op_begin[kMaxSegs];
+  uint32_t op_begin[kMaxSegs];  // segment s: top-level ops [op_begin, op_end)
   uint32_t op_end[kMaxSegs];
 };
 
@@ -121,91 +117,127 @@ struct ObsArgs {
 // device code
 // ---------------------------------------------------------------------------
 
-// Rotation exp(-i theta P / 2) with (c, s) = (cos theta/2, sin theta/2) on
-// register bit J, P = X/Y/Z by GEN; DAG applies the inverse (s -> -s).
-// 8 FMAs per pair; the generator is a template argument so that the pair
-// loop has no branches.
-template <int RB, int J, bool DAG, int GEN>
-__device__ __forceinline__ void brot(float2 *a, float c, float s) {
-  if (DAG) s = -s;
+// Rotations act in place on the register pairs of register bit J, with
+// coefficients (t, s) = (tan(phi/2), sin(phi)) from the binding's table
+// (built by k_bmats), phi = theta/2 reduced to [-pi/2, pi/2]: theta/2 - phi
+// is a multiple of pi, a global sign. Global signs cancel in <psi|H|psi>,
+// and in Im <lambda|P|phi> of the adjoint pass because phi and lambda carry
+// the same one, so neither the energies nor the gradients see them.
+// Every form is three shears (Paeth) of fused multiply-adds that update
+// one register from another, with no temporaries ((1 - t s) = cos phi and
+// t (1 + cos phi) = s):
+//   RY = [[c, -s], [s, c]]:           x0 -= t x1; x1 += s x0; x0 -= t x1;
+//   RX = [[c, -i s], [-i s, c]]:      x0 += -i t x1; x1 += -i s x0; x0 += -i t x1;
+//   RZ = diag(e^{-i phi}, e^{i phi}): each amplitude's (re, im) plane rotated
+//                                     by -phi (x0) and +phi (x1).
+// 6 FMAs per pair; the inverse negates (t, s).
+template <int RB, int J>
+__device__ __forceinline__ void bshear_y(float2 *a, float t, float s) {
 #pragma unroll
   for (int r = 0; r < (1 << RB); ++r) {
     if (r & (1 << J)) continue;
-    const int r1 = r | (1 << J);
-    const float2 v0 = a[r], v1 = a[r1];
-    if (GEN == 0) {  // RX = [[c, -i s], [-i s, c]]
-      a[r].x = c * v0.x + s * v1.y;
-      a[r].y = c * v0.y - s * v1.x;
-      a[r1].x = c * v1.x + s * v0.y;
-      a[r1].y = c * v1.y - s * v0.x;
-    } else if (GEN == 1) {  // RY = [[c, -s], [s, c]]
-      a[r].x = c * v0.x - s * v1.x;
-      a[r].y = c * v0.y - s * v1.y;
-      a[r1].x = s * v0.x + c * v1.x;
-      a[r1].y = s * v0.y + c * v1.y;
-    } else {  // RZ = diag(c - i s, c + i s)
-      a[r].x = c * v0.x + s * v0.y;
-      a[r].y = c * v0.y - s * v0.x;
-      a[r1].x = c * v1.x - s * v1.y;
-      a[r1].y = c * v1.y + s * v1.x;
-    }
+    float2 &x0 = a[r];
+    float2 &x1 = a[r | (1 << J)];
+    x0.x = fmaf(-t, x1.x, x0.x);
+    x0.y = fmaf(-t, x1.y, x0.y);
+    x1.x = fmaf(s, x0.x, x1.x);
+    x1.y = fmaf(s, x0.y, x1.y);
+    x0.x = fmaf(-t, x1.x, x0.x);
+    x0.y = fmaf(-t, x1.y, x0.y);
   }
 }
 
-template <int RB, bool DAG, int GEN>
-__device__ __forceinline__ void brot_j(float2 *a, float c, float s, int j) {
-  switch (j) {
-    case 0: brot<RB, 0, DAG, GEN>(a, c, s); break;
-    case 1: brot<RB, 1, DAG, GEN>(a, c, s); break;
-    case 2: brot<RB, 2, DAG, GEN>(a, c, s); break;
-    case 3: brot<RB, 3, DAG, GEN>(a, c, s); break;
-    default: break;
+template <int RB, int J>
+__device__ __forceinline__ void bshear_x(float2 *a, float t, float s) {
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    float2 &x0 = a[r];
+    float2 &x1 = a[r | (1 << J)];
+    x0.x = fmaf(t, x1.y, x0.x);
+    x0.y = fmaf(-t, x1.x, x0.y);
+    x1.x = fmaf(s, x0.y, x1.x);
+    x1.y = fmaf(-s, x0.x, x1.y);
+    x0.x = fmaf(t, x1.y, x0.x);
+    x0.y = fmaf(-t, x1.x, x0.y);
   }
 }
 
-template <int RB, bool DAG>
-__device__ __forceinline__ void brot_dispatch(float2 *a, uint32_t gen, float c, float s, int j) {
-  if (gen == 0u) brot_j<RB, DAG, 0>(a, c, s, j);
-  else if (gen == 1u) brot_j<RB, DAG, 1>(a, c, s, j);
-  else brot_j<RB, DAG, 2>(a, c, s, j);
+template <int RB, int J>
+__device__ __forceinline__ void bshear_z(float2 *a, float t, float s) {
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    float2 &x0 = a[r];
+    float2 &x1 = a[r | (1 << J)];
+    x0.x = fmaf(t, x0.y, x0.x);  // x0 e^{-i phi}
+    x1.x = fmaf(-t, x1.y, x1.x);  // x1 e^{i phi}
+    x0.y = fmaf(-s, x0.x, x0.y);
+    x1.y = fmaf(s, x1.x, x1.y);
+    x0.x = fmaf(t, x0.y, x0.x);
+    x1.x = fmaf(-t, x1.y, x1.x);
+  }
 }
 
-// Per-thread partial of Im <lambda|P|phi> over the register pairs of bit J.
+template <int RB, int J, int GEN, bool DAG>
+__device__ __forceinline__ void brot(float2 *a, float2 ts) {
+  const float t = DAG ? -ts.x : ts.x;
+  const float s = DAG ? -ts.y : ts.y;
+  if (GEN == 2) bshear_z<RB, J>(a, t, s);
+  else if (GEN == 1) bshear_y<RB, J>(a, t, s);
+  else bshear_x<RB, J>(a, t, s);
+}
+
+// Per-thread partial of Im <lambda|P|phi> over the register pairs of bit J
+// (P = X / Y / Z by GEN), as two FMA chains.
 template <int RB, int J, int GEN>
 __device__ __forceinline__ float bgrad(const float2 *a, const float2 *l) {
-  float g = 0.f;
+  float g0 = 0.f, g1 = 0.f;
 #pragma unroll
   for (int r = 0; r < (1 << RB); ++r) {
     if (r & (1 << J)) continue;
     const int r1 = r | (1 << J);
     const float2 a0 = a[r], a1 = a[r1], l0 = l[r], l1 = l[r1];
     if (GEN == 2) {  // Z: Im(conj(l0) a0) - Im(conj(l1) a1)
-      g += (l0.x * a0.y - l0.y * a0.x) - (l1.x * a1.y - l1.y * a1.x);
+      g0 = fmaf(l0.x, a0.y, g0);
+      g0 = fmaf(-l0.y, a0.x, g0);
+      g1 = fmaf(-l1.x, a1.y, g1);
+      g1 = fmaf(l1.y, a1.x, g1);
     } else if (GEN == 0) {  // X: Im(conj(l0) a1) + Im(conj(l1) a0)
-      g += (l0.x * a1.y - l0.y * a1.x) + (l1.x * a0.y - l1.y * a0.x);
+      g0 = fmaf(l0.x, a1.y, g0);
+      g0 = fmaf(-l0.y, a1.x, g0);
+      g1 = fmaf(l1.x, a0.y, g1);
+      g1 = fmaf(-l1.y, a0.x, g1);
     } else {  // Y: Re(conj(l1) a0) - Re(conj(l0) a1)
-      g += (l1.x * a0.x + l1.y * a0.y) - (l0.x * a1.x + l0.y * a1.y);
+      g0 = fmaf(l1.x, a0.x, g0);
+      g0 = fmaf(l1.y, a0.y, g0);
+      g1 = fmaf(-l0.x, a1.x, g1);
+      g1 = fmaf(-l0.y, a1.y, g1);
     }
   }
-  return g;
+  return g0 + g1;
 }
 
-template <int RB, int GEN>
-__device__ __forceinline__ float bgrad_j(const float2 *a, const float2 *l, int j) {
-  switch (j) {
-    case 0: return bgrad<RB, 0, GEN>(a, l);
-    case 1: return bgrad<RB, 1, GEN>(a, l);
-    case 2: return bgrad<RB, 2, GEN>(a, l);
-    case 3: return bgrad<RB, 3, GEN>(a, l);
-    default: return 0.f;
-  }
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-template <int RB>
-__device__ __forceinline__ float bgrad_dispatch(const float2 *a, const float2 *l, uint32_t gen, int j) {
-  if (gen == 0u) return bgrad_j<RB, 0>(a, l, j);
-  if (gen == 1u) return bgrad_j<RB, 1>(a, l, j);
-  return bgrad_j<RB, 2>(a, l, j);
+// Sum over the 64 lanes in a fixed order, uniform result: sums within each
+// 16-lane row by DPP (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror: after each step every lane holds the sum of
+// its 2, 4, 8, 16 lanes), then the four row sums in lane order.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v += dpp_f<0xB1>(v);
+  v += dpp_f<0x4E>(v);
+  v += dpp_f<0x141>(v);
+  v += dpp_f<0x140>(v);
+  const int b = __float_as_int(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(b, 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(b, 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(b, 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(b, 48));
+  return (r0 + r1) + (r2 + r3);
 }
 
 template <int RB>
@@ -225,12 +257,6 @@ __device__ __forceinline__ void bdiag(float2 *a, int ra, int rb, uint64_t amask,
   }
 }
 
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -240,36 +266,99 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Read-only tables indexed by wave-uniform values: through the constant
 // address space the loads are scalar (s_load), not per-lane vector loads.
 typedef const __attribute__((address_space(4))) BOp *BOpCP;
+typedef const __attribute__((address_space(4))) float *FCP;
 
-// The adjoint form holds phi and lambda (64 VGPRs of amplitudes, 160 in
-// all): both states exchange through one LDS buffer in turn, so its
-// registers, not the LDS, set the blocks per CU (three; holding it to 128
-// VGPRs for four spills and was 2.6x slower).
+// A fixed-phase op (CZ sign products or a two-qubit diagonal) on phi (and
+// lambda).
+template <int RB, bool ADJ>
+__device__ __forceinline__ void bphase_op(BOpCP opp, float2 *a, float2 *l, uint64_t gb) {
+  constexpr int R = 1 << RB;
+  if (opp->kind == BOP_CZ) {
+    uint32_t c = 0, lin = 0;
+    for (uint32_t k = 0; k < opp->count; ++k) {
+      const uint32_t e = opp->list[k];
+      const uint32_t ba = e & 255u, bb = (e >> 8) & 255u;
+      const uint32_t tb = static_cast<uint32_t>(gb >> bb) & 1u;
+      if (e >> 16) lin ^= tb << ba;
+      else c ^= static_cast<uint32_t>(gb >> ba) & tb;
+    }
+    const uint32_t qm = opp->qmask;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t par = (c ^ __popc(static_cast<uint32_t>(r) & lin) ^ (qm >> r)) & 1u;
+      a[r].x = par ? -a[r].x : a[r].x;
+      a[r].y = par ? -a[r].y : a[r].y;
+      if (ADJ) {
+        l[r].x = par ? -l[r].x : l[r].x;
+        l[r].y = par ? -l[r].y : l[r].y;
+      }
+    }
+  } else {  // BOP_DIAG
+    float dd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dd[k] = opp->d[k];
+    bdiag<RB>(a, opp->ra, opp->rb, opp->amask, opp->bmask, dd, gb);
+    if (ADJ) bdiag<RB>(l, opp->ra, opp->rb, opp->amask, opp->bmask, dd, gb);
+  }
+}
+
+// The stage's rotation on register bit J, if it has one (list[J] = its
+// record, or ~0). Forward: phi <- R phi. Adjoint: the gradient partial of
+// the rotation (one wave sum, stored by lane 0 at the rotation's row),
+// then phi <- R^dagger phi, lambda <- R^dagger lambda.
+template <int RB, int J, int GEN, bool ADJ>
+__device__ __forceinline__ void bstage_bit(BOpCP st, BOpCP cops, FCP mrow, float2 *a, float2 *l,
+                                           double *__restrict__ part_t, uint64_t row_stride) {
+  const uint32_t o = st->list[J];
+  if (o == ~0u) return;
+  const BOpCP opp = cops + o;
+  const float2 ts = make_float2(mrow[2u * opp->gate], mrow[2u * opp->gate + 1u]);
+  if (ADJ) {
+    const float g = wave_sum_dpp(bgrad<RB, J, GEN>(a, l));
+    if ((threadIdx.x & 63u) == 0u) part_t[static_cast<uint64_t>(opp->count) * row_stride] = g;
+    brot<RB, J, GEN, true>(a, ts);
+    brot<RB, J, GEN, true>(l, ts);
+  } else {
+    brot<RB, J, GEN, false>(a, ts);
+  }
+}
+
+// A stage: at most one rotation per register bit, all of one generator
+// (rotations on different qubits commute), as straight-line code.
+template <int RB, int GEN, bool ADJ>
+__device__ __forceinline__ void bstage(BOpCP st, BOpCP cops, FCP mrow, float2 *a, float2 *l,
+                                       double *__restrict__ part_t, uint64_t row_stride) {
+  bstage_bit<RB, 0, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
+  bstage_bit<RB, 1, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
+  bstage_bit<RB, 2, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
+  bstage_bit<RB, 3, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
+}
+
+// One sweep over every binding's tiles. Per segment the ops
+// [op_begin, op_end): fixed phases (backward: before the rotations), the
+// rotation stages, fixed phases (forward: after the rotations).
+// The adjoint form holds phi and lambda (64 VGPRs of amplitudes): both
+// states exchange through one LDS buffer in turn.
 template <int RB, bool ADJ>
 __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 *__restrict__ lam,
                                                const BOp *__restrict__ ops,
                                                const float2 *__restrict__ mats,
                                                double *__restrict__ part, const BArgs A) {
+  static_assert(RB == 4, "the rotation dispatch covers register bits 0..3");
   constexpr int R = 1 << RB;
   extern __shared__ unsigned char smem_raw[];
   float2 *lds_a = reinterpret_cast<float2 *>(smem_raw);
-  // the tile's rotations (cos, sin) staged once per tile
-  float2 *lds_m = lds_a + (A.nseg > 1 ? (1u << A.nlocal) : 0u);
   const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const BOpCP cops = (BOpCP)(ops);
+  // gradient partials: row g at part[(g * n_tiles + tile) * kBWaves + wave]
+  const uint64_t row_stride = A.n_tiles * kBWaves;
   for (uint64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
     const uint64_t base = tile_base(A, tile);
-    {
-      const float2 *src = mats + (base >> A.n) * A.n_gates + A.mat_lo;
-      __syncthreads();  // previous tile's readers are done
-      for (uint32_t i = tid; i < A.mat_cnt; i += blockDim.x) lds_m[i] = src[i];
-      __syncthreads();
-    }
+    double *const part_t = part + tile * kBWaves + wave;
+    // the binding's coefficients (tiles never straddle bindings)
+    const FCP mrow = (FCP)(mats) + 2u * ((base >> A.n) * A.n_gates + A.mat_lo);
     float2 a[R], l[R];
-    float acc[kMaxAcc];
-#pragma unroll
-    for (int k = 0; k < kMaxAcc; ++k) acc[k] = 0.f;
     for (uint32_t s = 0; s < A.nseg; ++s) {
       const TileSeg &sg = A.seg[s];
       uint64_t gstr[RB];
@@ -293,69 +382,13 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
       }  // ADJ: phi and lambda were read at the end of the previous segment
       for (uint32_t o = A.op_begin[s]; o < A.op_end[s]; ++o) {
         const BOpCP opp = cops + o;
-        const uint32_t kind = opp->kind;
-        if (kind == BOP_ROT) {
-          const float2 cs = lds_m[opp->gate];
-          brot_dispatch<RB, false>(a, opp->gen, cs.x, cs.y, opp->rpos);
-        } else if (kind == BOP_ROT_ADJ) {
-          const float2 cs = lds_m[opp->gate];
-          brot_dispatch<RB, true>(a, opp->gen, cs.x, cs.y, opp->rpos);
-          if (ADJ) brot_dispatch<RB, true>(l, opp->gen, cs.x, cs.y, opp->rpos);
-        } else if (kind == BOP_GRAD) {
-          if (ADJ) {
-            const float g = bgrad_dispatch<RB>(a, l, opp->gen, opp->rpos);
-            switch (opp->count) {
-              case 0: acc[0] += g; break;
-              case 1: acc[1] += g; break;
-              case 2: acc[2] += g; break;
-              case 3: acc[3] += g; break;
-              case 4: acc[4] += g; break;
-              case 5: acc[5] += g; break;
-              case 6: acc[6] += g; break;
-              default: acc[7] += g; break;
-            }
-          }
-        } else if (kind == BOP_FLUSH) {
-          if (ADJ) {
-            const uint32_t cnt = opp->count;
-            float w[kMaxAcc];
-#pragma unroll
-            for (int k = 0; k < kMaxAcc; ++k) w[k] = (k < static_cast<int>(cnt)) ? wave_sum_f(acc[k]) : 0.f;
-            if ((tid & 63u) == 0u) {
-#pragma unroll
-              for (int k = 0; k < kMaxAcc; ++k)
-                if (k < static_cast<int>(cnt))
-                  part[(static_cast<uint64_t>(opp->list[k]) * A.n_tiles + tile) * kBWaves + wave] = w[k];
-            }
-#pragma unroll
-            for (int k = 0; k < kMaxAcc; ++k) acc[k] = 0.f;
-          }
-        } else if (kind == BOP_CZ) {
-          uint32_t c = 0, lin = 0;
-          for (uint32_t k = 0; k < opp->count; ++k) {
-            const uint32_t e = opp->list[k];
-            const uint32_t ba = e & 255u, bb = (e >> 8) & 255u;
-            const uint32_t tb = static_cast<uint32_t>(gb >> bb) & 1u;
-            if (e >> 16) lin ^= tb << ba;
-            else c ^= static_cast<uint32_t>(gb >> ba) & tb;
-          }
-          const uint32_t qm = opp->qmask;
-#pragma unroll
-          for (int r = 0; r < R; ++r) {
-            const uint32_t par = (c ^ __popc(static_cast<uint32_t>(r) & lin) ^ (qm >> r)) & 1u;
-            a[r].x = par ? -a[r].x : a[r].x;
-            a[r].y = par ? -a[r].y : a[r].y;
-            if (ADJ) {
-              l[r].x = par ? -l[r].x : l[r].x;
-              l[r].y = par ? -l[r].y : l[r].y;
-            }
-          }
-        } else if (kind == BOP_DIAG) {
-          float dd[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) dd[k] = opp->d[k];
-          bdiag<RB>(a, opp->ra, opp->rb, opp->amask, opp->bmask, dd, gb);
-          if (ADJ) bdiag<RB>(l, opp->ra, opp->rb, opp->amask, opp->bmask, dd, gb);
+        if (opp->kind == BOP_STAGE) {
+          const uint32_t gen = opp->gen;
+          if (gen == 2u) bstage<RB, 2, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+          else if (gen == 1u) bstage<RB, 1, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+          else bstage<RB, 0, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+        } else {
+          bphase_op<RB, ADJ>(opp, a, l, gb);
         }
       }
       if (s + 1 == A.nseg) {
@@ -402,17 +435,18 @@ __global__ void __launch_bounds__(256) k_binit(float2 *__restrict__ phi, uint64_
     phi[i] = make_float2((i & mask) == 0 ? 1.f : 0.f, 0.f);
 }
 
-// mats[b][g] = (cos, sin) of half the binding's angle params[b][pidx[g]]
+// mats[b][g] = (tan(phi/2), sin(phi)) for rotation g of binding b, with
+// phi = theta/2 reduced to [-pi/2, pi/2] and theta = params[b][pidx[g]]
+// (the coefficients of brot).
 __global__ void __launch_bounds__(256) k_bmats(const double *__restrict__ params, int n_params,
-                                               const int32_t *__restrict__ pidx, int n_gates,
-                                               int batch, float2 *__restrict__ mats) {
+                                               const int32_t *__restrict__ pidx, int n_gates, int batch,
+                                               float2 *__restrict__ mats) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<int64_t>(batch) * n_gates) return;
   const int64_t b = t / n_gates;
   const int g = static_cast<int>(t % n_gates);
-  double s, c;
-  sincos(0.5 * params[b * n_params + pidx[g]], &s, &c);
-  mats[t] = make_float2(static_cast<float>(c), static_cast<float>(s));
+  const double ph = remainder(0.5 * params[b * n_params + pidx[g]], M_PI);
+  mats[t] = make_float2(static_cast<float>(tan(0.5 * ph)), static_cast<float>(sin(ph)));
 }
 
 // lambda = H phi for H = sum_t w_t Z^{zmask_t}; per-block FP64 partials of
@@ -692,7 +726,7 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
     A.n_tiles = static_cast<uint64_t>(batch) << (n - nlocal);
     (void)total_bits;
     // this sweep's rotations are a contiguous gate range (a block's
-    // rotations are consecutive gates); staged per tile in LDS
+    // rotations are consecutive gates)
     int mat_lo = 0, mat_hi = -1;
     for (int gi : sweep_rots[s]) {
       const int r = a->gates[gi].rot_no;
@@ -700,8 +734,6 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       mat_hi = std::max(mat_hi, r);
     }
     const int mat_cnt = mat_hi < 0 ? 0 : mat_hi - mat_lo + 1;
-    if (mat_cnt > kMaxSweepGates)
-      return set_error(MSV_E_UNSUPPORTED, "more than 256 rotations in one layer of the ansatz");
     A.mat_lo = static_cast<uint32_t>(mat_lo);
     A.mat_cnt = static_cast<uint32_t>(mat_cnt);
     bool all_cz = true;
@@ -722,7 +754,6 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
         sg.gi_bit[t] = static_cast<uint32_t>(local[k]);
         ++t;
       }
-      A.op_begin[w] = static_cast<uint32_t>(P->ops.size());
       auto rpos = [&](int bit) -> int {
         const int k = loc_of[bit];
         return (k >= 0 && k >= reg0 && k < reg0 + rb) ? k - reg0 : -1;
@@ -786,7 +817,6 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
           P->ops.push_back(op);
         }
       };
-      if (w == 0 && backward && put_phases) emit_phases();
       // rotations whose bit is a register bit of this window, not yet placed
       std::vector<int> here;
       for (size_t i = 0; i < sweep_rots[s].size(); ++i) {
@@ -797,34 +827,44 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
         }
       }
       if (backward) std::reverse(here.begin(), here.end());
-      BOp flush;
-      std::memset(&flush, 0, sizeof(flush));
-      flush.kind = BOP_FLUSH;
-      for (int gi : here) {
-        const BGate &g = a->gates[gi];
-        BOp op;
-        std::memset(&op, 0, sizeof(op));
-        op.rpos = rpos(bit_of(a, g.q0));
-        op.gen = static_cast<uint32_t>(g.kind);  // MSV_ROT_X/Y/Z = 0/1/2
-        op.gate = static_cast<uint32_t>(g.rot_no - mat_lo);
-        if (backward) {
-          op.kind = BOP_GRAD;
-          op.count = flush.count;
-          P->ops.push_back(op);
-          flush.list[flush.count++] = static_cast<uint32_t>(g.rot_no);
-          op.kind = BOP_ROT_ADJ;
-          op.count = 0;
-          P->ops.push_back(op);
-          if (flush.count == kMaxAcc) {
-            P->ops.push_back(flush);
-            flush.count = 0;
-          }
-        } else {
+      // per register bit, in application order
+      std::vector<std::vector<int>> seq(rb);
+      for (int gi : here) seq[rpos(bit_of(a, a->gates[gi].q0))].push_back(gi);
+      // stages: repeatedly take, on every bit whose next rotation has the
+      // generator most common among the bits' next rotations, that
+      // rotation; the rotation records go first, the segment's top-level
+      // ops after them
+      std::vector<BOp> stages;
+      std::vector<size_t> head(rb, 0);
+      while (true) {
+        int cnt[3] = {0, 0, 0};
+        for (int j = 0; j < rb; ++j)
+          if (head[j] < seq[j].size()) ++cnt[a->gates[seq[j][head[j]]].kind];
+        if (cnt[0] + cnt[1] + cnt[2] == 0) break;
+        const int gen = cnt[1] >= cnt[0] && cnt[1] >= cnt[2] ? 1 : (cnt[2] >= cnt[0] ? 2 : 0);
+        BOp st;
+        std::memset(&st, 0, sizeof(st));
+        st.kind = BOP_STAGE;
+        st.gen = static_cast<uint32_t>(gen);
+        for (int j = 0; j < rb; ++j) {
+          st.list[j] = ~0u;
+          if (head[j] >= seq[j].size() || a->gates[seq[j][head[j]]].kind != gen) continue;
+          const BGate &g = a->gates[seq[j][head[j]++]];
+          BOp op;
+          std::memset(&op, 0, sizeof(op));
           op.kind = BOP_ROT;
+          op.rpos = j;
+          op.gen = static_cast<uint32_t>(gen);
+          op.gate = static_cast<uint32_t>(g.rot_no - mat_lo);
+          op.count = static_cast<uint32_t>(g.rot_no);
+          st.list[j] = static_cast<uint32_t>(P->ops.size());
           P->ops.push_back(op);
         }
+        stages.push_back(st);
       }
-      if (flush.count > 0) P->ops.push_back(flush);
+      A.op_begin[w] = static_cast<uint32_t>(P->ops.size());
+      if (w == 0 && backward && put_phases) emit_phases();
+      for (const BOp &st : stages) P->ops.push_back(st);
       if (w + 1 == wins.size() && !backward && put_phases) emit_phases();
       A.op_end[w] = static_cast<uint32_t>(P->ops.size());
     }
@@ -832,8 +872,7 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       if (!p) return set_error(MSV_E_INVALID, "internal: rotation left unscheduled");
     if (put_phases) phases_done = true;
     sw.adj = backward;
-    sw.lds = (A.nseg > 1 ? (sizeof(float2) << nlocal) : 0) +
-             sizeof(float2) * static_cast<size_t>(mat_cnt);
+    sw.lds = A.nseg > 1 ? (sizeof(float2) << nlocal) : 0;
     out->push_back(sw);
   }
   return MSV_OK;
@@ -963,6 +1002,20 @@ int ensure_buffers(msv_ansatz *a, int batch, const msv_ansatz::Plan &P) {
   return MSV_OK;
 }
 
+int launch_sweep(msv_ansatz *a, const msv_ansatz::Plan &P, const BSweep &sw, int cu) {
+  const uint64_t cap = static_cast<uint64_t>(cu) * P.tune.blocks_per_cu;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(sw.args.n_tiles, cap)));
+  const uint32_t threads = 1u << (sw.args.nlocal - kBRB);
+  if (sw.adj)
+    hipLaunchKernelGGL((k_btile<kBRB, true>), dim3(grid), dim3(threads), sw.lds, a->stream, a->phi, a->lam,
+                       a->ops_d, a->mats, a->part, sw.args);
+  else
+    hipLaunchKernelGGL((k_btile<kBRB, false>), dim3(grid), dim3(threads), sw.lds, a->stream, a->phi, a->lam,
+                       a->ops_d, a->mats, a->part, sw.args);
+  BHIP(hipGetLastError());
+  return MSV_OK;
+}
+
 // Enqueues one evaluation on a->stream: params (pinned) -> device, matrices,
 // |0>, forward sweeps, observable, backward sweeps, reductions, results ->
 // pinned. Everything is a kernel or a memcpy node: capturable as a graph.
@@ -981,21 +1034,7 @@ int enqueue_eval(msv_ansatz *a, int batch, const msv_ansatz::Plan &P, bool with_
   hipLaunchKernelGGL(k_binit, dim3(static_cast<uint32_t>(std::min<uint64_t>((amps + 255) / 256, cu * 32ull))),
                      dim3(256), 0, st, a->phi, amps, static_cast<uint32_t>(a->n_eff));
   BHIP(hipGetLastError());
-  auto launch = [&](const BSweep &sw) -> int {
-    const uint64_t cap = static_cast<uint64_t>(cu) * P.tune.blocks_per_cu;
-    const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(sw.args.n_tiles, cap)));
-    const uint32_t threads = 1u << (sw.args.nlocal - kBRB);
-    const BOp *ops = a->ops_d + sw.op_off;
-    (void)ops;
-    if (sw.adj)
-      hipLaunchKernelGGL((k_btile<kBRB, true>), dim3(grid), dim3(threads), sw.lds, st, a->phi, a->lam,
-                         a->ops_d, a->mats, a->part, sw.args);
-    else
-      hipLaunchKernelGGL((k_btile<kBRB, false>), dim3(grid), dim3(threads), sw.lds, st, a->phi, a->lam,
-                         a->ops_d, a->mats, a->part, sw.args);
-    BHIP(hipGetLastError());
-    return MSV_OK;
-  };
+  auto launch = [&](const BSweep &sw) { return launch_sweep(a, P, sw, cu); };
   for (const BSweep &sw : P.fwd) BTRY(launch(sw));
   hipLaunchKernelGGL(k_bobs, dim3(P.obs_blocks), dim3(256), 0, st, a->phi, a->lam, a->epart, P.obs);
   BHIP(hipGetLastError());
@@ -1246,6 +1285,41 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
                            static_cast<int64_t>(P.bwd.size()), static_cast<int64_t>(P.ops.size()),
                            static_cast<int64_t>(P.tuned_ms * 1000.0)};
   for (int i = 0; i < n_out && i < 6; ++i) out[i] = vals[i];
+  return MSV_OK;
+}
+
+int msv_ansatz_time_sweeps(msv_ansatz *a, int batch, int with_grad, int reps, double *out_ms) {
+  if (!a || !out_ms || reps < 1) return set_error(MSV_E_INVALID, "null argument or reps < 1");
+  std::lock_guard<std::mutex> lk(a->mu);
+  auto it = a->plans.find(std::make_pair(batch, with_grad ? 1 : 0));
+  if (it == a->plans.end()) return set_error(MSV_E_INVALID, "no plan for this batch size yet");
+  const msv_ansatz::Plan &P = it->second;
+  BHIP(hipSetDevice(a->device));
+  BTRY(upload_ops(a, P));
+  const int cu = device_cu_count(a->device);
+  hipEvent_t ev[3];
+  for (hipEvent_t &e : ev) BHIP(hipEventCreate(&e));
+  double tot[2] = {0.0, 0.0};
+  int rc = MSV_OK;
+  for (int r = 0; r < reps && rc == MSV_OK; ++r) {
+    (void)hipEventRecord(ev[0], a->stream);
+    for (const BSweep &sw : P.fwd)
+      if (rc == MSV_OK) rc = launch_sweep(a, P, sw, cu);
+    (void)hipEventRecord(ev[1], a->stream);
+    for (const BSweep &sw : P.bwd)
+      if (rc == MSV_OK) rc = launch_sweep(a, P, sw, cu);
+    (void)hipEventRecord(ev[2], a->stream);
+    if (hipEventSynchronize(ev[2]) != hipSuccess && rc == MSV_OK) rc = set_error(MSV_E_HIP, "sweep timing failed");
+    float f = 0.f, b = 0.f;
+    (void)hipEventElapsedTime(&f, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+    tot[0] += f;
+    tot[1] += b;
+  }
+  for (hipEvent_t &e : ev) (void)hipEventDestroy(e);
+  if (rc != MSV_OK) return rc;
+  out_ms[0] = tot[0] / reps;
+  out_ms[1] = tot[1] / reps;
   return MSV_OK;
 }
 

@@ -59,6 +59,26 @@ def test_energies_and_gradients_match_oracle(gpu, n, layers, batch):
             assert np.max(np.abs(g[b] - want_g)) < TOL_G, (b, np.max(np.abs(g[b] - want_g)))
 
 
+def test_wide_angles_match_oracle(gpu):
+    """Angles far outside [-pi, pi] and at the points where the half angle
+    is reduced by a multiple of pi (+-pi, +-2 pi, 3 pi, ...): the dropped
+    global signs must leave energies and gradients unchanged."""
+    rng = np.random.default_rng(77)
+    n = 7
+    a, gates, terms = _random_ansatz(n, 3, rng)
+    with a:
+        params = rng.uniform(-9 * math.pi, 9 * math.pi, (4, a.n_params))
+        special = np.array([math.pi, -math.pi, 2 * math.pi, -2 * math.pi, 3 * math.pi, 0.0, 1e3, -47.25])
+        params[0, :] = special[np.arange(a.n_params) % len(special)]
+        params[1, :] = np.nextafter(special, np.inf)[np.arange(a.n_params) % len(special)]
+        e, g = a.evaluate(params)
+        for b in range(params.shape[0]):
+            want_e = O.ansatz_energy(n, gates, params[b], terms)
+            assert abs(e[b] - want_e) < TOL_E, (b, e[b], want_e)
+            want_g = O.ansatz_param_shift(n, gates, params[b], terms)
+            assert np.max(np.abs(g[b] - want_g)) < TOL_G, (b, np.max(np.abs(g[b] - want_g)))
+
+
 def test_gradients_match_central_differences(gpu):
     rng = np.random.default_rng(5)
     n = 5
