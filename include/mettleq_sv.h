@@ -329,7 +329,8 @@ int msv_ansatz_evaluate(msv_ansatz *a, const double *params, int batch, int n_pa
                         double *energies, double *grads);
 /* Chosen plan of a batch size: out[0] tile local bits, out[1] blocks per CU,
  * out[2] forward sweeps, out[3] backward sweeps, out[4] tile ops,
- * out[5] autotuned time in microseconds. */
+ * out[5] autotuned time in microseconds, out[6] sweeps run by a kernel
+ * compiled for their stage-generator program (the rest interpret it). */
 int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n_out);
 /* Times the cached plan's sweeps alone (no graph), averaged over `reps`:
  * out_ms[0] the forward sweeps, out_ms[1] the backward sweeps (0 without

@@ -91,10 +91,11 @@ class LayeredAnsatz:
         return (e, g) if grads else e
 
     def info(self, batch: int, grads: bool = True) -> dict:
-        out = (ctypes.c_int64 * 6)()
-        nat.check(nat.lib().msv_ansatz_info(self._h, int(batch), int(bool(grads)), out, 6), "ansatz info")
+        out = (ctypes.c_int64 * 7)()
+        nat.check(nat.lib().msv_ansatz_info(self._h, int(batch), int(bool(grads)), out, 7), "ansatz info")
         return {"tile_local_bits": out[0], "blocks_per_cu": out[1], "forward_sweeps": out[2],
-                "backward_sweeps": out[3], "tile_ops": out[4], "autotuned_ms": out[5] / 1000.0}
+                "backward_sweeps": out[3], "tile_ops": out[4], "autotuned_ms": out[5] / 1000.0,
+                "compiled_program_sweeps": out[6]}
 
     def sweep_times(self, batch: int, grads: bool = True, reps: int = 3) -> dict:
         """Forward and backward sweeps of the cached plan for `batch`, timed

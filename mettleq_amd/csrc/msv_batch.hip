@@ -101,6 +101,10 @@ struct BArgs {
   TileSeg seg[kMaxSegs];
   uint32_t op_begin[kMaxSegs];  // segment s: top-level ops [op_begin, op_end)
   uint32_t op_end[kMaxSegs];
+  // pattern sweeps: CZ ops before the stages, stages, CZ ops after
+  uint32_t n_pre[kMaxSegs];
+  uint32_t n_stage[kMaxSegs];
+  uint32_t n_post[kMaxSegs];
 };
 
 struct ObsArgs {
@@ -268,6 +272,31 @@ __device__ __forceinline__ double wave_sum(double v) {
 typedef const __attribute__((address_space(4))) BOp *BOpCP;
 typedef const __attribute__((address_space(4))) float *FCP;
 
+// CZ sign products of one op on phi (and lambda).
+template <int RB, bool ADJ>
+__device__ __forceinline__ void bcz(BOpCP opp, float2 *a, float2 *l, uint64_t gb) {
+  constexpr int R = 1 << RB;
+  uint32_t c = 0, lin = 0;
+  for (uint32_t k = 0; k < opp->count; ++k) {
+    const uint32_t e = opp->list[k];
+    const uint32_t ba = e & 255u, bb = (e >> 8) & 255u;
+    const uint32_t tb = static_cast<uint32_t>(gb >> bb) & 1u;
+    if (e >> 16) lin ^= tb << ba;
+    else c ^= static_cast<uint32_t>(gb >> ba) & tb;
+  }
+  const uint32_t qm = opp->qmask;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t par = (c ^ __popc(static_cast<uint32_t>(r) & lin) ^ (qm >> r)) & 1u;
+    a[r].x = par ? -a[r].x : a[r].x;
+    a[r].y = par ? -a[r].y : a[r].y;
+    if (ADJ) {
+      l[r].x = par ? -l[r].x : l[r].x;
+      l[r].y = par ? -l[r].y : l[r].y;
+    }
+  }
+}
+
 // A fixed-phase op (CZ sign products or a two-qubit diagonal) on phi (and
 // lambda).
 template <int RB, bool ADJ>
@@ -334,12 +363,27 @@ __device__ __forceinline__ void bstage(BOpCP st, BOpCP cops, FCP mrow, float2 *a
   bstage_bit<RB, 3, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
 }
 
+// Stage-generator programs known at compile time: PAT holds the
+// generators of a segment's stages in base-4 digits, first stage lowest
+// (digit = generator + 1, 0 ends the program); PAT = 0 is the generic
+// interpreter.
+constexpr uint32_t pat_len(uint32_t p) { return p == 0 ? 0 : 1 + pat_len(p >> 2); }
+constexpr int pat_gen(uint32_t p, int k) { return static_cast<int>((p >> (2 * k)) & 3u) - 1; }
+constexpr uint32_t pat_of(int g0, int g1 = -1, int g2 = -1) {
+  return static_cast<uint32_t>(g0 + 1) | static_cast<uint32_t>(g1 + 1) << 2 | static_cast<uint32_t>(g2 + 1) << 4;
+}
+constexpr int kMaxPatCz = 2;  // CZ ops before / after the stages in a pattern sweep
+
 // One sweep over every binding's tiles. Per segment the ops
 // [op_begin, op_end): fixed phases (backward: before the rotations), the
 // rotation stages, fixed phases (forward: after the rotations).
+// With PAT != 0 every segment holds n_pre[s] CZ ops, then no stages or the
+// stages of PAT, then n_post[s] CZ ops, and the program is straight-line
+// code: a loop over the ops would carry the amplitudes through a loop
+// latch, where the register allocator permutes them with copies.
 // The adjoint form holds phi and lambda (64 VGPRs of amplitudes): both
 // states exchange through one LDS buffer in turn.
-template <int RB, bool ADJ>
+template <int RB, bool ADJ, uint32_t PAT>
 __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 *__restrict__ lam,
                                                const BOp *__restrict__ ops,
                                                const float2 *__restrict__ mats,
@@ -380,16 +424,34 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = lds_a[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
       }  // ADJ: phi and lambda were read at the end of the previous segment
-      for (uint32_t o = A.op_begin[s]; o < A.op_end[s]; ++o) {
-        const BOpCP opp = cops + o;
-        if (opp->kind == BOP_STAGE) {
-          const uint32_t gen = opp->gen;
-          if (gen == 2u) bstage<RB, 2, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
-          else if (gen == 1u) bstage<RB, 1, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
-          else bstage<RB, 0, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
-        } else {
-          bphase_op<RB, ADJ>(opp, a, l, gb);
+      if constexpr (PAT == 0) {
+        for (uint32_t o = A.op_begin[s]; o < A.op_end[s]; ++o) {
+          const BOpCP opp = cops + o;
+          if (opp->kind == BOP_STAGE) {
+            const uint32_t gen = opp->gen;
+            if (gen == 2u) bstage<RB, 2, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+            else if (gen == 1u) bstage<RB, 1, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+            else bstage<RB, 0, ADJ>(opp, cops, mrow, a, l, part_t, row_stride);
+          } else {
+            bphase_op<RB, ADJ>(opp, a, l, gb);
+          }
         }
+      } else {
+        constexpr uint32_t L = pat_len(PAT);
+        const BOpCP o0 = cops + A.op_begin[s];
+        const uint32_t npre = A.n_pre[s];
+        if (npre > 0u) bcz<RB, ADJ>(o0, a, l, gb);
+        if (npre > 1u) bcz<RB, ADJ>(o0 + 1, a, l, gb);
+        const BOpCP st = o0 + npre;
+        if (A.n_stage[s] != 0u) {
+          bstage<RB, pat_gen(PAT, 0), ADJ>(st, cops, mrow, a, l, part_t, row_stride);
+          if constexpr (L > 1) bstage<RB, pat_gen(PAT, 1), ADJ>(st + 1, cops, mrow, a, l, part_t, row_stride);
+          if constexpr (L > 2) bstage<RB, pat_gen(PAT, 2), ADJ>(st + 2, cops, mrow, a, l, part_t, row_stride);
+        }
+        const BOpCP post = st + A.n_stage[s];
+        const uint32_t npost = A.n_post[s];
+        if (npost > 0u) bcz<RB, ADJ>(post, a, l, gb);
+        if (npost > 1u) bcz<RB, ADJ>(post + 1, a, l, gb);
       }
       if (s + 1 == A.nseg) {
         uint64_t gbs = gb;
@@ -520,6 +582,7 @@ struct BSweep {
   BArgs args;
   uint32_t op_off;  // first op in the device op buffer
   bool adj;
+  uint32_t pat;     // stage program of every segment (k_btile PAT), 0 = generic
   size_t lds;
 };
 
@@ -742,6 +805,7 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       all_cz = all_cz && g.kind - 100 == MSV_PHASE_CPHASE && std::abs(std::fabs(g.theta) - M_PI) < 1e-15;
     }
     std::vector<bool> placed(sweep_rots[s].size(), false);
+    uint32_t pat = ~0u;  // the sweep's stage program (~0: none seen yet, 0: generic)
     for (size_t w = 0; w < wins.size(); ++w) {
       TileSeg &sg = A.seg[w];
       const int reg0 = wins[w];
@@ -864,10 +928,29 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       }
       A.op_begin[w] = static_cast<uint32_t>(P->ops.size());
       if (w == 0 && backward && put_phases) emit_phases();
+      const size_t pre_end = P->ops.size();
       for (const BOp &st : stages) P->ops.push_back(st);
+      const size_t st_end = P->ops.size();
       if (w + 1 == wins.size() && !backward && put_phases) emit_phases();
       A.op_end[w] = static_cast<uint32_t>(P->ops.size());
+      // pattern form: CZ ops around one stage-generator program
+      A.n_pre[w] = static_cast<uint32_t>(pre_end - A.op_begin[w]);
+      A.n_stage[w] = static_cast<uint32_t>(st_end - pre_end);
+      A.n_post[w] = static_cast<uint32_t>(P->ops.size() - st_end);
+      uint32_t prog = 0;
+      for (size_t k = 0; k < stages.size() && k < 3; ++k) prog |= (stages[k].gen + 1u) << (2 * k);
+      bool cz_only = true;
+      for (size_t k = A.op_begin[w]; k < P->ops.size(); ++k)
+        if (P->ops[k].kind != BOP_STAGE && P->ops[k].kind != BOP_CZ) cz_only = false;
+      if (stages.size() > 3 || !cz_only || A.n_pre[w] > kMaxPatCz || A.n_post[w] > kMaxPatCz ||
+          (!stages.empty() && pat != ~0u && pat != 0u && pat != prog))
+        pat = 0;
+      else if (!stages.empty() && pat == ~0u)
+        pat = prog;
     }
+    // segments without stages fit any program; a sweep without rotations
+    // runs the generic form
+    sw.pat = pat == ~0u ? 0u : pat;
     for (bool p : placed)
       if (!p) return set_error(MSV_E_INVALID, "internal: rotation left unscheduled");
     if (put_phases) phases_done = true;
@@ -1002,16 +1085,39 @@ int ensure_buffers(msv_ansatz *a, int batch, const msv_ansatz::Plan &P) {
   return MSV_OK;
 }
 
+// Stage programs with a compiled kernel (single rotation layers, Euler
+// pairs and ZYZ, and their reverses for the adjoint sweeps).
+constexpr uint32_t kPats[] = {pat_of(0), pat_of(1), pat_of(2), pat_of(1, 2), pat_of(2, 1),
+                              pat_of(0, 2), pat_of(2, 0), pat_of(2, 1, 2)};
+
+template <bool ADJ, uint32_t PAT>
+void launch_k_btile(msv_ansatz *a, const BSweep &sw, uint32_t grid, uint32_t threads) {
+  hipLaunchKernelGGL((k_btile<kBRB, ADJ, PAT>), dim3(grid), dim3(threads), sw.lds, a->stream, a->phi, a->lam,
+                     a->ops_d, a->mats, a->part, sw.args);
+}
+
+template <bool ADJ, size_t I = 0>
+void launch_pat(msv_ansatz *a, const BSweep &sw, uint32_t grid, uint32_t threads) {
+  if constexpr (I == sizeof(kPats) / sizeof(kPats[0])) {
+    launch_k_btile<ADJ, 0>(a, sw, grid, threads);
+  } else {
+    if (sw.pat == kPats[I]) launch_k_btile<ADJ, kPats[I]>(a, sw, grid, threads);
+    else launch_pat<ADJ, I + 1>(a, sw, grid, threads);
+  }
+}
+
+bool pat_compiled(uint32_t pat) {
+  for (uint32_t p : kPats)
+    if (p == pat) return true;
+  return false;
+}
+
 int launch_sweep(msv_ansatz *a, const msv_ansatz::Plan &P, const BSweep &sw, int cu) {
   const uint64_t cap = static_cast<uint64_t>(cu) * P.tune.blocks_per_cu;
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(sw.args.n_tiles, cap)));
   const uint32_t threads = 1u << (sw.args.nlocal - kBRB);
-  if (sw.adj)
-    hipLaunchKernelGGL((k_btile<kBRB, true>), dim3(grid), dim3(threads), sw.lds, a->stream, a->phi, a->lam,
-                       a->ops_d, a->mats, a->part, sw.args);
-  else
-    hipLaunchKernelGGL((k_btile<kBRB, false>), dim3(grid), dim3(threads), sw.lds, a->stream, a->phi, a->lam,
-                       a->ops_d, a->mats, a->part, sw.args);
+  if (sw.adj) launch_pat<true>(a, sw, grid, threads);
+  else launch_pat<false>(a, sw, grid, threads);
   BHIP(hipGetLastError());
   return MSV_OK;
 }
@@ -1281,10 +1387,13 @@ int msv_ansatz_info(msv_ansatz *a, int batch, int with_grad, int64_t *out, int n
   auto it = a->plans.find(std::make_pair(batch, with_grad ? 1 : 0));
   if (it == a->plans.end()) return set_error(MSV_E_INVALID, "no plan for this batch size yet");
   const msv_ansatz::Plan &P = it->second;
-  const int64_t vals[6] = {P.tune.nlocal, P.tune.blocks_per_cu, static_cast<int64_t>(P.fwd.size()),
+  int64_t straight = 0;
+  for (const auto *v : {&P.fwd, &P.bwd})
+    for (const BSweep &sw : *v) straight += pat_compiled(sw.pat) ? 1 : 0;
+  const int64_t vals[7] = {P.tune.nlocal, P.tune.blocks_per_cu, static_cast<int64_t>(P.fwd.size()),
                            static_cast<int64_t>(P.bwd.size()), static_cast<int64_t>(P.ops.size()),
-                           static_cast<int64_t>(P.tuned_ms * 1000.0)};
-  for (int i = 0; i < n_out && i < 6; ++i) out[i] = vals[i];
+                           static_cast<int64_t>(P.tuned_ms * 1000.0), straight};
+  for (int i = 0; i < n_out && i < 7; ++i) out[i] = vals[i];
   return MSV_OK;
 }
 
