@@ -72,6 +72,8 @@ struct TileOp {
   uint64_t cmask_t; // U2: control bits that are not register bits (index mask)
   uint64_t amask;   // DIAG: index mask of bit a (when ra < 0)
   uint64_t bmask;   // DIAG: index mask of bit b (when rb < 0)
+  uint32_t pa;      // DIAG / SIGN: registers holding bit a = 1 (bit r set), 0 when ra < 0
+  uint32_t pb;      // DIAG / SIGN: same for bit b
   T m[8];
 };
 

@@ -192,53 +192,100 @@ __device__ __forceinline__ void qft_dispatch(typename Vec2<T>::type *a, typename
 }
 #undef MSV_QFT_CASE
 
+// Registers whose amplitude has the op's bit set, as a mask over r: the
+// register pattern `pat` of a register bit, or every register when a
+// thread / tile bit (index mask `imask`) is set in this thread's index.
+template <int RB>
+__device__ __forceinline__ uint32_t regs_with_bit(uint32_t pat, uint64_t imask, uint64_t gb) {
+  constexpr uint32_t all = (RB >= 5) ? ~0u : ((1u << (1 << RB)) - 1u);
+  return pat | ((gb & imask) ? all : 0u);
+}
+
+// Diagonal over 1 or 2 index bits (entries d0..d3 by index 2 ba + bb, or
+// d0, d1 by ba). Three forms, chosen per op (wave-uniform): both bits in
+// registers (the entry of each register is uniform, selected on the scalar
+// unit), one in registers (two candidate entries per thread, one uniform
+// choice per register), none (one entry per thread). Each amplitude then
+// costs one complex multiply.
 template <typename T, int RB>
 __device__ __forceinline__ void diag_apply(typename Vec2<T>::type *a, const TileOp<T> &op,
                                            uint64_t gb) {
   using V = typename Vec2<T>::type;
-  const int ra = op.ra, rb = op.rb;
   const bool two = op.nbits == 2;
-  const uint32_t ta = (gb & op.amask) ? 1u : 0u;
-  const uint32_t tb = (gb & op.bmask) ? 1u : 0u;
+  const bool areg = op.ra >= 0;
+  const bool breg = !two || op.rb >= 0;
   V d0, d1, d2, d3;
   d0.x = op.m[0]; d0.y = op.m[1];
   d1.x = op.m[2]; d1.y = op.m[3];
   d2.x = op.m[4]; d2.y = op.m[5];
   d3.x = op.m[6]; d3.y = op.m[7];
+  if (!two) {  // one bit: entries d0 (bit 0), d1 (bit 1)
+    d2 = d1;
+    d3 = d1;
+    d1 = d0;
+  }
+  const uint32_t pa = op.pa, pb = op.pb;
+  if (areg && breg) {
 #pragma unroll
-  for (int r = 0; r < (1 << RB); ++r) {
-    const uint32_t ba = (ra >= 0) ? ((static_cast<uint32_t>(r) >> ra) & 1u) : ta;
-    V dv;
-    if (two) {
-      const uint32_t bb = (rb >= 0) ? ((static_cast<uint32_t>(r) >> rb) & 1u) : tb;
-      const T lox = bb ? d1.x : d0.x, loy = bb ? d1.y : d0.y;
-      const T hix = bb ? d3.x : d2.x, hiy = bb ? d3.y : d2.y;
-      dv.x = ba ? hix : lox;
-      dv.y = ba ? hiy : loy;
-    } else {
-      dv.x = ba ? d1.x : d0.x;
-      dv.y = ba ? d1.y : d0.y;
+    for (int r = 0; r < (1 << RB); ++r) {
+      const bool ba = (pa >> r) & 1u, bb = (pb >> r) & 1u;
+      const V lo = bb ? d1 : d0, hi = bb ? d3 : d2;
+      a[r] = cmul(a[r], ba ? hi : lo);
     }
-    a[r] = cmul(a[r], dv);
+  } else if (areg || breg) {
+    // the thread's value of the non-register bit fixes the other half
+    const uint64_t tmask = areg ? op.bmask : op.amask;
+    const bool t = (gb & tmask) != 0;
+    V lo, hi;  // entries for the register bit = 0 / 1
+    if (areg) {
+      lo = t ? d1 : d0;
+      hi = t ? d3 : d2;
+    } else {
+      lo = t ? d2 : d0;
+      hi = t ? d3 : d1;
+    }
+    const uint32_t p = areg ? pa : pb;
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) a[r] = cmul(a[r], ((p >> r) & 1u) ? hi : lo);
+  } else {
+    const bool ta = (gb & op.amask) != 0, tb = two && (gb & op.bmask) != 0;
+    const V lo = tb ? d1 : d0, hi = tb ? d3 : d2;
+    const V d = ta ? hi : lo;
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) a[r] = cmul(a[r], d);
   }
 }
 
-// Diagonal with entries +1 / -1 (bit k of jl: entry k, index (bit a, bit b)).
+template <typename T> struct SignBits;
+template <> struct SignBits<float> {
+  __device__ static float flip(float v, uint32_t sb) { return __uint_as_float(__float_as_uint(v) ^ sb); }
+};
+template <> struct SignBits<double> {
+  __device__ static double flip(double v, uint32_t sb) {
+    return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<long long>(sb) << 32));
+  }
+};
+
+// Diagonal with entries +1 / -1 (bit k of jl: entry k, index 2 ba + bb or
+// ba is -1): the thread's registers to negate as a mask over r, then each
+// amplitude's sign bits flipped.
 template <typename T, int RB>
 __device__ __forceinline__ void sign_apply(typename Vec2<T>::type *a, const TileOp<T> &op, uint64_t gb) {
-  const int ra = op.ra, rb = op.rb;
-  const bool two = op.nbits == 2;
-  const uint32_t smask = op.jl;
-  const uint32_t ta = (gb & op.amask) ? 1u : 0u;
-  const uint32_t tb = (gb & op.bmask) ? 1u : 0u;
+  const uint32_t A = regs_with_bit<RB>(op.pa, op.amask, gb);
+  const uint32_t s = op.jl;
+  uint32_t M;
+  if (op.nbits == 2) {
+    const uint32_t B = regs_with_bit<RB>(op.pb, op.bmask, gb);
+    M = ((s & 1u) ? (~A & ~B) : 0u) | ((s & 2u) ? (~A & B) : 0u) | ((s & 4u) ? (A & ~B) : 0u) |
+        ((s & 8u) ? (A & B) : 0u);
+  } else {
+    M = ((s & 1u) ? ~A : 0u) | ((s & 2u) ? A : 0u);
+  }
 #pragma unroll
   for (int r = 0; r < (1 << RB); ++r) {
-    const uint32_t ba = (ra >= 0) ? ((static_cast<uint32_t>(r) >> ra) & 1u) : ta;
-    const uint32_t bb = (rb >= 0) ? ((static_cast<uint32_t>(r) >> rb) & 1u) : tb;
-    const uint32_t idx = two ? ((ba << 1) | bb) : ba;
-    const bool neg = (smask >> idx) & 1u;
-    a[r].x = neg ? -a[r].x : a[r].x;
-    a[r].y = neg ? -a[r].y : a[r].y;
+    const uint32_t sb = (M << (31 - r)) & 0x80000000u;
+    a[r].x = SignBits<T>::flip(a[r].x, sb);
+    a[r].y = SignBits<T>::flip(a[r].y, sb);
   }
 }
 

@@ -612,14 +612,25 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
         op.pbit = static_cast<uint32_t>(p);
         op.lb = static_cast<uint32_t>(po.lb);
       } else if (po.kind == OP_DIAG) {
+        // registers whose index has register bit j set
+        auto reg_pat = [&](int j) -> uint32_t {
+          uint32_t m = 0;
+          if (j >= 0)
+            for (int r = 0; r < (1 << rb); ++r)
+              if ((r >> j) & 1) m |= 1u << r;
+          return m;
+        };
         op.nbits = po.bbit >= 0 ? 2u : 1u;
         const int ja = reg_pos(po.abit);
         op.ra = static_cast<int8_t>(ja);
         op.amask = ja >= 0 ? 0ull : (1ull << po.abit);
+        op.pa = reg_pat(ja);
+        op.pb = 0;
         if (po.bbit >= 0) {
           const int jb = reg_pos(po.bbit);
           op.rb = static_cast<int8_t>(jb);
           op.bmask = jb >= 0 ? 0ull : (1ull << po.bbit);
+          op.pb = reg_pat(jb);
         }
         // entries of exactly +1 / -1 (CZ, Z) run as sign flips
         uint32_t smask = 0;
