@@ -111,6 +111,9 @@ struct TileArgs {
   TileSeg seg[kMaxSegs];
   T scale;                 // applied at the final store (QFT 1/sqrt2 factors)
   uint32_t prefetch;       // 1: load the next tile's first segment while this one runs
+  uint32_t from_basis;     // 1: the input is the basis state |basis>, generated instead of read
+  uint32_t pad;
+  uint64_t basis;
   T tab[TileCfg<T>::kTabVals];  // QFT: exp(2*pi*i*m/32), m = 0..31, (re, im) at 0
   TileOp<T> ops[TileCfg<T>::kMaxOps];
 };

@@ -281,6 +281,16 @@ __device__ __forceinline__ void sign_apply(typename Vec2<T>::type *a, const Tile
   } else {
     M = ((s & 1u) ? ~A : 0u) | ((s & 2u) ? A : 0u);
   }
+  if ((op.pa | op.pb) == 0u) {
+    // no register bits: one sign for all of the thread's amplitudes
+    const uint32_t sb = M << 31;
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) {
+      a[r].x = SignBits<T>::flip(a[r].x, sb);
+      a[r].y = SignBits<T>::flip(a[r].y, sb);
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < (1 << RB); ++r) {
     const uint32_t sb = (M << (31 - r)) & 0x80000000u;
@@ -303,8 +313,11 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
       if (!op.has_ctrl) {
         u2_dispatch<T, RB, false>(a, m, op.rpos, true, 0u);
       } else {
+        // controls only outside the registers: creg = 0 at compile time, so
+        // no per-pair scalar test of the register controls
         const bool tok = (gb & op.cmask_t) == op.cmask_t;
-        u2_dispatch<T, RB, true>(a, m, op.rpos, tok, op.cmask_r);
+        if (op.cmask_r == 0u) u2_dispatch<T, RB, true>(a, m, op.rpos, tok, 0u);
+        else u2_dispatch<T, RB, true>(a, m, op.rpos, tok, op.cmask_r);
       }
     } else if (kind == OP_QFT_F || kind == OP_QFT_I) {
       const uint32_t w = static_cast<uint32_t>(op.pbit) - op.lb;
@@ -318,7 +331,8 @@ __device__ __forceinline__ void run_ops(typename Vec2<T>::type *a, const TileArg
         x_dispatch<T, RB, false>(a, op.rpos, true, 0u);
       } else {
         const bool tok = (gb & op.cmask_t) == op.cmask_t;
-        x_dispatch<T, RB, true>(a, op.rpos, tok, op.cmask_r);
+        if (op.cmask_r == 0u) x_dispatch<T, RB, true>(a, op.rpos, tok, 0u);
+        else x_dispatch<T, RB, true>(a, op.rpos, tok, op.cmask_r);
       }
     } else if (kind == OP_SIGN) {
       sign_apply<T, RB>(a, op, gb);
@@ -385,7 +399,24 @@ __device__ __forceinline__ void st_f4(float4 *p, float4 a) {
   }
 }
 
-// First-segment amplitudes of a tile straight from HBM.
+// First-segment amplitudes of a tile: from HBM, or generated when the input
+// is a basis state (a deferred reset: nothing is read).
+template <typename T, int RB>
+__device__ __forceinline__ void load_seg0(const typename Vec2<T>::type *__restrict__ st,
+                                          const TileArgs<T> &A, uint64_t gb, const uint64_t *gstr,
+                                          typename Vec2<T>::type *a) {
+  if (A.from_basis) {
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) {
+      a[r].x = (gb + reg_goff<RB>(r, gstr)) == A.basis ? T(1) : T(0);
+      a[r].y = T(0);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < (1 << RB); ++r) a[r] = ld_amp(st + gb + reg_goff<RB>(r, gstr));
+  }
+}
+
 template <typename T, int RB>
 __device__ __forceinline__ void load_first(const typename Vec2<T>::type *__restrict__ st,
                                            const TileArgs<T> &A, uint32_t tid, uint64_t tile,
@@ -397,8 +428,7 @@ __device__ __forceinline__ void load_first(const typename Vec2<T>::type *__restr
   uint32_t tl, tm;
   uint64_t gb;
   seg_thread(sg, tid, tile_base(A, tile), tl, gb, tm);
-#pragma unroll
-  for (int r = 0; r < (1 << RB); ++r) a[r] = ld_amp(st + gb + reg_goff<RB>(r, gstr));
+  load_seg0<T, RB>(st, A, gb, gstr, a);
 }
 
 // Lookup-table index of phase window w for amplitude index i.
@@ -476,8 +506,7 @@ __device__ __forceinline__ void tile_sweep(typename Vec2<T>::type *__restrict__ 
           uint64_t gstr[RB];
 #pragma unroll
           for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
-#pragma unroll
-          for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
+          load_seg0<T, RB>(st, A, p.gb, gstr, a);
         }
       } else {
 #pragma unroll

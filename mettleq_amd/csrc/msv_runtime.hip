@@ -661,7 +661,15 @@ int launch_sweep(msv_state *s, const SweepPlan &plan) {
     }
   }
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(A.n_tiles, cap)));
+  // a deferred reset is consumed here: the sweep generates |basis> instead
+  // of reading the state (callers flush every other kind of deferred work)
+  if (s->basis_on) {
+    if (s->pend_on || s->perm_on || s->tail_on) return fail(MSV_E_INVALID, "internal: deferred work before a tile sweep");
+    A.from_basis = 1;
+    A.basis = s->basis_k;
+  }
   MSV_LAUNCH(s, MSV_K_TILE, launch_tile<T>(reinterpret_cast<V *>(s->buf), A, rb, grid, s->stream));
+  s->basis_on = false;  // cleared once enqueued
   s->sweeps += 1;
   return MSV_OK;
 }
@@ -3578,7 +3586,11 @@ int msv_apply_gates(msv_state *s, const msv_gate *gates, int n_gates) {
     }
   }
   MSV_TRY(set_dev(s));
-  MSV_TRY(flush_pending(s));
+  // A lone deferred basis state stays deferred: the gate list runs as tile
+  // sweeps only, and the first one generates the basis state instead of
+  // reading a written one. Other deferred work is flushed first.
+  const bool lone_basis = s->basis_on && !s->pend_on && !s->perm_on && !s->tail_on;
+  if (!lone_basis || ops.empty()) MSV_TRY(flush_pending(s));
   return MSV_DISPATCH(s, apply_plan_ops, s, ops);
 }
 
@@ -3586,7 +3598,6 @@ int msv_apply_circuit(msv_state *s, const int32_t *codes, const int32_t *wires, 
                       int n_ops) {
   TuneScope ts_;
   MSV_TRY(check_state(s));
-  MSV_TRY(flush_pending(s));
   if (n_ops < 0 || (n_ops > 0 && (!codes || !wires || !params)))
     return fail(MSV_E_INVALID, "circuit: null arrays");
   std::vector<msv_gate> gates;
