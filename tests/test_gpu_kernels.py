@@ -1006,6 +1006,31 @@ def test_deferred_basis_state_then_other_operations(gpu, after):
     assert np.max(np.abs(got - want)) < 2e-6
 
 
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("n", [9, 14, 21])
+def test_deferred_basis_state_then_gate_list_is_generated_by_the_first_tile_sweep(gpu, dtype, n):
+    """set_basis(k), then a served-style gate list (U3, CZ, ZZPHASE, CNOT):
+    the first tile sweep generates |k> instead of reading a written state
+    (no basis-state kernel), and the result equals the oracle's run from |k>."""
+    from mettleq_amd.circuit import encode_ops
+    from mettleq_amd.serving import random_circuit
+    rng = np.random.default_rng(9100 + n)
+    k = int(rng.integers(1 << n))
+    ops = random_circuit(n, 60, rng)
+    e = np.zeros(1 << n, dtype=np.complex128)
+    e[k] = 1
+    want = O.run_ops(n, ops, e)
+    with _state(n, dtype) as s:
+        s.set_basis(k)
+        s.dispatch_log(reset=True)
+        s.apply_circuit(*encode_ops(ops))
+        kinds = s.dispatch_log()
+        got = s.to_numpy()
+    assert kinds and set(kinds) == {"k_tile"}, kinds
+    tol = 5e-6 if dtype == "complex64" else TOL128
+    assert np.max(np.abs(got - want)) < tol
+
+
 @pytest.mark.parametrize("n", [6, 11, 15])
 @pytest.mark.parametrize("reorder", [0, 1])
 def test_gate_lists_with_controls_swaps_and_diagonals(gpu, n, reorder):
