@@ -298,8 +298,7 @@ hipError_t launch_pauli(const typename Vec2<T>::type *state, uint64_t n_amps,
 template <typename Ta, typename Tb>
 hipError_t launch_inner(const typename Vec2<Ta>::type *a, const typename Vec2<Tb>::type *b,
                         uint64_t n_amps, double *partials, uint32_t grid, hipStream_t stream);
-hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uint32_t grid,
-                             hipStream_t stream);
+hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, hipStream_t stream);
 // Sampler steps on the device (msv_sample_noalloc): the in-place sequential
 // prefix of the block sums at p[1..nb] (p[0] = 0), and per shot k of
 // [shot0, shot0 + count) its block and residual.

@@ -1417,19 +1417,22 @@ __global__ void __launch_bounds__(256) k_convert(const typename Vec2<Ts>::type *
   }
 }
 
-__global__ void __launch_bounds__(256) k_copy_probe(const float4 *__restrict__ src,
+// Copy probe (calibration of achievable bandwidth and of the FETCH_SIZE /
+// WRITE_SIZE counters): the access form of the engine's sweeps. Each block
+// copies one contiguous 128 KiB chunk (512 threads x 16 vectors of 16 B,
+// lane-consecutive), all loads in flight before the stores, nontemporal
+// like the sweeps' loads and stores.
+constexpr int kCopyVecs = 16;
+__global__ void __launch_bounds__(512) k_copy_probe(const float4 *__restrict__ src,
                                                     float4 *__restrict__ dst, uint64_t n_vec) {
-  const uint64_t nthreads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; base < n_vec;
-       base += nthreads * kUnroll) {
-    float4 v[kUnroll];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * (512u * kCopyVecs) + threadIdx.x;
+  float4 v[kCopyVecs];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-      if (base + u * nthreads < n_vec) v[u] = src[base + u * nthreads];
+  for (int k = 0; k < kCopyVecs; ++k)
+    if (base + k * 512u < n_vec) v[k] = ld_f4(src + base + k * 512u);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-      if (base + u * nthreads < n_vec) dst[base + u * nthreads] = v[u];
-  }
+  for (int k = 0; k < kCopyVecs; ++k)
+    if (base + k * 512u < n_vec) st_f4(dst + base + k * 512u, v[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2023,9 +2026,11 @@ hipError_t launch_sample_search(const double *prefix, uint64_t nb, double total,
   return hipGetLastError();
 }
 
-hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, uint32_t grid,
-                             hipStream_t stream) {
-  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, stream, src, dst, n_vec);
+hipError_t launch_copy_probe(const float4 *src, float4 *dst, uint64_t n_vec, hipStream_t stream) {
+  const uint64_t per = 512ull * kCopyVecs;
+  const uint64_t grid = (n_vec + per - 1) / per;
+  if (grid > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy_probe, dim3(static_cast<uint32_t>(grid)), dim3(512), 0, stream, src, dst, n_vec);
   return hipGetLastError();
 }
 

@@ -3891,14 +3891,13 @@ int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_p
   MSV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   MSV_HIP(hipMemsetAsync(a, 0, bytes, st));
   const uint64_t nvec = bytes / 16;
-  const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((nvec + 255) / 256, dev_cu(device) * 16ull));
-  MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, grid, st));
+  MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, st));
   hipEvent_t e0, e1;
   MSV_HIP(hipEventCreate(&e0));
   MSV_HIP(hipEventCreate(&e1));
   MSV_HIP(hipEventRecord(e0, st));
   for (int i = 0; i < iters; ++i)
-    MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, grid, st));
+    MSV_HIP(launch_copy_probe(static_cast<const float4 *>(a), static_cast<float4 *>(b), nvec, st));
   MSV_HIP(hipEventRecord(e1, st));
   MSV_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
