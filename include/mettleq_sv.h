@@ -376,6 +376,10 @@ int msv_ansatz_time_sweeps(msv_ansatz *a, int batch, int with_grad, int reps, do
                                          XOR permutation (0 off; 1 on, and without a
                                          pending permutation for states up to 64 GiB;
                                          2 on, also above) */
+#define MSV_TUNE_LANE_SWAP 17         /* rotation sweeps: lane bits 4 and 5 trade
+                                         places with register bits inside the
+                                         wavefront (v_permlane16/32_swap), so a
+                                         sweep needs fewer LDS exchanges (0/1) */
 int msv_set_tuning(int key, int64_t value);
 int msv_get_tuning(int key, int64_t *value);
 

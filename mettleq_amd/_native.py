@@ -55,6 +55,7 @@ MSV_TUNE_DEFER_PERM = 13
 MSV_TUNE_FUSED_BLOCKS_PER_CU = 14
 MSV_TUNE_REORDER_GATES = 15
 MSV_TUNE_DEFER_TAIL = 16
+MSV_TUNE_LANE_SWAP = 17
 MSV_PENDING_BASIS = 1
 MSV_PENDING_DIAG = 2
 MSV_PENDING_PERM = 4

@@ -136,6 +136,8 @@ struct PhaseArgs {
 
 constexpr int kMaxTileBits = 32;   // bits of a tile number
 constexpr int kMaxTileGroups = 8;  // 4-bit groups of a tile number
+constexpr int kThrTabOff = 2 * kMaxTileGroups * 16;  // thread parts in RotArgs::tile_tab
+constexpr int kThrTabLen = 512;          // one entry per thread-id value of a 512-thread tile
 constexpr int kFusedRdep = 4;      // deferred-diagonal windows a fused sweep evaluates per register
 constexpr int kFusedWins = 8;      // deferred-diagonal windows a fused sweep takes in all
 
@@ -177,9 +179,25 @@ struct RotArgs {
   uint32_t has_post;
   uint32_t ntgrp;             // 4-bit groups of the tile number in tile_tab
   uint32_t pad;
-  const typename Vec2<T>::type *tile_tab;  // [2][kMaxTileGroups][16]: pre, post (unit entries past ntgrp)
+  // [2][kMaxTileGroups][16]: pre, post tile-number groups (unit entries past
+  // ntgrp); then at kThrTabOff the thread parts, [2][512]: pre, post, entry
+  // tm = the product of the factors of the set thread-id bits of tm
+  const typename Vec2<T>::type *tile_tab;
   TileSeg seg[kMaxSegs];
   T shear[kMaxSegs][5][2];
+  // In-wavefront swaps (k_rot only; zero elsewhere). After segment s's
+  // rotations, swp[s] bit 0 swaps register bit 0 with lane bit 4 and bit 1
+  // register bit 1 with lane bit 5 (v_permlane16/32_swap), and the bit each
+  // brings into its register is rotated by shear2[s][0|1]. The segment's
+  // LDS write then uses its layout after the swaps: thread-id bit 4 / 5
+  // adds tfix[s][0] / [1] to the thread's local index (XORed), register
+  // bit j sits at local bit mask wimg[s][j]. `out` is the last segment's
+  // layout after its swaps (store addresses and the post diagonal).
+  uint32_t swp[kMaxSegs];
+  uint32_t tfix[kMaxSegs][2];
+  uint32_t wimg[kMaxSegs][5];
+  T shear2[kMaxSegs][2][2];
+  TileSeg out;
   T pre[kDtabVals];
   T post[kDtabVals];
   FusedRead fr;                         // k_rot<.., PERM = true> and k_rot_relabel only

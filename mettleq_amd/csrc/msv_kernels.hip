@@ -447,6 +447,21 @@ struct SegPos {
   uint64_t gb;
 };
 
+// Local index bits / amplitude index of a thread (thread-id bits in use:
+// tm) in segment sg.
+__device__ __forceinline__ uint32_t seg_local(const TileSeg &sg, uint32_t tm) {
+  uint32_t tl = 0;
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) tl |= ((tm >> k) & 1u) << sg.tl_bit[k];
+  return tl;
+}
+__device__ __forceinline__ uint64_t seg_index(const TileSeg &sg, uint32_t tm, uint64_t base) {
+  uint64_t gb = base;
+#pragma unroll
+  for (int k = 0; k < kTileThreadBits; ++k) gb |= static_cast<uint64_t>((tm >> k) & 1u) << sg.gi_bit[k];
+  return gb;
+}
+
 template <int RB>
 __device__ __forceinline__ SegPos seg_pos(const TileSeg &sg, uint32_t tid, uint64_t base) {
   SegPos p;
@@ -454,6 +469,7 @@ __device__ __forceinline__ SegPos seg_pos(const TileSeg &sg, uint32_t tid, uint6
   p.tsw = lds_swz<RB>(p.tl);
   return p;
 }
+
 
 // One tile per block and iteration (persistent grid): the first segment
 // loads from HBM, later segments exchange through LDS, the last one stores.
@@ -583,32 +599,62 @@ __device__ __forceinline__ void shear_pairs(typename Vec2<T>::type *a, T sa, T s
   }
 }
 
-// Diagonal over all index bits, applied to a segment's amplitudes: tab[0..63]
-// holds the register part d(r) for the segment's register bits (constant
-// included), tab[64..81] the factor of each thread-id bit when it is set; the
-// thread part (thread_factor) is the product of the factors of its set bits,
-// multiplied by the tile's factor before dtab_apply_f.
-template <typename T>
-__device__ __forceinline__ typename Vec2<T>::type thread_factor(const T *tab, uint32_t tm) {
-  using V = typename Vec2<T>::type;
-  V f;
-  f.x = T(1);
-  f.y = T(0);
-#pragma unroll
-  for (int k = 0; k < kTileThreadBits; ++k) {
-    V q;
-    q.x = tab[64 + 2 * k];
-    q.y = tab[65 + 2 * k];
-    const V h = cmul(f, q);
-    const bool on = (tm >> k) & 1u;
-    f.x = on ? h.x : f.x;
-    f.y = on ? h.y : f.y;
+// Exchanges register bit J with lane bit LB (4 or 5) inside the wavefront:
+// for each register pair (r, r | 2^J) the lanes whose bit LB is 1 trade
+// their r amplitude for the r | 2^J amplitude of the lane whose bit LB is
+// 0. v_permlane16_swap trades the odd 16-lane rows of its first operand
+// with the even rows of its second, v_permlane32_swap the upper 32 lanes of
+// the first with the lower 32 of the second: afterwards register bit J
+// holds what lane bit LB held and the reverse. Needs full waves.
+template <int LB>
+__device__ __forceinline__ void swap_word(uint32_t &lo, uint32_t &hi) {
+  if constexpr (LB == 4) {
+    const auto p = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+    lo = p[0];
+    hi = p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+    lo = p[0];
+    hi = p[1];
   }
-  return f;
 }
 
-// Applies the diagonal with the thread and tile parts already multiplied
-// into `f` (thread_factor, once per block, times the tile's factor).
+template <int LB>
+__device__ __forceinline__ void swap_val(float &lo, float &hi) {
+  uint32_t a = __float_as_uint(lo), b = __float_as_uint(hi);
+  swap_word<LB>(a, b);
+  lo = __uint_as_float(a);
+  hi = __uint_as_float(b);
+}
+
+template <int LB>
+__device__ __forceinline__ void swap_val(double &lo, double &hi) {
+  const uint64_t a = static_cast<uint64_t>(__double_as_longlong(lo));
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(hi));
+  uint32_t al = static_cast<uint32_t>(a), ah = static_cast<uint32_t>(a >> 32);
+  uint32_t bl = static_cast<uint32_t>(b), bh = static_cast<uint32_t>(b >> 32);
+  swap_word<LB>(al, bl);
+  swap_word<LB>(ah, bh);
+  lo = __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(ah) << 32) | al));
+  hi = __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(bh) << 32) | bl));
+}
+
+template <typename T, int RB, int J, int LB>
+__device__ __forceinline__ void lane_swap(typename Vec2<T>::type *a) {
+#pragma unroll
+  for (int r = 0; r < (1 << RB); ++r) {
+    if (r & (1 << J)) continue;
+    const int r1 = r | (1 << J);
+    swap_val<LB>(a[r].x, a[r1].x);
+    swap_val<LB>(a[r].y, a[r1].y);
+  }
+}
+
+// Diagonal over all index bits, applied to a segment's amplitudes: tab[0..63]
+// holds the register part d(r) for the segment's register bits (constant
+// included); `f` is the thread part (one entry of a host table indexed by
+// the thread-id bits in use, formed in double precision) times the tile's
+// factor.
 template <typename T, int RB>
 __device__ __forceinline__ void dtab_apply_f(typename Vec2<T>::type *a, const T *tab, typename Vec2<T>::type f) {
   using V = typename Vec2<T>::type;
@@ -620,27 +666,6 @@ __device__ __forceinline__ void dtab_apply_f(typename Vec2<T>::type *a, const T 
     a[r] = cmul(a[r], cmul(d, f));
   }
 }
-
-// Applies the diagonal whose tile part is `g`: the thread part's factors
-// multiply into g, then each register's table entry.
-template <typename T, int RB>
-__device__ __forceinline__ void dtab_apply_g(typename Vec2<T>::type *a, const T *tab, uint32_t tm,
-                                             typename Vec2<T>::type g) {
-  using V = typename Vec2<T>::type;
-  V f = g;
-#pragma unroll
-  for (int k = 0; k < kTileThreadBits; ++k) {
-    V q;
-    q.x = tab[64 + 2 * k];
-    q.y = tab[65 + 2 * k];
-    const V h = cmul(f, q);
-    const bool on = (tm >> k) & 1u;
-    f.x = on ? h.x : f.x;
-    f.y = on ? h.y : f.y;
-  }
-  dtab_apply_f<T, RB>(a, tab, f);
-}
-
 
 // Per-tile factor of a diagonal: product over the 4-bit groups of the tile
 // number (wave-uniform reads of a read-only table: scalar loads).
@@ -809,7 +834,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepW
           for (int r = 0; r < R; ++r) a[r] = ld_amp(st + p.gb + reg_goff<RB>(r, gstr));
         }
         if (A.has_pre)
-          dtab_apply_g<T, RB>(a, A.pre, p.tm, tile_factor<T>(ttab, A.ntgrp, tile));
+          dtab_apply_f<T, RB>(a, A.pre, cmul(ttab[kThrTabOff + p.tm], tile_factor<T>(ttab, A.ntgrp, tile)));
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -830,17 +855,43 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(SweepW
           }
         }
       }
+      // lane bits 4 / 5 into register bits 0 / 1, and their rotations
+      if constexpr (RB >= 2) {
+        const uint32_t sw = A.swp[s];
+        if (sw & 1u) {
+          lane_swap<T, RB, 0, 4>(a);
+          shear_pairs<T, RB, 0>(a, A.shear2[s][0][0], A.shear2[s][0][1]);
+        }
+        if (sw & 2u) {
+          lane_swap<T, RB, 1, 5>(a);
+          shear_pairs<T, RB, 1>(a, A.shear2[s][1][0], A.shear2[s][1][1]);
+        }
+      }
       if (s == last) break;
+      // the LDS write in the layout after the swaps
+      const uint32_t tw = lds_swz<RB>(p.tl ^ ((tid & 16u) ? A.tfix[s][0] : 0u) ^ ((tid & 32u) ? A.tfix[s][1] : 0u));
+      uint32_t rw[RB];
+#pragma unroll
+      for (int j = 0; j < RB; ++j) rw[j] = lds_swz<RB>(A.wimg[s][j]);
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)] = a[r];
+      for (int r = 0; r < R; ++r) {
+        uint32_t ro = 0;
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+          if ((r >> j) & 1) ro ^= rw[j];
+        lds[tw ^ ro] = a[r];
+      }
       __syncthreads();
     }
     {
-      const TileSeg &sg = A.seg[last];
+      // the last segment's layout after its swaps
+      const TileSeg &sg = A.out;
       const SegPos p = seg_pos<RB>(sg, tid, base);
       if (A.has_post)
-        dtab_apply_g<T, RB>(a, A.post, p.tm, tile_factor<T>(ttab + kMaxTileGroups * 16, A.ntgrp, tile));
+        dtab_apply_f<T, RB>(a, A.post,
+                            cmul(ttab[kThrTabOff + kThrTabLen + p.tm],
+                                 tile_factor<T>(ttab + kMaxTileGroups * 16, A.ntgrp, tile)));
       uint64_t gstr[RB];
 #pragma unroll
       for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
@@ -889,20 +940,25 @@ k_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> A, const typename Ve
     V a[R];
     for (uint32_t s = 0;; ++s) {
       const TileSeg &sg = A.seg[s];
-      const SegPos p = seg_pos<RB>(sg, tid, base);
+      // the index bits only where they are used (the load, the windows)
+      SegPos p;
+      p.tm = tid & sg.tmask;
+      p.tl = seg_local(sg, p.tm);
+      p.tsw = lds_swz<RB>(p.tl);
       if (s == 0) {
+        const uint64_t gb = seg_index(sg, p.tm, base);
         uint64_t gstr[RB];
 #pragma unroll
         for (int j = 0; j < RB; ++j) gstr[j] = sg.gstr[j];
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = ld_amp(src + p.gb + reg_goff<RB>(r, gstr));
+        for (int r = 0; r < R; ++r) a[r] = ld_amp(src + gb + reg_goff<RB>(r, gstr));
       } else {
 #pragma unroll
         for (int r = 0; r < R; ++r)
           a[r] = lds[p.tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << sg.reg0)];
         if (s == so + 1 && A.has_pre) {
-          // the thread part is formed per tile here: kept across tiles it spills registers
-          dtab_apply_f<T, RB>(a, A.pre, thread_factor<T>(A.pre, p.tm));
+          // the thread part is read per tile (a table hit): kept across tiles it spills registers
+          dtab_apply_f<T, RB>(a, A.pre, A.tile_tab[kThrTabOff + p.tm]);
         }
       }
       const uint32_t gm = A.gmask[s];
@@ -923,13 +979,14 @@ k_rot_relabel(typename Vec2<T>::type *dst, const RotArgs<T> A, const typename Ve
       if (s == last) break;
       if (s == so) {
         if (A.has_post) {
-          dtab_apply_f<T, RB>(a, A.post, thread_factor<T>(A.post, p.tm));
+          dtab_apply_f<T, RB>(a, A.post, A.tile_tab[kThrTabOff + kThrTabLen + p.tm]);
         }
         if (F.ph.nwin) {
           // this thread's index in segment so (old frame) without its register bits
+          const uint64_t gb = seg_index(sg, p.tm, base);
           uint32_t ix[kFusedWins];
 #pragma unroll
-          for (int w = 0; w < kFusedWins; ++w) ix[w] = win_bits(F.ph.win[w], p.gb);
+          for (int w = 0; w < kFusedWins; ++w) ix[w] = win_bits(F.ph.win[w], gb);
           fused_diag<T, RB>(a, F, lut, ix);
         }
         const uint64_t e = F.pc ^ ptab[tile & 255u] ^ ptab[256 + ((tile >> 8) & 255u)] ^
@@ -1737,6 +1794,15 @@ __global__ void __launch_bounds__(256) k_inner(const typename Vec2<Ta>::type *__
 // launchers
 // ---------------------------------------------------------------------------
 
+// Dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU) needs the kernel's
+// limit raised; done once per kernel.
+template <typename K>
+hipError_t allow_lds(K kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+}
+
 template <typename T>
 hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, int rb,
                        uint32_t grid, hipStream_t stream) {
@@ -1761,6 +1827,8 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
       break;
     case 5:
       if constexpr (TileCfg<T>::kRB >= 5) {
+        const hipError_t e = allow_lds(k_tile<T, 5>, lds);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_tile<T, 5>), dim3(grid), dim3(threads), lds, stream, state, args);
         break;
       }
@@ -1768,15 +1836,6 @@ hipError_t launch_tile(typename Vec2<T>::type *state, const TileArgs<T> &args, i
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
-}
-
-// Dynamic LDS beyond 64 KiB (gfx950 has 160 KiB per CU) needs the kernel's
-// limit raised; done once per kernel.
-template <typename K>
-hipError_t allow_lds(K kernel, size_t lds) {
-  if (lds <= 65536) return hipSuccess;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
 }
 
 template <typename T, int RB>
@@ -1793,6 +1852,8 @@ hipError_t launch_rot_rb(typename Vec2<T>::type *state, const RotArgs<T> &args, 
     hipLaunchKernelGGL((k_rot<T, RB, true>), dim3(grid), dim3(threads), lds, stream, state,
                        args.tile_tab, args, perm_src, ph_lut, ptab);
   } else {
+    const hipError_t e = allow_lds(k_rot<T, RB, false>, xlds);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_rot<T, RB, false>), dim3(grid), dim3(threads), xlds, stream, state,
                        args.tile_tab, args, static_cast<const V *>(nullptr), static_cast<const V *>(nullptr),
                        static_cast<const uint64_t *>(nullptr));
@@ -1871,6 +1932,11 @@ hipError_t launch_qft(typename Vec2<T>::type *state, const QftArgs<T> &args, int
   const V *nov = nullptr;
   const uint64_t *nou = nullptr;
 #define MSV_QFT_LAUNCH(RBV)                                                                          \
+  {                                                                                                  \
+    const hipError_t e = inverse ? allow_lds(k_qft<T, RBV, true, false>, lds)                        \
+                                 : allow_lds(k_qft<T, RBV, false, false>, lds);                      \
+    if (e != hipSuccess) return e;                                                                   \
+  }                                                                                                  \
   if (inverse)                                                                                       \
     hipLaunchKernelGGL((k_qft<T, RBV, true, false>), dim3(grid), dim3(threads), lds, stream, state,  \
                        args, nov, nov, nou);                                                         \

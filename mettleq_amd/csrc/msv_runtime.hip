@@ -84,6 +84,7 @@ struct Tuning {
   int reorder_gates = 1;    // gate lists: commuting gates are regrouped into fewer tile programs
   int defer_tail = 1;       // 1q layers leave their last sweep's rotations to the next layer's first sweep
                             // (1: without a pending permutation only up to 64 GiB states, 2: always)
+  int lane_swap = 1;        // rotation sweeps reach lane bits 4 and 5 through in-wavefront swaps
 };
 Tuning g_tune_store;  // written by msv_set_tuning under g_tune_mu
 std::mutex g_tune_mu;
@@ -375,13 +376,18 @@ template <typename T> int rb_of() {
 template <typename T> int local_max() {
   return std::is_same<T, float>::value ? tune().local_bits_c64 : tune().local_bits_c128;
 }
+// Local bits of a tile: the tuned size, at most what one 512-thread block
+// holds (9 thread-id bits + the register bits).
+template <typename T> int max_local() {
+  return std::min(std::min(local_max<T>(), TileCfg<T>::kMaxLocal), kTileThreadBits + rb_of<T>());
+}
 template <typename T> int col_bits() { return std::is_same<T, float>::value ? tune().col_bits_c64 : 3; }
 
 // Partition of index bits [0, n) into a low sweep [0, L) and high chunks of
 // between rb and gate_bits bits each (ascending).
 template <typename T>
 void partition(int n, int *low_len, std::vector<std::pair<int, int>> *chunks) {
-  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int NL = max_local<T>();
   const int RB = rb_of<T>();
   const int G = std::max(RB, std::min(tune().gate_bits, NL - col_bits<T>()));
   chunks->clear();
@@ -410,7 +416,7 @@ void partition(int n, int *low_len, std::vector<std::pair<int, int>> *chunks) {
 // leave room in the tile, for longer contiguous rows), then the gates.
 template <typename T>
 std::vector<int> high_sweep_local_bits(const std::vector<int> &gates) {
-  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int NL = max_local<T>();
   const int lowest = gates.empty() ? 0 : gates.front();
   const int C = std::min(std::max(col_bits<T>(), NL - static_cast<int>(gates.size())), lowest);
   std::vector<int> local;
@@ -702,6 +708,76 @@ std::vector<int> commuting_windows(uint32_t need, int nlocal, int rb, int gf) {
   return {};
 }
 
+// Segments of a rotation sweep (k_rot): register windows and, per segment,
+// the local bits that lane bits 4 and 5 bring into register bits 0 and 1
+// by in-wavefront swaps after the window's rotations (-1: no swap). With
+// the swaps a segment reaches 6 of the 13 bits of a 512-thread tile
+// without LDS, so a sweep of 9 gate bits needs one LDS exchange instead of
+// two. Swap-ins are bits no window covers, taken from the segment's
+// non-window bits above its lowest 4 (thread bits 0-3 keep those, so the
+// first and last segments stay coalesced).
+struct RotSegs {
+  std::vector<int> wins;
+  std::vector<std::array<int, 2>> swaps;
+};
+
+// Thread-bit order of a segment with register window [w, w + rb): the
+// non-window local bits ascending, except that thread bits 4 and 5 hold
+// the swap-ins x[0] and x[1] when they are set.
+std::vector<int> seg_thread_order(int nlocal, int w, int rb, const std::array<int, 2> &x) {
+  std::vector<int> nw;
+  for (int k = 0; k < nlocal; ++k)
+    if (k < w || k >= w + rb) nw.push_back(k);
+  const size_t lo = std::min<size_t>(4, nw.size());
+  std::vector<int> ord(nw.begin(), nw.begin() + lo);
+  std::vector<int> rest;
+  for (size_t i = lo; i < nw.size(); ++i)
+    if (nw[i] != x[0] && nw[i] != x[1]) rest.push_back(nw[i]);
+  size_t at = 0;
+  for (int i = 0; i < 2 && ord.size() < nw.size(); ++i) ord.push_back(x[i] >= 0 ? x[i] : rest[at++]);
+  for (; at < rest.size(); ++at) ord.push_back(rest[at]);
+  return ord;
+}
+
+RotSegs rot_segments(uint32_t need, int nlocal, int rb, int gf, bool lanes) {
+  RotSegs plain;
+  plain.wins = commuting_windows(need, nlocal, rb, gf);
+  plain.swaps.assign(plain.wins.size(), {-1, -1});
+  if (!lanes || rb < 2 || nlocal - rb < 6 || plain.wins.size() < 2) return plain;
+  const int nw = nlocal - rb + 1;
+  const uint32_t wmask = (1u << rb) - 1u;
+  // only plans with fewer segments than the plain cover are worth swaps
+  for (int m = 1; m < static_cast<int>(plain.wins.size()); ++m) {
+    std::vector<int> w(m, gf);
+    for (int i = 1; i + 1 < m; ++i) w[i] = 0;
+    while (true) {
+      uint32_t cov = 0;
+      for (int x : w) cov |= wmask << x;
+      std::vector<std::array<int, 2>> sw(m, {-1, -1});
+      for (int i = 0; i < m; ++i) {
+        int got = 0, seen = 0;
+        for (int k = 0; k < nlocal && got < 2; ++k) {
+          if (k >= w[i] && k < w[i] + rb) continue;
+          if (seen++ < 4) continue;  // the segment's thread bits 0-3
+          if (((need >> k) & 1u) && !((cov >> k) & 1u)) {
+            sw[i][got++] = k;
+            cov |= 1u << k;
+          }
+        }
+      }
+      if ((cov & need) == need) return {w, sw};
+      int i = 0;
+      for (; i < m; ++i) {
+        const int lo = (i == 0 || i == m - 1) ? gf : 0;
+        if (++w[i] < nw) break;
+        w[i] = lo;
+      }
+      if (i == m) break;
+    }
+  }
+  return plain;
+}
+
 // Commuting per-bit ops (one op per gate local bit, any order).
 // `pre` runs before every op of the sweep (first segment), `post` after
 // every op (last segment); both are diagonals (or absent: kind 0).
@@ -850,10 +926,14 @@ int fill_tile_geometry(int n, const std::vector<int> &local, const std::vector<i
 // factors in `tab` (register entries at 0..63, thread-bit factors at
 // 64..81, as dtab_apply_g reads them), tile-number groups in `tgrp`
 // (kMaxTileGroups x 16 entries, unit past ntgrp).
+// rloc: local bit of each register bit (null: the window sg.reg0 + j).
+// thr: receives the thread part, kThrTabLen entries (entry tm = the product
+// of the factors of the set bits of tm), the table the kernels index.
 template <typename T, typename Args>
 void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &local, int rb,
                       const std::vector<std::complex<double>> &ratio, std::complex<double> cst,
-                      T *tab, int ntgrp, std::vector<typename Vec2<T>::type> *tgrp) {
+                      T *tab, int ntgrp, std::vector<typename Vec2<T>::type> *tgrp,
+                      std::vector<typename Vec2<T>::type> *thr, const int *rloc = nullptr) {
   using C = std::complex<double>;
   using V = typename Vec2<T>::type;
   const int nlocal = static_cast<int>(local.size());
@@ -862,16 +942,27 @@ void build_layer_diag(const Args &A, const TileSeg &sg, const std::vector<int> &
     if (r < (1 << rb)) {
       d = cst;
       for (int j = 0; j < rb; ++j)
-        if ((r >> j) & 1) d *= ratio[local[sg.reg0 + j]];
+        if ((r >> j) & 1) d *= ratio[local[rloc ? rloc[j] : static_cast<int>(sg.reg0) + j]];
     }
     tab[2 * r] = static_cast<T>(d.real());
     tab[2 * r + 1] = static_cast<T>(d.imag());
   }
+  C q[kTileThreadBits];
   for (int k = 0; k < kTileThreadBits; ++k) {
-    C q = 1.0;
-    if (k < nlocal - rb) q = ratio[local[sg.tl_bit[k]]];
-    tab[64 + 2 * k] = static_cast<T>(q.real());
-    tab[65 + 2 * k] = static_cast<T>(q.imag());
+    q[k] = 1.0;
+    if (k < nlocal - rb) q[k] = ratio[local[sg.tl_bit[k]]];
+    tab[64 + 2 * k] = static_cast<T>(q[k].real());
+    tab[65 + 2 * k] = static_cast<T>(q[k].imag());
+  }
+  {
+    std::vector<C> f(kThrTabLen, 1.0);
+    for (int tm = 1; tm < kThrTabLen; ++tm) f[tm] = f[tm & (tm - 1)] * q[__builtin_ctz(tm)];
+    for (int tm = 0; tm < kThrTabLen; ++tm) {
+      V e;
+      e.x = static_cast<T>(f[tm].real());
+      e.y = static_cast<T>(f[tm].imag());
+      thr->push_back(e);
+    }
   }
   // tile-number bit t lands on index bit run_dst + (t - run_src)
   std::vector<int> tbit_index;
@@ -1074,24 +1165,57 @@ int rot_sweep_T(msv_state *s, const std::vector<int> &local, const std::vector<i
     if (std::find(gates.begin(), gates.end(), local[k]) != gates.end()) gl.push_back(k);
   uint32_t need = 0;
   for (int k : gl) need |= 1u << k;
-  const std::vector<int> wins = commuting_windows(need, nlocal, rb, good_from(nlocal, rb));
+  const RotSegs plan = rot_segments(need, nlocal, rb, good_from(nlocal, rb), tune().lane_swap != 0);
+  const std::vector<int> &wins = plan.wins;
   if (wins.empty()) return fail(MSV_E_INVALID, "internal: no register window cover");
   RotArgs<T> A;
   std::memset(&A, 0, sizeof(A));
   MSV_TRY(fill_tile_geometry(n, local, wins, rb, &A));
   assign_rotations<T>(A, local, gl, wins, 0, rb, sh);
+  int rloc[5] = {0, 0, 0, 0, 0};  // the last segment's register bits after its swaps
+  for (size_t si = 0; si < wins.size(); ++si) {
+    TileSeg &sg = A.seg[si];
+    const std::array<int, 2> &x = plan.swaps[si];
+    // thread bits 4 and 5 hold the swap-ins before the swaps
+    const std::vector<int> ord = seg_thread_order(nlocal, wins[si], rb, x);
+    for (size_t t = 0; t < ord.size(); ++t) {
+      sg.tl_bit[t] = static_cast<uint32_t>(ord[t]);
+      sg.gi_bit[t] = static_cast<uint32_t>(local[ord[t]]);
+    }
+    int rl[5];
+    for (int j = 0; j < rb; ++j) rl[j] = wins[si] + j;
+    TileSeg after = sg;
+    for (int i = 0; i < 2; ++i) {
+      if (x[i] < 0) continue;
+      // register bit i <-> thread bit 4 + i
+      A.swp[si] |= 1u << i;
+      A.tfix[si][i] = (1u << x[i]) ^ (1u << (wins[si] + i));
+      A.shear2[si][i][0] = static_cast<T>(sh[local[x[i]]][0]);
+      A.shear2[si][i][1] = static_cast<T>(sh[local[x[i]]][1]);
+      rl[i] = x[i];
+      after.tl_bit[4 + i] = static_cast<uint32_t>(wins[si] + i);
+      after.gi_bit[4 + i] = static_cast<uint32_t>(local[wins[si] + i]);
+      after.gstr[i] = 1ull << local[x[i]];
+    }
+    for (int j = 0; j < rb; ++j) A.wimg[si][j] = 1u << rl[j];
+    if (si + 1 == wins.size()) {
+      A.out = after;
+      for (int j = 0; j < rb; ++j) rloc[j] = rl[j];
+    }
+  }
   const int ntgrp = (n - nlocal + 3) / 4;
   if (ntgrp > kMaxTileGroups) return fail(MSV_E_INVALID, "internal: too many tile-number bits");
   A.ntgrp = static_cast<uint32_t>(ntgrp);
-  std::vector<V> tgrp;
-  tgrp.reserve(2 * 16 * kMaxTileGroups);
+  std::vector<V> tgrp, thr;
+  tgrp.reserve(2 * 16 * kMaxTileGroups + 2 * kThrTabLen);
   const std::vector<C> one(n, 1.0);
   A.has_pre = pre ? 1u : 0u;
-  build_layer_diag<T>(A, A.seg[0], local, rb, pre ? *pre : one, 1.0, A.pre, ntgrp, &tgrp);
+  build_layer_diag<T>(A, A.seg[0], local, rb, pre ? *pre : one, 1.0, A.pre, ntgrp, &tgrp, &thr);
   A.has_post = post ? 1u : 0u;
-  build_layer_diag<T>(A, A.seg[wins.size() - 1], local, rb, post ? *post : one, post ? cst : C(1.0), A.post,
-                      ntgrp, &tgrp);
+  build_layer_diag<T>(A, A.out, local, rb, post ? *post : one, post ? cst : C(1.0), A.post, ntgrp, &tgrp,
+                      &thr, rloc);
   if (pre || post) {
+    tgrp.insert(tgrp.end(), thr.begin(), thr.end());  // thread parts at kThrTabOff
     void *d = nullptr;
     MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &d));
     A.tile_tab = reinterpret_cast<const V *>(d);
@@ -1576,12 +1700,13 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
   A.ntgrp = 0;
   std::vector<C> pre_n(n + 1, 1.0);
   for (int k : gl_n) pre_n[nloc[k]] = pre[nloc[k]];
-  std::vector<V> tgrp;
-  tgrp.reserve(2 * 16 * kMaxTileGroups);
+  std::vector<V> tgrp, thr;
+  tgrp.reserve(2 * 16 * kMaxTileGroups + 2 * kThrTabLen);
   A.has_pre = pre_on ? 1u : 0u;
-  build_layer_diag<T>(A, A.seg[so + 1], nloc, rb, pre_n, 1.0, A.pre, 0, &tgrp);
+  build_layer_diag<T>(A, A.seg[so + 1], nloc, rb, pre_n, 1.0, A.pre, 0, &tgrp, &thr);
   A.has_post = 1u;
-  build_layer_diag<T>(A, A.seg[so], t.vloc, rb, s->tail_post, s->tail_cst, A.post, 0, &tgrp);
+  build_layer_diag<T>(A, A.seg[so], t.vloc, rb, s->tail_post, s->tail_cst, A.post, 0, &tgrp, &thr);
+  tgrp.insert(tgrp.end(), thr.begin(), thr.end());  // thread parts at kThrTabOff
   // the crossing: images Q u of segment so's bits, per-tile rep | delta << 48
   const TileSeg &sgo = A.seg[so];
   for (int k = 0; k < kTileThreadBits; ++k)
@@ -1718,7 +1843,11 @@ int cross_sweep_T(msv_state *s, const CrossTile &t, const std::vector<bool> &act
     }
   }
   if (!launch) return MSV_OK;
-  A.tile_tab = nullptr;
+  {
+    void *dt = nullptr;
+    MSV_TRY(stage(s, tgrp.data(), tgrp.size() * sizeof(V), &dt));
+    A.tile_tab = reinterpret_cast<const V *>(dt);
+  }
   std::vector<uint64_t> ptab(8 * 256, 0);
   for (int b = 0; b < 4; ++b)
     for (int v = 0; v < 256; ++v) {
@@ -1788,7 +1917,7 @@ void split_rest(int n, const std::vector<int> &rest, int G, bool defer, const st
   sets->clear();
   tail->clear();
   std::vector<int> plain = rest;
-  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int NL = max_local<T>();
   const int cb = std::max(0, std::min(col_bits<T>(), n));
   if (defer && rest.size() > 1) {
     const int k = std::min<int>(G, static_cast<int>(rest.size()) - 1);
@@ -1856,7 +1985,7 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
     cst *= zb[b].a1;
     pre_on |= std::abs(zb[b].b2 - 1.0) > 0.0;
   }
-  const int NL = std::min(local_max<T>(), TileCfg<T>::kMaxLocal);
+  const int NL = max_local<T>();
   const int cb = std::max(0, std::min(col_bits<T>(), n));
   const int G = std::max(rb_of<T>(), std::min(tune().gate_bits, NL - col_bits<T>()));
   const bool defer = tune().defer_tail != 0;
@@ -1943,7 +2072,8 @@ int run_rot_layer(msv_state *s, const std::vector<bool> &act_bit, const std::vec
     uint32_t need = 0;
     for (int k = 0; k < nl0; ++k)
       if (std::find(std_sets[0].begin(), std_sets[0].end(), local0[k]) != std_sets[0].end()) need |= 1u << k;
-    const std::vector<int> w0 = commuting_windows(need, nl0, rb0, good_from(nl0, rb0));
+    const std::vector<int> w0 =
+        rot_segments(need, nl0, rb0, good_from(nl0, rb0), tune().lane_swap != 0).wins;
     fuse = !w0.empty() && can_fuse_read<T>(s, local0, w0[0], rb0);
   }
   if (!fuse) MSV_TRY(flush_pending_T<T>(s));
@@ -2200,7 +2330,7 @@ template <typename T>
 int flush_program(msv_state *s, const std::vector<PlanOp> &ops) {
   if (ops.empty()) return MSV_OK;
   const int n = s->n;
-  const int NL = std::min(std::min(local_max<T>(), TileCfg<T>::kMaxLocal), n);
+  const int NL = std::min(max_local<T>(), n);
   std::vector<int> targets;
   for (const PlanOp &po : ops)
     if (po.tbit >= 0 && std::find(targets.begin(), targets.end(), po.tbit) == targets.end())
@@ -2280,7 +2410,7 @@ int flush_program(msv_state *s, const std::vector<PlanOp> &ops) {
 template <typename T>
 int count_segments(const msv_state *s, const std::vector<PlanOp> &ops) {
   const int n = s->n;
-  const int NL = std::min(std::min(local_max<T>(), TileCfg<T>::kMaxLocal), n);
+  const int NL = std::min(max_local<T>(), n);
   std::vector<int> targets;
   for (const PlanOp &po : ops)
     if (po.tbit >= 0 && std::find(targets.begin(), targets.end(), po.tbit) == targets.end())
@@ -3806,6 +3936,10 @@ int msv_set_tuning(int key, int64_t value) {
       if (value < 0 || value > 2) return fail(MSV_E_INVALID, "defer tail must be 0, 1 or 2");
       g_tune_store.defer_tail = static_cast<int>(value);
       return MSV_OK;
+    case MSV_TUNE_LANE_SWAP:
+      if (value < 0 || value > 1) return fail(MSV_E_INVALID, "lane swap must be 0 or 1");
+      g_tune_store.lane_swap = static_cast<int>(value);
+      return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }
@@ -3830,6 +3964,7 @@ int msv_get_tuning(int key, int64_t *value) {
     case MSV_TUNE_FUSED_BLOCKS_PER_CU: *value = g_tune_store.fused_blocks_per_cu; return MSV_OK;
     case MSV_TUNE_REORDER_GATES: *value = g_tune_store.reorder_gates; return MSV_OK;
     case MSV_TUNE_DEFER_TAIL: *value = g_tune_store.defer_tail; return MSV_OK;
+    case MSV_TUNE_LANE_SWAP: *value = g_tune_store.lane_swap; return MSV_OK;
     default: return fail(MSV_E_INVALID, "unknown tuning key");
   }
 }

@@ -69,6 +69,50 @@ def test_1q_layer_matches_oracle(gpu, n, dtype):
     assert np.max(np.abs(got - want)) < tol
 
 
+@pytest.mark.parametrize("lane_swap", [0, 1])
+@pytest.mark.parametrize("n", [10, 13, 16, 21])
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_1q_layer_lane_swaps(gpu, lane_swap, n, dtype):
+    """Rotation sweeps whose segments reach lane bits 4 and 5 through
+    in-wavefront swaps (MSV_TUNE_LANE_SWAP=1) against the oracle, and
+    against the LDS-only segments: a plain 1q layer, a subset of its bits,
+    and a layer read through a deferred RZZ ring + CNOT chain (fused first
+    sweep). The swaps change where segments meet, never the sweep count."""
+    from mettleq_amd import _native as nat
+    rng = np.random.default_rng(900 + n)
+    psi = _rand_state(n, n + 7)
+    mats = [_rand_u(rng) for _ in range(n)]
+    subset = sorted(int(q) for q in rng.choice(n, max(1, (2 * n) // 3), replace=False))
+    thetas = rng.uniform(0, math.pi, n)
+    bonds = [(i, (i + 1) % n) for i in range(n)]
+    rows, c = _chain(n)
+    want = O.apply_1q_layer(psi, n, mats)
+    sub = [mats[q] if q in subset else np.eye(2) for q in range(n)]
+    want_sub = O.apply_1q_layer(psi, n, sub)
+    fused = O.xor_affine_gather(O.zz_layer_diag(n, bonds, thetas) * psi, n, rows, c)
+    fused = O.apply_1q_layer(fused, n, mats)
+    tol = TOL64 if dtype == "complex64" else TOL128
+    with _Tuned(LANE_SWAP=lane_swap, DEFER_TAIL=0), _state(n, dtype) as s:
+        s.from_numpy(psi)
+        s0 = s.sweep_count()
+        s.apply_1q_layer(mats)
+        sweeps = s.sweep_count() - s0
+        assert np.max(np.abs(s.to_numpy() - want)) < tol
+        s.from_numpy(psi)
+        s.apply_1q_layer(mats, subset)
+        assert np.max(np.abs(s.to_numpy() - want_sub)) < tol
+        s.from_numpy(psi)
+        s.apply_phase_layer([nat.MSV_PHASE_ZZ] * n, bonds, list(thetas))
+        s.apply_xor_affine(rows, c)
+        s.apply_1q_layer(mats)
+        assert np.max(np.abs(s.to_numpy() - fused)) < tol
+    with _Tuned(LANE_SWAP=1 - lane_swap, DEFER_TAIL=0), _state(n, dtype) as s:
+        s.from_numpy(psi)
+        s0 = s.sweep_count()
+        s.apply_1q_layer(mats)
+        assert s.sweep_count() - s0 == sweeps
+
+
 @pytest.mark.parametrize("reg_bits", [4, 5])
 @pytest.mark.parametrize("n", [13, 15, 19])
 def test_tile_register_bits_both_shapes(gpu, reg_bits, n):
