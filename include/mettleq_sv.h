@@ -447,6 +447,61 @@ int msv_state_timer_stop(msv_state *s, double *ms);
  * HBM bandwidth and hardware byte counters. No reference counterpart. */
 int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_per_copy);
 
+/* ---- matrix product states ---------------------------------------------- */
+/* The reference's MPS method (mps_state.py:250-1549 with its native core
+ * _mps_native.cpp): site tensors (dl, 2, dr) in complex64, resident on the
+ * device; two-site updates through an SVD on the device (rocSOLVER) with the
+ * reference's truncation rule; the orthogonality center moves before each
+ * update like the reference's QR moves. Site q is qubit q. */
+typedef struct msv_mps msv_mps;
+
+typedef struct {
+  int64_t svd_calls;        /* SVDs run (two-site updates and center moves) */
+  int64_t two_site_calls;   /* two-site updates (gates and swaps) */
+  int64_t swap_calls;       /* swaps of the routing network */
+  int64_t trunc_events;     /* two-site updates that kept fewer than their rank */
+  int64_t truncated;        /* 1 once any update truncated */
+  int64_t max_bond_ever;    /* largest bond dimension reached */
+  int64_t renormalization_count;
+  int64_t center;           /* site of the orthogonality center */
+  double local_discarded_weight_sum;    /* sum of dropped s^2 over events */
+  double local_discarded_weight_max;
+  double relative_discarded_weight_sum; /* dropped s^2 / total s^2 */
+  double relative_discarded_weight_max;
+} msv_mps_stats;
+
+/* MPSState.__init__ / MPSOptions(dmax, eps, renormalize_splits)
+ * (mps_state.py:57-64, 256-286): |0...0> with every bond 1. */
+int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int device, msv_mps **out);
+int msv_mps_destroy(msv_mps *m);
+/* MPSState.reset (mps_state.py:288-332). */
+int msv_mps_reset(msv_mps *m);
+/* MPSState.apply_single (mps_state.py:546-566): u is the 2x2 row-major
+ * complex matrix as 8 doubles (re, im). */
+int msv_mps_apply_1q(msv_mps *m, int q, const double *u);
+/* MPSState.apply_two (mps_state.py:744-768): u is the 4x4 row-major complex
+ * matrix (32 doubles) in the basis |q0 q1>; non-adjacent operands are
+ * brought together by a swap network and restored after the gate. */
+int msv_mps_apply_2q(msv_mps *m, int q0, int q1, const double *u);
+/* MPSState.bond_dims (mps_state.py:1119-1120): n - 1 entries. */
+int msv_mps_bond_dims(msv_mps *m, int32_t *out);
+/* MPSState.truncation_diagnostics counters (mps_state.py:1137-1230). */
+int msv_mps_get_stats(msv_mps *m, msv_mps_stats *out);
+/* Copies site q (dl x 2 x dr complex64, row-major) to host; host may be null
+ * to query the shape. Test and diagnostics path. */
+int msv_mps_site_tensor(msv_mps *m, int q, void *host, int32_t *dl, int32_t *dr);
+/* MPSState.norm (mps_state.py:1284-1288): the norm of the center tensor. */
+int msv_mps_norm(msv_mps *m, double *out);
+/* MPSState.expectation_product (mps_state.py:1301-1330): <psi| x_i O_i |psi>
+ * / <psi|psi> for 2x2 operators ops (8 doubles each) on distinct wires,
+ * by left-to-right transfer matrices on the device. */
+int msv_mps_expval_product(msv_mps *m, const int32_t *wires, const double *ops, int k, double *re,
+                           double *im);
+/* MPSState.to_statevector (mps_state.py:1417-1440): contracts the MPS into
+ * dst (a complex64 state of the same width), normalized. */
+int msv_mps_to_state(msv_mps *m, msv_state *dst);
+int msv_mps_sync(msv_mps *m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,23 @@ class msv_device_info(ctypes.Structure):
     ]
 
 
+class msv_mps_stats(ctypes.Structure):
+    _fields_ = [
+        ("svd_calls", ctypes.c_int64),
+        ("two_site_calls", ctypes.c_int64),
+        ("swap_calls", ctypes.c_int64),
+        ("trunc_events", ctypes.c_int64),
+        ("truncated", ctypes.c_int64),
+        ("max_bond_ever", ctypes.c_int64),
+        ("renormalization_count", ctypes.c_int64),
+        ("center", ctypes.c_int64),
+        ("local_discarded_weight_sum", ctypes.c_double),
+        ("local_discarded_weight_max", ctypes.c_double),
+        ("relative_discarded_weight_sum", ctypes.c_double),
+        ("relative_discarded_weight_max", ctypes.c_double),
+    ]
+
+
 class msv_preflight_limits(ctypes.Structure):
     _fields_ = [
         ("max_buffer_bytes", ctypes.c_uint64),
@@ -192,6 +209,18 @@ SIGNATURES = {
     "msv_ansatz_evaluate": (_I, [_P, _DP, _I, _I, _DP, _DP]),
     "msv_ansatz_info": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_int64), _I]),
     "msv_ansatz_time_sweeps": (_I, [_P, _I, _I, _I, ctypes.POINTER(ctypes.c_double)]),
+    "msv_mps_create": (_I, [_I, _I, ctypes.c_double, _I, _I, ctypes.POINTER(_P)]),
+    "msv_mps_destroy": (_I, [_P]),
+    "msv_mps_reset": (_I, [_P]),
+    "msv_mps_apply_1q": (_I, [_P, _I, _DP]),
+    "msv_mps_apply_2q": (_I, [_P, _I, _I, _DP]),
+    "msv_mps_bond_dims": (_I, [_P, _I32P]),
+    "msv_mps_get_stats": (_I, [_P, ctypes.POINTER(msv_mps_stats)]),
+    "msv_mps_site_tensor": (_I, [_P, _I, _P, _I32P, _I32P]),
+    "msv_mps_norm": (_I, [_P, _DP]),
+    "msv_mps_expval_product": (_I, [_P, _I32P, _DP, _I, _DP, _DP]),
+    "msv_mps_to_state": (_I, [_P, _P]),
+    "msv_mps_sync": (_I, [_P]),
 }
 
 _lib = None

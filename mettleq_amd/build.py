@@ -27,7 +27,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libmettleq_sv.so"
-SOURCES = ["msv_kernels.hip", "msv_runtime.hip", "msv_batch.hip", "msv_preflight.hip"]
+SOURCES = ["msv_kernels.hip", "msv_runtime.hip", "msv_batch.hip", "msv_preflight.hip", "msv_mps.hip"]
 HEADERS = ["msv_internal.h", "msv_device.h", "msv_philox.h"]
 
 HIP_FLAGS = [
@@ -51,7 +51,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     cmd = [hipcc(), *HIP_FLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(OUT) + ".tmp",
-           "-L/opt/rocm/lib", "-lhsa-runtime64"]
+           "-L/opt/rocm/lib", "-lhsa-runtime64", "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
