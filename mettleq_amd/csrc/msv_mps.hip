@@ -342,6 +342,8 @@ int move_center(msv_mps *m, int target) {
 // (mps_state.py:193-247: keep r = min(rank, max(1, min(dmax, #{s >= eps s_0}))),
 // discarded weight = sum of the dropped s^2, singular values divided by the
 // kept norm when renormalizing), split with the center at i + 1.
+constexpr double kZeroFloor = 1e-30;  // relative singular value treated as an exact zero
+
 int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
   MPS_TRY(move_center(m, i));
   const int dl = m->dim[i], dm = m->dim[i + 1], dr = m->dim[i + 2];
@@ -353,13 +355,21 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
   const int rows = 2 * dl, cols = 2 * dr, kmin = std::min(rows, cols);
   MPS_TRY(svd(m, rows, cols));
   const std::vector<float> &s = m->h_sv;
-  int r_eps = kmin;
+  // Exact zeros are dropped even at eps = 0: the device SVD's vectors for
+  // them are not reliable, and they carry no weight. At eps = 0 (where the
+  // reference keeps every value) dropping them is no truncation event; at
+  // eps > 0 an event is any split below the full rank, as in the reference.
+  int r_nz = kmin, r_eps = kmin;
   if (kmin > 0) {
-    const double thresh = eps * static_cast<double>(s[0]);
-    r_eps = 0;
-    for (int k = 0; k < kmin; ++k) r_eps += static_cast<double>(s[k]) >= thresh ? 1 : 0;
+    r_nz = r_eps = 0;
+    const double s0 = static_cast<double>(s[0]);
+    for (int k = 0; k < kmin; ++k) {
+      r_nz += static_cast<double>(s[k]) >= kZeroFloor * s0 ? 1 : 0;
+      r_eps += static_cast<double>(s[k]) >= eps * s0 ? 1 : 0;
+    }
+    r_nz = std::max(1, r_nz);
   }
-  const int r = std::min(kmin, std::max(1, std::min(m->dmax, r_eps)));
+  const int r = std::min(r_nz, std::max(1, std::min(m->dmax, r_eps)));
   double total = 0, kept = 0;
   for (int k = 0; k < kmin; ++k) {
     const double w = static_cast<double>(s[k]) * s[k];
@@ -369,7 +379,7 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
   if (!(kept > 1e-300) || !std::isfinite(kept))
     return set_error(MSV_E_INVALID, "MPS truncation retained a zero or non-finite singular spectrum");
   const double discarded = total - kept;
-  if (r < kmin) {
+  if (r < (eps > 0.0 ? kmin : r_nz)) {
     m->st.truncated = 1;
     m->st.trunc_events += 1;
     m->st.local_discarded_weight_sum += discarded;

@@ -29,19 +29,26 @@ class Device:
         self.wires = int(wires)
         self.shots = int(shots)
         backend = (backend or os.environ.get("METTLEQ_BACKEND", "sv")).strip().lower()
-        if backend not in ("sv", "statevector"):
-            raise ValueError("this engine implements the exact statevector backend only ('sv')")
+        if backend not in ("sv", "statevector", "mps"):
+            raise ValueError("backend must be 'sv' (exact statevector) or 'mps' (matrix product state)")
         if execution_device is not None and str(execution_device).strip().lower() not in ("gpu", "auto"):
             raise ValueError("execution_device must be 'gpu' (the HIP engine) or None")
-        self.backend = "sv"
         self.execution_device = "gpu"
         self._allow_unsafe_statevector = allow_unsafe_statevector
         self.dtype = dtype
+        self.last_execution_plan: Optional[Dict[str, Any]] = None
+        if backend == "mps":
+            # device.py:91-128: the MPS method with MPSOptions(dmax, eps)
+            from .mps import MPSOptions, MPSState
+            self.backend = "mps"
+            self.sim = MPSState(self.wires, mps_opts or MPSOptions(), device=device_index)
+            self.statevector_preflight = None
+            return
+        self.backend = "sv"
         self.sim = StateVectorSimulator(self.wires, allow_unsafe_statevector=allow_unsafe_statevector,
                                         dtype=dtype, device=device_index,
                                         preflight_mode=preflight_mode, reserved_bytes=reserved_bytes)
         self.statevector_preflight = self.sim.preflight
-        self.last_execution_plan: Optional[Dict[str, Any]] = None
 
     # -- lifecycle ---------------------------------------------------------
     def reset(self) -> None:
@@ -52,9 +59,13 @@ class Device:
 
     @property
     def native(self):
+        if self.backend == "mps":
+            raise AttributeError("the MPS backend has no statevector; use Device.sim")
         return self.sim.native
 
     def synchronize(self):
+        if self.backend == "mps":
+            return self.sim
         if self.last_execution_plan is None:
             self.native.sync()
             return self.native
@@ -67,6 +78,15 @@ class Device:
     # -- execution ---------------------------------------------------------
     def execute(self, operations: List[Dict[str, Any]], *, report: Optional[bool] = None):
         ops = [dict(op) for op in operations]
+        if self.backend == "mps":
+            # op by op through the reference's routing to device gates
+            # (device.py:1138-1290); each gate is one MPS update
+            from .fusion import gates_for_op
+            from .mps import apply_device_gate
+            for op in ops:
+                for g in gates_for_op(op, self.wires):
+                    apply_device_gate(self.sim, g)
+            return self.sim
         fused = self.fuse(ops)
         if report is None:
             report = os.environ.get("METTLEQ_EXECUTION_REPORT", "0") == "1"

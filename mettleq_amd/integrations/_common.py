@@ -3,8 +3,9 @@
 Restates the reference's adapter seam (integrations/_common.py:29-427):
 canonical op validation, ``execute_operations`` with a per-width Device
 cache, global phase, statevector / marginal / sample read-back and Pauli
-expectations, all over the HIP engine. Only the exact statevector method is
-available here; requesting "mps" raises ValueError.
+expectations, all over the HIP engine: the exact statevector, or the MPS
+method (mettleq_amd.mps) when the caller asks for it or allows approximation
+(mettleq_amd.planning.select_execution).
 """
 from __future__ import annotations
 
@@ -50,15 +51,31 @@ def execute_operations(n_qubits: int, operations: Sequence[dict], *, shots: int 
                        report: bool = False, allow_unsafe_statevector: Optional[bool] = None,
                        method: str = "automatic", execution_device: str = "auto",
                        execution_cache: Optional[dict] = None, dtype: str = "complex64",
+                       allow_approximation: bool = False, mps_max_bond_dimension: int = 64,
+                       mps_truncation_threshold: float = 1e-10,
+                       automatic_mps_min_qubits: Optional[int] = None,
                        **_ignored) -> Device:
     """Execute canonical ops on a (cached per width) Device (_common.py:118-228)."""
-    method = str(method).strip().lower()
-    if method not in ("automatic", "statevector"):
-        raise ValueError("this engine provides method='statevector' (or 'automatic') only")
+    from ..planning import DEFAULT_AUTOMATIC_MPS_MIN_QUBITS, normalize_method, select_execution
+    method = normalize_method(method)
     device_key = str(execution_device).strip().lower()
     if device_key not in ("auto", "gpu"):
         raise ValueError("execution_device must be 'auto' or 'gpu' for the HIP engine")
     ops = list(operations)
+    if method == "matrix_product_state" or allow_approximation:
+        sel = select_execution(int(n_qubits), ops, method=method, device="auto", dtype=dtype,
+                               allow_approximation=bool(allow_approximation),
+                               allow_unsafe_statevector=allow_unsafe_statevector,
+                               automatic_mps_min_qubits=(DEFAULT_AUTOMATIC_MPS_MIN_QUBITS
+                                                         if automatic_mps_min_qubits is None
+                                                         else int(automatic_mps_min_qubits)))
+        if sel.selected_method == "matrix_product_state":
+            from ..mps import MPSOptions
+            opts = MPSOptions(dmax=int(mps_max_bond_dimension), eps=float(mps_truncation_threshold))
+            device = Device(n_qubits, shots=shots, backend="mps", mps_opts=opts)
+            device.execute(ops)
+            device.execution_selection = {**sel.to_dict(), "engine": "hip_mps"}
+            return device
     needs_second_buffer = any(str(op.get("name", "")).upper() in ("CNOT", "CX", "X", "SWAP")
                               for op in ops)
     mode = "out_of_place" if needs_second_buffer else "in_place"
@@ -83,11 +100,17 @@ def execute_operations(n_qubits: int, operations: Sequence[dict], *, shots: int 
 def apply_global_phase(device: Device, phase: float) -> None:
     """_common.py:231-238, as one device scale pass."""
     if phase:
-        device.native.scale(complex(np.cos(float(phase)), np.sin(float(phase))))
+        f = complex(np.cos(float(phase)), np.sin(float(phase)))
+        if device.backend == "mps":
+            device.sim.apply_single(np.eye(2) * f, 0)
+        else:
+            device.native.scale(f)
 
 
 def statevector_numpy(device: Device) -> np.ndarray:
     device.synchronize()
+    if device.backend == "mps":
+        return device.sim.to_statevector()
     return device.native.to_numpy().astype(np.complex64, copy=False)
 
 
@@ -127,6 +150,10 @@ def pauli_product_expectations(device: Device, words: Sequence[dict]) -> list:
             if p not in ("X", "Y", "Z"):
                 raise ValueError("Pauli products support only I, X, Y, and Z")
     device.synchronize()
+    if device.backend == "mps":
+        mats = {"X": np.array([[0, 1], [1, 0]], dtype=complex), "Y": np.array([[0, -1j], [1j, 0]]),
+                "Z": np.diag([1.0, -1.0]).astype(complex)}
+        return [device.sim.expectation_product({w: mats[p] for w, p in word.items()}) for word in normalized]
     vals = device.native.expval_pauli(normalized)
     return [complex(v) for v in vals]
 

@@ -69,14 +69,22 @@ class MettleQDevice(PennyLaneDevice):
 
     def __init__(self, wires, shots=None, *, seed=None, method: str = "automatic", device: str = "auto",
                  execution_report: bool = False, allow_unsafe_statevector: Optional[bool] = None,
-                 dtype: str = "complex64", **_ignored) -> None:
+                 dtype: str = "complex64", allow_approximation: bool = False,
+                 mps_max_bond_dimension: int = 64, mps_truncation_threshold: float = 1e-10,
+                 **_ignored) -> None:
         if wires is None:
             raise DeviceError("MettleQDevice requires an explicit wire set")
         super().__init__(wires=wires, shots=shots)
         if self.wires is None or len(self.wires) <= 0:
             raise DeviceError("MettleQDevice requires at least one wire")
-        if str(method).lower() not in ("automatic", "statevector"):
-            raise DeviceError("this engine provides method='statevector' (or 'automatic') only")
+        method = str(method).lower()
+        method = {"mps": "matrix_product_state", "sv": "statevector"}.get(method, method)
+        if method not in ("automatic", "statevector", "matrix_product_state"):
+            raise DeviceError("method must be one of: automatic, statevector, matrix_product_state")
+        self._method = method
+        self._mps_options = {"allow_approximation": bool(allow_approximation),
+                             "mps_max_bond_dimension": int(mps_max_bond_dimension),
+                             "mps_truncation_threshold": float(mps_truncation_threshold)}
         if str(device).lower() not in ("auto", "gpu"):
             raise DeviceError("device must be 'auto' or 'gpu' for the HIP engine")
         self._rng = np.random.default_rng(seed)
@@ -198,7 +206,8 @@ class MettleQDevice(PennyLaneDevice):
         device = execute_operations(len(self.wires), operations, shots=default_shots,
                                     report=self._execution_report,
                                     allow_unsafe_statevector=self._allow_unsafe_statevector,
-                                    execution_cache=cache, dtype=self._dtype)
+                                    execution_cache=cache, dtype=self._dtype, method=self._method,
+                                    **self._mps_options)
         apply_global_phase(device, phase)
         self.last_execution_plan = device.last_execution_plan
         self.statevector_preflight = device.statevector_preflight

@@ -10,8 +10,9 @@ on this engine:
   structural guess;
 * double precision stays on the GPU (complex128 is a native mode of every
   kernel here), where the reference must fall back to a CPU engine;
-* MPS is not part of this engine: Qiskit requests go to Aer's MPS, PennyLane
-  requests are refused with the reason;
+* MPS requests go to Aer's MPS for Qiskit (as in the reference, no GPU
+  crossover is established for MPS) and to this engine's device MPS
+  (mettleq_amd.mps) for PennyLane, where the reference uses its CPU MPS;
 * crossovers come from this engine's measured per-circuit latency (see
   mettleq_amd.planning): 16 qubits through Qiskit and 14 through PennyLane,
   where the SDK translation and fusion add less host time per gate.
@@ -166,15 +167,15 @@ def recommend_sdk_engine(
         reasons.append("high_planned_launch_density")
     cpu_engine = "qiskit_aer_cpu_statevector" if sdk == "qiskit" else "pennylane_lightning_cpu"
     gpu_engine = "mettleq_gpu_statevector" + ("_complex128" if precision == "double" else "")
+    mps_engine = "qiskit_aer_cpu_mps" if sdk == "qiskit" else "mettleq_gpu_mps"
     if method in {"matrix_product_state", "mps"}:
-        if sdk == "qiskit":
-            engine, reason = "qiskit_aer_cpu_mps", "MPS is not part of this engine; Aer's MPS is the CPU path"
-        else:
-            engine, reason = "refused", "MPS is not part of this engine and PennyLane has no CPU MPS fallback here"
+        engine = mps_engine
+        reason = ("MPS GPU crossover is not established; Aer's MPS is the Qiskit path" if sdk == "qiskit"
+                  else "this engine's MPS (device SVDs) is the PennyLane path")
     elif not preflight["allowed"]:
-        if sdk == "qiskit" and allow_approximation and profile.topology_class in {
+        if allow_approximation and profile.topology_class in {
                 "single_qubit_only", "nearest_neighbor", "mostly_local"}:
-            engine = "qiskit_aer_cpu_mps"
+            engine = mps_engine
             reason = "dense preflight refused; bounded local MPS fallback was explicitly allowed"
         else:
             engine = "refused"

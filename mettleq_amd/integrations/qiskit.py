@@ -44,15 +44,19 @@ _QISKIT_TO_CORE = {
     "rxx": "XXPHASE", "ryy": "YYPHASE", "rzz": "ZZPHASE",
 }
 _TARGET_GATES = tuple(n for n in _QISKIT_TO_CORE if n not in ("ccx", "cswap")) + ("id", "measure")
-_METHODS = {"automatic", "statevector"}
+_METHODS = {"automatic", "statevector", "matrix_product_state"}
+_MPS_OPTIONS = ("allow_approximation", "mps_max_bond_dimension", "mps_truncation_threshold",
+                "automatic_mps_min_qubits")
 
 
 def _normalize_method(method: str) -> str:
     m = str(method).strip().lower().replace("-", "_")
     if m in ("sv", "state_vector"):
         m = "statevector"
+    if m == "mps":
+        m = "matrix_product_state"
     if m not in _METHODS:
-        raise ValueError("this engine provides method='statevector' (or 'automatic') only")
+        raise ValueError("method must be one of: automatic, statevector, matrix_product_state")
     return m
 
 
@@ -89,9 +93,10 @@ class MettleQBackend(BackendV2):
                  device: str = "auto", allow_unsafe_statevector: Optional[bool] = None,
                  dtype: str = "complex64", metal_checkpoint_budget_bytes: Optional[int] = None,
                  **fields) -> None:
-        mps_fields = {k for k in list(fields) if k.startswith("mps_") or k in (
-            "allow_approximation", "statevector_gpu_min_qubits", "automatic_mps_min_qubits")}
-        for k in mps_fields:
+        # the MPS method's options (qiskit.py:214-316 of the reference); the
+        # reference's CPU-driver and routing options have no counterpart here
+        self._mps_options = {k: fields.pop(k) for k in list(fields) if k in _MPS_OPTIONS}
+        for k in [k for k in fields if k.startswith("mps_") or k == "statevector_gpu_min_qubits"]:
             fields.pop(k)
         super().__init__(provider=provider, name="mettleq",
                          description="MettleQ statevector simulator on AMD Instinct MI355X (HIP)",
@@ -212,7 +217,8 @@ class MettleQBackend(BackendV2):
                                     report=bool(options["execution_report"]),
                                     allow_unsafe_statevector=self._allow_unsafe_statevector,
                                     method=options["method"], execution_device=options["device"],
-                                    execution_cache=execution_cache, dtype=options.get("dtype", "complex64"))
+                                    execution_cache=execution_cache, dtype=options.get("dtype", "complex64"),
+                                    **self._mps_options)
         apply_global_phase(device, phase)
         return device, measured, phase
 

@@ -271,16 +271,57 @@ class MPSState:
         with self.to_native_state() as st:
             return np.asarray(st.probabilities(wires), dtype=np.float64)
 
-    def sample(self, shots: int, wires: Optional[List[int]] = None, seed: int = 0) -> np.ndarray:
-        """Shot indices over `wires` (MSB-first), drawn by the engine's sampler."""
-        with self.to_native_state() as st:
-            return st.sample(int(shots), wires, seed=seed)
-
-    def sample_counts(self, shots: int, wires: Optional[List[int]] = None, seed: int = 0) -> Dict[str, int]:
+    def sample_array(self, shots: int, wires: Optional[List[int]] = None, *, rng=None) -> np.ndarray:
+        """Shot bits, first wire = MSB (mps_state.py:1442-1540): the per-call
+        seed is drawn from ``rng`` like the statevector simulator's; the
+        engine's counter-based sampler draws from the contracted state."""
+        from .sim import unpack_bits
+        shots = int(shots)
+        if shots < 0:
+            raise ValueError("shots must be non-negative")
         wires = list(range(self.n)) if wires is None else [int(w) for w in wires]
-        idx = self.sample(shots, wires, seed)
+        if len(set(wires)) != len(wires) or any(w < 0 or w >= self.n for w in wires):
+            raise ValueError("Qubit index out of range or duplicated")
+        rng = np.random.default_rng() if rng is None else rng
+        seed = int(rng.integers(0, np.iinfo(np.uint32).max, dtype=np.uint32))
+        with self.to_native_state() as st:
+            packed = st.sample(shots, wires, seed=seed)
+        return unpack_bits(packed, len(wires))
+
+    def sample(self, shots: int, wires: Optional[List[int]] = None) -> List[List[int]]:
+        return self.sample_array(shots, wires).tolist()
+
+    def sample_counts(self, shots: int, wires: Optional[List[int]] = None) -> Dict[str, int]:
         counts: Dict[str, int] = {}
-        for v in idx:
-            key = format(int(v), f"0{len(wires)}b")
+        for bits in self.sample(shots, wires):
+            key = "".join(str(b) for b in bits)
             counts[key] = counts.get(key, 0) + 1
         return counts
+
+
+def apply_device_gate(mps: MPSState, g) -> None:
+    """One device-gate record (msv_gate, from the reference's op routing in
+    mettleq_amd.fusion.gates_for_op) on the MPS: a 2x2 with at most one
+    control, a one- or two-qubit diagonal, or an uncontrolled SWAP. Gates on
+    three qubits (Toffoli, Fredkin) are not MPS operations here."""
+    kind, nc = int(g.kind), int(g.ncontrols)
+    m = [complex(g.m[2 * i], g.m[2 * i + 1]) for i in range(4)]
+    if kind == nat.MSV_G_U:
+        u = np.array(m, dtype=np.complex128).reshape(2, 2)
+        if nc == 0:
+            mps.apply_single(u, g.q[0])
+        elif nc == 1:
+            cu = np.eye(4, dtype=np.complex128)
+            cu[2:, 2:] = u
+            mps.apply_two(cu, g.q[1], g.q[0])
+        else:
+            raise ValueError("the MPS method applies gates on at most two qubits")
+    elif kind == nat.MSV_G_DIAG1:
+        mps.apply_single(np.diag(m[:2]), g.q[0])
+    elif kind == nat.MSV_G_DIAG2:
+        mps.apply_two(np.diag(m), g.q[0], g.q[1])
+    elif kind == nat.MSV_G_SWAP and nc == 0:
+        sw = np.eye(4, dtype=np.complex128)[[0, 2, 1, 3]]
+        mps.apply_two(sw, g.q[0], g.q[1])
+    else:
+        raise ValueError("the MPS method applies gates on at most two qubits")

@@ -157,8 +157,10 @@ def test_mps_gate_forms_and_readout(gpu):
         pw = np.abs(got.astype(np.complex128)) ** 2
         pw = pw.reshape([2] * n).sum(axis=tuple(q for q in range(n) if q not in (0, 3))).reshape(-1)
         assert np.max(np.abs(p - pw)) < 1e-6
-        shots = mps.sample(500, [0, 3], seed=9)
-        assert shots.shape == (500,) and shots.max() < 4
+        shots = mps.sample_array(500, [0, 3], rng=np.random.default_rng(9))
+        assert shots.shape == (500, 2) and set(np.unique(shots)) <= {0, 1}
+        counts = mps.sample_counts(200, [3])
+        assert sum(counts.values()) == 200 and set(counts) <= {"0", "1"}
         nrm = mps.norm()
         mps.normalize()
         assert abs(mps.norm() - 1.0) < 1e-5 and nrm > 0
@@ -181,3 +183,76 @@ def test_mps_invalid_arguments(gpu):
             mps.expectation_product({0: np.eye(2), 7: np.eye(2)})
         mps.reset()
         assert mps.bond_dims() == [1, 1]
+
+
+def test_device_mps_backend_runs_op_lists(gpu):
+    """Device(backend="mps") runs the reference's op dicts (device.py:91-128,
+    1138-1290) on the MPS; the result matches the dense oracle."""
+    from mettleq_amd.device import Device
+    from mettleq_amd.mps import MPSOptions
+    n = 8
+    rng = np.random.default_rng(21)
+    names1 = ["H", "X", "S", "T", "RX", "RY", "RZ", "U3"]
+    ops = []
+    for _ in range(60):
+        if rng.random() < 0.5:
+            nm = names1[rng.integers(len(names1))]
+            params = {"RX": 1, "RY": 1, "RZ": 1, "U3": 3}.get(nm, 0)
+            ops.append({"name": nm, "wires": [int(rng.integers(n))],
+                        "parameters": [float(x) for x in rng.uniform(0, 6, params)]})
+        else:
+            nm = ["CNOT", "CZ", "CPHASE", "SWAP", "CRY", "ZZPHASE", "ISWAP"][rng.integers(7)]
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            params = [float(rng.uniform(0, 3))] if nm in ("CPHASE", "CRY", "ZZPHASE") else []
+            ops.append({"name": nm, "wires": [a, b], "parameters": params})
+    want = O.run_ops(n, ops)
+    dev = Device(n, backend="mps", mps_opts=MPSOptions(dmax=64, eps=0.0))
+    try:
+        dev.execute(ops)
+        got = dev.sim.to_statevector()
+        p = dev.sim.probabilities_array([2, 5])
+    finally:
+        dev.close()
+    assert _overlap(want, got) > 1 - 1e-5
+    pw = (np.abs(want) ** 2).reshape([2] * n).sum(axis=tuple(q for q in range(n) if q not in (2, 5))).reshape(-1)
+    assert np.max(np.abs(p - pw)) < 1e-5
+    with pytest.raises(ValueError):
+        Device(4, backend="mps").execute([{"name": "CCX", "wires": [0, 1, 2]}])
+
+
+def test_sdk_seam_runs_the_mps_method(gpu):
+    """execute_operations(method="mps") and the approximate fallback of an
+    automatic request (_common.py:118-228, planning.py:108-204): statevector,
+    marginals and Pauli words read from the MPS."""
+    from mettleq_amd.integrations import _common as C
+    n = 6
+    ops = [C.core_operation("H", [0])] + [C.core_operation("CNOT", [i, i + 1]) for i in range(n - 1)]
+    ops += [C.core_operation("RY", [2], [0.4]), C.core_operation("CZ", [1, 4])]
+    want = O.run_ops(n, ops)
+    dev = C.execute_operations(n, ops, method="mps")
+    try:
+        assert dev.backend == "mps"
+        assert dev.execution_selection["selected_method"] == "matrix_product_state"
+        got = C.statevector_numpy(dev)
+        assert _overlap(want, got) > 1 - 1e-5
+        p = C.marginal_probabilities(dev, [0, 2])
+        pw = (np.abs(want) ** 2).reshape([2] * n).sum(axis=(1, 3, 4, 5)).reshape(-1)
+        assert np.max(np.abs(p - pw)) < 1e-5
+        zz = C.pauli_product_expectation(dev, {0: "Z", 5: "Z"})
+        psi_z = O.apply_dense(O.apply_dense(want, n, np.diag([1, -1]), [0]), n, np.diag([1, -1]), [5])
+        assert abs(zz - np.vdot(want, psi_z)) < 1e-4
+        C.apply_global_phase(dev, 0.3)
+        assert abs(np.vdot(got, C.statevector_numpy(dev)) - np.exp(0.3j)) < 1e-4
+    finally:
+        dev.close()
+    # 40 qubits: the preflight refuses no statevector here, but a local chain
+    # above automatic_mps_min_qubits with approximation allowed runs as MPS
+    n = 40
+    chain = [C.core_operation("H", [0])] + [C.core_operation("CNOT", [i, i + 1]) for i in range(n - 1)]
+    dev = C.execute_operations(n, chain, allow_approximation=True, automatic_mps_min_qubits=24)
+    try:
+        assert dev.backend == "mps"
+        vals = C.pauli_product_expectations(dev, [{0: "Z", 39: "Z"}, {q: "X" for q in range(n)}, {7: "Z"}])
+        assert abs(vals[0] - 1) < 1e-4 and abs(vals[1] - 1) < 1e-4 and abs(vals[2]) < 1e-4
+    finally:
+        dev.close()

@@ -49,8 +49,39 @@ def test_no_gpu_and_explicit_requests(monkeypatch):
         planning.select_execution(4, [], method="hybrid")
     with pytest.raises(ValueError, match="device must be"):
         planning.select_execution(4, [], device="neural-engine")
-    with pytest.raises(ValueError, match="statevector method only"):
-        planning.select_execution(4, [], method="mps")
+    sel = planning.select_execution(4, [], method="mps")
+    assert sel.selected_method == "matrix_product_state"
+    assert sel.reason.startswith("explicit_matrix_product_state_request")
+
+
+def _local_chain(n):
+    return [{"name": "CNOT", "wires": [w, w + 1], "parameters": []} for w in range(n - 1)]
+
+
+def test_automatic_mps_requires_opt_in_and_local_circuits(monkeypatch):
+    """The reference's rules (tests/mlxQExecutionPlanningTest.py:42-70), with
+    the MPS selected on the GPU, where this engine runs its SVDs."""
+    monkeypatch.setattr(planning, "_gpu_available", lambda: True)
+    compatible = planning.select_execution(12, _local_chain(12), allow_approximation=True,
+                                           automatic_mps_min_qubits=12)
+    exact = planning.select_execution(12, _local_chain(12), allow_approximation=False,
+                                      automatic_mps_min_qubits=12)
+    nonlocal_circuit = planning.select_execution(12, [{"name": "CNOT", "wires": [0, 11], "parameters": []}],
+                                                 allow_approximation=True, automatic_mps_min_qubits=12)
+    assert compatible.selected_method == "matrix_product_state"
+    assert compatible.selected_device == "gpu" and compatible.mps_svd_device == "gpu"
+    assert exact.selected_method == "statevector"
+    assert nonlocal_circuit.selected_method == "statevector"
+    assert "contains_non_adjacent_two_qubit_operation" in nonlocal_circuit.mps_compatibility_reasons
+    deep = planning.select_execution(4, _local_chain(4) * 5, allow_approximation=True, automatic_mps_min_qubits=2)
+    assert "entangling_operation_count_exceeds_auto_mps_limit" in deep.mps_compatibility_reasons
+    # a statevector the preflight refuses falls back to MPS only when allowed
+    assert planning.select_execution(40, _local_chain(40), allow_approximation=True,
+                                     automatic_mps_min_qubits=64).selected_method == "matrix_product_state"
+    with pytest.raises(ValueError, match="runs on the GPU"):
+        planning.select_execution(3, [], method="mps", device="cpu")
+    with pytest.raises(TypeError):
+        planning.select_execution(3, [], allow_approximation="yes")
 
 
 def test_refused_statevector_keeps_the_exact_request(monkeypatch):
@@ -87,7 +118,7 @@ def test_gpu_above_the_crossover_and_double_precision_stays_on_gpu():
 
 def test_mps_requests():
     assert recommend_sdk_engine("qiskit", 40, _layer(40), method="mps").engine == "qiskit_aer_cpu_mps"
-    assert recommend_sdk_engine("pennylane", 40, _layer(40), method="mps").engine == "refused"
+    assert recommend_sdk_engine("pennylane", 40, _layer(40), method="mps").engine == "mettleq_gpu_mps"
 
 
 def test_refusal_and_bounded_local_fallback():
