@@ -56,11 +56,12 @@ def parse():
     ap.add_argument("--no-oracle", action="store_true", help="skip the complex128 fidelity check")
     ap.add_argument("--json-out", default=None)
     ap.add_argument("--workload", default="layered30",
-                    choices=["layered30", "brick20", "memlimit", "vqe16", "serve"],
+                    choices=["layered30", "brick20", "memlimit", "vqe16", "serve", "mps"],
                     help="layered30 (default): the 30-qubit layered circuit; brick20: 20-qubit brick "
                          "circuit with probabilities and shots vs the CPU restatement; memlimit: 34q "
                          "complex64 and 33q complex128 through the preflight; vqe16: 1024 bindings of a "
-                         "16-qubit ansatz with adjoint gradients; serve: Poisson request stream")
+                         "16-qubit ansatz with adjoint gradients; serve: Poisson request stream; mps: 40-qubit "
+                         "brick circuit by the MPS method")
     ap.add_argument("--no-extra", action="store_true",
                     help="layered30 only: skip the short brick20 / vqe16 / serve summaries")
     ap.add_argument("--rates", default="50,100,200,500,1000,2000")
@@ -268,6 +269,61 @@ def bench_brick20(args, quick=False):
     }
 
 
+def bench_mps(args, quick=False):
+    """MPS method: brick layers (U3-all + CZ brick) on 40 qubits, depth 16 (12 quick),
+    dmax 64, eps 1e-10, on the device engine (mettleq_amd.mps) and on the
+    NumPy restatement of the reference's CPU MPS (oracle/mps_oracle.py, the
+    reference's algorithm with LAPACK SVDs), same circuit; <Z_0 Z_1> from both."""
+    from mettleq_amd import gates as G
+    from mettleq_amd.mps import MPSOptions, MPSState
+    from mettleq_amd.workloads import brick_20q
+    from oracle.mps_oracle import MPSOracle
+    n, depth, dmax = 40, (12 if quick else 16), 64
+    layers = brick_20q(n, depth)
+    cz = np.diag([1, 1, 1, -1]).astype(complex)
+
+    def run(mps):
+        for layer in layers:
+            if layer["type"] == "u3":
+                for q in range(n):
+                    mps.apply_single(G.single_qubit_matrix("U3", layer["angles"][q]), q)
+            else:
+                for a, b in layer["bonds"]:
+                    mps.apply_two(cz, a, b)
+
+    z = np.diag([1.0, -1.0])
+    with MPSState(n, MPSOptions(dmax=dmax, eps=1e-10)) as warm:
+        run(warm)
+    with MPSState(n, MPSOptions(dmax=dmax, eps=1e-10)) as mps:
+        t0 = time.perf_counter()
+        run(mps)
+        zz = mps.expectation_product({0: z, 1: z})
+        gpu_s = time.perf_counter() - t0
+        d = mps.truncation_diagnostics()
+        bonds = mps.bond_dims()
+    ref = MPSOracle(n, dmax=dmax, eps=1e-10)
+    t0 = time.perf_counter()
+    run(ref)
+    zz_ref = ref.expectation_product({0: z, 1: z})
+    cpu_s = time.perf_counter() - t0
+    updates = sum(len(l["bonds"]) for l in layers if l["type"] == "cz")
+    return {
+        "metric": "MPS brick circuit: seconds per circuit (device engine vs the reference's CPU algorithm)",
+        "workload": f"{n} qubits, {depth} layers U3-all + CZ brick, dmax {dmax}, eps 1e-10",
+        "gpu_s": round(gpu_s, 3),
+        "cpu_s": round(cpu_s, 3),
+        "two_qubit_gates": updates,
+        "svd_calls": d["svd_calls"],
+        "max_bond": d["maximum_bond_dimension_reached"],
+        "bond_mid": bonds[n // 2],
+        "truncation_events": d["events"],
+        "relative_discarded_weight_sum": d["relative_discarded_weight_sum"],
+        "zz01_gpu": round(float(zz.real), 6),
+        "zz01_cpu": round(float(zz_ref.real), 6),
+        "cpu_kind": "port (NumPy/LAPACK restatement of the reference's CPU MPS)",
+    }
+
+
 def bench_memlimit(args):
     """34q complex64 (128 GiB) and 33q complex128 (128 GiB) through the
     preflight, in place; 1-F between 33q complex64 and the 33q complex128
@@ -444,7 +500,8 @@ def extra_summaries(args):
                      ("brick20", lambda: bench_brick20(args, quick=True)),
                      ("vqe16_adjoint", lambda: bench_vqe16(args, quick=True)),
                      ("serve_sweep", lambda: bench_serve(args, rates=[50.0, 200.0, 1000.0, 2000.0],
-                                                         n_requests=400, quick=True))):
+                                                         n_requests=400, quick=True)),
+                     ("mps40", lambda: bench_mps(args, quick=True))):
         try:
             t0 = time.perf_counter()
             res[name] = fn()
@@ -462,7 +519,7 @@ def main():
             print("bench.py measures one GPU; run with --gpus 1.", file=sys.stderr)
             sys.exit(2)
         out = {"brick20": bench_brick20, "memlimit": bench_memlimit, "vqe16": bench_vqe16,
-               "serve": bench_serve}[args.workload](args)
+               "serve": bench_serve, "mps": bench_mps}[args.workload](args)
         line = json.dumps(out)
         if args.json_out:
             with open(args.json_out, "w") as fh:

@@ -450,8 +450,8 @@ int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_p
 /* ---- matrix product states ---------------------------------------------- */
 /* The reference's MPS method (mps_state.py:250-1549 with its native core
  * _mps_native.cpp): site tensors (dl, 2, dr) in complex64, resident on the
- * device; two-site updates through an SVD on the device (rocSOLVER) with the
- * reference's truncation rule; the orthogonality center moves before each
+ * device; two-site updates through an SVD on the device (one-sided Jacobi,
+ * one workgroup) with the reference's truncation rule; the orthogonality center moves before each
  * update like the reference's QR moves. Site q is qubit q. */
 typedef struct msv_mps msv_mps;
 

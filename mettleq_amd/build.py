@@ -51,7 +51,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> Path:
     cmd = [hipcc(), *HIP_FLAGS, *[str(CSRC / s) for s in SOURCES], "-o", str(OUT) + ".tmp",
-           "-L/opt/rocm/lib", "-lhsa-runtime64", "-lrocsolver", "-lrocblas"]
+           "-L/opt/rocm/lib", "-lhsa-runtime64"]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,7 +7,8 @@
 // Per two-site update (sites i, i + 1):
 //   k_mps_merge   theta = G . (A_i A_{i+1}), written column-major as the
 //                 (2 dl) x (2 dr) matrix the SVD takes;
-//   rocsolver_cgesvd on the device (the reference: LAPACK cgesdd on the CPU);
+//   k_mps_svd     one-sided Jacobi SVD in one workgroup on the device (the
+//                 reference: LAPACK cgesdd on the CPU);
 //   host           the reference's truncation rule on the singular values
 //                 (mps_state.py:193-247), read back (2 min(dl, dr) floats);
 //   k_mps_split    A_i = U[:, :r], A_{i+1} = diag(S / norm) Vh[:r, :]
@@ -16,8 +17,6 @@
 // (mps_state.py:347-399): one-site SVDs without truncation, the factor
 // carried into the neighbour by k_mps_absorb_*.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -216,6 +215,164 @@ __global__ void __launch_bounds__(256) k_mps_transfer(const double2 *__restrict_
   out[idx] = make_double2(re, im);
 }
 
+// One-sided Jacobi (Hestenes) SVD of the column-major rows x cols matrix A
+// in one 1024-thread workgroup. With k = min(rows, cols) it works on W = A
+// (or A^H when cols > rows, so that W has k columns of kk = max rows each)
+// and V = I (k x k): each sweep visits every column pair (p, q) once, in
+// k - 1 rounds of disjoint pairs (round-robin order, one wavefront per
+// pair), and rotates the pair so that w_p^H w_q = 0 (phase e^{i phi} of
+// w_p^H w_q, then the real Jacobi rotation t = sign(z) / (|z| + sqrt(1 + z^2)),
+// z = (|w_q|^2 - |w_p|^2) / (2 |w_p^H w_q|)). Dot products accumulate in
+// double precision. Sweeps stop when no pair is rotated (|w_p^H w_q| below
+// kJacobiTol |w_p| |w_q|). Pairs with a column of norm below kJacobiFloor
+// |A|_F are left alone: such a column's entries may be float denormals,
+// whose few significant bits cannot reach kJacobiTol (the sweeps would
+// rotate it forever), and its singular value lies below the exact-zero
+// floor of the truncation, so its vector is never kept by a split. Then
+// sigma_j = |w_j|, sorted descending:
+//   A = U diag(S) Vh,  U (rows x k, column-major), Vh (k x cols, column-major,
+//   leading dimension k); info = sweeps run, or -1 without convergence.
+constexpr double kJacobiTol = 3e-7;
+constexpr double kJacobiFloor = 1e-32;
+constexpr int kJacobiMaxSweeps = 60;
+constexpr int kSvdThreads = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(kSvdThreads) k_mps_svd(const C *__restrict__ A, int rows, int cols,
+                                                         C *__restrict__ W, C *__restrict__ V, C *__restrict__ U,
+                                                         float *__restrict__ S, C *__restrict__ Vh,
+                                                         int *__restrict__ info) {
+  __shared__ int rotated;
+  __shared__ double wave_fro[kSvdThreads / 64];
+  __shared__ float sig[2048];
+  __shared__ int rank[2048];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const bool trans = cols > rows;
+  const int kk = trans ? cols : rows;  // rows of W
+  const int k = trans ? rows : cols;   // columns of W (= min(rows, cols))
+  double fro = 0;
+  for (int idx = tid; idx < kk * k; idx += blockDim.x) {
+    const int i = idx % kk, j = idx / kk;
+    C a;
+    if (!trans) {
+      a = A[i + static_cast<size_t>(j) * rows];
+    } else {
+      a = A[j + static_cast<size_t>(i) * rows];  // W = A^H: W[i, j] = conj(A[j, i])
+      a.y = -a.y;
+    }
+    W[idx] = a;
+    fro += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  fro = wave_sum(fro);
+  if (lane == 0) wave_fro[wave] = fro;
+  for (int idx = tid; idx < k * k; idx += blockDim.x)
+    V[idx] = make_float2((idx % k) == (idx / k) ? 1.f : 0.f, 0.f);
+  __syncthreads();
+  fro = 0;
+  for (int w = 0; w < nwave; ++w) fro += wave_fro[w];
+  const double floor2 = fmax(kJacobiFloor * kJacobiFloor * fro, 1e-66);
+  const int np = k + (k & 1);  // players of the round-robin (one bye when k is odd)
+  int sweeps = 0;
+  bool converged = k < 2;
+  for (; sweeps < kJacobiMaxSweeps && !converged; ++sweeps) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < np - 1; ++r) {
+      for (int pi = wave; pi < np / 2; pi += nwave) {
+        // circle method: position 0 fixed, the others rotate by r
+        const int a = pi == 0 ? 0 : 1 + (pi - 1 + r) % (np - 1);
+        const int b = 1 + (np - 2 - pi + r) % (np - 1);
+        const int p = a < b ? a : b, q = a < b ? b : a;
+        if (q >= k) continue;
+        C *wp = W + static_cast<size_t>(p) * kk, *wq = W + static_cast<size_t>(q) * kk;
+        double al = 0, be = 0, gr = 0, gi = 0;
+        for (int i = lane; i < kk; i += 64) {
+          const C x = wp[i], y = wq[i];
+          al += (double)x.x * x.x + (double)x.y * x.y;
+          be += (double)y.x * y.x + (double)y.y * y.y;
+          gr += (double)x.x * y.x + (double)x.y * y.y;  // conj(x) y
+          gi += (double)x.x * y.y - (double)x.y * y.x;
+        }
+        al = wave_sum(al);
+        be = wave_sum(be);
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        const double g = sqrt(gr * gr + gi * gi);
+        if (!(g > kJacobiTol * sqrt(al * be)) || !(fmin(al, be) > floor2)) continue;
+        const double z = (be - al) / (2.0 * g);
+        const double t = (z >= 0 ? 1.0 : -1.0) / (fabs(z) + sqrt(1.0 + z * z));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+        const double er = gr / g, ei = -gi / g;  // e^{-i phi}
+        // w_p' = c w_p - s e^{-i phi} w_q,  w_q' = s w_p + c e^{-i phi} w_q
+        const float fc = (float)c, fs = (float)sn;
+        const float fr = (float)er, fi = (float)ei;
+        for (int i = lane; i < kk; i += 64) {
+          const C x = wp[i], y = wq[i];
+          const C ey = make_float2(fr * y.x - fi * y.y, fr * y.y + fi * y.x);
+          wp[i] = make_float2(fc * x.x - fs * ey.x, fc * x.y - fs * ey.y);
+          wq[i] = make_float2(fs * x.x + fc * ey.x, fs * x.y + fc * ey.y);
+        }
+        C *vp = V + static_cast<size_t>(p) * k, *vq = V + static_cast<size_t>(q) * k;
+        for (int i = lane; i < k; i += 64) {
+          const C x = vp[i], y = vq[i];
+          const C ey = make_float2(fr * y.x - fi * y.y, fr * y.y + fi * y.x);
+          vp[i] = make_float2(fc * x.x - fs * ey.x, fc * x.y - fs * ey.y);
+          vq[i] = make_float2(fs * x.x + fc * ey.x, fs * x.y + fc * ey.y);
+        }
+        if (lane == 0) rotated = 1;
+      }
+      __syncthreads();
+    }
+    converged = rotated == 0;
+    __syncthreads();
+  }
+  // singular values (column norms) and their descending order
+  for (int j = wave; j < k; j += nwave) {
+    double s2 = 0;
+    for (int i = lane; i < kk; i += 64) {
+      const C x = W[static_cast<size_t>(j) * kk + i];
+      s2 += (double)x.x * x.x + (double)x.y * x.y;
+    }
+    s2 = wave_sum(s2);
+    if (lane == 0) sig[j] = (float)sqrt(s2);
+  }
+  __syncthreads();
+  for (int j = tid; j < k; j += blockDim.x) {
+    int rk = 0;
+    for (int l = 0; l < k; ++l) rk += (sig[l] > sig[j] || (sig[l] == sig[j] && l < j)) ? 1 : 0;
+    rank[j] = rk;
+    S[rk] = sig[j];
+  }
+  __syncthreads();
+  // outputs: the left vectors of W are w_j / sigma_j, its right vectors V
+  for (int idx = tid; idx < kk * k; idx += blockDim.x) {
+    const int i = idx % kk, j = idx / kk;
+    const float inv = sig[j] > 0.f ? 1.f / sig[j] : 0.f;
+    const C w = W[idx];
+    const C u = make_float2(w.x * inv, w.y * inv);
+    if (!trans) {
+      U[i + static_cast<size_t>(rank[j]) * rows] = u;        // U[:, rank j] = w_j / sigma_j
+    } else {
+      Vh[rank[j] + static_cast<size_t>(i) * k] = make_float2(u.x, -u.y);  // Vh[rank j, :] = (w_j / sigma_j)^H
+    }
+  }
+  for (int idx = tid; idx < k * k; idx += blockDim.x) {
+    const int i = idx % k, j = idx / k;
+    const C v = V[idx];
+    if (!trans) {
+      Vh[rank[j] + static_cast<size_t>(i) * k] = make_float2(v.x, -v.y);  // Vh[rank j, :] = v_j^H
+    } else {
+      U[i + static_cast<size_t>(rank[j]) * rows] = v;       // U[:, rank j] = v_j
+    }
+  }
+  if (tid == 0) *info = converged ? sweeps : -1;
+}
+
 constexpr int kThreads = 256;
 uint32_t blocks_for(uint64_t n) { return static_cast<uint32_t>((n + kThreads - 1) / kThreads); }
 
@@ -228,13 +385,12 @@ struct msv_mps {
   bool renormalize = true;
   int device = 0;
   hipStream_t stream = nullptr;
-  rocblas_handle blas = nullptr;
   size_t cap = 0;                 // complex entries per site buffer (dmax x 2 x dmax)
   C *sites = nullptr;             // n site buffers
   C *spare = nullptr;             // one site buffer (out-of-place absorb)
-  C *work = nullptr;              // SVD input / U / Vh (3 x 4 dmax^2)
+  C *work = nullptr;              // SVD input / U / Vh / Jacobi W / Jacobi V (5 x 4 dmax^2)
   float *sv = nullptr;            // singular values and superdiagonal (2 x 2 dmax)
-  rocblas_int *info = nullptr;
+  int *info = nullptr;
   std::vector<int> dim;           // bond dimensions, dim[0] = dim[n] = 1
   int center = 0;
   msv_mps_stats st{};
@@ -269,21 +425,16 @@ C *site_ptr(msv_mps *m, int i) { return m->sites + static_cast<size_t>(i) * m->c
 int svd(msv_mps *m, int rows, int cols) {
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   const int kmin = std::min(rows, cols);
-  auto *A = reinterpret_cast<rocblas_float_complex *>(m->work);
-  auto *U = reinterpret_cast<rocblas_float_complex *>(m->work + slot);
-  auto *V = reinterpret_cast<rocblas_float_complex *>(m->work + 2 * slot);
-  const rocblas_status rs = rocsolver_cgesvd(m->blas, rocblas_svect_singular, rocblas_svect_singular, rows, cols,
-                                             A, rows, m->sv, U, rows, V, kmin, m->sv + 2 * m->dmax,
-                                             rocblas_outofplace, m->info);
-  if (rs != rocblas_status_success)
-    return set_error(MSV_E_HIP, "rocsolver_cgesvd failed (status " + std::to_string(static_cast<int>(rs)) + ")");
-  rocblas_int h_info = 0;
+  hipLaunchKernelGGL(k_mps_svd, dim3(1), dim3(kSvdThreads), 0, m->stream, m->work, rows, cols, m->work + 3 * slot,
+                     m->work + 4 * slot, m->work + slot, m->sv, m->work + 2 * slot, m->info);
+  MPS_HIP(hipGetLastError());
+  int h_info = 0;
   m->h_sv.resize(kmin);
   MPS_HIP(hipMemcpyAsync(m->h_sv.data(), m->sv, sizeof(float) * kmin, hipMemcpyDeviceToHost, m->stream));
-  MPS_HIP(hipMemcpyAsync(&h_info, m->info, sizeof(rocblas_int), hipMemcpyDeviceToHost, m->stream));
+  MPS_HIP(hipMemcpyAsync(&h_info, m->info, sizeof(int), hipMemcpyDeviceToHost, m->stream));
   MPS_HIP(hipStreamSynchronize(m->stream));
   m->st.svd_calls += 1;
-  if (h_info != 0) return set_error(MSV_E_INVALID, "MPS SVD did not converge");
+  if (h_info < 0) return set_error(MSV_E_INVALID, "MPS SVD did not converge");
   for (int k = 0; k < kmin; ++k)
     if (!std::isfinite(m->h_sv[k])) return set_error(MSV_E_INVALID, "MPS SVD produced non-finite singular values");
   return MSV_OK;
@@ -428,6 +579,8 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
   *out = nullptr;
   if (n_qubits < 1 || n_qubits > 4096) return set_error(MSV_E_INVALID, "MPS qubit count out of range");
   if (dmax < 1 || dmax > 1024) return set_error(MSV_E_INVALID, "MPS dmax must be in [1, 1024]");
+  // the Jacobi SVD keeps one float and one int per column of the
+  // (2 dmax)-column two-site matrix in LDS
   if (!(eps >= 0.0)) return set_error(MSV_E_INVALID, "MPS eps must be non-negative");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
@@ -447,14 +600,11 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
   };
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(set_error(MSV_E_HIP, "stream creation failed"));
-  if (rocblas_create_handle(&m->blas) != rocblas_status_success)
-    return cleanup(set_error(MSV_E_HIP, "rocblas_create_handle failed"));
-  rocblas_set_stream(m->blas, m->stream);
   if (hipMalloc(reinterpret_cast<void **>(&m->sites), sizeof(C) * m->cap * n_qubits) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->spare), sizeof(C) * m->cap) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&m->work), sizeof(C) * slot * 3) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&m->work), sizeof(C) * slot * 5) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->sv), sizeof(float) * 4 * dmax) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&m->info), sizeof(rocblas_int)) != hipSuccess)
+      hipMalloc(reinterpret_cast<void **>(&m->info), sizeof(int)) != hipSuccess)
     return cleanup(set_error(MSV_E_OOM, "MPS device allocation failed"));
   const int rc = msv_mps_reset(m);
   if (rc != MSV_OK) return cleanup(rc);
@@ -466,7 +616,6 @@ int msv_mps_destroy(msv_mps *m) {
   if (!m) return MSV_OK;
   hipSetDevice(m->device);
   if (m->stream) hipStreamSynchronize(m->stream);
-  if (m->blas) rocblas_destroy_handle(m->blas);
   hipFree(m->sites);
   hipFree(m->spare);
   hipFree(m->work);

@@ -4,7 +4,7 @@ The reference's MPS method (src/mettleq/mps_state.py:57-1549, native core
 _mps_native.cpp) with the same option names, gate conventions and
 diagnostics. Site tensors (dl, 2, dr) live in device memory; each two-site
 update merges the pair with the gate on the device, takes the SVD on the
-device (rocSOLVER) and applies the reference's truncation rule
+device (a one-sided Jacobi kernel) and applies the reference's truncation rule
 (mps_state.py:193-247) before splitting, with the orthogonality center moved
 to the pair first (mps_state.py:347-399). Non-adjacent two-qubit gates go
 through the reference's restoring swap network (mps_state.py:744-768).
