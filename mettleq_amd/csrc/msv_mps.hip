@@ -4,18 +4,25 @@
 // row-major (index (a * 2 + s) * dr + b), complex64 like the reference's
 // storage. Site i is qubit i (qubit 0 = the most significant amplitude bit).
 //
+// Every factorization here is "orthonormal basis Q of one side, then the
+// projection X = Q^H M": for M = U S Vh, Q = U (the columns kept) and
+// X = S Vh, so no right-hand vectors are ever accumulated.
 // Per two-site update (sites i, i + 1):
-//   k_mps_merge   theta = G . (A_i A_{i+1}), written column-major as the
-//                 (2 dl) x (2 dr) matrix the SVD takes;
-//   k_mps_svd     one-sided Jacobi SVD in one workgroup on the device (the
-//                 reference: LAPACK cgesdd on the CPU);
-//   host           the reference's truncation rule on the singular values
-//                 (mps_state.py:193-247), read back (2 min(dl, dr) floats);
-//   k_mps_split    A_i = U[:, :r], A_{i+1} = diag(S / norm) Vh[:r, :]
-//                 (or U diag(S) and Vh when the center moves left).
+//   k_mps_merge     theta = G . (A_i A_{i+1}), written column-major as the
+//                   (2 dl) x (2 dr) matrix M;
+//   k_mps_basis_lds one-sided Jacobi in one workgroup with the matrix in LDS
+//                   (both sides up to 128; k_mps_svd, in global memory,
+//                   beyond): Q = U sorted by singular value, and S (the
+//                   reference: LAPACK cgesdd on the CPU);
+//   host            the reference's truncation rule on the singular values
+//                   (mps_state.py:193-247), read back (min(2 dl, 2 dr) floats);
+//   k_mps_project   X = Q^H M;
+//   k_mps_split     A_i = Q[:, :r], A_{i+1} = X[:r, :] / norm.
 // The orthogonality center moves like the reference's QR moves
-// (mps_state.py:347-399): one-site SVDs without truncation, the factor
-// carried into the neighbour by k_mps_absorb_*.
+// (mps_state.py:347-399), without truncation: right, Q of the site as a
+// (2 dl) x dr matrix and X carried into the next site; left, the same on
+// the conjugate transpose of the site as a dl x (2 dr) matrix
+// (k_mps_absorb_*).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,88 +91,102 @@ __global__ void __launch_bounds__(256) k_mps_merge(const C *__restrict__ L, cons
 }
 
 // Site tensor viewed as a (rows x cols) row-major matrix, copied into a
-// column-major one (the SVD's input).
+// column-major one: M = A (rows x cols), or M = A^H (cols x rows) when herm.
 __global__ void __launch_bounds__(256) k_mps_to_colmajor(const C *__restrict__ A, C *__restrict__ M, int rows,
-                                                         int cols) {
+                                                         int cols, int herm) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rows * cols) return;
   const int r = idx / cols, c = idx % cols;
-  M[r + static_cast<size_t>(c) * rows] = A[idx];
+  const C a = A[idx];
+  if (herm)
+    M[c + static_cast<size_t>(r) * cols] = make_float2(a.x, -a.y);
+  else
+    M[r + static_cast<size_t>(c) * rows] = a;
 }
 
-// Split after a two-site SVD: U (m x kmin, column-major), S, Vh (kmin x n,
-// column-major, ldv = kmin). left (dl, 2, r) = U[:, :r] (x S / norm when the
-// singular values go left), right (r, 2, dr) = (S / norm) Vh[:r, :] (or Vh).
-__global__ void __launch_bounds__(256) k_mps_split(const C *__restrict__ U, const float *__restrict__ S,
-                                                   const C *__restrict__ Vh, C *__restrict__ left,
-                                                   C *__restrict__ right, int dl, int dr, int kmin, int r,
-                                                   float scale, int s_left) {
+// X = Q^H M (kmin x n, column-major, leading dimension kmin) for Q (m x kmin)
+// and M (m x n), both column-major; sums in double precision. With M = U S Vh
+// and Q = U this is S Vh.
+__global__ void __launch_bounds__(256) k_mps_project(const C *__restrict__ Q, const C *__restrict__ M,
+                                                     C *__restrict__ X, int m, int n, int kmin) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kmin * n) return;
+  const int j = idx % kmin, c = idx / kmin;
+  const C *q = Q + static_cast<size_t>(j) * m;
+  const C *a = M + static_cast<size_t>(c) * m;
+  double re = 0, im = 0;
+  for (int i = 0; i < m; ++i) {
+    const C u = q[i], v = a[i];
+    re += (double)u.x * v.x + (double)u.y * v.y;
+    im += (double)u.x * v.y - (double)u.y * v.x;
+  }
+  X[idx] = make_float2((float)re, (float)im);
+}
+
+// Split after a two-site factorization of M = theta ((2 dl) x (2 dr)):
+// left (dl, 2, r) = Q[:, :r], right (r, 2, dr) = scale X[:r, :], with Q
+// (2 dl x kmin) and X (kmin x 2 dr, leading dimension kmin).
+__global__ void __launch_bounds__(256) k_mps_split(const C *__restrict__ Q, const C *__restrict__ X,
+                                                   C *__restrict__ left, C *__restrict__ right, int dl, int dr,
+                                                   int kmin, int r, float scale) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   const int m = 2 * dl, n = 2 * dr;
   const int nl = m * r, nr = r * n;
   if (idx < nl) {
     const int row = idx / r, j = idx % r;  // row = a * 2 + s
-    C u = U[row + static_cast<size_t>(j) * m];
-    if (s_left) {
-      const float f = S[j] * scale;
-      u.x *= f;
-      u.y *= f;
-    }
-    left[idx] = u;
+    left[idx] = Q[row + static_cast<size_t>(j) * m];
   } else if (idx < nl + nr) {
     const int k = idx - nl;
     const int j = k / n, col = k % n;  // col = t * dr + b; right index (j * 2 + t) * dr + b = j * n + col
-    C v = Vh[j + static_cast<size_t>(col) * kmin];
-    if (!s_left) {
-      const float f = S[j] * scale;
-      v.x *= f;
-      v.y *= f;
-    }
+    C v = X[j + static_cast<size_t>(col) * kmin];
+    v.x *= scale;
+    v.y *= scale;
     right[k] = v;
   }
 }
 
-// Center moves right: A_i <- U (rows x r, from the column-major U), and the
-// next site absorbs X = diag(S) Vh (r x cols): next'[j, t, b] = sum_c X[j, c] next[c, t, b].
-__global__ void __launch_bounds__(256) k_mps_absorb_right(const C *__restrict__ U, const float *__restrict__ S,
-                                                          const C *__restrict__ Vh, C *__restrict__ site,
-                                                          const C *__restrict__ next, C *__restrict__ next_out,
-                                                          int rows, int cols, int kmin, int r, int dn) {
+// Center moves right: the site as M ((2 dl) x dr) = Q X; A_i <- Q
+// ((2 dl) x kmin) and the next site absorbs X (kmin x dr):
+// next'[j, t, b] = sum_c X[j, c] next[c, t, b].
+__global__ void __launch_bounds__(256) k_mps_absorb_right(const C *__restrict__ Q, const C *__restrict__ X,
+                                                          C *__restrict__ site, const C *__restrict__ next,
+                                                          C *__restrict__ next_out, int rows, int cols, int kmin,
+                                                          int dn) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nu = rows * r, nn = r * 2 * dn;
+  const int nu = rows * kmin, nn = kmin * 2 * dn;
   if (idx < nu) {
-    const int row = idx / r, j = idx % r;
-    site[idx] = U[row + static_cast<size_t>(j) * rows];
+    const int row = idx / kmin, j = idx % kmin;
+    site[idx] = Q[row + static_cast<size_t>(j) * rows];
   } else if (idx < nu + nn) {
     const int k = idx - nu;
     const int j = k / (2 * dn), tb = k % (2 * dn);
     C acc = make_float2(0, 0);
-    for (int c = 0; c < cols; ++c) acc = cadd(acc, cmul(Vh[j + static_cast<size_t>(c) * kmin], next[c * 2 * dn + tb]));
-    acc.x *= S[j];
-    acc.y *= S[j];
+    for (int c = 0; c < cols; ++c) acc = cadd(acc, cmul(X[j + static_cast<size_t>(c) * kmin], next[c * 2 * dn + tb]));
     next_out[k] = acc;
   }
 }
 
-// Center moves left: A_i (rows = dl) x (cols = 2 dr) = U S Vh; A_i <- Vh[:r, :]
-// and the previous site absorbs Y = U diag(S) (rows x r):
-// prev'[a, t, j] = sum_c prev[a, t, c] Y[c, j].
-__global__ void __launch_bounds__(256) k_mps_absorb_left(const C *__restrict__ U, const float *__restrict__ S,
-                                                         const C *__restrict__ Vh, C *__restrict__ site,
-                                                         const C *__restrict__ prev, C *__restrict__ prev_out,
-                                                         int rows, int cols, int kmin, int r, int dp) {
+// Center moves left: the site (dl x (2 dr)) enters as M = site^H
+// ((2 dr) x dl) = Q X, so site = X^H Q^H; A_i <- Q^H (kmin x (2 dr)) and the
+// previous site absorbs X^H (dl x kmin): prev'[a, t, j] = sum_c prev[a, t, c] conj(X[j, c]).
+__global__ void __launch_bounds__(256) k_mps_absorb_left(const C *__restrict__ Q, const C *__restrict__ X,
+                                                         C *__restrict__ site, const C *__restrict__ prev,
+                                                         C *__restrict__ prev_out, int dl, int cols, int kmin,
+                                                         int dp) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nv = r * cols, np = dp * 2 * r;
+  const int nv = kmin * cols, np = dp * 2 * kmin;  // cols = 2 dr, the rows of Q
   if (idx < nv) {
     const int j = idx / cols, c = idx % cols;
-    site[idx] = Vh[j + static_cast<size_t>(c) * kmin];
+    const C q = Q[c + static_cast<size_t>(j) * cols];
+    site[idx] = make_float2(q.x, -q.y);
   } else if (idx < nv + np) {
     const int k = idx - nv;
-    const int at = k / r, j = k % r;  // at = a * 2 + t
+    const int at = k / kmin, j = k % kmin;  // at = a * 2 + t
     C acc = make_float2(0, 0);
-    for (int c = 0; c < rows; ++c) acc = cadd(acc, cmul(prev[at * rows + c], U[c + static_cast<size_t>(j) * rows]));
-    acc.x *= S[j];
-    acc.y *= S[j];
+    for (int c = 0; c < dl; ++c) {
+      const C x = X[j + static_cast<size_t>(c) * kmin];
+      acc = cadd(acc, cmul(prev[at * dl + c], make_float2(x.x, -x.y)));
+    }
     prev_out[k] = acc;
   }
 }
@@ -349,10 +370,11 @@ __global__ void __launch_bounds__(kSvdThreads) k_mps_svd(const C *__restrict__ A
     S[rk] = sig[j];
   }
   __syncthreads();
-  // outputs: the left vectors of W are w_j / sigma_j, its right vectors V
+  // outputs: the left vectors of W are w_j / sigma_j, its right vectors V;
+  // columns at or below the floor never rotated and stay zero
   for (int idx = tid; idx < kk * k; idx += blockDim.x) {
     const int i = idx % kk, j = idx / kk;
-    const float inv = sig[j] > 0.f ? 1.f / sig[j] : 0.f;
+    const float inv = (double)sig[j] * sig[j] > floor2 ? 1.f / sig[j] : 0.f;
     const C w = W[idx];
     const C u = make_float2(w.x * inv, w.y * inv);
     if (!trans) {
@@ -368,6 +390,227 @@ __global__ void __launch_bounds__(kSvdThreads) k_mps_svd(const C *__restrict__ A
       Vh[rank[j] + static_cast<size_t>(i) * k] = make_float2(v.x, -v.y);  // Vh[rank j, :] = v_j^H
     } else {
       U[i + static_cast<size_t>(rank[j]) * rows] = v;       // U[:, rank j] = v_j
+    }
+  }
+  if (tid == 0) *info = converged ? sweeps : -1;
+}
+
+// Fast path of the factorization for m, n <= kBasisMaxDim: the same
+// one-sided Jacobi (same pair order, rotation, tolerance and floor as
+// k_mps_svd) with the working matrix in LDS. Each 16-lane DPP row owns one
+// column pair per round and holds 8 rows of each column in registers; the
+// four dot products reduce over the row by DPP (all 16 lanes end with the
+// same bits, so the rotate/skip decision is uniform in the row).
+//   !kAccum: W = M (m x n); the columns of W converge to U S, so
+//            Q[:, rank j] = w_j / |w_j| (the first kmin of them);
+//   kAccum:  W = M^H (n x m) for wide M, with the rotations accumulated in
+//            V (m x m, also in LDS); W V = V_M S, so Q[:, rank j] = v_j.
+// S gets the kmin largest column norms in descending order; info = sweeps
+// run, or -1 without convergence.
+constexpr int kBasisMaxDim = 128;
+constexpr int kBasisLds = 16384;  // complex64 entries of W (+ V): 128 KiB
+constexpr int kPairLanes = 16;
+constexpr int kRowsPerLane = kBasisMaxDim / kPairLanes;
+
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), Ctrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes of a DPP row: lane pairs (i, i ^ 1), (i, i ^ 2), then
+// the mirrored half-rows and halves, so each add sees the same two operands
+// in both lanes of the pair.
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror
+  v += dpp_f64<0x140>(v);  // row_mirror
+  return v;
+}
+
+// 1 / sqrt(x) and 1 / x in double from the hardware approximations plus
+// one Newton step (relative error far below float rounding).
+__device__ __forceinline__ double rsq_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return y * (2.0 - x * y);
+}
+
+template <bool kAccum, int RPL>
+__global__ void __launch_bounds__(1024) k_mps_basis_lds(const C *__restrict__ M, int m, int n, C *__restrict__ Q,
+                                                        float *__restrict__ S, int *__restrict__ info) {
+  static_assert(RPL >= 1 && RPL <= kRowsPerLane, "row slots per lane");
+  __shared__ C lds[kBasisLds];
+  __shared__ float sig[kBasisMaxDim];
+  __shared__ int rank[kBasisMaxDim];
+  __shared__ double wave_fro[kSvdThreads / 64];
+  __shared__ int rotated;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int l16 = tid & (kPairLanes - 1), group = tid / kPairLanes, ngroups = blockDim.x / kPairLanes;
+  const int kk = kAccum ? n : m;  // rows of W
+  const int k = kAccum ? m : n;   // columns of W
+  C *W = lds;
+  C *V = lds + kk * k;
+  double fro = 0;
+  for (int idx = tid; idx < kk * k; idx += blockDim.x) {
+    C a;
+    if (!kAccum) {
+      a = M[idx];
+    } else {
+      const int i = idx % kk, j = idx / kk;
+      a = M[j + static_cast<size_t>(i) * m];  // W[i, j] = conj(M[j, i])
+      a.y = -a.y;
+    }
+    W[idx] = a;
+    fro += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  if (kAccum)
+    for (int idx = tid; idx < k * k; idx += blockDim.x)
+      V[idx] = make_float2((idx % k) == (idx / k) ? 1.f : 0.f, 0.f);
+  fro = wave_sum(fro);
+  if (lane == 0) wave_fro[wave] = fro;
+  __syncthreads();
+  fro = 0;
+  for (int w = 0; w < nwave; ++w) fro += wave_fro[w];
+  const double floor2 = fmax(kJacobiFloor * kJacobiFloor * fro, 1e-66);
+  const int np = k + (k & 1), npairs = np / 2;
+  const int pi = group;  // one pair slot per 16-lane row (the launch has npairs rows or more)
+  int sweeps = 0;
+  bool converged = k < 2;
+  for (; sweeps < kJacobiMaxSweeps && !converged; ++sweeps) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    // round-robin positions at round 0: a = pi (0 stays fixed), b = np - 1 - pi; both step
+    // through 1 .. np - 1 cyclically from round to round
+    int a = pi, b = np - 1 - pi;
+    for (int r = 0; r < np - 1; ++r) {
+      if (pi < npairs) {
+        const int p = a < b ? a : b, q = a < b ? b : a;
+        if (q < k) {
+          C *wp = W + p * kk, *wq = W + q * kk;
+          // RPL * 16 >= kk row slots; slots past kk read row 0 and count as zero,
+          // so every read is issued before the first use
+          C x[RPL], y[RPL];
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int i = l16 + kPairLanes * t;
+            const int ii = i < kk ? i : 0;
+            x[t] = wp[ii];
+            y[t] = wq[ii];
+          }
+          double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            if (l16 + kPairLanes * t >= kk) x[t] = y[t] = make_float2(0.f, 0.f);
+            al += (double)x[t].x * x[t].x + (double)x[t].y * x[t].y;
+            be += (double)y[t].x * y[t].x + (double)y[t].y * y[t].y;
+            gr += (double)x[t].x * y[t].x + (double)x[t].y * y[t].y;  // conj(x) y
+            gi += (double)x[t].x * y[t].y - (double)x[t].y * y[t].x;
+          }
+          al = row16_sum(al);
+          be = row16_sum(be);
+          gr = row16_sum(gr);
+          gi = row16_sum(gi);
+          const double g2 = gr * gr + gi * gi;
+          if (g2 > kJacobiTol * kJacobiTol * (al * be) && fmin(al, be) > floor2) {
+            // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (be - al) / (2 g), and
+            // e^{-i phi} = conj(gamma) / g, to float accuracy (they rotate
+            // float data): hardware reciprocal and reciprocal square root
+            // with one Newton step each
+            const double ig = rsq_nr(g2);
+            const double z = 0.5 * (be - al) * ig;
+            const double w = 1.0 + z * z;
+            const double tn = rcp_nr(fabs(z) + w * rsq_nr(w));
+            // c and s in double, each rounded once: c^2 + s^2 stays within
+            // float rounding of 1, which the column norms accumulate over
+            // every rotation they take part in
+            const double cd = rsq_nr(1.0 + tn * tn);
+            const float fc = (float)cd, fs = (float)(z >= 0 ? cd * tn : -cd * tn);
+            const float fr = (float)(gr * ig), fi = (float)(-gi * ig);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+              const int i = l16 + kPairLanes * t;
+              if (i < kk) {
+                const C ey = make_float2(fr * y[t].x - fi * y[t].y, fr * y[t].y + fi * y[t].x);
+                wp[i] = make_float2(fc * x[t].x - fs * ey.x, fc * x[t].y - fs * ey.y);
+                wq[i] = make_float2(fs * x[t].x + fc * ey.x, fs * x[t].y + fc * ey.y);
+              }
+            }
+            if (kAccum) {
+              // V has k = m < n = kk rows, so RPL slots cover them too
+              C *vp = V + p * k, *vq = V + q * k;
+              C vx[RPL], vy[RPL];
+#pragma unroll
+              for (int t = 0; t < RPL; ++t) {
+                const int i = l16 + kPairLanes * t;
+                const int ii = i < k ? i : 0;
+                vx[t] = vp[ii];
+                vy[t] = vq[ii];
+              }
+#pragma unroll
+              for (int t = 0; t < RPL; ++t) {
+                const int i = l16 + kPairLanes * t;
+                if (i < k) {
+                  const C xv = vx[t], yv = vy[t];
+                  const C ey = make_float2(fr * yv.x - fi * yv.y, fr * yv.y + fi * yv.x);
+                  vp[i] = make_float2(fc * xv.x - fs * ey.x, fc * xv.y - fs * ey.y);
+                  vq[i] = make_float2(fs * xv.x + fc * ey.x, fs * xv.y + fc * ey.y);
+                }
+              }
+            }
+            if (l16 == 0) rotated = 1;
+          }
+        }
+        if (pi != 0) a = a == np - 1 ? 1 : a + 1;
+        b = b == np - 1 ? 1 : b + 1;
+      }
+      __syncthreads();
+    }
+    converged = rotated == 0;
+    __syncthreads();
+  }
+  for (int j = group; j < k; j += ngroups) {
+    double s2 = 0;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int i = l16 + kPairLanes * t;
+      if (i < kk) {
+        const C x = W[j * kk + i];
+        s2 += (double)x.x * x.x + (double)x.y * x.y;
+      }
+    }
+    s2 = row16_sum(s2);
+    if (l16 == 0) sig[j] = (float)sqrt(s2);
+  }
+  __syncthreads();
+  const int kmin = m < n ? m : n;
+  for (int j = tid; j < k; j += blockDim.x) {
+    int rk = 0;
+    for (int l = 0; l < k; ++l) rk += (sig[l] > sig[j] || (sig[l] == sig[j] && l < j)) ? 1 : 0;
+    rank[j] = rk;
+    if (rk < kmin) S[rk] = sig[j];
+  }
+  __syncthreads();
+  if (!kAccum) {
+    // columns at or below the floor never rotated: their directions are not
+    // orthogonal to the others, so they stay zero in Q (weight <= 1e-32 |M|)
+    for (int idx = tid; idx < kk * k; idx += blockDim.x) {
+      const int i = idx % kk, j = idx / kk, rk = rank[j];
+      if (rk < kmin) {
+        const float inv = (double)sig[j] * sig[j] > floor2 ? 1.f / sig[j] : 0.f;
+        const C w = W[idx];
+        Q[i + static_cast<size_t>(rk) * m] = make_float2(w.x * inv, w.y * inv);
+      }
+    }
+  } else {
+    for (int idx = tid; idx < k * k; idx += blockDim.x) {
+      const int i = idx % k, j = idx / k;
+      Q[i + static_cast<size_t>(rank[j]) * m] = V[idx];
     }
   }
   if (tid == 0) *info = converged ? sweeps : -1;
@@ -420,13 +663,56 @@ int check_site(const msv_mps *m, int q) {
 }
 C *site_ptr(msv_mps *m, int i) { return m->sites + static_cast<size_t>(i) * m->cap; }
 
-// The SVD of the (rows x cols) column-major matrix in work[0]: U at work[1]
-// (rows x kmin), Vh at work[2] (kmin x cols), S on the device and in h_sv.
-int svd(msv_mps *m, int rows, int cols) {
+template <bool kAccum>
+void launch_basis_rpl(int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
+                      int *info) {
+  switch (rpl) {
+    case 1:
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 1>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 2>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 4>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      break;
+    default:
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 8>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      break;
+  }
+}
+
+void launch_basis(bool accum, int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
+                  int *info) {
+  if (accum)
+    launch_basis_rpl<true>(rpl, threads, stream, M, m, n, Q, S, info);
+  else
+    launch_basis_rpl<false>(rpl, threads, stream, M, m, n, Q, S, info);
+}
+
+// Factorizes M (rows x cols, column-major in work slot 0) as M = Q X: Q
+// (rows x kmin, orthonormal columns in descending singular-value order) in
+// slot 1, the singular values in sv and h_sv, X = Q^H M (kmin x cols,
+// leading dimension kmin) in slot 2.
+int factor(msv_mps *m, int rows, int cols) {
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   const int kmin = std::min(rows, cols);
-  hipLaunchKernelGGL(k_mps_svd, dim3(1), dim3(kSvdThreads), 0, m->stream, m->work, rows, cols, m->work + 3 * slot,
-                     m->work + 4 * slot, m->work + slot, m->sv, m->work + 2 * slot, m->info);
+  if (rows <= kBasisMaxDim && cols <= kBasisMaxDim) {
+    // wide matrices orthogonalize the rows of M (fewer columns) when W and V fit in LDS together
+    const bool accum = cols > rows && cols * rows + rows * rows <= kBasisLds;
+    const int k = accum ? rows : cols;
+    const int npairs = (k + (k & 1)) / 2;
+    const int threads = std::min(kSvdThreads, std::max(64, (npairs * kPairLanes + 63) / 64 * 64));
+    const int kk = accum ? cols : rows;  // rows of the working matrix: 16 * RPL >= kk
+    launch_basis(accum, kk <= 16 ? 1 : kk <= 32 ? 2 : kk <= 64 ? 4 : 8, threads, m->stream, m->work, rows, cols,
+                 m->work + slot, m->sv, m->info);
+  } else {
+    hipLaunchKernelGGL(k_mps_svd, dim3(1), dim3(kSvdThreads), 0, m->stream, m->work, rows, cols, m->work + 3 * slot,
+                       m->work + 4 * slot, m->work + slot, m->sv, m->work + 2 * slot, m->info);
+  }
+  MPS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_mps_project, dim3(blocks_for(static_cast<uint64_t>(kmin) * cols)), dim3(kThreads), 0,
+                     m->stream, m->work + slot, m->work, m->work + 2 * slot, rows, cols, kmin);
   MPS_HIP(hipGetLastError());
   int h_info = 0;
   m->h_sv.resize(kmin);
@@ -442,42 +728,42 @@ int svd(msv_mps *m, int rows, int cols) {
 
 // Moves the orthogonality center one site right (from i to i + 1) or left
 // (from i to i - 1) without truncation: the reference's QR moves
-// (mps_state.py:347-389), here through a one-site SVD.
+// (mps_state.py:347-389), here through the factorization M = Q X.
 int move_right(msv_mps *m, int i) {
   const int dl = m->dim[i], dr = m->dim[i + 1], dn = m->dim[i + 2];
   const int rows = 2 * dl, cols = dr, kmin = std::min(rows, cols);
-  MPS_TRY(m->n > 0 ? MSV_OK : MSV_E_INVALID);
   hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(rows) * cols)), dim3(kThreads), 0,
-                     m->stream, site_ptr(m, i), m->work, rows, cols);
+                     m->stream, site_ptr(m, i), m->work, rows, cols, 0);
   MPS_HIP(hipGetLastError());
-  MPS_TRY(svd(m, rows, cols));
+  MPS_TRY(factor(m, rows, cols));
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
-  const int r = kmin;
-  hipLaunchKernelGGL(k_mps_absorb_right, dim3(blocks_for(static_cast<uint64_t>(rows) * r + r * 2 * dn)),
-                     dim3(kThreads), 0, m->stream, m->work + slot, m->sv, m->work + 2 * slot, site_ptr(m, i),
-                     site_ptr(m, i + 1), m->spare, rows, cols, kmin, r, dn);
+  hipLaunchKernelGGL(k_mps_absorb_right, dim3(blocks_for(static_cast<uint64_t>(rows) * kmin + kmin * 2 * dn)),
+                     dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
+                     site_ptr(m, i + 1), m->spare, rows, cols, kmin, dn);
   MPS_HIP(hipGetLastError());
-  MPS_HIP(hipMemcpyAsync(site_ptr(m, i + 1), m->spare, sizeof(C) * r * 2 * dn, hipMemcpyDeviceToDevice, m->stream));
-  m->dim[i + 1] = r;
+  MPS_HIP(hipMemcpyAsync(site_ptr(m, i + 1), m->spare, sizeof(C) * kmin * 2 * dn, hipMemcpyDeviceToDevice,
+                         m->stream));
+  m->dim[i + 1] = kmin;
   m->center = i + 1;
   return MSV_OK;
 }
 
 int move_left(msv_mps *m, int i) {
   const int dl = m->dim[i], dr = m->dim[i + 1], dp = m->dim[i - 1];
-  const int rows = dl, cols = 2 * dr, kmin = std::min(rows, cols);
+  // M = site^H, (2 dr) x dl
+  const int rows = 2 * dr, cols = dl, kmin = std::min(rows, cols);
   hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(rows) * cols)), dim3(kThreads), 0,
-                     m->stream, site_ptr(m, i), m->work, rows, cols);
+                     m->stream, site_ptr(m, i), m->work, dl, 2 * dr, 1);
   MPS_HIP(hipGetLastError());
-  MPS_TRY(svd(m, rows, cols));
+  MPS_TRY(factor(m, rows, cols));
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
-  const int r = kmin;
-  hipLaunchKernelGGL(k_mps_absorb_left, dim3(blocks_for(static_cast<uint64_t>(r) * cols + dp * 2 * r)),
-                     dim3(kThreads), 0, m->stream, m->work + slot, m->sv, m->work + 2 * slot, site_ptr(m, i),
-                     site_ptr(m, i - 1), m->spare, rows, cols, kmin, r, dp);
+  hipLaunchKernelGGL(k_mps_absorb_left, dim3(blocks_for(static_cast<uint64_t>(kmin) * rows + dp * 2 * kmin)),
+                     dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
+                     site_ptr(m, i - 1), m->spare, dl, rows, kmin, dp);
   MPS_HIP(hipGetLastError());
-  MPS_HIP(hipMemcpyAsync(site_ptr(m, i - 1), m->spare, sizeof(C) * dp * 2 * r, hipMemcpyDeviceToDevice, m->stream));
-  m->dim[i] = r;
+  MPS_HIP(hipMemcpyAsync(site_ptr(m, i - 1), m->spare, sizeof(C) * dp * 2 * kmin, hipMemcpyDeviceToDevice,
+                         m->stream));
+  m->dim[i] = kmin;
   m->center = i - 1;
   return MSV_OK;
 }
@@ -504,7 +790,7 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
                      site_ptr(m, i), site_ptr(m, i + 1), m->work, dl, dm, dr, G);
   MPS_HIP(hipGetLastError());
   const int rows = 2 * dl, cols = 2 * dr, kmin = std::min(rows, cols);
-  MPS_TRY(svd(m, rows, cols));
+  MPS_TRY(factor(m, rows, cols));
   const std::vector<float> &s = m->h_sv;
   // Exact zeros are dropped even at eps = 0: the device SVD's vectors for
   // them are not reliable, and they carry no weight. At eps = 0 (where the
@@ -543,8 +829,8 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
   if (m->renormalize) m->st.renormalization_count += 1;
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   hipLaunchKernelGGL(k_mps_split, dim3(blocks_for(static_cast<uint64_t>(rows) * r + static_cast<uint64_t>(r) * cols)),
-                     dim3(kThreads), 0, m->stream, m->work + slot, m->sv, m->work + 2 * slot, site_ptr(m, i),
-                     site_ptr(m, i + 1), dl, dr, kmin, r, scale, 0);
+                     dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
+                     site_ptr(m, i + 1), dl, dr, kmin, r, scale);
   MPS_HIP(hipGetLastError());
   m->dim[i + 1] = r;
   m->center = i + 1;
