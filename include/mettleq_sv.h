@@ -456,7 +456,7 @@ int msv_probe_copy_bandwidth(int device, uint64_t bytes, int iters, double *ms_p
 typedef struct msv_mps msv_mps;
 
 typedef struct {
-  int64_t svd_calls;        /* SVDs run (two-site updates and center moves) */
+  int64_t svd_calls;        /* SVDs run: one per two-site update, as the reference counts them */
   int64_t two_site_calls;   /* two-site updates (gates and swaps) */
   int64_t swap_calls;       /* swaps of the routing network */
   int64_t trunc_events;     /* two-site updates that kept fewer than their rank */

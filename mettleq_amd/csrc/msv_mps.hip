@@ -663,6 +663,144 @@ int check_site(const msv_mps *m, int q) {
 }
 C *site_ptr(msv_mps *m, int i) { return m->sites + static_cast<size_t>(i) * m->cap; }
 
+// Reduced QR of M (rows x cols, column-major; rows * cols + rows * kmin <=
+// kBasisLds) in one workgroup with M in LDS, for the center moves (the
+// reference's economic QR, mps_state.py:347-399): Householder reflectors
+// H_j = I - beta_j u_j u_j^H (Hermitian and unitary) with
+// u_j = x + e^{i arg x_0} |x| e_0 for x = M[j:, j], so H_j x = -e^{i arg x_0} |x| e_0;
+// dot products in double precision. Q = H_0 ... H_{kmin-1} [I; 0] (rows x kmin)
+// is orthonormal whatever the rank of M; R = Q^H M (kmin x cols, leading
+// dimension kmin, upper trapezoidal). A column whose x is at or below the
+// floor (1e-32 |M|_F) takes no reflector (beta = 0).
+// Column c belongs to the 16-lane DPP row c mod (rows of the launch); each
+// step applies H_j to the owned columns c > j, and the owner of column j + 1
+// forms the next reflector as soon as its update is done, so each step (and
+// each step of the accumulation of Q) ends in one barrier.
+template <int RPL>
+__device__ __forceinline__ void hh_form(C *col, int j, int rows, int l16, double floor2, double *beta,
+                                        C *rdiag) {
+  double s2 = 0;
+  C x0 = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = j + l16 + kPairLanes * t;
+    if (r < rows) {
+      const C x = col[r];
+      s2 += (double)x.x * x.x + (double)x.y * x.y;
+    }
+  }
+  s2 = row16_sum(s2);
+  x0 = col[j];
+  if (!(s2 > floor2)) {
+    if (l16 == 0) {
+      beta[j] = 0.0;
+      rdiag[j] = x0;
+    }
+    return;
+  }
+  const double nx = sqrt(s2);
+  const double a0 = sqrt((double)x0.x * x0.x + (double)x0.y * x0.y);
+  // e^{i arg x_0} (1 when x_0 = 0)
+  const double er = a0 > 0 ? x0.x / a0 : 1.0, ei = a0 > 0 ? x0.y / a0 : 0.0;
+  if (l16 == 0) {
+    col[j] = make_float2((float)(x0.x + er * nx), (float)(x0.y + ei * nx));  // u_0
+    rdiag[j] = make_float2((float)(-er * nx), (float)(-ei * nx));
+    beta[j] = 1.0 / (nx * (nx + a0));  // 2 / (u^H u), u^H u = 2 |x| (|x| + |x_0|)
+  }
+}
+
+// col[j:] <- H col[j:] for the reflector u stored in ucol[j:].
+template <int RPL>
+__device__ __forceinline__ void hh_apply(const C *ucol, C *col, int j, int rows, int l16, double beta) {
+  C u[RPL], x[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = j + l16 + kPairLanes * t;
+    const int rr = r < rows ? r : j;
+    u[t] = ucol[rr];
+    x[t] = col[rr];
+  }
+  double wr = 0, wi = 0;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    if (j + l16 + kPairLanes * t < rows) {
+      wr += (double)u[t].x * x[t].x + (double)u[t].y * x[t].y;  // conj(u) x
+      wi += (double)u[t].x * x[t].y - (double)u[t].y * x[t].x;
+    }
+  }
+  wr = row16_sum(wr) * beta;
+  wi = row16_sum(wi) * beta;
+  const float fr = (float)wr, fi = (float)wi;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = j + l16 + kPairLanes * t;
+    if (r < rows) {
+      // x -= u w
+      col[r] = make_float2(x[t].x - (u[t].x * fr - u[t].y * fi), x[t].y - (u[t].x * fi + u[t].y * fr));
+    }
+  }
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(1024) k_mps_qr_lds(const C *__restrict__ M, int rows, int cols,
+                                                     C *__restrict__ Q, C *__restrict__ R) {
+  __shared__ C lds[kBasisLds];
+  __shared__ double beta[kBasisMaxDim];
+  __shared__ C rdiag[kBasisMaxDim];
+  __shared__ double wave_part[kSvdThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int l16 = tid & (kPairLanes - 1), group = tid / kPairLanes, ngroups = blockDim.x / kPairLanes;
+  const int kmin = rows < cols ? rows : cols;
+  C *A = lds;
+  C *Qs = lds + rows * cols;
+  double fro = 0;
+  for (int idx = tid; idx < rows * cols; idx += blockDim.x) {
+    const C a = M[idx];
+    A[idx] = a;
+    fro += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  for (int idx = tid; idx < rows * kmin; idx += blockDim.x)
+    Qs[idx] = make_float2((idx % rows) == (idx / rows) ? 1.f : 0.f, 0.f);
+  fro = wave_sum(fro);
+  if (lane == 0) wave_part[wave] = fro;
+  __syncthreads();
+  fro = 0;
+  for (int w = 0; w < nwave; ++w) fro += wave_part[w];
+  const double floor2 = fmax(kJacobiFloor * kJacobiFloor * fro, 1e-66);
+  if (group == 0) hh_form<RPL>(A, 0, rows, l16, floor2, beta, rdiag);
+  __syncthreads();
+  for (int j = 0; j < kmin; ++j) {
+    const double bj = beta[j];
+    const C *u = A + j * rows;
+    // the owner of column j + 1 updates it first, then forms the next reflector
+    for (int c = j + 1 + ((group - (j + 1)) % ngroups + ngroups) % ngroups; c < cols; c += ngroups) {
+      C *col = A + c * rows;
+      if (bj != 0.0) hh_apply<RPL>(u, col, j, rows, l16, bj);
+      if (c == j + 1 && j + 1 < kmin) hh_form<RPL>(col, j + 1, rows, l16, floor2, beta, rdiag);
+    }
+    __syncthreads();
+  }
+  // Q = H_0 ... H_{kmin-1} [I; 0]: H_j touches rows j.. and, applied in this
+  // order, only the columns c >= j
+  for (int j = kmin - 1; j >= 0; --j) {
+    const double bj = beta[j];
+    if (bj != 0.0) {
+      const C *u = A + j * rows;
+      for (int c = j + ((group - j) % ngroups + ngroups) % ngroups; c < kmin; c += ngroups)
+        hh_apply<RPL>(u, Qs + c * rows, j, rows, l16, bj);
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < rows * kmin; idx += blockDim.x) Q[idx] = Qs[idx];
+  for (int idx = tid; idx < kmin * cols; idx += blockDim.x) {
+    const int i = idx % kmin, c = idx / kmin;
+    C v = make_float2(0.f, 0.f);
+    if (i == c) v = rdiag[i];
+    else if (i < c) v = A[i + c * rows];
+    R[idx] = v;
+  }
+}
+
 template <bool kAccum>
 void launch_basis_rpl(int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
                       int *info) {
@@ -719,23 +857,44 @@ int factor(msv_mps *m, int rows, int cols) {
   MPS_HIP(hipMemcpyAsync(m->h_sv.data(), m->sv, sizeof(float) * kmin, hipMemcpyDeviceToHost, m->stream));
   MPS_HIP(hipMemcpyAsync(&h_info, m->info, sizeof(int), hipMemcpyDeviceToHost, m->stream));
   MPS_HIP(hipStreamSynchronize(m->stream));
-  m->st.svd_calls += 1;
   if (h_info < 0) return set_error(MSV_E_INVALID, "MPS SVD did not converge");
   for (int k = 0; k < kmin; ++k)
     if (!std::isfinite(m->h_sv[k])) return set_error(MSV_E_INVALID, "MPS SVD produced non-finite singular values");
   return MSV_OK;
 }
 
+// M = Q R for a center move: k_mps_qr_lds when M and Q fit in LDS (no host
+// round trip: nothing is truncated), else the SVD factorization (R = S Vh).
+int factor_move(msv_mps *m, int rows, int cols) {
+  if (rows > kBasisMaxDim || cols > kBasisMaxDim ||
+      static_cast<int64_t>(rows) * cols + static_cast<int64_t>(rows) * std::min(rows, cols) > kBasisLds)
+    return factor(m, rows, cols);
+  const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
+  const int kmin = std::min(rows, cols);
+  const int threads = std::min(kSvdThreads, std::max(64, (cols * kPairLanes + 63) / 64 * 64));
+  C *q = m->work + slot, *r = m->work + 2 * slot;
+  if (rows <= 16)
+    hipLaunchKernelGGL(k_mps_qr_lds<1>, dim3(1), dim3(threads), 0, m->stream, m->work, rows, cols, q, r);
+  else if (rows <= 32)
+    hipLaunchKernelGGL(k_mps_qr_lds<2>, dim3(1), dim3(threads), 0, m->stream, m->work, rows, cols, q, r);
+  else if (rows <= 64)
+    hipLaunchKernelGGL(k_mps_qr_lds<4>, dim3(1), dim3(threads), 0, m->stream, m->work, rows, cols, q, r);
+  else
+    hipLaunchKernelGGL(k_mps_qr_lds<8>, dim3(1), dim3(threads), 0, m->stream, m->work, rows, cols, q, r);
+  MPS_HIP(hipGetLastError());
+  return MSV_OK;
+}
+
 // Moves the orthogonality center one site right (from i to i + 1) or left
 // (from i to i - 1) without truncation: the reference's QR moves
-// (mps_state.py:347-389), here through the factorization M = Q X.
+// (mps_state.py:347-389), M = Q R with R carried into the neighbour.
 int move_right(msv_mps *m, int i) {
   const int dl = m->dim[i], dr = m->dim[i + 1], dn = m->dim[i + 2];
   const int rows = 2 * dl, cols = dr, kmin = std::min(rows, cols);
   hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(rows) * cols)), dim3(kThreads), 0,
                      m->stream, site_ptr(m, i), m->work, rows, cols, 0);
   MPS_HIP(hipGetLastError());
-  MPS_TRY(factor(m, rows, cols));
+  MPS_TRY(factor_move(m, rows, cols));
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   hipLaunchKernelGGL(k_mps_absorb_right, dim3(blocks_for(static_cast<uint64_t>(rows) * kmin + kmin * 2 * dn)),
                      dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
@@ -755,7 +914,7 @@ int move_left(msv_mps *m, int i) {
   hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(rows) * cols)), dim3(kThreads), 0,
                      m->stream, site_ptr(m, i), m->work, dl, 2 * dr, 1);
   MPS_HIP(hipGetLastError());
-  MPS_TRY(factor(m, rows, cols));
+  MPS_TRY(factor_move(m, rows, cols));
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   hipLaunchKernelGGL(k_mps_absorb_left, dim3(blocks_for(static_cast<uint64_t>(kmin) * rows + dp * 2 * kmin)),
                      dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
@@ -791,6 +950,7 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
   MPS_HIP(hipGetLastError());
   const int rows = 2 * dl, cols = 2 * dr, kmin = std::min(rows, cols);
   MPS_TRY(factor(m, rows, cols));
+  m->st.svd_calls += 1;
   const std::vector<float> &s = m->h_sv;
   // Exact zeros are dropped even at eps = 0: the device SVD's vectors for
   // them are not reliable, and they carry no weight. At eps = 0 (where the
