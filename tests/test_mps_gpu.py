@@ -89,6 +89,32 @@ def test_mps_truncation_matches_oracle(gpu, n, dmax, eps):
     assert _overlap(ref.to_statevector(), got) > 1 - 1e-4
 
 
+def test_mps_large_bonds_take_the_global_memory_paths(gpu):
+    """Bonds past 64: a 14-qubit brick circuit of random two-qubit gates in
+    the exact regime (dmax 128 = the largest Schmidt rank) reaches a bond of
+    128, so two-site matrices of 64 x 256 go through the global-memory SVD
+    and 128 x 128 center moves through the Jacobi factorization (beyond the
+    LDS QR). The state must match the dense oracle."""
+    from mettleq_amd.mps import MPSOptions, MPSState
+    n = 14
+    rng = np.random.default_rng(1400)
+    ops = []
+    for layer in range(9):
+        for q in range(n):
+            ops.append((q, _rand_u(rng, 1)))
+        for a in range(layer % 2, n - 1, 2):
+            ops.append(((a, a + 1), _rand_u(rng, 2)))
+    want = O.zero_state(n)
+    for q, u in ops:
+        want = O.apply_dense(want, n, u, [q] if np.ndim(q) == 0 else list(q))
+    with MPSState(n, MPSOptions(dmax=128, eps=0.0)) as mps:
+        _run(mps, ops)
+        assert max(mps.bond_dims()) == 128
+        assert not mps.truncated()
+        got = mps.to_statevector()
+    assert _overlap(want, got) > 1 - 1e-5
+
+
 def test_mps_wide_ghz_expectations(gpu):
     """A 48-qubit GHZ state: bonds of 2, and <Z_0 Z_47> = <X^48> = 1,
     <Z_0> = 0, read on the MPS by transfer matrices (no dense state)."""
