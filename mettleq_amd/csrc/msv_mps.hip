@@ -207,28 +207,40 @@ __global__ void __launch_bounds__(256) k_mps_contract(const C *__restrict__ in, 
   out[idx] = acc;
 }
 
-// Transfer step of <psi| O_1 x ... x O_n |psi>: env' = sum_{a, a', s, t}
-// conj(A[a, s, b]) env[a, a'] O[s, t] A[a', t, b'] (complex128 environment).
-__global__ void __launch_bounds__(256) k_mps_transfer(const double2 *__restrict__ env, const C *__restrict__ A,
-                                                      double2 *__restrict__ out, int dm, int dn, Gate2 O) {
+// Transfer step of <psi| O_1 x ... x O_n |psi> (complex128 environment) in
+// two contractions of O(d^3) each:
+//   k_mps_transfer_a  T[a, t, b'] = sum_a' env[a, a'] A[a', t, b'];
+//   k_mps_transfer_b  env'[b, b'] = sum_{a, s} conj(A[a, s, b]) sum_t O[s, t] T[a, t, b'].
+__global__ void __launch_bounds__(256) k_mps_transfer_a(const double2 *__restrict__ env, const C *__restrict__ A,
+                                                        double2 *__restrict__ T, int dm, int dn) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= dm * 2 * dn) return;
+  const int a = idx / (2 * dn), tb = idx % (2 * dn);
+  double re = 0, im = 0;
+  for (int ap = 0; ap < dm; ++ap) {
+    const double2 e = env[a * dm + ap];
+    const C x = A[ap * 2 * dn + tb];
+    re += e.x * x.x - e.y * x.y;
+    im += e.x * x.y + e.y * x.x;
+  }
+  T[idx] = make_double2(re, im);
+}
+
+__global__ void __launch_bounds__(256) k_mps_transfer_b(const double2 *__restrict__ T, const C *__restrict__ A,
+                                                        double2 *__restrict__ out, int dm, int dn, Gate2 O) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= dn * dn) return;
   const int b = idx / dn, bp = idx % dn;
   double re = 0, im = 0;
   for (int a = 0; a < dm; ++a) {
+    const double2 t0 = T[(a * 2) * dn + bp], t1 = T[(a * 2 + 1) * dn + bp];
+#pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const C ca = A[(a * 2 + s) * dn + b];  // conj taken below
-      // y = sum_{a', t} env[a, a'] O[s, t] A[a', t, b']
-      double yr = 0, yi = 0;
-      for (int ap = 0; ap < dm; ++ap) {
-        const double2 e = env[a * dm + ap];
-        const C x0 = A[(ap * 2) * dn + bp], x1 = A[(ap * 2 + 1) * dn + bp];
-        const C o0 = O.u[s * 2], o1 = O.u[s * 2 + 1];
-        const double tr = (double)o0.x * x0.x - (double)o0.y * x0.y + (double)o1.x * x1.x - (double)o1.y * x1.y;
-        const double ti = (double)o0.x * x0.y + (double)o0.y * x0.x + (double)o1.x * x1.y + (double)o1.y * x1.x;
-        yr += e.x * tr - e.y * ti;
-        yi += e.x * ti + e.y * tr;
-      }
+      const C o0 = O.u[s * 2], o1 = O.u[s * 2 + 1];
+      // y = O[s, 0] T[a, 0, b'] + O[s, 1] T[a, 1, b']
+      const double yr = o0.x * t0.x - o0.y * t0.y + o1.x * t1.x - o1.y * t1.y;
+      const double yi = o0.x * t0.y + o0.y * t0.x + o1.x * t1.y + o1.y * t1.x;
+      const C ca = A[(a * 2 + s) * dn + b];  // conj taken here
       re += (double)ca.x * yr + (double)ca.y * yi;
       im += (double)ca.x * yi - (double)ca.y * yr;
     }
@@ -1190,7 +1202,9 @@ int msv_mps_expval_product(msv_mps *m, const int32_t *wires, const double *ops, 
   }
   double2 *env = nullptr;
   const size_t emax = static_cast<size_t>(m->dmax) * m->dmax;
-  MPS_HIP(hipMallocAsync(reinterpret_cast<void **>(&env), sizeof(double2) * 2 * emax, m->stream));
+  // two environments (dn x dn each) and the intermediate T (dm x 2 x dn)
+  MPS_HIP(hipMallocAsync(reinterpret_cast<void **>(&env), sizeof(double2) * 4 * emax, m->stream));
+  double2 *tmid = env + 2 * emax;
   const double2 one = make_double2(1.0, 0.0);
   hipError_t e = hipMemcpyAsync(env, &one, sizeof(double2), hipMemcpyHostToDevice, m->stream);
   int cur = 0;
@@ -1204,8 +1218,10 @@ int msv_mps_expval_product(msv_mps *m, const int32_t *wires, const double *ops, 
       O.u[1] = O.u[2] = make_float2(0, 0);
     }
     const int dm = m->dim[i], dn = m->dim[i + 1];
-    hipLaunchKernelGGL(k_mps_transfer, dim3(blocks_for(static_cast<uint64_t>(dn) * dn)), dim3(kThreads), 0,
-                       m->stream, env + cur * emax, site_ptr(m, i), env + (1 - cur) * emax, dm, dn, O);
+    hipLaunchKernelGGL(k_mps_transfer_a, dim3(blocks_for(static_cast<uint64_t>(dm) * 2 * dn)), dim3(kThreads), 0,
+                       m->stream, env + cur * emax, site_ptr(m, i), tmid, dm, dn);
+    hipLaunchKernelGGL(k_mps_transfer_b, dim3(blocks_for(static_cast<uint64_t>(dn) * dn)), dim3(kThreads), 0,
+                       m->stream, tmid, site_ptr(m, i), env + (1 - cur) * emax, dm, dn, O);
     e = hipGetLastError();
     cur = 1 - cur;
   }
