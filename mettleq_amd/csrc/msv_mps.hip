@@ -1072,16 +1072,23 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
 
 int msv_mps_destroy(msv_mps *m) {
   if (!m) return MSV_OK;
-  hipSetDevice(m->device);
-  if (m->stream) hipStreamSynchronize(m->stream);
-  hipFree(m->sites);
-  hipFree(m->spare);
-  hipFree(m->work);
-  hipFree(m->sv);
-  hipFree(m->info);
-  if (m->stream) hipStreamDestroy(m->stream);
+  // everything is released whatever fails; the first error is reported
+  hipError_t e = hipSetDevice(m->device);
+  if (m->stream) {
+    const hipError_t es = hipStreamSynchronize(m->stream);
+    if (e == hipSuccess) e = es;
+  }
+  for (void *p : {static_cast<void *>(m->sites), static_cast<void *>(m->spare), static_cast<void *>(m->work),
+                  static_cast<void *>(m->sv), static_cast<void *>(m->info)}) {
+    const hipError_t ef = hipFree(p);
+    if (e == hipSuccess) e = ef;
+  }
+  if (m->stream) {
+    const hipError_t ed = hipStreamDestroy(m->stream);
+    if (e == hipSuccess) e = ed;
+  }
   delete m;
-  return MSV_OK;
+  return e == hipSuccess ? MSV_OK : fail_hip(e, "MPS destroy");
 }
 
 int msv_mps_reset(msv_mps *m) {
