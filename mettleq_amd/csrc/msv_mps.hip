@@ -643,8 +643,9 @@ struct msv_mps {
   size_t cap = 0;                 // complex entries per site buffer (dmax x 2 x dmax)
   C *sites = nullptr;             // n site buffers
   C *spare = nullptr;             // one site buffer (out-of-place absorb)
-  C *work = nullptr;              // SVD input / U / Vh / Jacobi W / Jacobi V (5 x 4 dmax^2)
-  float *sv = nullptr;            // singular values and superdiagonal (2 x 2 dmax)
+  C *work = nullptr;              // 5 slots of 4 dmax^2: the factorization's input M, Q, X = Q^H M,
+                                  // and W and V of the global-memory Jacobi (k_mps_svd)
+  float *sv = nullptr;            // singular values (up to 2 dmax)
   int *info = nullptr;
   std::vector<int> dim;           // bond dimensions, dim[0] = dim[n] = 1
   int center = 0;
@@ -1036,9 +1037,9 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
   if (!out) return set_error(MSV_E_INVALID, "null output");
   *out = nullptr;
   if (n_qubits < 1 || n_qubits > 4096) return set_error(MSV_E_INVALID, "MPS qubit count out of range");
+  // dmax <= 1024: the global-memory Jacobi keeps a norm and a rank per column
+  // of the two-site matrix (2 dmax columns at most) in LDS
   if (dmax < 1 || dmax > 1024) return set_error(MSV_E_INVALID, "MPS dmax must be in [1, 1024]");
-  // the Jacobi SVD keeps one float and one int per column of the
-  // (2 dmax)-column two-site matrix in LDS
   if (!(eps >= 0.0)) return set_error(MSV_E_INVALID, "MPS eps must be non-negative");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
