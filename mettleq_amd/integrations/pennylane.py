@@ -48,6 +48,7 @@ except ImportError as exc:  # pragma: no cover - exercised without the extra
 from ._common import (apply_global_phase, bit_counts, core_operation, execute_operations,
                       local_expectation, marginal_probabilities, pauli_product_expectation,
                       pauli_product_expectations, sample_bits, statevector_numpy)
+from .policy import PENNYLANE_GPU_CROSSOVER_QUBITS, adaptive_plan, engine_target
 
 _PENNYLANE_TO_CORE = {
     "Hadamard": "H", "PauliX": "X", "PauliY": "Y", "PauliZ": "Z", "S": "S", "T": "T", "SX": "SX",
@@ -199,15 +200,15 @@ class MettleQDevice(PennyLaneDevice):
             for dev in cache.values():
                 dev.close()
 
-    def _execute_circuit(self, circuit, cache):
+    def _execute_circuit(self, circuit, cache, *, method: Optional[str] = None, dtype: Optional[str] = None):
         operations, phase = self._canonical_operations(circuit)
         shots = circuit.shots if circuit.shots else self.shots
         default_shots = shots.total_shots or 1000
         device = execute_operations(len(self.wires), operations, shots=default_shots,
                                     report=self._execution_report,
                                     allow_unsafe_statevector=self._allow_unsafe_statevector,
-                                    execution_cache=cache, dtype=self._dtype, method=self._method,
-                                    **self._mps_options)
+                                    execution_cache=cache, dtype=dtype or self._dtype,
+                                    method=method or self._method, **self._mps_options)
         apply_global_phase(device, phase)
         self.last_execution_plan = device.last_execution_plan
         self.statevector_preflight = device.statevector_preflight
@@ -351,3 +352,81 @@ class MettleQDevice(PennyLaneDevice):
         if abs(value.imag) > tol:
             raise DeviceError(f"Hermitian {label} developed an imaginary component {value.imag:.3e}")
         return float(value.real)
+
+
+@simulator_tracking
+@single_tape_support
+class AdaptivePennyLaneDevice(MettleQDevice):
+    """Each tape to Lightning's CPU simulator or to the MI355X engine
+    (pennylane.py:646-741): recommend_sdk_engine on the tape's gate stream,
+    the precision, the output contract its measurements imply and the dense
+    preflight, with the crossover calibrated on MI355X. Without
+    pennylane_lightning every tape stays on this engine; the decisions are
+    kept in last_adaptive_decisions."""
+
+    def __init__(self, wires, shots=None, *, precision: str = "single",
+                 pennylane_gpu_crossover_qubits: int = PENNYLANE_GPU_CROSSOVER_QUBITS, **kwargs) -> None:
+        super().__init__(wires=wires, shots=shots, **kwargs)
+        precision = str(precision).strip().lower()
+        if precision not in ("single", "double"):
+            raise DeviceError("precision must be 'single' or 'double'")
+        self._adaptive_precision = precision
+        self._adaptive_gpu_crossover = int(pennylane_gpu_crossover_qubits)
+        self.last_adaptive_decisions = []
+
+    @property
+    def name(self) -> str:
+        return "mettleq.adaptive"
+
+    @staticmethod
+    def _lightning_available() -> bool:
+        try:
+            import pennylane_lightning  # noqa: F401
+        except ImportError:
+            return False
+        return True
+
+    @staticmethod
+    def _output_contract(circuit) -> str:
+        kinds = [type(m) for m in circuit.measurements]
+        for mp, contract in ((StateMP, "statevector"), (ExpectationMP, "expectation"),
+                             (ProbabilityMP, "probabilities")):
+            if any(issubclass(k, mp) for k in kinds):
+                return contract
+        return "samples"
+
+    def execute(self, circuits, execution_config=None):
+        config = self.setup_execution_config(execution_config)
+        circuits = list(circuits)
+        batch = [(len(self.wires), self._canonical_operations(c)[0], self._output_contract(c)) for c in circuits]
+        try:
+            decisions, _ = adaptive_plan(
+                "pennylane", batch, precision=self._adaptive_precision, method=self._method,
+                allow_approximation=self._mps_options["allow_approximation"],
+                fallback_available=self._lightning_available(),
+                allow_unsafe_statevector=self._allow_unsafe_statevector,
+                pennylane_gpu_crossover_qubits=self._adaptive_gpu_crossover)
+        except ValueError as exc:
+            raise DeviceError(str(exc)) from exc
+        self.last_adaptive_decisions = [d.to_dict() for d in decisions]
+        lightning: dict = {}
+        cache: dict = {}
+        results = []
+        try:
+            for circuit, decision in zip(circuits, decisions):
+                target = engine_target(decision.engine)
+                if target.sdk_cpu:
+                    key = repr(circuit.shots)
+                    if key not in lightning:
+                        lightning[key] = qml.device("lightning.qubit", wires=self.wires, shots=circuit.shots)
+                    results.append(lightning[key].execute(circuit, config))
+                    self.last_execution_plan = None
+                    self.statevector_preflight = decision.statevector_preflight
+                    self.last_execution_selection = decision.to_dict()
+                else:
+                    results.append(self._execute_circuit(circuit, cache, method=target.method, dtype=target.dtype))
+        finally:
+            for dev in cache.values():
+                dev.close()
+        return tuple(results)
+

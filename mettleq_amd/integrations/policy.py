@@ -194,3 +194,45 @@ def recommend_sdk_engine(
         calibrated_gpu_crossover_qubits=base, adjusted_gpu_crossover_qubits=adjusted,
         statevector_preflight=preflight, profile=profile, fallback_available=bool(fallback_available),
     )
+
+
+@dataclass(frozen=True)
+class EngineTarget:
+    """Where an engine name of recommend_sdk_engine runs: the SDK's own CPU
+    simulator (Aer / Lightning) or this package's engine, with the method
+    and state type to run it with."""
+    sdk_cpu: bool
+    method: str
+    dtype: str
+
+
+def engine_target(engine: str) -> EngineTarget:
+    if engine in ("qiskit_aer_cpu_statevector", "pennylane_lightning_cpu"):
+        return EngineTarget(True, "statevector", "complex64")
+    if engine == "qiskit_aer_cpu_mps":
+        return EngineTarget(True, "matrix_product_state", "complex64")
+    if engine == "mettleq_gpu_statevector":
+        return EngineTarget(False, "statevector", "complex64")
+    if engine == "mettleq_gpu_statevector_complex128":
+        return EngineTarget(False, "statevector", "complex128")
+    if engine == "mettleq_gpu_mps":
+        return EngineTarget(False, "matrix_product_state", "complex64")
+    raise ValueError(f"engine {engine!r} does not run anything")
+
+
+def adaptive_plan(sdk: str, circuits: Sequence[tuple], **policy) -> tuple:
+    """The per-batch part of the reference's adaptive SDK front ends
+    (qiskit.py:244-316 AdaptiveQiskitBackend.run, pennylane.py:690-738
+    AdaptivePennyLaneDevice.execute): one recommend_sdk_engine decision per
+    circuit ((n_qubits, operations, output_contract) each, the remaining
+    keywords passed through), and the circuit indices grouped by engine in
+    order of first use. A refused circuit raises ValueError with its reason
+    before anything runs. Returns (decisions, groups)."""
+    decisions = [recommend_sdk_engine(sdk, n, ops, output_contract=contract, **policy)
+                 for n, ops, contract in circuits]
+    groups: dict = {}
+    for index, decision in enumerate(decisions):
+        if decision.engine == "refused":
+            raise ValueError(decision.reason)
+        groups.setdefault(decision.engine, []).append(index)
+    return decisions, groups

@@ -6,8 +6,10 @@ already-DONE jobs, final-measurement-only translation with wire reversal
 (core wire = n-1-qiskit qubit, qiskit.py:576-581), pair-phase angle halving
 for rxx/ryy/rzz (qiskit.py:572-575), hex-keyed counts packed by classical bit
 (qiskit.py:683-695) and the optional statevector / execution-report fields.
-Only the exact statevector method is provided; MPS and midpoint-MPO options
-are accepted for signature compatibility and rejected if selected.
+method="matrix_product_state" runs this package's MPS engine (mettleq_amd.mps)
+with the reference's MPS options. AdaptiveQiskitBackend (qiskit.py:115-316)
+sends each circuit to Aer's CPU simulator or to this engine by the
+MI355X-calibrated policy (integrations/policy.py).
 """
 from __future__ import annotations
 
@@ -35,6 +37,7 @@ from .. import __version__
 from ..execution import StatevectorMemoryError  # noqa: F401  (re-export for callers)
 from ._common import (apply_global_phase, core_operation, execute_operations,
                       pauli_product_expectations, sample_bits, statevector_numpy)
+from .policy import QISKIT_GPU_CROSSOVER_QUBITS, adaptive_plan, engine_target
 
 _QISKIT_TO_CORE = {
     "h": "H", "x": "X", "y": "Y", "z": "Z", "s": "S", "sdg": "SDG", "t": "T", "tdg": "TDG",
@@ -259,6 +262,117 @@ class MettleQBackend(BackendV2):
         }
         return {"name": circuit.name, "seed_simulator": options["seed_simulator"], "shots": shots,
                 "data": data, "status": "DONE", "success": True, "header": header}
+
+
+class AdaptiveQiskitBackend(BackendV2):
+    """Each circuit to Aer's CPU simulator or to the MI355X engine
+    (qiskit.py:115-316): the decision is recommend_sdk_engine on the
+    translated gate stream, the precision, the output contract and the dense
+    preflight, with the crossover calibrated on MI355X. Without qiskit_aer the
+    policy keeps every circuit on this engine. Circuits of one engine run as
+    one job; the results come back in the input order, and the decisions are
+    kept in last_adaptive_decisions."""
+
+    version = 2
+
+    def __init__(self, provider=None, *, precision: str = "single", output_contract: str = "samples",
+                 qiskit_gpu_crossover_qubits: int = QISKIT_GPU_CROSSOVER_QUBITS,
+                 mettleq_options: Optional[dict] = None, **fields) -> None:
+        super().__init__(provider=provider, name="mettleq_adaptive",
+                         description="Circuit-aware Aer CPU / MettleQ MI355X dispatcher",
+                         backend_version=__version__, **fields)
+        precision = str(precision).strip().lower()
+        if precision not in ("single", "double"):
+            raise ValueError("precision must be 'single' or 'double'")
+        self.precision = precision
+        self.output_contract = str(output_contract).strip().lower()
+        self.qiskit_gpu_crossover_qubits = int(qiskit_gpu_crossover_qubits)
+        self._mettleq = MettleQBackend(**dict(mettleq_options or {}))
+        self.last_adaptive_decisions = []
+
+    @classmethod
+    def _default_options(cls):
+        return Options(shots=1024, memory=False, seed_simulator=None, return_statevector=False,
+                       execution_report=False, precision=None, output_contract=None)
+
+    @property
+    def target(self) -> Target:
+        return self._mettleq.target
+
+    @property
+    def max_circuits(self):
+        return None
+
+    @staticmethod
+    def _aer_available() -> bool:
+        try:
+            import qiskit_aer  # noqa: F401
+        except ImportError:
+            return False
+        return True
+
+    def _aer_job(self, circuits, target, precision, options):
+        from qiskit_aer import AerSimulator
+        sim = AerSimulator(method=target.method, device="CPU", precision=precision, enable_truncation=False)
+        batch = []
+        for circuit in circuits:
+            circuit = circuit.copy()
+            if options["return_statevector"]:
+                circuit.save_statevector()
+            batch.append(circuit)
+        kwargs = {"shots": int(options["shots"]), "memory": bool(options["memory"])}
+        if options["seed_simulator"] is not None:
+            kwargs["seed_simulator"] = options["seed_simulator"]
+        return sim.run(batch, **kwargs)
+
+    def run(self, run_input, **run_options) -> _CompletedJob:
+        circuits = ([run_input] if isinstance(run_input, QuantumCircuit)
+                    else list(run_input) if isinstance(run_input, Sequence) else None)
+        if not circuits or not all(isinstance(c, QuantumCircuit) for c in circuits):
+            raise QiskitError("AdaptiveQiskitBackend.run expects a QuantumCircuit or a non-empty "
+                              "sequence of QuantumCircuit objects")
+        unknown = set(run_options) - set(self.options)
+        if unknown:
+            raise QiskitError(f"Unsupported adaptive run option(s): {', '.join(sorted(unknown))}")
+        options = {name: getattr(self.options, name) for name in self.options}
+        options.update(run_options)
+        precision = str(options["precision"] or self.precision).strip().lower()
+        contract = options["output_contract"] or ("statevector" if options["return_statevector"]
+                                                  else self.output_contract)
+        batch = [(c.num_qubits, MettleQBackend._translate_circuit(c)[0], contract) for c in circuits]
+        try:
+            decisions, groups = adaptive_plan(
+                "qiskit", batch, precision=precision, method="statevector",
+                fallback_available=self._aer_available(),
+                qiskit_gpu_crossover_qubits=self.qiskit_gpu_crossover_qubits)
+        except ValueError as exc:
+            raise QiskitError(str(exc)) from exc
+        self.last_adaptive_decisions = [d.to_dict() for d in decisions]
+        slots = [None] * len(circuits)
+        success, elapsed = True, 0.0
+        for engine, indices in groups.items():
+            target = engine_target(engine)
+            chosen = [circuits[i] for i in indices]
+            if target.sdk_cpu:
+                job = self._aer_job(chosen, target, precision, options)
+            else:
+                job = self._mettleq.run(chosen, shots=options["shots"], memory=options["memory"],
+                                        seed_simulator=options["seed_simulator"],
+                                        return_statevector=options["return_statevector"],
+                                        execution_report=options["execution_report"],
+                                        method=target.method, dtype=target.dtype)
+            payload = job.result().to_dict()
+            success = success and bool(payload.get("success", True))
+            elapsed += float(payload.get("time_taken") or 0.0)
+            for i, experiment in zip(indices, payload.get("results", [])):
+                slots[i] = experiment
+        job_id = str(uuid.uuid4())
+        result = Result.from_dict({
+            "backend_name": self.name, "backend_version": self.backend_version, "qobj_id": None,
+            "job_id": job_id, "success": success, "results": slots,
+            "status": "COMPLETED" if success else "ERROR", "time_taken": elapsed,
+        })
+        return _CompletedJob(self, job_id, result)
 
 
 class MettleQSamplerV2(BackendSamplerV2):

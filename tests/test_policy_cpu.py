@@ -124,3 +124,46 @@ def test_mps_requests():
 def test_refusal_and_bounded_local_fallback():
     assert recommend_sdk_engine("qiskit", 38, _layer(38)).engine == "refused"
     assert recommend_sdk_engine("qiskit", 38, _layer(38), allow_approximation=True).engine == "qiskit_aer_cpu_mps"
+
+
+def test_engine_targets():
+    from mettleq_amd.integrations.policy import engine_target
+    assert engine_target("qiskit_aer_cpu_statevector").sdk_cpu
+    assert engine_target("pennylane_lightning_cpu").sdk_cpu
+    assert engine_target("qiskit_aer_cpu_mps").method == "matrix_product_state"
+    g = engine_target("mettleq_gpu_statevector")
+    assert not g.sdk_cpu and (g.method, g.dtype) == ("statevector", "complex64")
+    assert engine_target("mettleq_gpu_statevector_complex128").dtype == "complex128"
+    m = engine_target("mettleq_gpu_mps")
+    assert not m.sdk_cpu and m.method == "matrix_product_state"
+    with pytest.raises(ValueError):
+        engine_target("refused")
+
+
+def test_adaptive_plan_groups_circuits_by_engine():
+    """The per-batch part of the reference's adaptive front ends
+    (qiskit.py:244-316, pennylane.py:690-738): one decision per circuit, the
+    indices grouped by engine in order of first use."""
+    from mettleq_amd.integrations.policy import adaptive_plan
+    batch = [(22, _layer(22), "samples"), (10, _layer(10), "samples"), (24, _layer(24), "expectation"),
+             (12, _layer(12), "statevector")]
+    decisions, groups = adaptive_plan("qiskit", batch)
+    engines = [d.engine for d in decisions]
+    assert engines == ["mettleq_gpu_statevector", "qiskit_aer_cpu_statevector",
+                       "mettleq_gpu_statevector", "qiskit_aer_cpu_statevector"]
+    assert list(groups) == ["mettleq_gpu_statevector", "qiskit_aer_cpu_statevector"]
+    assert groups["mettleq_gpu_statevector"] == [0, 2] and groups["qiskit_aer_cpu_statevector"] == [1, 3]
+    # without the SDK's CPU engine everything stays on this one
+    _, only = adaptive_plan("pennylane", batch, fallback_available=False)
+    assert list(only) == ["mettleq_gpu_statevector"] and only["mettleq_gpu_statevector"] == [0, 1, 2, 3]
+    # double precision keeps complex128 on the GPU
+    d2, _ = adaptive_plan("qiskit", batch[:1], precision="double")
+    assert d2[0].engine == "mettleq_gpu_statevector_complex128"
+
+
+def test_adaptive_plan_refuses_before_running_anything():
+    from mettleq_amd.integrations.policy import adaptive_plan
+    with pytest.raises(ValueError, match="dense preflight refused"):
+        adaptive_plan("qiskit", [(10, _layer(10), "samples"), (38, _layer(38), "samples")])
+    decisions, groups = adaptive_plan("qiskit", [(38, _layer(38), "samples")], allow_approximation=True)
+    assert list(groups) == ["qiskit_aer_cpu_mps"]
