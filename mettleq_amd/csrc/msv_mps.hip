@@ -415,8 +415,9 @@ __global__ void __launch_bounds__(kSvdThreads) k_mps_svd(const C *__restrict__ A
 // same bits, so the rotate/skip decision is uniform in the row).
 //   !kAccum: W = M (m x n); the columns of W converge to U S, so
 //            Q[:, rank j] = w_j / |w_j| (the first kmin of them);
-//   kAccum:  W = M^H (n x m) for wide M, with the rotations accumulated in
-//            V (m x m, also in LDS); W V = V_M S, so Q[:, rank j] = v_j.
+//   kAccum:  W = M^H (n x m), with the rotations accumulated in V (m x m;
+//            in LDS, or in global memory at Vg when kVGlobal);
+//            W V = V_M S, so Q[:, rank j] = v_j.
 // S gets the kmin largest column norms in descending order; info = sweeps
 // run, or -1 without convergence.
 constexpr int kBasisMaxDim = 128;
@@ -453,9 +454,10 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y * (2.0 - x * y);
 }
 
-template <bool kAccum, int RPL>
+template <bool kAccum, int RPL, bool kVGlobal = false>
 __global__ void __launch_bounds__(1024) k_mps_basis_lds(const C *__restrict__ M, int m, int n, C *__restrict__ Q,
-                                                        float *__restrict__ S, int *__restrict__ info) {
+                                                        float *__restrict__ S, int *__restrict__ info,
+                                                        C *__restrict__ Vg = nullptr) {
   static_assert(RPL >= 1 && RPL <= kRowsPerLane, "row slots per lane");
   __shared__ C lds[kBasisLds];
   __shared__ float sig[kBasisMaxDim];
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(1024) k_mps_basis_lds(const C *__restrict__ M,
   const int kk = kAccum ? n : m;  // rows of W
   const int k = kAccum ? m : n;   // columns of W
   C *W = lds;
-  C *V = lds + kk * k;
+  C *V = kVGlobal ? Vg : lds + kk * k;  // the rotations' accumulation (kAccum)
   double fro = 0;
   for (int idx = tid; idx < kk * k; idx += blockDim.x) {
     C a;
@@ -646,6 +648,8 @@ struct msv_mps {
   C *work = nullptr;              // 5 slots of 4 dmax^2: the factorization's input M, Q, X = Q^H M,
                                   // and W and V of the global-memory Jacobi (k_mps_svd)
   float *sv = nullptr;            // singular values (up to 2 dmax)
+  double *hh_beta = nullptr;      // reflector factors of the pivoted QR (up to 2 dmax)
+  int *hh_piv = nullptr;          // its pivots (up to 2 dmax)
   int *info = nullptr;
   std::vector<int> dim;           // bond dimensions, dim[0] = dim[n] = 1
   int center = 0;
@@ -814,23 +818,190 @@ __global__ void __launch_bounds__(1024) k_mps_qr_lds(const C *__restrict__ M, in
   }
 }
 
-template <bool kAccum>
+// Householder QR with column pivoting (Businger-Golub) of M (rows x cols,
+// column-major, rows * cols <= kBasisLds) in one workgroup, the
+// preconditioner of the two-site factorization: one-sided Jacobi on R^H of
+// a pivoted QR converges in a few sweeps where Jacobi on M itself needs many
+// for a graded spectrum (Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29
+// (2008)). Step j takes the unpicked column of largest norm over rows j..;
+// columns stay in place in LDS and piv[] names the physical column of each
+// logical one. Outputs: R = Q^H M P (kmin x cols, leading dimension kmin,
+// pivoted column order); the reflectors H_j = I - beta_j u_j u_j^H as A
+// (u_j in rows j.. of physical column piv[j], rows x cols), beta and piv,
+// for k_mps_hh_apply. Each step is two barriers: every thread finds the
+// pivot from the norms of the step before and the pivot's owner forms the
+// reflector; then the owners update their unpicked columns and their norms
+// over rows j + 1.., and the pivot's owner marks it picked.
+template <int RPL>
+__device__ __forceinline__ double col_norm2_from(const C *col, int r0, int rows, int l16) {
+  double s2 = 0;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = r0 + l16 + kPairLanes * t;
+    if (r < rows) {
+      const C x = col[r];
+      s2 += (double)x.x * x.x + (double)x.y * x.y;
+    }
+  }
+  return row16_sum(s2);
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, int rows, int cols,
+                                                       C *__restrict__ R, C *__restrict__ Aout,
+                                                       double *__restrict__ beta_out, int *__restrict__ piv_out) {
+  __shared__ C A[kBasisLds];
+  __shared__ double beta[kBasisMaxDim];
+  __shared__ C rdiag[kBasisMaxDim];
+  __shared__ double cn2[kBasisMaxDim];  // remaining norm^2 of each physical column
+  __shared__ int picked[kBasisMaxDim];
+  __shared__ int piv[kBasisMaxDim];     // physical column of each logical column
+  __shared__ double wave_part[kSvdThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int l16 = tid & (kPairLanes - 1), group = tid / kPairLanes, ngroups = blockDim.x / kPairLanes;
+  const int kmin = rows < cols ? rows : cols;
+  double fro = 0;
+  for (int idx = tid; idx < rows * cols; idx += blockDim.x) {
+    const C a = M[idx];
+    A[idx] = a;
+    fro += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  fro = wave_sum(fro);
+  if (lane == 0) wave_part[wave] = fro;
+  __syncthreads();
+  for (int c = group; c < cols; c += ngroups) {
+    const double s2 = col_norm2_from<RPL>(A + c * rows, 0, rows, l16);
+    if (l16 == 0) {
+      cn2[c] = s2;
+      picked[c] = 0;
+    }
+  }
+  fro = 0;
+  for (int w = 0; w < nwave; ++w) fro += wave_part[w];
+  const double floor2 = fmax(kJacobiFloor * kJacobiFloor * fro, 1e-66);
+  __syncthreads();
+  for (int j = 0; j < kmin; ++j) {
+    // the pivot: argmax over the unpicked columns, lowest index on ties (a
+    // NaN norm counts as 0), reduced across the wavefront; every wavefront
+    // gets the same result
+    double bv = -1.0;
+    int bi = 0;
+    for (int c = lane; c < cols; c += 64) {
+      if (picked[c]) continue;
+      const double v = cn2[c] >= 0.0 ? cn2[c] : 0.0;
+      if (v > bv) {
+        bv = v;
+        bi = c;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    const int p = bi;
+    if (p % ngroups == group) hh_form<RPL>(A + p * rows, j, rows, l16, floor2, beta, rdiag);
+    __syncthreads();
+    const double bj = beta[j];
+    const C *u = A + p * rows;
+    for (int c = group; c < cols; c += ngroups) {
+      if (c == p) {
+        if (l16 == 0) {
+          picked[c] = 1;
+          piv[j] = c;
+        }
+        continue;
+      }
+      if (picked[c]) continue;
+      C *col = A + c * rows;
+      if (bj != 0.0) hh_apply<RPL>(u, col, j, rows, l16, bj);
+      const double s2 = col_norm2_from<RPL>(col, j + 1, rows, l16);
+      if (l16 == 0) cn2[c] = s2;
+    }
+    __syncthreads();
+  }
+  // logical columns past kmin (wide M): the unpicked ones in index order
+  if (tid == 0) {
+    int at = kmin;
+    for (int c = 0; c < cols; ++c)
+      if (!picked[c]) piv[at++] = c;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < rows * cols; idx += blockDim.x) Aout[idx] = A[idx];
+  for (int j = tid; j < kmin; j += blockDim.x) {
+    beta_out[j] = beta[j];
+    piv_out[j] = piv[j];
+  }
+  for (int idx = tid; idx < kmin * cols; idx += blockDim.x) {
+    const int i = idx % kmin, c = idx / kmin;
+    C v = make_float2(0.f, 0.f);
+    if (i == c) v = rdiag[i];
+    else if (i < c) v = A[i + piv[c] * rows];
+    R[idx] = v;
+  }
+}
+
+// U = H_0 ... H_{kmin-1} [Ur; 0] (rows x kmin, column-major) for the
+// reflectors of k_mps_qrcp_lds and Ur (kmin x kmin): with R = Ur S Vr^H,
+// M P = Q R gives M's left singular vectors U = Q Ur. U sits in LDS; each
+// step stages u_j there and updates every column of U.
+template <int RPL>
+__global__ void __launch_bounds__(1024) k_mps_hh_apply(const C *__restrict__ A, const double *__restrict__ beta,
+                                                       const int *__restrict__ piv, const C *__restrict__ Ur,
+                                                       C *__restrict__ U, int rows, int kmin) {
+  __shared__ C Us[kBasisLds];
+  __shared__ C us[kBasisMaxDim];
+  const int tid = threadIdx.x;
+  const int l16 = tid & (kPairLanes - 1), group = tid / kPairLanes, ngroups = blockDim.x / kPairLanes;
+  for (int idx = tid; idx < rows * kmin; idx += blockDim.x) {
+    const int i = idx % rows, j = idx / rows;
+    Us[idx] = i < kmin ? Ur[i + static_cast<size_t>(j) * kmin] : make_float2(0.f, 0.f);
+  }
+  for (int j = kmin - 1; j >= 0; --j) {
+    const double bj = beta[j];
+    if (bj == 0.0) continue;  // the same for every thread
+    const C *ucol = A + static_cast<size_t>(piv[j]) * rows;
+    __syncthreads();  // the previous step's updates (or the load) are done with us
+    for (int r = j + tid; r < rows; r += blockDim.x) us[r] = ucol[r];
+    __syncthreads();
+    for (int c = group; c < kmin; c += ngroups) hh_apply<RPL>(us, Us + c * rows, j, rows, l16, bj);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < rows * kmin; idx += blockDim.x) U[idx] = Us[idx];
+}
+
+template <bool kAccum, bool kVGlobal = false>
 void launch_basis_rpl(int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
-                      int *info) {
+                      int *info, C *Vg = nullptr) {
   switch (rpl) {
     case 1:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 1>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 1, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
+                         info, Vg);
       break;
     case 2:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 2>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 2, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
+                         info, Vg);
       break;
     case 4:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 4>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 4, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
+                         info, Vg);
       break;
     default:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 8>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 8, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
+                         info, Vg);
       break;
   }
+}
+
+int rpl_for(int rows) { return rows <= 16 ? 1 : rows <= 32 ? 2 : rows <= 64 ? 4 : 8; }
+
+int threads_for_pairs(int k) {
+  const int npairs = (k + (k & 1)) / 2;
+  return std::min(kSvdThreads, std::max(64, (npairs * kPairLanes + 63) / 64 * 64));
 }
 
 void launch_basis(bool accum, int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
@@ -841,22 +1012,66 @@ void launch_basis(bool accum, int rpl, int threads, hipStream_t stream, const C 
     launch_basis_rpl<false>(rpl, threads, stream, M, m, n, Q, S, info);
 }
 
+// Smaller side from which a two-site factorization takes the pivoted-QR
+// preconditioner (below it, plain Jacobi on M is as fast).
+constexpr int kPrecondMin = 32;
+
 // Factorizes M (rows x cols, column-major in work slot 0) as M = Q X: Q
 // (rows x kmin, orthonormal columns in descending singular-value order) in
 // slot 1, the singular values in sv and h_sv, X = Q^H M (kmin x cols,
 // leading dimension kmin) in slot 2.
-int factor(msv_mps *m, int rows, int cols) {
+int factor(msv_mps *m, int rows, int cols, bool precondition = false) {
   const size_t slot = static_cast<size_t>(4) * m->dmax * m->dmax;
   const int kmin = std::min(rows, cols);
-  if (rows <= kBasisMaxDim && cols <= kBasisMaxDim) {
+  const bool fits_qrcp = rows <= kBasisMaxDim && cols <= kBasisMaxDim;  // M (and then U) in LDS
+  if (precondition && kmin >= kPrecondMin && fits_qrcp) {
+    // M P = Q R (pivoted QR), then Jacobi on R^H with the rotations
+    // accumulated: R = U_R S V_R^H and U = Q U_R. Slots: the reflectors in
+    // 3, R (kmin x cols) in 4, U_R in 2, V (when it does not fit in LDS
+    // beside R^H) in 1, U in 1.
+    C *refl = m->work + 3 * slot, *r = m->work + 4 * slot, *ur = m->work + 2 * slot, *u = m->work + slot;
+    const int qthreads = std::min(kSvdThreads, std::max(64, (cols * kPairLanes + 63) / 64 * 64));
+    const int hthreads = std::min(kSvdThreads, std::max(64, (kmin * kPairLanes + 63) / 64 * 64));
+    const int rp = rpl_for(rows);
+    if (rp == 1)
+      hipLaunchKernelGGL(k_mps_qrcp_lds<1>, dim3(1), dim3(qthreads), 0, m->stream, m->work, rows, cols, r, refl,
+                         m->hh_beta, m->hh_piv);
+    else if (rp == 2)
+      hipLaunchKernelGGL(k_mps_qrcp_lds<2>, dim3(1), dim3(qthreads), 0, m->stream, m->work, rows, cols, r, refl,
+                         m->hh_beta, m->hh_piv);
+    else if (rp == 4)
+      hipLaunchKernelGGL(k_mps_qrcp_lds<4>, dim3(1), dim3(qthreads), 0, m->stream, m->work, rows, cols, r, refl,
+                         m->hh_beta, m->hh_piv);
+    else
+      hipLaunchKernelGGL(k_mps_qrcp_lds<8>, dim3(1), dim3(qthreads), 0, m->stream, m->work, rows, cols, r, refl,
+                         m->hh_beta, m->hh_piv);
+    MPS_HIP(hipGetLastError());
+    // W = R^H (cols x kmin), V (kmin x kmin)
+    const int threads = threads_for_pairs(kmin);
+    if (static_cast<int64_t>(cols) * kmin + static_cast<int64_t>(kmin) * kmin <= kBasisLds)
+      launch_basis_rpl<true>(rpl_for(cols), threads, m->stream, r, kmin, cols, ur, m->sv, m->info);
+    else
+      launch_basis_rpl<true, true>(rpl_for(cols), threads, m->stream, r, kmin, cols, ur, m->sv, m->info, u);
+    MPS_HIP(hipGetLastError());
+    if (rp == 1)
+      hipLaunchKernelGGL(k_mps_hh_apply<1>, dim3(1), dim3(hthreads), 0, m->stream, refl, m->hh_beta, m->hh_piv, ur,
+                         u, rows, kmin);
+    else if (rp == 2)
+      hipLaunchKernelGGL(k_mps_hh_apply<2>, dim3(1), dim3(hthreads), 0, m->stream, refl, m->hh_beta, m->hh_piv, ur,
+                         u, rows, kmin);
+    else if (rp == 4)
+      hipLaunchKernelGGL(k_mps_hh_apply<4>, dim3(1), dim3(hthreads), 0, m->stream, refl, m->hh_beta, m->hh_piv, ur,
+                         u, rows, kmin);
+    else
+      hipLaunchKernelGGL(k_mps_hh_apply<8>, dim3(1), dim3(hthreads), 0, m->stream, refl, m->hh_beta, m->hh_piv, ur,
+                         u, rows, kmin);
+  } else if (rows <= kBasisMaxDim && cols <= kBasisMaxDim) {
     // wide matrices orthogonalize the rows of M (fewer columns) when W and V fit in LDS together
     const bool accum = cols > rows && cols * rows + rows * rows <= kBasisLds;
     const int k = accum ? rows : cols;
-    const int npairs = (k + (k & 1)) / 2;
-    const int threads = std::min(kSvdThreads, std::max(64, (npairs * kPairLanes + 63) / 64 * 64));
     const int kk = accum ? cols : rows;  // rows of the working matrix: 16 * RPL >= kk
-    launch_basis(accum, kk <= 16 ? 1 : kk <= 32 ? 2 : kk <= 64 ? 4 : 8, threads, m->stream, m->work, rows, cols,
-                 m->work + slot, m->sv, m->info);
+    launch_basis(accum, rpl_for(kk), threads_for_pairs(k), m->stream, m->work, rows, cols, m->work + slot, m->sv,
+                 m->info);
   } else {
     hipLaunchKernelGGL(k_mps_svd, dim3(1), dim3(kSvdThreads), 0, m->stream, m->work, rows, cols, m->work + 3 * slot,
                        m->work + 4 * slot, m->work + slot, m->sv, m->work + 2 * slot, m->info);
@@ -962,7 +1177,7 @@ int two_site(msv_mps *m, int i, const std::complex<double> *g, double eps) {
                      site_ptr(m, i), site_ptr(m, i + 1), m->work, dl, dm, dr, G);
   MPS_HIP(hipGetLastError());
   const int rows = 2 * dl, cols = 2 * dr, kmin = std::min(rows, cols);
-  MPS_TRY(factor(m, rows, cols));
+  MPS_TRY(factor(m, rows, cols, true));
   m->st.svd_calls += 1;
   const std::vector<float> &s = m->h_sv;
   // Exact zeros are dropped even at eps = 0: the device SVD's vectors for
@@ -1063,7 +1278,9 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
       hipMalloc(reinterpret_cast<void **>(&m->spare), sizeof(C) * m->cap) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->work), sizeof(C) * slot * 5) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->sv), sizeof(float) * 4 * dmax) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&m->info), sizeof(int)) != hipSuccess)
+      hipMalloc(reinterpret_cast<void **>(&m->info), sizeof(int)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&m->hh_beta), sizeof(double) * 2 * dmax) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&m->hh_piv), sizeof(int) * 2 * dmax) != hipSuccess)
     return cleanup(set_error(MSV_E_OOM, "MPS device allocation failed"));
   const int rc = msv_mps_reset(m);
   if (rc != MSV_OK) return cleanup(rc);
@@ -1080,7 +1297,8 @@ int msv_mps_destroy(msv_mps *m) {
     if (e == hipSuccess) e = es;
   }
   for (void *p : {static_cast<void *>(m->sites), static_cast<void *>(m->spare), static_cast<void *>(m->work),
-                  static_cast<void *>(m->sv), static_cast<void *>(m->info)}) {
+                  static_cast<void *>(m->sv), static_cast<void *>(m->info), static_cast<void *>(m->hh_beta),
+                  static_cast<void *>(m->hh_piv)}) {
     const hipError_t ef = hipFree(p);
     if (e == hipSuccess) e = ef;
   }
