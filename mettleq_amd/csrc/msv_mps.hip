@@ -13,7 +13,12 @@
 //   k_mps_basis_lds one-sided Jacobi in one workgroup with the matrix in LDS
 //                   (both sides up to 128; k_mps_svd, in global memory,
 //                   beyond): Q = U sorted by singular value, and S (the
-//                   reference: LAPACK cgesdd on the CPU);
+//                   reference: LAPACK cgesdd on the CPU). With both sides
+//                   >= 32, M is first factored by k_mps_qrcp_lds (pivoted
+//                   QR, M P = Q_p R), the Jacobi runs on R^H with its
+//                   rotations accumulated (U_R), and k_mps_hh_apply forms
+//                   U = Q_p U_R: a few sweeps instead of many on graded
+//                   spectra;
 //   host            the reference's truncation rule on the singular values
 //                   (mps_state.py:193-247), read back (min(2 dl, 2 dr) floats);
 //   k_mps_project   X = Q^H M;
