@@ -648,8 +648,10 @@ struct msv_mps {
   int device = 0;
   hipStream_t stream = nullptr;
   size_t cap = 0;                 // complex entries per site buffer (dmax x 2 x dmax)
-  C *sites = nullptr;             // n site buffers
-  C *spare = nullptr;             // one site buffer (out-of-place absorb)
+  C *sites = nullptr;             // the n site buffers' allocation
+  C *spare_block = nullptr;       // one more site buffer's allocation
+  std::vector<C *> site;          // buffer of each site: a center move absorbs into the spare
+  C *spare = nullptr;             // buffer, then trades places with the site's (no copy)
   C *work = nullptr;              // 5 slots of 4 dmax^2: the factorization's input M, Q, X = Q^H M,
                                   // and W and V of the global-memory Jacobi (k_mps_svd)
   float *sv = nullptr;            // singular values (up to 2 dmax)
@@ -683,7 +685,7 @@ int check_site(const msv_mps *m, int q) {
   if (q < 0 || q >= m->n) return set_error(MSV_E_INVALID, "MPS qubit index out of range");
   return MSV_OK;
 }
-C *site_ptr(msv_mps *m, int i) { return m->sites + static_cast<size_t>(i) * m->cap; }
+C *site_ptr(msv_mps *m, int i) { return m->site[i]; }
 
 // Reduced QR of M (rows x cols, column-major; rows * cols + rows * kmin <=
 // kBasisLds) in one workgroup with M in LDS, for the center moves (the
@@ -1133,8 +1135,7 @@ int move_right(msv_mps *m, int i) {
                      dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
                      site_ptr(m, i + 1), m->spare, rows, cols, kmin, dn);
   MPS_HIP(hipGetLastError());
-  MPS_HIP(hipMemcpyAsync(site_ptr(m, i + 1), m->spare, sizeof(C) * kmin * 2 * dn, hipMemcpyDeviceToDevice,
-                         m->stream));
+  std::swap(m->site[i + 1], m->spare);  // later writes to the old buffer are ordered after its reads
   m->dim[i + 1] = kmin;
   m->center = i + 1;
   return MSV_OK;
@@ -1153,8 +1154,7 @@ int move_left(msv_mps *m, int i) {
                      dim3(kThreads), 0, m->stream, m->work + slot, m->work + 2 * slot, site_ptr(m, i),
                      site_ptr(m, i - 1), m->spare, dl, rows, kmin, dp);
   MPS_HIP(hipGetLastError());
-  MPS_HIP(hipMemcpyAsync(site_ptr(m, i - 1), m->spare, sizeof(C) * dp * 2 * kmin, hipMemcpyDeviceToDevice,
-                         m->stream));
+  std::swap(m->site[i - 1], m->spare);
   m->dim[i] = kmin;
   m->center = i - 1;
   return MSV_OK;
@@ -1280,13 +1280,16 @@ int msv_mps_create(int n_qubits, int dmax, double eps, int renormalize, int devi
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(set_error(MSV_E_HIP, "stream creation failed"));
   if (hipMalloc(reinterpret_cast<void **>(&m->sites), sizeof(C) * m->cap * n_qubits) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void **>(&m->spare), sizeof(C) * m->cap) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void **>(&m->spare_block), sizeof(C) * m->cap) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->work), sizeof(C) * slot * 5) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->sv), sizeof(float) * 4 * dmax) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->info), sizeof(int)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->hh_beta), sizeof(double) * 2 * dmax) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&m->hh_piv), sizeof(int) * 2 * dmax) != hipSuccess)
     return cleanup(set_error(MSV_E_OOM, "MPS device allocation failed"));
+  m->site.resize(n_qubits);
+  for (int i = 0; i < n_qubits; ++i) m->site[i] = m->sites + static_cast<size_t>(i) * m->cap;
+  m->spare = m->spare_block;
   const int rc = msv_mps_reset(m);
   if (rc != MSV_OK) return cleanup(rc);
   *out = m;
@@ -1301,7 +1304,7 @@ int msv_mps_destroy(msv_mps *m) {
     const hipError_t es = hipStreamSynchronize(m->stream);
     if (e == hipSuccess) e = es;
   }
-  for (void *p : {static_cast<void *>(m->sites), static_cast<void *>(m->spare), static_cast<void *>(m->work),
+  for (void *p : {static_cast<void *>(m->sites), static_cast<void *>(m->spare_block), static_cast<void *>(m->work),
                   static_cast<void *>(m->sv), static_cast<void *>(m->info), static_cast<void *>(m->hh_beta),
                   static_cast<void *>(m->hh_piv)}) {
     const hipError_t ef = hipFree(p);
