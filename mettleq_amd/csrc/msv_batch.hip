@@ -64,6 +64,7 @@ enum BOpKind : uint32_t {
   BOP_ROT = 2,    // rotation record of a stage (never a top-level op)
   BOP_DIAG = 4,   // diagonal on index bits a (MSB) and b of phi (and lambda)
   BOP_CZ = 5,     // product of CZ signs over a bond list (phi and lambda)
+  BOP_ZDIAG = 6,  // the RZ rotations that end their qubits' rotations in a sweep (see bzdiag)
 };
 
 constexpr int kMaxCzList = 16; // non register-register CZ bonds per op
@@ -81,8 +82,10 @@ struct BOp {
   uint64_t amask, bmask;  // DIAG: index masks of a / b when not register bits
   float d[8];             // DIAG: d(a,b) for ab = 00, 01, 10, 11 as (re, im)
   // STAGE: list[j] = the rotation record on register bit j, or ~0; CZ: bond k = a | b << 8 | kind << 16 (kind 0: index bits a, b; kind 1:
-  // register bit a, index bit b)
+  // register bit a, index bit b); ZDIAG: list[k] = index bit | rotation << 8 (rotation number minus the
+  // sweep's first), the k < count rotations whose qubit is not a register bit
   uint32_t list[kMaxCzList];
+  uint32_t zreg[4];  // ZDIAG: rotation (minus the sweep's first) on register bit j, or ~0
 };
 
 struct BArgs {
@@ -94,7 +97,7 @@ struct BArgs {
   uint32_t n_gates;  // rotations per binding
   uint32_t mat_lo;   // this sweep's rotations: [mat_lo, mat_lo + mat_cnt) of the binding's table
   uint32_t mat_cnt;
-  uint32_t pad;
+  uint32_t zop;      // the sweep's BOP_ZDIAG op, or ~0
   uint32_t run_src[kMaxRuns];
   uint32_t run_width[kMaxRuns];
   uint32_t run_dst[kMaxRuns];
@@ -363,6 +366,114 @@ __device__ __forceinline__ void bstage(BOpCP st, BOpCP cops, FCP mrow, float2 *a
   bstage_bit<RB, 3, GEN, ADJ>(st, cops, mrow, a, l, part_t, row_stride);
 }
 
+// The RZ rotations that end their qubits' rotations in a sweep commute with
+// everything else in it (rotations on other qubits, the fixed phases), so
+// they run as one diagonal D: amplitude x is multiplied by e^{-i phi_k} or
+// e^{+i phi_k} for each such rotation k (its qubit's bit of x clear / set),
+// with e^{i phi} = (1 - t s, s) from the rotation's (t, s). D is a thread
+// factor (rotations on thread and tile bits) spread over the registers by
+// doubling with the register bits' factors: 30 complex products where the
+// shears of 12 rotations cost 576 FMAs.
+// Forward: phi <- D phi after the last segment's ops. Adjoint, before the
+// first segment's ops: the gradient partial of rotation k,
+// sum over the thread's amplitudes of z_k(x) Im(conj(lambda(x)) phi(x))
+// (z_k = +1 / -1 for its bit clear / set; D's other factors commute with
+// Z_k, so this is the partial just after the rotation), as signed register
+// sums of w = Im(conj(lambda) phi) for register bits and +-(sum of w) for
+// the others, one wave sum each; then phi, lambda <- D^dagger (phi, lambda).
+// a conj(b)
+__device__ __forceinline__ float2 cmul_conj(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+
+template <int RB, bool ADJ>
+__device__ __forceinline__ void bzdiag(BOpCP zop, FCP mrow, float2 *a, float2 *l, uint64_t gb,
+                                       double *__restrict__ part_t, uint64_t row_stride, uint32_t mat_lo) {
+  static_assert(RB == 4, "the register sums and factors cover register bits 0..3");
+  constexpr int R = 1 << RB;
+  const uint32_t nt = zop->count;
+  if (ADJ) {
+    float w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) w[r] = fmaf(l[r].x, a[r].y, -l[r].y * a[r].x);
+    // one level per register bit: pair sums go on, pair differences sum
+    // to that bit's signed sum
+    float p1[8], m1[8], p2[4], m2[4];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      p1[r] = w[2 * r] + w[2 * r + 1];
+      m1[r] = w[2 * r] - w[2 * r + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      p2[r] = p1[2 * r] + p1[2 * r + 1];
+      m2[r] = p1[2 * r] - p1[2 * r + 1];
+    }
+    float sj[RB];
+    sj[0] = ((m1[0] + m1[1]) + (m1[2] + m1[3])) + ((m1[4] + m1[5]) + (m1[6] + m1[7]));
+    sj[1] = (m2[0] + m2[1]) + (m2[2] + m2[3]);
+    sj[2] = (p2[0] - p2[1]) + (p2[2] - p2[3]);
+    sj[3] = (p2[0] + p2[1]) - (p2[2] + p2[3]);
+    const float wsum = (p2[0] + p2[1]) + (p2[2] + p2[3]);
+#pragma unroll 1
+    for (uint32_t k = 0; k < nt; ++k) {
+      const uint32_t e = zop->list[k];
+      const float g = wave_sum_dpp(((gb >> (e & 63u)) & 1ull) ? -wsum : wsum);
+      if ((threadIdx.x & 63u) == 0u) part_t[static_cast<uint64_t>(mat_lo + (e >> 8)) * row_stride] = g;
+    }
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const uint32_t o = zop->zreg[j];
+      if (o == ~0u) continue;
+      const float g = wave_sum_dpp(sj[j]);
+      if ((threadIdx.x & 63u) == 0u) part_t[static_cast<uint64_t>(mat_lo + o) * row_stride] = g;
+    }
+  }
+  float2 f = make_float2(1.f, 0.f);
+#pragma unroll 1
+  for (uint32_t k = 0; k < nt; ++k) {
+    const uint32_t e = zop->list[k];
+    const uint32_t o = e >> 8;
+    const float t = mrow[2u * o], s = mrow[2u * o + 1u];
+    f = cmul(f, make_float2(fmaf(-t, s, 1.f), ((gb >> (e & 63u)) & 1ull) ? s : -s));
+  }
+  // register factors e^{i phi_j}; rotation 0 of the sweep stands in for a
+  // register bit without one (always a valid row), its factor replaced by 1
+  float2 e[RB];
+#pragma unroll
+  for (int j = 0; j < RB; ++j) {
+    const uint32_t o = zop->zreg[j];
+    const uint32_t oo = o == ~0u ? 0u : o;
+    const float t = mrow[2u * oo], sv = mrow[2u * oo + 1u];
+    e[j] = make_float2(o == ~0u ? 1.f : fmaf(-t, sv, 1.f), o == ~0u ? 0.f : sv);
+  }
+  // D at register index r = G[r & 3] H[r >> 2]: G holds the thread factor
+  // and register bits 0, 1, H register bits 2, 3 (two 4-entry tables keep
+  // the registers a 16-entry one would take)
+  float2 G[4], H[4];
+  {
+    const float2 f0 = cmul_conj(f, e[0]), f1 = cmul(f, e[0]);
+    G[0] = cmul_conj(f0, e[1]);
+    G[1] = cmul_conj(f1, e[1]);
+    G[2] = cmul(f0, e[1]);
+    G[3] = cmul(f1, e[1]);
+    const float2 h0 = make_float2(e[2].x, -e[2].y);
+    H[0] = cmul_conj(h0, e[3]);
+    H[1] = cmul_conj(e[2], e[3]);
+    H[2] = cmul(h0, e[3]);
+    H[3] = cmul(e[2], e[3]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (ADJ) {
+      a[r] = cmul_conj(cmul_conj(a[r], G[r & 3]), H[r >> 2]);
+      l[r] = cmul_conj(cmul_conj(l[r], G[r & 3]), H[r >> 2]);
+    } else {
+      a[r] = cmul(cmul(a[r], G[r & 3]), H[r >> 2]);
+    }
+  }
+}
+
 // Stage-generator programs known at compile time: PAT holds the
 // generators of a segment's stages in base-4 digits, first stage lowest
 // (digit = generator + 1, 0 ends the program); PAT = 0 is the generic
@@ -424,6 +535,8 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = lds_a[tsw ^ lds_swz<RB>(static_cast<uint32_t>(r) << reg0)];
       }  // ADJ: phi and lambda were read at the end of the previous segment
+      if (ADJ && s == 0 && A.zop != ~0u)
+        bzdiag<RB, true>(cops + A.zop, mrow, a, l, gb, part_t, row_stride, A.mat_lo);
       if constexpr (PAT == 0) {
         for (uint32_t o = A.op_begin[s]; o < A.op_end[s]; ++o) {
           const BOpCP opp = cops + o;
@@ -453,6 +566,8 @@ __global__ void __launch_bounds__(256) k_btile(float2 *__restrict__ phi, float2 
         if (npost > 0u) bcz<RB, ADJ>(post, a, l, gb);
         if (npost > 1u) bcz<RB, ADJ>(post + 1, a, l, gb);
       }
+      if (!ADJ && s + 1 == A.nseg && A.zop != ~0u)
+        bzdiag<RB, false>(cops + A.zop, mrow, a, l, gb, part_t, row_stride, A.mat_lo);
       if (s + 1 == A.nseg) {
         uint64_t gbs = gb;
         asm volatile("" : "+v"(gbs));
@@ -730,6 +845,15 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
   int last_used = -1;
   for (size_t s = 0; s < sweeps.size(); ++s)
     if (!sweep_rots[s].empty()) last_used = static_cast<int>(s);
+  // an RZ rotation that ends its qubit's rotations in the block commutes
+  // with the rest of its sweep: it joins the sweep's diagonal (bzdiag)
+  std::vector<char> in_zdiag(a->gates.size(), 0);
+  {
+    std::vector<int> last(static_cast<size_t>(a->n), -1);
+    for (int gi : rots) last[static_cast<size_t>(a->gates[gi].q0)] = gi;
+    for (int gi : last)
+      if (gi >= 0 && a->gates[gi].kind == MSV_ROT_Z) in_zdiag[static_cast<size_t>(gi)] = 1;
+  }
   const int phase_sweep = backward ? -1 : last_used;  // forward: phases after the last rotations
   bool phases_done = phases.empty();
   const uint64_t total_bits = static_cast<uint64_t>(n);
@@ -755,7 +879,8 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
     for (int i = 0; i < 64; ++i) loc_of[i] = -1;
     for (int k = 0; k < nlocal; ++k) loc_of[local[k]] = k;
     uint32_t need = 0;
-    for (int gi : sweep_rots[s]) need |= 1u << loc_of[bit_of(a, a->gates[gi].q0)];
+    for (int gi : sweep_rots[s])
+      if (!in_zdiag[static_cast<size_t>(gi)]) need |= 1u << loc_of[bit_of(a, a->gates[gi].q0)];
     std::vector<int> wins = need ? windows_for(need, nlocal, rb) : std::vector<int>{nlocal - rb};
     if (wins.empty()) return set_error(MSV_E_INVALID, "internal: no register window cover");
     BSweep sw;
@@ -805,6 +930,12 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
       all_cz = all_cz && g.kind - 100 == MSV_PHASE_CPHASE && std::abs(std::fabs(g.theta) - M_PI) < 1e-15;
     }
     std::vector<bool> placed(sweep_rots[s].size(), false);
+    for (size_t i = 0; i < sweep_rots[s].size(); ++i) placed[i] = in_zdiag[static_cast<size_t>(sweep_rots[s][i])] != 0;
+    BOp zd;
+    std::memset(&zd, 0, sizeof(zd));
+    zd.kind = BOP_ZDIAG;
+    for (int j = 0; j < 4; ++j) zd.zreg[j] = ~0u;
+    bool has_zd = false;
     uint32_t pat = ~0u;  // the sweep's stage program (~0: none seen yet, 0: generic)
     for (size_t w = 0; w < wins.size(); ++w) {
       TileSeg &sg = A.seg[w];
@@ -881,6 +1012,26 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
           P->ops.push_back(op);
         }
       };
+      // the diagonal in the layout of the segment that runs it: the last
+      // (forward) or the first (adjoint)
+      if (w == (backward ? 0 : wins.size() - 1)) {
+        for (int gi : sweep_rots[s]) {
+          if (!in_zdiag[static_cast<size_t>(gi)]) continue;
+          const BGate &g = a->gates[gi];
+          const int bit = bit_of(a, g.q0);
+          const int j = rpos(bit);
+          const uint32_t o = static_cast<uint32_t>(g.rot_no - mat_lo);
+          if (o >= (1u << 24)) return set_error(MSV_E_INVALID, "internal: diagonal rotation range");
+          if (j >= 0) {
+            if (zd.zreg[j] != ~0u) return set_error(MSV_E_INVALID, "internal: two diagonal rotations on one bit");
+            zd.zreg[j] = o;
+          } else {
+            if (zd.count == kMaxCzList) return set_error(MSV_E_INVALID, "internal: diagonal list");
+            zd.list[zd.count++] = static_cast<uint32_t>(bit) | o << 8;
+          }
+          has_zd = true;
+        }
+      }
       // rotations whose bit is a register bit of this window, not yet placed
       std::vector<int> here;
       for (size_t i = 0; i < sweep_rots[s].size(); ++i) {
@@ -951,6 +1102,11 @@ int plan_block(const msv_ansatz *a, const std::vector<int> &rots, const std::vec
     // segments without stages fit any program; a sweep without rotations
     // runs the generic form
     sw.pat = pat == ~0u ? 0u : pat;
+    A.zop = ~0u;
+    if (has_zd) {
+      A.zop = static_cast<uint32_t>(P->ops.size());
+      P->ops.push_back(zd);
+    }
     for (bool p : placed)
       if (!p) return set_error(MSV_E_INVALID, "internal: rotation left unscheduled");
     if (put_phases) phases_done = true;
