@@ -116,6 +116,61 @@ def test_compiled_stage_programs_match_oracle(gpu, seq):
             assert np.max(np.abs(g[b] - want_g)) < TOL_G, (b, np.max(np.abs(g[b] - want_g)))
 
 
+def test_rz_diagonal_layouts_match_oracle(gpu):
+    """RZ rotations that end their qubits' rotations in a block run as one
+    diagonal per sweep (forward after the last segment, adjoint before the
+    first). Blocks here: RZ only (sweeps with a diagonal and no stage),
+    RX/RY followed by RZ on some qubits and ending on RX/RY on others, and a
+    run of two RZ on one qubit sharing one parameter (the first stays a
+    stage rotation, the second joins the diagonal); CZ rings and ZZ phases
+    between blocks; 14 qubits, so every block has a low and a high sweep
+    and the diagonal holds register and thread bits."""
+    from mettleq_amd.ansatz import LayeredAnsatz
+    n, batch = 14, 2
+    rng = np.random.default_rng(1414)
+    blocks = [
+        [("RZ",)] * n,
+        [("RX", "RZ") if q % 3 else ("RY",) for q in range(n)],
+        [("RY", "RZ", "RZ") if q % 2 else ("RZ", "RX") for q in range(n)],
+        [("RZ",) if q < 7 else ("RY", "RZ") for q in range(n)],
+    ]
+    a = LayeredAnsatz(n)
+    gates, p = [], 0
+    for k, block in enumerate(blocks):
+        for q, seq in enumerate(block):
+            shared = None
+            for name in seq:
+                if name == "RZ" and len(seq) == 3:
+                    # both RZ of the run take one parameter
+                    if shared is None:
+                        shared = p
+                        p += 1
+                    idx = shared
+                else:
+                    idx = p
+                    p += 1
+                a.rotation(name, q, idx)
+                gates.append((name, q, idx))
+        for q in range(n):
+            if k % 2 == 0:
+                a.cz(q, (q + 1) % n)
+                gates.append(("CPHASE", q, (q + 1) % n, math.pi))
+            else:
+                th = float(rng.uniform(0, 3))
+                a.phase("ZZPHASE", (q + 1) % n, q, th)
+                gates.append(("ZZPHASE", (q + 1) % n, q, th))
+    terms = [((q, (q + 1) % n), float(rng.uniform(-1, 1))) for q in range(n)] + [((5,), 0.4), ((12,), -0.3)]
+    a.observable(terms)
+    with a:
+        params = rng.uniform(-2 * math.pi, 2 * math.pi, (batch, a.n_params))
+        e, g = a.evaluate(params)
+        for b in range(batch):
+            want_e = O.ansatz_energy(n, gates, params[b], terms)
+            assert abs(e[b] - want_e) < TOL_E, (b, e[b], want_e)
+            want_g = O.ansatz_param_shift(n, gates, params[b], terms)
+            assert np.max(np.abs(g[b] - want_g)) < TOL_G, (b, np.max(np.abs(g[b] - want_g)))
+
+
 def test_gradients_match_central_differences(gpu):
     rng = np.random.default_rng(5)
     n = 5
