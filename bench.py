@@ -500,7 +500,7 @@ def extra_summaries(args):
                      ("brick20", lambda: bench_brick20(args, quick=True)),
                      ("vqe16_adjoint", lambda: bench_vqe16(args, quick=True)),
                      ("serve_sweep", lambda: bench_serve(args, rates=[50.0, 200.0, 1000.0, 2000.0],
-                                                         n_requests=400, quick=True)),
+                                                         n_requests=2000, quick=True)),
                      ("mps40", lambda: bench_mps(args, quick=True))):
         try:
             t0 = time.perf_counter()
