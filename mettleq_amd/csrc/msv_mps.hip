@@ -15,10 +15,12 @@
 //                   beyond): Q = U sorted by singular value, and S (the
 //                   reference: LAPACK cgesdd on the CPU). With both sides
 //                   >= 32, M is first factored by k_mps_qrcp_lds (pivoted
-//                   QR, M P = Q_p R), the Jacobi runs on R^H with its
-//                   rotations accumulated (U_R), and k_mps_hh_apply forms
-//                   U = Q_p U_R: a few sweeps instead of many on graded
-//                   spectra;
+//                   QR, M P = Q_p R), then R^T by the same kernel without
+//                   pivoting (R^T = Q_2 R_2, R_2 only); the Jacobi runs on
+//                   L = R_2^T, whose normalized columns are U_R, the left
+//                   singular vectors of R (R = L Q_2^T), and k_mps_hh_apply
+//                   forms U = Q_p U_R: a few sweeps instead of many on
+//                   graded spectra, with no rotations accumulated;
 //   host            the reference's truncation rule on the singular values
 //                   (mps_state.py:193-247), read back (min(2 dl, 2 dr) floats);
 //   k_mps_project   X = Q^H M;
@@ -421,7 +423,7 @@ __global__ void __launch_bounds__(kSvdThreads) k_mps_svd(const C *__restrict__ A
 //   !kAccum: W = M (m x n); the columns of W converge to U S, so
 //            Q[:, rank j] = w_j / |w_j| (the first kmin of them);
 //   kAccum:  W = M^H (n x m), with the rotations accumulated in V (m x m;
-//            in LDS, or in global memory at Vg when kVGlobal);
+//            in LDS beside W);
 //            W V = V_M S, so Q[:, rank j] = v_j.
 // S gets the kmin largest column norms in descending order; info = sweeps
 // run, or -1 without convergence.
@@ -459,10 +461,9 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y * (2.0 - x * y);
 }
 
-template <bool kAccum, int RPL, bool kVGlobal = false>
+template <bool kAccum, int RPL>
 __global__ void __launch_bounds__(1024) k_mps_basis_lds(const C *__restrict__ M, int m, int n, C *__restrict__ Q,
-                                                        float *__restrict__ S, int *__restrict__ info,
-                                                        C *__restrict__ Vg = nullptr) {
+                                                        float *__restrict__ S, int *__restrict__ info) {
   static_assert(RPL >= 1 && RPL <= kRowsPerLane, "row slots per lane");
   __shared__ C lds[kBasisLds];
   __shared__ float sig[kBasisMaxDim];
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(1024) k_mps_basis_lds(const C *__restrict__ M,
   const int kk = kAccum ? n : m;  // rows of W
   const int k = kAccum ? m : n;   // columns of W
   C *W = lds;
-  C *V = kVGlobal ? Vg : lds + kk * k;  // the rotations' accumulation (kAccum)
+  C *V = lds + kk * k;  // the rotations' accumulation (kAccum)
   double fro = 0;
   for (int idx = tid; idx < kk * k; idx += blockDim.x) {
     C a;
@@ -827,9 +828,10 @@ __global__ void __launch_bounds__(1024) k_mps_qr_lds(const C *__restrict__ M, in
 
 // Householder QR with column pivoting (Businger-Golub) of M (rows x cols,
 // column-major, rows * cols <= kBasisLds) in one workgroup, the
-// preconditioner of the two-site factorization: one-sided Jacobi on R^H of
-// a pivoted QR converges in a few sweeps where Jacobi on M itself needs many
-// for a graded spectrum (Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29
+// preconditioner of the two-site factorization: one-sided Jacobi on the
+// triangular factor of a pivoted QR (after a second, unpivoted QR of R^T)
+// converges in a few sweeps where Jacobi on M itself needs many for a
+// graded spectrum (Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29
 // (2008)). Step j takes the unpicked column of largest norm over rows j..;
 // columns stay in place in LDS and piv[] names the physical column of each
 // logical one. Outputs: R = Q^H M P (kmin x cols, leading dimension kmin,
@@ -839,6 +841,9 @@ __global__ void __launch_bounds__(1024) k_mps_qr_lds(const C *__restrict__ M, in
 // pivot from the norms of the step before and the pivot's owner forms the
 // reflector; then the owners update their unpicked columns and their norms
 // over rows j + 1.., and the pivot's owner marks it picked.
+// kPivot = false: plain Householder QR (step j reflects column j; the norms
+// are not kept), for the second factorization of the preconditioner, which
+// needs R only (Aout == nullptr: no reflectors, beta or piv written).
 template <int RPL>
 __device__ __forceinline__ double col_norm2_from(const C *col, int r0, int rows, int l16) {
   double s2 = 0;
@@ -853,7 +858,7 @@ __device__ __forceinline__ double col_norm2_from(const C *col, int r0, int rows,
   return row16_sum(s2);
 }
 
-template <int RPL>
+template <int RPL, bool kPivot = true>
 __global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, int rows, int cols,
                                                        C *__restrict__ R, C *__restrict__ Aout,
                                                        double *__restrict__ beta_out, int *__restrict__ piv_out) {
@@ -877,7 +882,7 @@ __global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, 
   if (lane == 0) wave_part[wave] = fro;
   __syncthreads();
   for (int c = group; c < cols; c += ngroups) {
-    const double s2 = col_norm2_from<RPL>(A + c * rows, 0, rows, l16);
+    const double s2 = kPivot ? col_norm2_from<RPL>(A + c * rows, 0, rows, l16) : 0.0;
     if (l16 == 0) {
       cn2[c] = s2;
       picked[c] = 0;
@@ -891,26 +896,29 @@ __global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, 
     // the pivot: argmax over the unpicked columns, lowest index on ties (a
     // NaN norm counts as 0), reduced across the wavefront; every wavefront
     // gets the same result
-    double bv = -1.0;
-    int bi = 0;
-    for (int c = lane; c < cols; c += 64) {
-      if (picked[c]) continue;
-      const double v = cn2[c] >= 0.0 ? cn2[c] : 0.0;
-      if (v > bv) {
-        bv = v;
-        bi = c;
+    int p = j;
+    if (kPivot) {
+      double bv = -1.0;
+      int bi = 0;
+      for (int c = lane; c < cols; c += 64) {
+        if (picked[c]) continue;
+        const double v = cn2[c] >= 0.0 ? cn2[c] : 0.0;
+        if (v > bv) {
+          bv = v;
+          bi = c;
+        }
       }
-    }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bv, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > bv || (ov == bv && oi < bi)) {
-        bv = ov;
-        bi = oi;
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
       }
+      p = bi;
     }
-    const int p = bi;
     if (p % ngroups == group) hh_form<RPL>(A + p * rows, j, rows, l16, floor2, beta, rdiag);
     __syncthreads();
     const double bj = beta[j];
@@ -926,8 +934,10 @@ __global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, 
       if (picked[c]) continue;
       C *col = A + c * rows;
       if (bj != 0.0) hh_apply<RPL>(u, col, j, rows, l16, bj);
-      const double s2 = col_norm2_from<RPL>(col, j + 1, rows, l16);
-      if (l16 == 0) cn2[c] = s2;
+      if (kPivot) {
+        const double s2 = col_norm2_from<RPL>(col, j + 1, rows, l16);
+        if (l16 == 0) cn2[c] = s2;
+      }
     }
     __syncthreads();
   }
@@ -938,10 +948,12 @@ __global__ void __launch_bounds__(1024) k_mps_qrcp_lds(const C *__restrict__ M, 
       if (!picked[c]) piv[at++] = c;
   }
   __syncthreads();
-  for (int idx = tid; idx < rows * cols; idx += blockDim.x) Aout[idx] = A[idx];
-  for (int j = tid; j < kmin; j += blockDim.x) {
-    beta_out[j] = beta[j];
-    piv_out[j] = piv[j];
+  if (Aout != nullptr) {
+    for (int idx = tid; idx < rows * cols; idx += blockDim.x) Aout[idx] = A[idx];
+    for (int j = tid; j < kmin; j += blockDim.x) {
+      beta_out[j] = beta[j];
+      piv_out[j] = piv[j];
+    }
   }
   for (int idx = tid; idx < kmin * cols; idx += blockDim.x) {
     const int i = idx % kmin, c = idx / kmin;
@@ -981,25 +993,21 @@ __global__ void __launch_bounds__(1024) k_mps_hh_apply(const C *__restrict__ A, 
   for (int idx = tid; idx < rows * kmin; idx += blockDim.x) U[idx] = Us[idx];
 }
 
-template <bool kAccum, bool kVGlobal = false>
+template <bool kAccum>
 void launch_basis_rpl(int rpl, int threads, hipStream_t stream, const C *M, int m, int n, C *Q, float *S,
-                      int *info, C *Vg = nullptr) {
+                      int *info) {
   switch (rpl) {
     case 1:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 1, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
-                         info, Vg);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 1>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
       break;
     case 2:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 2, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
-                         info, Vg);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 2>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
       break;
     case 4:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 4, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
-                         info, Vg);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 4>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
       break;
     default:
-      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 8, kVGlobal>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S,
-                         info, Vg);
+      hipLaunchKernelGGL((k_mps_basis_lds<kAccum, 8>), dim3(1), dim3(threads), 0, stream, M, m, n, Q, S, info);
       break;
   }
 }
@@ -1032,10 +1040,9 @@ int factor(msv_mps *m, int rows, int cols, bool precondition = false) {
   const int kmin = std::min(rows, cols);
   const bool fits_qrcp = rows <= kBasisMaxDim && cols <= kBasisMaxDim;  // M (and then U) in LDS
   if (precondition && kmin >= kPrecondMin && fits_qrcp) {
-    // M P = Q R (pivoted QR), then Jacobi on R^H with the rotations
-    // accumulated: R = U_R S V_R^H and U = Q U_R. Slots: the reflectors in
-    // 3, R (kmin x cols) in 4, U_R in 2, V (when it does not fit in LDS
-    // beside R^H) in 1, U in 1.
+    // M P = Q R (pivoted QR), then U_R, the left singular vectors of R
+    // (R = U_R S V_R^H), and U = Q U_R. Slots: the reflectors in 3, R
+    // (kmin x cols) in 4, U_R in 2, U in 1.
     C *refl = m->work + 3 * slot, *r = m->work + 4 * slot, *ur = m->work + 2 * slot, *u = m->work + slot;
     const int qthreads = std::min(kSvdThreads, std::max(64, (cols * kPairLanes + 63) / 64 * 64));
     const int hthreads = std::min(kSvdThreads, std::max(64, (kmin * kPairLanes + 63) / 64 * 64));
@@ -1053,12 +1060,35 @@ int factor(msv_mps *m, int rows, int cols, bool precondition = false) {
       hipLaunchKernelGGL(k_mps_qrcp_lds<8>, dim3(1), dim3(qthreads), 0, m->stream, m->work, rows, cols, r, refl,
                          m->hh_beta, m->hh_piv);
     MPS_HIP(hipGetLastError());
-    // W = R^H (cols x kmin), V (kmin x kmin)
-    const int threads = threads_for_pairs(kmin);
-    if (static_cast<int64_t>(cols) * kmin + static_cast<int64_t>(kmin) * kmin <= kBasisLds)
-      launch_basis_rpl<true>(rpl_for(cols), threads, m->stream, r, kmin, cols, ur, m->sv, m->info);
+    // Second QR (Drmac and Veselic): R^T = Q2 R2 (R2 only) and L = R2^T, so
+    // R = L Q2^T and the left singular vectors of L are U_R. Jacobi on L's
+    // columns converges like Jacobi on R^H and gives them as its normalized
+    // columns, with no rotations to keep (on R^H they would have to be
+    // accumulated, in global memory once both exceed LDS). R^T (cols x
+    // kmin) in slot 1, R2 (kmin x kmin) in 2, L in 1.
+    C *rt = u, *r2 = ur, *lt = u;
+    hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(kmin) * cols)), dim3(kThreads), 0,
+                       m->stream, r, rt, cols, kmin, 0);
+    MPS_HIP(hipGetLastError());
+    const int q2threads = std::min(kSvdThreads, std::max(64, (kmin * kPairLanes + 63) / 64 * 64));
+    const int rp2 = rpl_for(cols);
+    if (rp2 == 1)
+      hipLaunchKernelGGL((k_mps_qrcp_lds<1, false>), dim3(1), dim3(q2threads), 0, m->stream, rt, cols, kmin, r2,
+                         nullptr, nullptr, nullptr);
+    else if (rp2 == 2)
+      hipLaunchKernelGGL((k_mps_qrcp_lds<2, false>), dim3(1), dim3(q2threads), 0, m->stream, rt, cols, kmin, r2,
+                         nullptr, nullptr, nullptr);
+    else if (rp2 == 4)
+      hipLaunchKernelGGL((k_mps_qrcp_lds<4, false>), dim3(1), dim3(q2threads), 0, m->stream, rt, cols, kmin, r2,
+                         nullptr, nullptr, nullptr);
     else
-      launch_basis_rpl<true, true>(rpl_for(cols), threads, m->stream, r, kmin, cols, ur, m->sv, m->info, u);
+      hipLaunchKernelGGL((k_mps_qrcp_lds<8, false>), dim3(1), dim3(q2threads), 0, m->stream, rt, cols, kmin, r2,
+                         nullptr, nullptr, nullptr);
+    MPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_mps_to_colmajor, dim3(blocks_for(static_cast<uint64_t>(kmin) * kmin)), dim3(kThreads), 0,
+                       m->stream, r2, lt, kmin, kmin, 0);
+    MPS_HIP(hipGetLastError());
+    launch_basis_rpl<false>(rpl_for(kmin), threads_for_pairs(kmin), m->stream, lt, kmin, kmin, ur, m->sv, m->info);
     MPS_HIP(hipGetLastError());
     if (rp == 1)
       hipLaunchKernelGGL(k_mps_hh_apply<1>, dim3(1), dim3(hthreads), 0, m->stream, refl, m->hh_beta, m->hh_piv, ur,
