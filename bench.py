@@ -317,7 +317,10 @@ def bench_mps(args, quick=False):
         "max_bond": d["maximum_bond_dimension_reached"],
         "bond_mid": bonds[n // 2],
         "truncation_events": d["events"],
+        "truncation_events_cpu": ref.trunc_events,
         "relative_discarded_weight_sum": d["relative_discarded_weight_sum"],
+        "discarded_weight_sum": d["local_discarded_weight_sum"],
+        "discarded_weight_sum_cpu": float(ref.discarded_sum),
         "zz01_gpu": round(float(zz.real), 6),
         "zz01_cpu": round(float(zz_ref.real), 6),
         "cpu_kind": "port (NumPy/LAPACK restatement of the reference's CPU MPS)",
